@@ -1,0 +1,251 @@
+"""ctypes binding of libbwpm_hip.so: the module surface of the reference's `bwpm._core`.
+
+Reference: cpp/python/bindings.cpp:122-180 (`transmission_matrix`, `reflection_matrix`,
+`longitudinal_wavenumber`, `CudaSolver(...).run`, `HAS_CUDA`, `cuda_device_count`). Here the solver
+is `GpuSolver` (alias `CudaSolver`), computing on an MI355X through the C ABI of
+include/bwpm_hip.h. There is no CPU fallback: if the library is missing the import fails, and
+without a GPU every compute call raises RuntimeError.
+
+Error mapping as in the reference binding: invalid arguments → ValueError; device, allocation,
+non-convergence → RuntimeError.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+from typing import Any
+
+import numpy as np
+
+_LIB_PATH = Path(os.environ.get("BWPM_HIP_LIB", Path(__file__).resolve().parent / "lib" / "libbwpm_hip.so"))
+if not _LIB_PATH.exists():
+    raise ImportError(
+        f"libbwpm_hip.so not found at {_LIB_PATH}; build it with `python -m bwpm_amd.build`"
+    )
+_lib = ctypes.CDLL(str(_LIB_PATH))
+
+OK, EINVAL, ENOCONV, EDEVICE, ENOMEM, EINTERNAL = 0, 1, 2, 3, 4, 5
+PRECISIONS = {"fp64": 64, "fp32": 32, "fp32_plain": 31}
+
+_d = ctypes.c_double
+_i64 = ctypes.c_int64
+_p = ctypes.c_void_p
+_int = ctypes.c_int
+
+
+def _sig(name: str, *args: Any) -> Any:
+    f = getattr(_lib, name)
+    f.argtypes = list(args)
+    f.restype = ctypes.c_int
+    return f
+
+
+_lib.bwpm_last_error.restype = ctypes.c_char_p
+_lib.bwpm_last_error.argtypes = []
+_device_count = _sig("bwpm_device_count", ctypes.POINTER(_int))
+_tmat = _sig("bwpm_transmission_matrix", _p, _p, _i64, _d, _d, _d, _d, _d, _p)
+_rmat = _sig("bwpm_reflection_matrix", _p, _p, _i64, _d, _d, _d, _d, _d, _p)
+_kz = _sig("bwpm_longitudinal_wavenumber", _d, _d, _d, _p, _i64, _p)
+_create = _sig("bwpm_solver_create", _i64, _i64, _d, _d, _d, _d, _p, _i64, _p, _i64, _d, _d, _p, _p, _int,
+               _int, ctypes.POINTER(_p))
+_destroy = _sig("bwpm_solver_destroy", _p)
+_run = _sig("bwpm_solver_run", _p, _p, _int, _p, _i64, ctypes.POINTER(_p))
+_res_field = _sig("bwpm_result_field", _p, _int, _p)
+_res_iters = _sig("bwpm_result_iterations", _p, ctypes.POINTER(_i64))
+_res_iter_field = _sig("bwpm_result_iter_field", _p, _int, _i64, _p)
+_res_plane = _sig("bwpm_result_plane", _p, _int, _i64, _p, ctypes.POINTER(_int))
+_res_stored = _sig("bwpm_result_stored_values", _p, ctypes.POINTER(_i64))
+_res_stats = _sig("bwpm_result_stats", _p, ctypes.POINTER(_d), ctypes.POINTER(_i64), ctypes.POINTER(_i64))
+_res_free = _sig("bwpm_result_free", _p)
+_upload = _sig("bwpm_solver_upload", _p, _p)
+_run_resident = _sig("bwpm_solver_run_resident", _p, _int, _int, ctypes.POINTER(_d), ctypes.POINTER(_d),
+                     ctypes.POINTER(_d), ctypes.POINTER(_i64), ctypes.POINTER(_i64))
+_dev_bytes = _sig("bwpm_solver_device_bytes", _p, ctypes.POINTER(_i64))
+
+LIBRARY = str(_LIB_PATH)
+
+
+def _check(status: int) -> None:
+    if status == OK:
+        return
+    msg = (_lib.bwpm_last_error() or b"").decode("utf-8", "replace")
+    if status == EINVAL:
+        raise ValueError(msg)
+    raise RuntimeError(msg or f"bwpm_hip error {status}")
+
+
+def gpu_device_count() -> int:
+    n = _int(0)
+    _check(_device_count(ctypes.byref(n)))
+    return int(n.value)
+
+
+HAS_GPU = True
+HAS_CUDA = True  # name of the reference's flag: "the compiled core has a GPU backend"
+cuda_device_count = gpu_device_count
+
+
+def _ptr(a: np.ndarray) -> int:
+    return a.ctypes.data
+
+
+def _f64(a: Any) -> np.ndarray:
+    return np.array(a, dtype=np.float64, order="C", copy=True)
+
+
+def _matrices(fn: Any, k_x: Any, k_y: Any, index_in: complex, index_out: complex, k0: float) -> np.ndarray:
+    kx, ky = _f64(k_x), _f64(k_y)
+    if kx.shape != ky.shape:
+        raise ValueError("k_x and k_y must have the same shape")
+    out = np.empty(kx.shape + (2, 2), dtype=np.complex128)
+    a, b = complex(index_in), complex(index_out)
+    _check(fn(_ptr(kx), _ptr(ky), kx.size, a.real, a.imag, b.real, b.imag, float(k0), _ptr(out)))
+    return out
+
+
+def transmission_matrix(k_x: Any, k_y: Any, index_in: complex, index_out: complex, k0: float) -> np.ndarray:
+    """T of wende2023 eqs. (2), (3) without propagation factor, shape (..., 2, 2), complex128."""
+    return _matrices(_tmat, k_x, k_y, index_in, index_out, k0)
+
+
+def reflection_matrix(k_x: Any, k_y: Any, index_in: complex, index_out: complex, k0: float) -> np.ndarray:
+    """R of wende2023 eqs. (8), (9) without propagation factor, shape (..., 2, 2), complex128."""
+    return _matrices(_rmat, k_x, k_y, index_in, index_out, k0)
+
+
+def longitudinal_wavenumber(index: complex, k0: float, k_perp_sq: Any) -> Any:
+    """k_z = sqrt(ñ²k₀² − k⊥²) with Im ≥ 0 and the grazing rule; scalar in → complex out."""
+    kp2 = _f64(k_perp_sq)
+    out = np.empty(kp2.shape, dtype=np.complex128)
+    n = complex(index)
+    _check(_kz(n.real, n.imag, float(k0), _ptr(kp2), kp2.size, _ptr(out)))
+    return complex(out) if out.ndim == 0 else out
+
+
+class GpuSolver:
+    """FPWPM/BWPM on one MI355X: `GpuSolver(ny, nx, dy, dx, dz, k0, indices, labels, index_before,
+    mask, device=0, fluctuation=None, precision="fp64")` with `.run(field, iterations, keep=[])`.
+
+    Arguments and result dict as the reference's `CudaSolver` (cpp/python/bindings.cpp:158-173);
+    `precision` selects complex128 ("fp64") or the compensated complex64 bulk path ("fp32").
+    """
+
+    def __init__(self, ny: int, nx: int, dy: float, dx: float, dz: float, k0: float, indices: Any,
+                 labels: Any, index_before: complex, mask: Any = None, device: int = 0,
+                 fluctuation: Any = None, precision: str = "fp64") -> None:
+        self._h = _p(None)
+        if precision not in PRECISIONS:
+            raise ValueError(f"precision must be one of {sorted(PRECISIONS)}")
+        lab = np.asarray(labels)
+        if lab.ndim != 3:
+            raise ValueError("labels must have shape (nz, ny, nx)")
+        lab = np.ascontiguousarray(lab, dtype=np.int32)
+        idx = np.ascontiguousarray(np.asarray(indices, dtype=np.complex128).ravel())
+        m = None if mask is None else _f64(mask)
+        if m is not None and m.size != int(ny) * int(nx):
+            raise ValueError("mask must have shape (ny, nx)")
+        f = None
+        if fluctuation is not None:
+            f = np.ascontiguousarray(np.asarray(fluctuation, dtype=np.complex128))
+            if f.shape != lab.shape:
+                raise ValueError("fluctuation must have the shape of labels")
+        if lab.shape[1:] != (int(ny), int(nx)):
+            raise ValueError("labels must have shape (nz, ny, nx)")
+        ib = complex(index_before)
+        self.ny, self.nx, self.nz = int(ny), int(nx), int(lab.shape[0])
+        self.precision = precision
+        self.device = int(device)
+        _check(_create(int(ny), int(nx), float(dy), float(dx), float(dz), float(k0), _ptr(idx), idx.size,
+                       _ptr(lab), lab.shape[0], ib.real, ib.imag, None if m is None else _ptr(m),
+                       None if f is None else _ptr(f), int(device), PRECISIONS[precision], ctypes.byref(self._h)))
+
+    def close(self) -> None:
+        if self._h and self._h.value:
+            _destroy(self._h)
+            self._h = _p(None)
+
+    def __del__(self) -> None:
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _field(self, field: Any) -> np.ndarray:
+        a = np.asarray(field)
+        if a.ndim != 3 or a.shape[2] != 2 or a.shape[:2] != (self.ny, self.nx):
+            raise ValueError("field must have shape (ny, nx, 2)")
+        return np.ascontiguousarray(a, dtype=np.complex128)
+
+    def run(self, field: Any, iterations: int, keep: Any = ()) -> dict[str, Any]:
+        """Bidirectional solve; the dict of cpp/python/bindings.cpp:71-85 plus `gpu_ms`."""
+        e = self._field(field)
+        if int(iterations) < 0:
+            raise ValueError("iterations must be >= 0")
+        keep_list = [int(k) for k in keep]
+        kp = np.ascontiguousarray(np.asarray(keep_list, dtype=np.int64))
+        res = _p(None)
+        _check(_run(self._h, _ptr(e), int(iterations), _ptr(kp) if kp.size else None, kp.size, ctypes.byref(res)))
+        try:
+            shape = (self.ny, self.nx, 2)
+
+            def field_of(which: int) -> np.ndarray:
+                out = np.empty(shape, dtype=np.complex128)
+                _check(_res_field(res, which, _ptr(out)))
+                return out
+
+            out: dict[str, Any] = {"transmitted": field_of(0), "reflected": field_of(1)}
+            n = _i64(0)
+            _check(_res_iters(res, ctypes.byref(n)))
+            ti, ri = [], []
+            for it in range(n.value):
+                for which, lst in ((0, ti), (1, ri)):
+                    a = np.empty(shape, dtype=np.complex128)
+                    _check(_res_iter_field(res, which, it, _ptr(a)))
+                    lst.append(a)
+            out["transmitted_iter"] = ti
+            out["reflected_iter"] = ri
+            fw: dict[int, np.ndarray] = {}
+            bw: dict[int, np.ndarray] = {}
+            present = _int(0)
+            for p in sorted(set(keep_list)):
+                for which, dct in ((0, fw), (1, bw)):
+                    a = np.empty(shape, dtype=np.complex128)
+                    _check(_res_plane(res, which, p, _ptr(a), ctypes.byref(present)))
+                    if present.value:
+                        dct[p] = a
+            out["forward"] = fw
+            out["backward"] = bw
+            sv = _i64(0)
+            _check(_res_stored(res, ctypes.byref(sv)))
+            out["stored_values"] = int(sv.value)
+            ms, steps, gm = _d(0), _i64(0), _i64(0)
+            _check(_res_stats(res, ctypes.byref(ms), ctypes.byref(steps), ctypes.byref(gm)))
+            out["gpu_ms"] = float(ms.value)
+            out["slab_steps"] = int(steps.value)
+            out["gmres_iterations"] = int(gm.value)
+            return out
+        finally:
+            _res_free(res)
+
+    # device-resident timing path (fields stay in HBM)
+    def upload(self, field: Any) -> None:
+        e = self._field(field)
+        _check(_upload(self._h, _ptr(e)))
+
+    def run_resident(self, iterations: int, time_kernels: bool = False) -> dict[str, float]:
+        ms, cms, rms = _d(0), _d(0), _d(0)
+        cl, rl = _i64(0), _i64(0)
+        _check(_run_resident(self._h, int(iterations), 1 if time_kernels else 0, ctypes.byref(ms),
+                             ctypes.byref(cms), ctypes.byref(rms), ctypes.byref(cl), ctypes.byref(rl)))
+        return {"gpu_ms": ms.value, "col_ms": cms.value, "row_ms": rms.value, "col_launches": cl.value,
+                "row_launches": rl.value}
+
+    def device_bytes(self) -> int:
+        b = _i64(0)
+        _check(_dev_bytes(self._h, ctypes.byref(b)))
+        return int(b.value)
+
+
+CudaSolver = GpuSolver  # the reference's name for its GPU solver class

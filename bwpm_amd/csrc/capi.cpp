@@ -1,0 +1,234 @@
+// extern "C" entry points of libbwpm_hip.so (declared in include/bwpm_hip.h).
+#include <cstring>
+#include <string>
+
+#include "../../include/bwpm_hip.h"
+#include "engine.hpp"
+
+using bwpm::c128;
+
+struct bwpm_solver {
+  std::unique_ptr<bwpm::EngineBase> engine;
+  int device = 0;
+};
+
+struct bwpm_result {
+  bwpm::RunOutput out;
+  long long npix = 0;
+};
+
+namespace {
+
+thread_local std::string g_error;
+
+template <typename F>
+int guarded(F&& f) {
+  try {
+    f();
+    g_error.clear();
+    return BWPM_OK;
+  } catch (const bwpm::Error& e) {
+    g_error = e.what();
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    g_error = "host allocation failed";
+    return BWPM_ENOMEM;
+  } catch (const std::exception& e) {
+    g_error = e.what();
+    return BWPM_EINTERNAL;
+  } catch (...) {
+    g_error = "unknown error";
+    return BWPM_EINTERNAL;
+  }
+}
+
+void copy_field(const std::vector<c128>& v, double* out) {
+  std::memcpy(out, v.data(), v.size() * sizeof(c128));
+}
+
+int interface_call(int which, const double* kx, const double* ky, int64_t n, double a, double b, double c,
+                   double d, double k0, double* out) {
+  return guarded([&] {
+    bwpm::require(n >= 0, "n must be >= 0");
+    bwpm::require(n == 0 || (kx && ky && out), "null array");
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    bwpm::interface_matrices(which, kx, ky, n, c128(a, b), c128(c, d), k0, reinterpret_cast<c128*>(out), dev);
+  });
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* bwpm_last_error(void) { return g_error.c_str(); }
+
+int bwpm_device_count(int* count) {
+  return guarded([&] {
+    bwpm::require(count != nullptr, "null pointer");
+    int n = 0;
+    const hipError_t e = hipGetDeviceCount(&n);
+    *count = e == hipSuccess ? n : 0;
+  });
+}
+
+int bwpm_transmission_matrix(const double* kx, const double* ky, int64_t n, double n_in_re, double n_in_im,
+                             double n_out_re, double n_out_im, double k0, double* out) {
+  return interface_call(0, kx, ky, n, n_in_re, n_in_im, n_out_re, n_out_im, k0, out);
+}
+
+int bwpm_reflection_matrix(const double* kx, const double* ky, int64_t n, double n_in_re, double n_in_im,
+                           double n_out_re, double n_out_im, double k0, double* out) {
+  return interface_call(1, kx, ky, n, n_in_re, n_in_im, n_out_re, n_out_im, k0, out);
+}
+
+int bwpm_longitudinal_wavenumber(double n_re, double n_im, double k0, const double* kp2, int64_t n, double* out) {
+  return guarded([&] {
+    bwpm::require(n >= 0, "n must be >= 0");
+    bwpm::require(n == 0 || (kp2 && out), "null array");
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    bwpm::longitudinal_wavenumbers(c128(n_re, n_im), k0, kp2, n, reinterpret_cast<c128*>(out), dev);
+  });
+}
+
+int bwpm_solver_create(int64_t ny, int64_t nx, double dy, double dx, double dz, double k0, const double* indices,
+                       int64_t m, const int32_t* labels, int64_t nz, double ib_re, double ib_im, const double* mask,
+                       const double* fluct, int device, int precision, bwpm_solver** out) {
+  return guarded([&] {
+    bwpm::require(out != nullptr, "null output handle");
+    *out = nullptr;
+    bwpm::require(ny >= 1 && nx >= 1 && nz >= 1, "labels must have shape (nz, ny, nx)");
+    bwpm::require(m >= 0 && (m == 0 || indices), "indices missing");
+    bwpm::require(labels != nullptr, "labels missing");
+    bwpm::Spec s;
+    s.ny = ny;
+    s.nx = nx;
+    s.nz = nz;
+    s.dy = dy;
+    s.dx = dx;
+    s.dz = dz;
+    s.k0 = k0;
+    s.indices.assign(reinterpret_cast<const c128*>(indices), reinterpret_cast<const c128*>(indices) + m);
+    s.labels.assign(labels, labels + nz * ny * nx);
+    s.index_before = c128(ib_re, ib_im);
+    if (mask) s.mask.assign(mask, mask + ny * nx);
+    if (fluct) {
+      const c128* f = reinterpret_cast<const c128*>(fluct);
+      s.fluct.assign(f, f + nz * ny * nx);
+    }
+    s.precision = precision;
+    s.device = device;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0)
+      throw bwpm::Error(bwpm::kEdevice, "no GPU device available");
+    bwpm::require(device >= 0 && device < count, "device index out of range");
+    auto h = std::make_unique<bwpm_solver>();
+    h->engine = bwpm::make_engine(s);
+    h->device = device;
+    *out = h.release();
+  });
+}
+
+int bwpm_solver_destroy(bwpm_solver* s) {
+  return guarded([&] { delete s; });
+}
+
+int bwpm_solver_run(bwpm_solver* s, const double* field, int iterations, const int64_t* keep, int64_t nkeep,
+                    bwpm_result** out) {
+  return guarded([&] {
+    bwpm::require(s && field && out, "null argument");
+    bwpm::require(iterations >= 0, "iterations must be >= 0");
+    *out = nullptr;
+    std::vector<long long> k;
+    for (int64_t i = 0; i < nkeep; ++i) k.push_back(keep[i]);
+    auto r = std::make_unique<bwpm_result>();
+    s->engine->run(reinterpret_cast<const c128*>(field), iterations, k, r->out);
+    r->npix = static_cast<long long>(r->out.transmitted.size() / 2);
+    *out = r.release();
+  });
+}
+
+int bwpm_result_field(const bwpm_result* r, int which, double* out) {
+  return guarded([&] {
+    bwpm::require(r && out, "null argument");
+    copy_field(which == 0 ? r->out.transmitted : r->out.reflected, out);
+  });
+}
+
+int bwpm_result_iterations(const bwpm_result* r, int64_t* n) {
+  return guarded([&] {
+    bwpm::require(r && n, "null argument");
+    *n = static_cast<int64_t>(r->out.transmitted_iter.size());
+  });
+}
+
+int bwpm_result_iter_field(const bwpm_result* r, int which, int64_t it, double* out) {
+  return guarded([&] {
+    bwpm::require(r && out, "null argument");
+    const auto& v = which == 0 ? r->out.transmitted_iter : r->out.reflected_iter;
+    bwpm::require(it >= 0 && it < static_cast<int64_t>(v.size()), "iteration out of range");
+    copy_field(v[it], out);
+  });
+}
+
+int bwpm_result_plane(const bwpm_result* r, int which, int64_t p, double* out, int* present) {
+  return guarded([&] {
+    bwpm::require(r && out && present, "null argument");
+    const auto& m = which == 0 ? r->out.forward : r->out.backward;
+    auto it = m.find(p);
+    *present = it != m.end() ? 1 : 0;
+    if (it != m.end()) copy_field(it->second, out);
+  });
+}
+
+int bwpm_result_stored_values(const bwpm_result* r, int64_t* v) {
+  return guarded([&] {
+    bwpm::require(r && v, "null argument");
+    *v = r->out.stored_values;
+  });
+}
+
+int bwpm_result_stats(const bwpm_result* r, double* gpu_ms, int64_t* slab_steps, int64_t* gmres_iterations) {
+  return guarded([&] {
+    bwpm::require(r != nullptr, "null argument");
+    if (gpu_ms) *gpu_ms = r->out.gpu_ms;
+    if (slab_steps) *slab_steps = r->out.slab_steps;
+    if (gmres_iterations) *gmres_iterations = r->out.gmres_iterations;
+  });
+}
+
+int bwpm_result_free(bwpm_result* r) {
+  return guarded([&] { delete r; });
+}
+
+int bwpm_solver_upload(bwpm_solver* s, const double* field) {
+  return guarded([&] {
+    bwpm::require(s && field, "null argument");
+    s->engine->upload(reinterpret_cast<const c128*>(field));
+  });
+}
+
+int bwpm_solver_run_resident(bwpm_solver* s, int iterations, int time_kernels, double* gpu_ms, double* col_ms,
+                             double* row_ms, int64_t* col_launches, int64_t* row_launches) {
+  return guarded([&] {
+    bwpm::require(s != nullptr, "null argument");
+    bwpm::require(iterations >= 0, "iterations must be >= 0");
+    bwpm::KernelTimes kt;
+    const double ms = s->engine->run_resident(iterations, time_kernels != 0, &kt);
+    if (gpu_ms) *gpu_ms = ms;
+    if (col_ms) *col_ms = kt.col_ms;
+    if (row_ms) *row_ms = kt.row_ms;
+    if (col_launches) *col_launches = kt.col_launches;
+    if (row_launches) *row_launches = kt.row_launches;
+  });
+}
+
+int bwpm_solver_device_bytes(const bwpm_solver* s, int64_t* bytes) {
+  return guarded([&] {
+    bwpm::require(s && bytes, "null argument");
+    *bytes = static_cast<int64_t>(s->engine->device_bytes());
+  });
+}
+
+}  // extern "C"

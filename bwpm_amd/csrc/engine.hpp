@@ -1,0 +1,68 @@
+// Host side of one solve: set-up of the per-boundary label-pair structure, operator tables and
+// device buffers, and the device-resident bidirectional driver.
+#pragma once
+
+#include <complex>
+#include <map>
+#include <memory>
+#include <vector>
+
+#include "common.hpp"
+
+namespace bwpm {
+
+using c128 = std::complex<double>;
+
+struct Spec {
+  long long ny = 0, nx = 0, nz = 0;
+  double dy = 0, dx = 0, dz = 0, k0 = 0;
+  std::vector<c128> indices;
+  std::vector<int32_t> labels;  // (nz, ny, nx)
+  c128 index_before{1.0, 0.0};
+  std::vector<double> mask;     // (ny, nx) or empty
+  std::vector<c128> fluct;      // (nz, ny, nx) or empty
+  int precision = 32;           // 64: FP64; 32: FP32 compensated; 31: FP32 plain
+  int device = 0;
+};
+
+struct RunOutput {
+  std::vector<c128> transmitted, reflected;           // (ny, nx, 2)
+  std::vector<std::vector<c128>> transmitted_iter, reflected_iter;
+  std::map<long long, std::vector<c128>> forward, backward;
+  long long stored_values = 0;
+  // diagnostics of the last run
+  double gpu_ms = 0;            // device time of the whole run (HIP events)
+  long long slab_steps = 0;     // slab steps executed
+  long long gmres_iterations = 0;
+};
+
+// Launch statistics of the dominant kernels, measured with HIP events on the solve's stream.
+struct KernelTimes {
+  double col_ms = 0, row_ms = 0;
+  long long col_launches = 0, row_launches = 0;
+};
+
+class EngineBase {
+ public:
+  virtual ~EngineBase() = default;
+  // Full solve from a host field (ny, nx, 2) complex128.
+  virtual void run(const c128* field, int iterations, const std::vector<long long>& keep,
+                   RunOutput& out) = 0;
+  // Device-resident variant for timing: the field is already on the device (set by upload);
+  // results stay on the device. Returns the device time in ms.
+  virtual void upload(const c128* field) = 0;
+  virtual double run_resident(int iterations, bool time_kernels, KernelTimes* kt) = 0;
+  virtual void download(RunOutput& out, int iterations, const std::vector<long long>& keep) = 0;
+  virtual hipStream_t stream() const = 0;
+  virtual size_t device_bytes() const = 0;
+};
+
+std::unique_ptr<EngineBase> make_engine(const Spec& spec);
+
+// FP64 evaluation of the exported interface functions on the device.
+void interface_matrices(int which, const double* kx, const double* ky, long long n, c128 n_in,
+                        c128 n_out, double k0, c128* out, int device);
+void longitudinal_wavenumbers(c128 index, double k0, const double* kp2, long long n, c128* out,
+                              int device);
+
+}  // namespace bwpm

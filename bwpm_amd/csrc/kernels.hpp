@@ -1,0 +1,339 @@
+// Slab-step kernels of the bidirectional engine (templated on precision T and on COMP, the
+// compensated FP32 arithmetic of the bulk path).
+//
+// One slab step  out = Σ_pairs I_{r,s} ⊙ F⁻¹{M_{kind,r,s}·F{in}}  (reference: Solver::apply,
+// cpp/src/solver.cpp:384-417) is split over two kernels so that each moves the (E_x, E_y) plane
+// through HBM once:
+//
+//   k_col_step  X (rows already FFT'd along x)  → column FFT → for each product o (one per label
+//               pair, T kinds first, then R kinds): M_o·Ŝ or phase_o·Ŝ → column IFFT → plane P_o
+//   k_row_step  for each T product: row IFFT of P_o, keep the pixels of its pair; mask; Born
+//               increment; sparse reflection sources and dense Born planes added; the plane is
+//               written out only where needed (kept planes, pass ends, Born inputs); R products:
+//               row IFFT and gather of the interface pixels into the sparse store; finally the row
+//               FFT of the result for the next slab step (fused, so the field never makes an
+//               extra trip).
+#pragma once
+
+#include "fft.hpp"
+#include "interface_math.hpp"
+
+namespace bwpm {
+
+extern __shared__ __attribute__((aligned(16))) unsigned char g_lds[];
+
+template <typename T>
+__device__ __forceinline__ cx<T>* lds() {
+  return reinterpret_cast<cx<T>*>(g_lds);
+}
+
+// Operator product descriptor: scalar phase table (type 0, one complex per k) or symmetric 2×2
+// matrix table (type 1, three complex per k: m00, m01, m11), each as hi and lo parts.
+template <typename T>
+struct Prod {
+  int type;
+  const cx<T>* hi;
+  const cx<T>* lo;
+};
+
+inline constexpr int kMaxProd = 64;
+
+template <typename T>
+struct ColStep {
+  const px2<T>* in;
+  px2<T>* out;          // nprod planes of ny·nx pixels
+  long long plane;      // pixels per plane
+  int ny, nx, cols, ls;
+  int nprod;
+  const Prod<T>* prods; // device array of nprod entries
+  FftPlan plan;         // length ny
+};
+
+template <typename T, bool COMP>
+__global__ __launch_bounds__(kThreads) void k_col_step(ColStep<T> a) {
+  cx<T>* A = lds<T>();
+  const int W = a.cols, ny = a.ny, nx = a.nx, ls = a.ls;
+  const int lines = 2 * W;
+  cx<T>* B = A + static_cast<long long>(ny) * ls;
+  const int x0 = blockIdx.x * W;
+  const int count = ny * W;
+  for (int i = threadIdx.x; i < count; i += kThreads) {
+    const int y = i / W, c = i - y * W, x = x0 + c;
+    px2<T> v{};
+    if (x < nx) v = a.in[static_cast<long long>(y) * nx + x];
+    A[y * ls + c] = v.a;
+    A[y * ls + W + c] = v.b;
+  }
+  __syncthreads();
+  fft_lines<COMP, false>(A, ls, lines, a.plan);
+  for (int o = 0; o < a.nprod; ++o) {
+    const Prod<T> pd = a.prods[o];
+    cx<T>* D = a.nprod == 1 ? A : B;
+    for (int i = threadIdx.x; i < count; i += kThreads) {
+      const int y = i / W, c = i - y * W, x = x0 + c;
+      if (x >= nx) continue;
+      const long long k = static_cast<long long>(y) * nx + x;
+      const cx<T> ea = A[y * ls + c], eb = A[y * ls + W + c];
+      cx<T> ra, rb;
+      if (pd.type == 0) {
+        const cx<T> ph = pd.hi[k], pl = pd.lo[k];
+        ra = cmul2<COMP>(ea, ph, pl);
+        rb = cmul2<COMP>(eb, ph, pl);
+      } else {
+        const cx<T>* h = pd.hi + 3 * k;
+        const cx<T>* l = pd.lo + 3 * k;
+        const cx<T> h00 = h[0], h01 = h[1], h11 = h[2];
+        const cx<T> l00 = l[0], l01 = l[1], l11 = l[2];
+        ra = cmul2<COMP>(ea, h00, l00) + cmul2<COMP>(eb, h01, l01);
+        rb = cmul2<COMP>(ea, h01, l01) + cmul2<COMP>(eb, h11, l11);
+      }
+      D[y * ls + c] = ra;
+      D[y * ls + W + c] = rb;
+    }
+    __syncthreads();
+    fft_lines<COMP, true>(D, ls, lines, a.plan);
+    px2<T>* out = a.out + o * a.plane;
+    for (int i = threadIdx.x; i < count; i += kThreads) {
+      const int y = i / W, c = i - y * W, x = x0 + c;
+      if (x < nx) out[static_cast<long long>(y) * nx + x] = px2<T>{D[y * ls + c], D[y * ls + W + c]};
+    }
+    __syncthreads();
+  }
+}
+
+template <typename T>
+struct RowStep {
+  int ny, nx, rows, ls;
+  FftPlan plan;               // length nx
+  const px2<T>* prod;         // nt + nr product planes
+  long long plane;
+  int nt, nr;
+  const uint8_t* tsel;        // T pair of each pixel (nt > 1), plane of ny·nx
+  const T* mask;              // transmission mask or nullptr
+  const px2<T>* e_in;         // nt == 0: start plane (nullptr: zero)
+  const px2<T>* inc;          // Born increment, multiplied by mask if mask != nullptr
+  const int* rowptr;          // ny + 1 offsets into the boundary's sparse list (nullptr: none)
+  const int* sp_pix;          // pixel index of each sparse entry
+  const uint8_t* sp_rid;      // R product of each sparse entry
+  const px2<T>* sp_add;       // source values added at the sparse pixels (nullptr: none)
+  px2<T>* sp_store;           // reflection values stored at the sparse pixels (nullptr: none)
+  const px2<T>* dense;        // dense Born plane added (nullptr: none)
+  px2<T>* e_out;              // real-space result (nullptr: not written)
+  px2<T>* e_out2;             // second copy (kept plane)
+  px2<T>* x_out;              // row FFT of the result for the next step (nullptr: none)
+};
+
+template <typename T>
+__device__ __forceinline__ void load_rows(cx<T>* buf, const px2<T>* src, int y0, int nrow, int nx,
+                                          int rows, int ls) {
+  const int count = nrow * nx;
+  for (int i = threadIdx.x; i < count; i += kThreads) {
+    const int r = i / nx, x = i - r * nx;
+    const px2<T> v = src[static_cast<long long>(y0 + r) * nx + x];
+    buf[x * ls + r] = v.a;
+    buf[x * ls + rows + r] = v.b;
+  }
+}
+
+template <typename T, bool COMP>
+__global__ __launch_bounds__(kThreads) void k_row_step(RowStep<T> a) {
+  const int nx = a.nx, rows = a.rows, ls = a.ls;
+  const int y0 = blockIdx.x * rows;
+  const int nrow = min(rows, a.ny - y0);
+  const int lines = 2 * rows;
+  const int count = nrow * nx;
+  cx<T>* acc = lds<T>();
+  cx<T>* work = acc + static_cast<long long>(nx) * ls;
+
+  // 1. transmitted field (or the start plane)
+  if (a.nt == 0) {
+    if (a.e_in) {
+      load_rows(acc, a.e_in, y0, nrow, nx, rows, ls);
+    } else {
+      for (int i = threadIdx.x; i < count; i += kThreads) {
+        const int r = i / nx, x = i - r * nx;
+        acc[x * ls + r] = cx<T>{0, 0};
+        acc[x * ls + rows + r] = cx<T>{0, 0};
+      }
+    }
+    __syncthreads();
+  } else if (a.nt == 1) {
+    load_rows(acc, a.prod, y0, nrow, nx, rows, ls);
+    __syncthreads();
+    fft_lines<COMP, true>(acc, ls, lines, a.plan);
+  } else {
+    for (int o = 0; o < a.nt; ++o) {
+      load_rows(work, a.prod + o * a.plane, y0, nrow, nx, rows, ls);
+      __syncthreads();
+      fft_lines<COMP, true>(work, ls, lines, a.plan);
+      for (int i = threadIdx.x; i < count; i += kThreads) {
+        const int r = i / nx, x = i - r * nx;
+        if (a.tsel[static_cast<long long>(y0 + r) * nx + x] == o) {
+          acc[x * ls + r] = work[x * ls + r];
+          acc[x * ls + rows + r] = work[x * ls + rows + r];
+        }
+      }
+      __syncthreads();
+    }
+  }
+
+  // 2. epilogue in the order of the reference: mask, Born increment, sparse sources, Born plane
+  for (int i = threadIdx.x; i < count; i += kThreads) {
+    const int r = i / nx, x = i - r * nx;
+    const long long q = static_cast<long long>(y0 + r) * nx + x;
+    cx<T> ea = acc[x * ls + r], eb = acc[x * ls + rows + r];
+    if (a.mask) {
+      const T m = a.mask[q];
+      ea = ea * m;
+      eb = eb * m;
+      if (a.inc) {
+        const px2<T> d = a.inc[q];
+        ea = ea + d.a * m;
+        eb = eb + d.b * m;
+      }
+    } else if (a.inc) {
+      const px2<T> d = a.inc[q];
+      ea = ea + d.a;
+      eb = eb + d.b;
+    }
+    acc[x * ls + r] = ea;
+    acc[x * ls + rows + r] = eb;
+  }
+  __syncthreads();
+  if (a.rowptr && a.sp_add) {
+    const int e0 = a.rowptr[y0], e1 = a.rowptr[y0 + nrow];
+    for (int e = e0 + threadIdx.x; e < e1; e += kThreads) {
+      const int pix = a.sp_pix[e];
+      const int r = pix / nx - y0, x = pix - (pix / nx) * nx;
+      const px2<T> v = a.sp_add[e];
+      acc[x * ls + r] = acc[x * ls + r] + v.a;
+      acc[x * ls + rows + r] = acc[x * ls + rows + r] + v.b;
+    }
+    __syncthreads();
+  }
+  if (a.dense) {
+    for (int i = threadIdx.x; i < count; i += kThreads) {
+      const int r = i / nx, x = i - r * nx;
+      const px2<T> d = a.dense[static_cast<long long>(y0 + r) * nx + x];
+      acc[x * ls + r] = acc[x * ls + r] + d.a;
+      acc[x * ls + rows + r] = acc[x * ls + rows + r] + d.b;
+    }
+    __syncthreads();
+  }
+
+  // 3. reflection products: row IFFT and gather of the interface pixels
+  if (a.nr > 0 && a.rowptr && a.sp_store) {
+    const int e0 = a.rowptr[y0], e1 = a.rowptr[y0 + nrow];
+    for (int o = 0; o < a.nr; ++o) {
+      load_rows(work, a.prod + (a.nt + o) * a.plane, y0, nrow, nx, rows, ls);
+      __syncthreads();
+      fft_lines<COMP, true>(work, ls, lines, a.plan);
+      for (int e = e0 + threadIdx.x; e < e1; e += kThreads) {
+        if (a.sp_rid[e] != o) continue;
+        const int pix = a.sp_pix[e];
+        const int r = pix / nx - y0, x = pix - (pix / nx) * nx;
+        a.sp_store[e] = px2<T>{work[x * ls + r], work[x * ls + rows + r]};
+      }
+      __syncthreads();
+    }
+  }
+
+  // 4. real-space outputs
+  if (a.e_out || a.e_out2) {
+    for (int i = threadIdx.x; i < count; i += kThreads) {
+      const int r = i / nx, x = i - r * nx;
+      const long long q = static_cast<long long>(y0 + r) * nx + x;
+      const px2<T> v{acc[x * ls + r], acc[x * ls + rows + r]};
+      if (a.e_out) a.e_out[q] = v;
+      if (a.e_out2) a.e_out2[q] = v;
+    }
+  }
+
+  // 5. row FFT for the next slab step
+  if (a.x_out) {
+    fft_lines<COMP, false>(acc, ls, lines, a.plan);
+    for (int i = threadIdx.x; i < count; i += kThreads) {
+      const int r = i / nx, x = i - r * nx;
+      a.x_out[static_cast<long long>(y0 + r) * nx + x] = px2<T>{acc[x * ls + r], acc[x * ls + rows + r]};
+    }
+  }
+}
+
+// ---- operator tables (FP64 math, stored as T hi + lo) ---------------------------------------
+
+template <typename T>
+__device__ __forceinline__ void split(zd v, cx<T>* hi, cx<T>* lo) {
+  const T hx = static_cast<T>(v.x), hy = static_cast<T>(v.y);
+  *hi = cx<T>{hx, hy};
+  *lo = cx<T>{static_cast<T>(v.x - static_cast<double>(hx)), static_cast<T>(v.y - static_cast<double>(hy))};
+}
+
+// Homogeneous slab (index(r) = index(s) = n): exp(i k_z dz)/N.
+template <typename T>
+__global__ void k_build_phase(const double* kx, const double* ky, int ny, int nx, zd n, double k0,
+                              double dz, double scale, cx<T>* hi, cx<T>* lo) {
+  const long long k = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;
+  if (k >= static_cast<long long>(ny) * nx) return;
+  const int y = static_cast<int>(k / nx), x = static_cast<int>(k - static_cast<long long>(y) * nx);
+  const double kp2 = kx[x] * kx[x] + ky[y] * ky[y];
+  const zd p = zmul(propagation(kz_of(n, k0, kp2), dz, 1.0), scale);
+  split(p, hi + k, lo + k);
+}
+
+// Interface pair: M_{kind, r, s}(k)/N of table 1.
+template <typename T>
+__global__ void k_build_matrix(const double* kx, const double* ky, int ny, int nx, int kind, zd nr,
+                               zd ns, double k0, double dz, double scale, cx<T>* hi, cx<T>* lo) {
+  const long long k = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;
+  if (k >= static_cast<long long>(ny) * nx) return;
+  const int y = static_cast<int>(k / nx), x = static_cast<int>(k - static_cast<long long>(y) * nx);
+  const Mat2 m = operator_matrix(kind, kx[x], ky[y], nr, ns, k0, dz);
+  split(zmul(m.m00, scale), hi + 3 * k, lo + 3 * k);
+  split(zmul(m.m01, scale), hi + 3 * k + 1, lo + 3 * k + 1);
+  split(zmul(m.m11, scale), hi + 3 * k + 2, lo + 3 * k + 2);
+}
+
+// Exported interface functions on arbitrary k arrays: full row-major 2×2 (which = 0: T, 1: R).
+__global__ void k_interface_eval(const double* kx, const double* ky, long long n, int which, zd n1,
+                                 zd n2, double k0, zd* out) {
+  const long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;
+  if (i >= n) return;
+  const Mat2 m = which == 0 ? transmission(kx[i], ky[i], n1, n2, k0) : reflection(kx[i], ky[i], n1, n2, k0);
+  out[4 * i] = m.m00;
+  out[4 * i + 1] = m.m01;
+  out[4 * i + 2] = m.m01;
+  out[4 * i + 3] = m.m11;
+}
+
+__global__ void k_kz_eval(zd n, double k0, const double* kp2, long long cnt, zd* out) {
+  const long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;
+  if (i < cnt) out[i] = kz_of(n, k0, kp2[i]);
+}
+
+// ---- small elementwise helpers ---------------------------------------------------------------
+
+// Convert interleaved complex128 (E_x, E_y) pixels to the working precision.
+template <typename T>
+__global__ void k_from_c128(const zd* src, px2<T>* dst, long long npix) {
+  const long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;
+  if (i >= npix) return;
+  const zd a = src[2 * i], b = src[2 * i + 1];
+  dst[i] = px2<T>{cx<T>{static_cast<T>(a.x), static_cast<T>(a.y)}, cx<T>{static_cast<T>(b.x), static_cast<T>(b.y)}};
+}
+
+template <typename T>
+__global__ void k_to_c128(const px2<T>* src, zd* dst, long long npix) {
+  const long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;
+  if (i >= npix) return;
+  const px2<T> v = src[i];
+  dst[2 * i] = zd{static_cast<double>(v.a.x), static_cast<double>(v.a.y)};
+  dst[2 * i + 1] = zd{static_cast<double>(v.b.x), static_cast<double>(v.b.y)};
+}
+
+template <typename T>
+__global__ void k_fill_zero(px2<T>* dst, long long npix) {
+  const long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;
+  if (i < npix) dst[i] = px2<T>{};
+}
+
+}  // namespace bwpm

@@ -1,0 +1,105 @@
+/* C ABI of the MI355X BWPM engine (libbwpm_hip.so).
+ *
+ * Drop-in boundary for the reference's compiled module `bwpm._core`
+ * (cpp/python/bindings.cpp:122-180): the interface matrices, the longitudinal wavenumber and the
+ * solver object with its `run`. Plain C types only; complex numbers are pairs of doubles
+ * (re, im) in C order, fields are (ny, nx, 2) complex128, i.e. 4·ny·nx doubles.
+ *
+ * Conventions (as the reference binding):
+ * - every input is copied before the call returns; no caller memory is retained;
+ * - status codes: BWPM_EINVAL for invalid arguments (the reference's std::invalid_argument →
+ *   ValueError), BWPM_ENOCONV, BWPM_EDEVICE, BWPM_ENOMEM, BWPM_EINTERNAL (std::runtime_error →
+ *   RuntimeError); bwpm_last_error() returns the message of the last failure of the calling thread;
+ * - one solver handle must not run concurrently from two threads (the reference Solver is not
+ *   re-entrant either, cpp/src/solver.cpp:73-106); distinct handles may run in parallel.
+ */
+#ifndef BWPM_HIP_H
+#define BWPM_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BWPM_OK 0
+#define BWPM_EINVAL 1
+#define BWPM_ENOCONV 2
+#define BWPM_EDEVICE 3
+#define BWPM_ENOMEM 4
+#define BWPM_EINTERNAL 5
+
+/* precision of the bulk fields */
+#define BWPM_FP64 64       /* complex128 throughout (parity mode against the FP64 reference) */
+#define BWPM_FP32 32       /* complex64 bulk, FP64 tables, compensated twiddles/operators */
+#define BWPM_FP32_PLAIN 31 /* complex64 bulk without compensation (diagnostic) */
+
+typedef struct bwpm_solver bwpm_solver;
+typedef struct bwpm_result bwpm_result;
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* bwpm_last_error(void);
+
+/* Number of visible GPUs. Replaces `_core.cuda_device_count` (cpp/python/bindings.cpp:157,176). */
+int bwpm_device_count(int* count);
+
+/* T of wende2023 eqs. (2), (3) at n points, FP64, row-major 2×2 per point (8n doubles out).
+ * Replaces `_core.transmission_matrix` (cpp/python/bindings.cpp:124-130, cpp/src/interface.cpp:30-40). */
+int bwpm_transmission_matrix(const double* kx, const double* ky, int64_t n, double n_in_re,
+                             double n_in_im, double n_out_re, double n_out_im, double k0,
+                             double* out);
+
+/* R of wende2023 eqs. (8), (9); layout as above.
+ * Replaces `_core.reflection_matrix` (cpp/python/bindings.cpp:131-137, cpp/src/interface.cpp:42-52). */
+int bwpm_reflection_matrix(const double* kx, const double* ky, int64_t n, double n_in_re,
+                           double n_in_im, double n_out_re, double n_out_im, double k0,
+                           double* out);
+
+/* k_z at n values of k⊥² (2n doubles out), with the grazing rule.
+ * Replaces `_core.longitudinal_wavenumber` (cpp/python/bindings.cpp:178-179, cpp/src/interface.cpp:8-18). */
+int bwpm_longitudinal_wavenumber(double n_re, double n_im, double k0, const double* kp2, int64_t n,
+                                 double* out);
+
+/* Solver over a labelled volume. labels: (nz, ny, nx) int32 into `indices` (m complex);
+ * mask: (ny, nx) or NULL; fluct: δn (nz, ny, nx) complex or NULL; precision: BWPM_FP64/FP32.
+ * Replaces `_core.CudaSolver(...)` / `_core.Solver(...)` (cpp/python/bindings.cpp:139-151,158-171,
+ * cpp/src/solver.cpp:27-63; cpp/src/solver_cuda.cu:643-719). */
+int bwpm_solver_create(int64_t ny, int64_t nx, double dy, double dx, double dz, double k0,
+                       const double* indices, int64_t m, const int32_t* labels, int64_t nz,
+                       double index_before_re, double index_before_im, const double* mask,
+                       const double* fluct, int device, int precision, bwpm_solver** out);
+
+int bwpm_solver_destroy(bwpm_solver* s);
+
+/* Bidirectional solve (iterations = 0: forward only) of field (ny, nx, 2), keeping the planes in
+ * `keep`. Replaces `.run(field, iterations, keep)` (cpp/python/bindings.cpp:60-86,152-153,172-173;
+ * cpp/src/solver.cpp:444-528). */
+int bwpm_solver_run(bwpm_solver* s, const double* field, int iterations, const int64_t* keep,
+                    int64_t nkeep, bwpm_result** out);
+
+/* Result accessors (the dict of cpp/python/bindings.cpp:71-85). which: 0 transmitted, 1 reflected. */
+int bwpm_result_field(const bwpm_result* r, int which, double* out);
+int bwpm_result_iterations(const bwpm_result* r, int64_t* n);
+int bwpm_result_iter_field(const bwpm_result* r, int which, int64_t it, double* out);
+/* Kept plane p; which: 0 forward, 1 backward. *present = 0 if the plane was not kept. */
+int bwpm_result_plane(const bwpm_result* r, int which, int64_t p, double* out, int* present);
+int bwpm_result_stored_values(const bwpm_result* r, int64_t* v);
+/* Device time of the run (HIP events on the solve's stream), slab steps and GMRES iterations. */
+int bwpm_result_stats(const bwpm_result* r, double* gpu_ms, int64_t* slab_steps,
+                      int64_t* gmres_iterations);
+int bwpm_result_free(bwpm_result* r);
+
+/* Device-resident timing path (no counterpart in the reference; for benchmarks and serving):
+ * upload a field once, run with the field and all state resident in HBM. With time_kernels != 0
+ * every slab-step kernel is bracketed by HIP events (slower; for per-kernel durations). */
+int bwpm_solver_upload(bwpm_solver* s, const double* field);
+int bwpm_solver_run_resident(bwpm_solver* s, int iterations, int time_kernels, double* gpu_ms,
+                             double* col_ms, double* row_ms, int64_t* col_launches,
+                             int64_t* row_launches);
+int bwpm_solver_device_bytes(const bwpm_solver* s, int64_t* bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BWPM_HIP_H */

@@ -1,0 +1,190 @@
+"""GPU parity of the slab-stepping path through the C ABI, against the reference's golden fixtures
+and the FP64 oracle.
+
+Tolerances: FP64 mode as the reference's own GPU tests (rel 1e-10, abs 1e-11·max,
+tests/test_core_cuda.py:34-36); FP32 mode by relative field L2 ≤ 1e-5 (the bulk-precision bar).
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from conftest import load_golden, rel_l2
+from helpers import fixture_2d_args, fixture_3d_args, random_volume
+from oracle import bwpm_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+L2_FP32 = 1e-5
+
+
+def _core():
+    from bwpm_amd import _core
+
+    assert _core.gpu_device_count() > 0, "no GPU visible"
+    return _core
+
+
+def _close64(a: np.ndarray, b: np.ndarray) -> None:
+    assert a.shape == b.shape
+    atol = 1e-11 * float(np.max(np.abs(b)) or 1.0)
+    err = float(np.max(np.abs(a - b)))
+    assert np.allclose(a, b, rtol=1e-10, atol=atol), err
+
+
+def _solver(args: dict, precision: str):
+    c = _core()
+    return c.GpuSolver(args["ny"], args["nx"], args["dy"], args["dx"], args["dz"], args["k0"],
+                       args["indices"], args["labels"], args["index_before"], args.get("mask"),
+                       fluctuation=args.get("fluctuation"), precision=precision)
+
+
+def test_interface_matrices_fixture() -> None:
+    c = _core()
+    d = load_golden("interface_matrices")
+    kx, ky = d["inputs"]["k_x_per_mm"], d["inputs"]["k_y_per_mm"]
+    k0 = d["parameters"]["k0_per_mm"]
+    tol = d["tolerance"]
+    for j, (n1, n2) in enumerate(d["inputs"]["index_pairs"]):
+        s = n1.real
+        t = c.transmission_matrix(kx * s, ky * s, n1, n2, k0)
+        r = c.reflection_matrix(kx * s, ky * s, n1, n2, k0)
+        assert np.allclose(t, d["expected"]["T"][j], rtol=tol["rel"], atol=tol["abs"])
+        assert np.allclose(r, d["expected"]["R"][j], rtol=tol["rel"], atol=tol["abs"])
+
+
+@pytest.mark.parametrize(("n1", "n2"), [(1.0, 1.5), (1.5, 1.0), (1.0, 0.213 + 6.697j), (1.3, 1.3)])
+def test_interface_on_fft_grid_with_grazing_points(n1: complex, n2: complex) -> None:
+    """k⊥ = 0, propagating, exactly grazing and evanescent points (test_core_interface.py:42-56)."""
+    c = _core()
+    k0 = 2 * np.pi / 0.5e-3
+    k = np.arange(-8, 8) * k0 / 4
+    kx, ky = np.meshgrid(k, k)
+    for fn, ref in ((c.transmission_matrix, O.transmission_matrix), (c.reflection_matrix, O.reflection_matrix)):
+        a, b = fn(kx, ky, n1, n2, k0), ref(kx, ky, n1, n2, k0)
+        assert a.shape == (*kx.shape, 2, 2)
+        assert np.all(np.isfinite(a))
+        assert np.allclose(a, b, rtol=1e-13, atol=1e-13)
+
+
+@pytest.mark.parametrize(("n1", "n2"), [(1.0, 1.5), (1.5, 1.0)])
+def test_grazing_rule_is_independent_of_rounding(n1: complex, n2: complex) -> None:
+    """Perturbing k⊥ by a few ulp at |k⊥| = n₁k₀ must not change the matrices
+    (test_core_interface.py:68-85)."""
+    c = _core()
+    k0 = 2 * np.pi / 0.5e-3
+    for angle in (0.0, 0.3, 1.1):
+        k = n1 * k0
+        for rel in (0.0, 3e-16, -3e-16, 1e-14):
+            kx = np.array([k * (1 + rel) * np.cos(angle)])
+            ky = np.array([k * (1 + rel) * np.sin(angle)])
+            for fn, ref in ((c.transmission_matrix, O.transmission_matrix), (c.reflection_matrix, O.reflection_matrix)):
+                assert np.allclose(fn(kx, ky, n1, n2, k0), ref(kx, ky, n1, n2, k0), rtol=1e-13, atol=1e-13)
+
+
+def test_longitudinal_wavenumber_matches_oracle() -> None:
+    c = _core()
+    k0 = 2 * np.pi / 0.8e-3
+    kp2 = np.linspace(0, 4 * k0 * k0, 101)
+    for n in (1.0, 1.5, 0.13 + 4.0j, 2.1 + 0.01j):
+        a = c.longitudinal_wavenumber(n, k0, kp2)
+        b = O.longitudinal_wavenumber(n, k0, kp2)
+        assert np.allclose(a, b, rtol=1e-14, atol=1e-12 * k0)
+    assert isinstance(c.longitudinal_wavenumber(1.5, k0, 0.0), complex)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_fpwpm2d_interface(precision: str) -> None:
+    d = load_golden("fpwpm2d_interface")
+    a = fixture_2d_args(d)
+    nz = a["labels"].shape[0]
+    out = _solver(a, precision).run(a["field"], 0, list(range(nz + 1)))
+    planes = np.stack([out["forward"][p][:, 0] for p in range(nz + 1)])
+    if precision == "fp64":
+        _close64(planes, d["expected"]["planes"])
+    else:
+        assert rel_l2(planes, d["expected"]["planes"]) < L2_FP32
+    assert out["backward"] == {} and out["stored_values"] == 0
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_bwpm2d_tilted_silver(precision: str) -> None:
+    d = load_golden("bwpm2d_tilted_silver")
+    a = fixture_2d_args(d)
+    nz = a["labels"].shape[0]
+    out = _solver(a, precision).run(a["field"], 2, list(range(nz + 1)))
+    fwd = np.stack([out["forward"][p][:, 0] for p in range(nz + 1)])
+    bwd = np.stack([out["backward"][p][:, 0] for p in range(nz + 1)])
+    p = d["parameters"]
+    last = complex(d["inputs"]["indices"][int(d["inputs"]["labels"][-1][p["ny"] // 2])])
+    res = O.energy_residual_2d(d["inputs"]["field"], [t[:, 0] for t in out["transmitted_iter"]],
+                               [r[:, 0] for r in out["reflected_iter"]], p["ny"], p["dy_mm"], a["k0"], 1.0, last)
+    if precision == "fp64":
+        _close64(fwd, d["expected"]["forward"])
+        _close64(bwd, d["expected"]["backward"])
+        assert np.allclose(res, d["expected"]["energy_residual"], rtol=1e-10, atol=1e-12)
+    else:
+        assert rel_l2(fwd, d["expected"]["forward"]) < L2_FP32
+        assert rel_l2(bwd, d["expected"]["backward"]) < L2_FP32
+        assert np.allclose(res, d["expected"]["energy_residual"], atol=1e-4)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_bwpm2d_slab_exact(precision: str) -> None:
+    d = load_golden("bwpm2d_slab_exact")
+    a = fixture_2d_args(d)
+    out = _solver(a, precision).run(a["field"], 40)
+    t, r = out["transmitted"][:, 0], out["reflected"][:, 0]
+    if precision == "fp64":
+        _close64(t, d["expected"]["transmitted"])
+        _close64(r, d["expected"]["reflected"])
+        rr = complex(np.mean(r[:, 0] / d["inputs"]["field"][:, 0]))
+        assert abs(rr - d["reference"]["r_te_airy"]) < d["reference_tolerance"]["abs"]
+    else:
+        assert rel_l2(t, d["expected"]["transmitted"]) < L2_FP32
+        assert rel_l2(r, d["expected"]["reflected"]) < L2_FP32
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_bwpm3d_tilted(precision: str) -> None:
+    d = load_golden("bwpm3d_tilted")
+    a = fixture_3d_args(d)
+    nz = a["labels"].shape[0]
+    keep = (0, 10, nz)
+    out = _solver(a, precision).run(a["field"], 2, keep)
+    fwd = np.stack([out["forward"][q] for q in keep])
+    bwd = np.stack([out["backward"][q] for q in keep])
+    e = d["expected"]
+    if precision == "fp64":
+        _close64(out["transmitted"], e["transmitted"])
+        _close64(out["reflected"], e["reflected"])
+        _close64(fwd, e["forward_planes"])
+        _close64(bwd, e["backward_planes"])
+    else:
+        for got, want in ((out["transmitted"], e["transmitted"]), (out["reflected"], e["reflected"]),
+                          (fwd, e["forward_planes"]), (bwd, e["backward_planes"])):
+            assert rel_l2(got, want) < L2_FP32
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_random_volumes_equal_oracle(seed: int, precision: str) -> None:
+    """Random volumes with four materials incl. silver, odd and even sizes, dx ≠ dy (grazing
+    points), 2 iterations; the reference's core-vs-Python test (test_core_solver.py:99-127)."""
+    a = random_volume(seed)
+    nz = a["labels"].shape[0]
+    keep = (0, 5, nz)
+    ref = O.Solver(a["ny"], a["nx"], a["dy"], a["dx"], a["dz"], a["k0"], a["indices"], a["labels"],
+                   a["index_before"], a["mask"]).run(a["field"], 2, keep)
+    out = _solver(a, precision).run(a["field"], 2, keep)
+    assert out["stored_values"] == ref["stored_values"]
+    pairs = [(out["transmitted"], ref["transmitted"]), (out["reflected"], ref["reflected"])]
+    for q in keep:
+        pairs += [(out["forward"][q], ref["forward"][q]), (out["backward"][q], ref["backward"][q])]
+    scale = float(np.max(np.abs(a["field"])))
+    for got, want in pairs:
+        if precision == "fp64":
+            assert np.max(np.abs(got - want)) < 1e-10 * scale
+        else:
+            assert rel_l2(got, want) < L2_FP32
