@@ -1,0 +1,164 @@
+#!/usr/bin/env python3
+"""Benchmark of the 3D vector BWPM slab-stepping path on one MI355X.
+
+Workload (default): the tilted-interface 3D vector geometry of the reference's plain config
+(λ = 1.55 µm, dx = dy = λ/6, dz = λ/4, interface 1.0 → 1.46 tilted 10°, pixel-centre labels,
+Gaussian w₀ = 8 µm, 4 bidirectional iterations) at 512² × 1024 slabs, FP32 bulk. One step is one
+batch of `--batch` independent solves (E_x and E_y polarised alternately) run concurrently on the
+GPU with fields and state resident in HBM; the timed region covers `--steps` such batches.
+
+Printed: one JSON line with grid-point·slab updates/s (whole job), the roofline of the slab-step
+kernels (HIP-event durations over a measured solve, algorithmic 32 B per update), and the
+reference's CPU core timed on this host over a bounded sample of the same geometry.
+
+    python bench.py [--steps K] [--warmup W] [--batch B] [--n 512] [--nz 1024] [--iterations 4]
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from concurrent.futures import ThreadPoolExecutor
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBPS = 8000.0  # MI355X HBM3E
+BYTES_PER_UPDATE = 32   # read + write of the complex64 (E_x, E_y) plane per slab step
+
+
+def cpu_baseline(n: int, nz_sample: int, threads: int) -> dict:
+    """The reference's compiled CPU core (oracle/_ref) on a bounded sample of the workload, or the
+    NumPy oracle when that build is absent."""
+    from bwpm_amd import workloads
+
+    prob = workloads.tilted_3d(n=n, nz=nz_sample, iterations=1)
+    args = (prob.ny, prob.nx, prob.dy, prob.dx, prob.dz, prob.k0, np.asarray(prob.indices, complex),
+            prob.labels.astype(np.int32), prob.index_before, None)
+    ref_dir = ROOT / "oracle" / "_ref"
+    so = sorted(ref_dir.glob("_core*.so")) if ref_dir.exists() else []
+    if so:
+        import importlib.util
+
+        spec = importlib.util.spec_from_file_location("_core", so[0])
+        mod = importlib.util.module_from_spec(spec)
+        spec.loader.exec_module(mod)
+        solver = mod.Solver(*args, threads)
+        kind, cores = "reference", threads
+    else:
+        from oracle import bwpm_oracle
+
+        solver = bwpm_oracle.Solver(*args)
+        kind, cores = "port", 1
+    t0 = time.perf_counter()
+    solver.run(prob.field, prob.iterations, [])
+    dt = time.perf_counter() - t0
+    return {"value": prob.updates / dt, "unit": "grid-point·slab updates/s", "cores": cores, "kind": kind,
+            "sample": f"tilted_3d {n}x{n}x{nz_sample} slabs, 1 iteration ({prob.updates} updates) in {dt:.2f} s"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--batch", type=int, default=4)
+    ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--nz", type=int, default=1024)
+    ap.add_argument("--iterations", type=int, default=4)
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64", "fp32_plain"])
+    ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "0") or 0))
+    ap.add_argument("--cpu-nz", type=int, default=48)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU figures are not produced",
+                          "n_gpus": args.gpus}))
+        return 2
+
+    from bwpm_amd import _core, workloads
+
+    probs = [workloads.tilted_3d(n=args.n, nz=args.nz, iterations=args.iterations,
+                                 polarisation="x" if i % 2 == 0 else "y") for i in range(args.batch)]
+    solvers = []
+    for p in probs:
+        s = _core.GpuSolver(*p.solver_args(), precision=args.precision)
+        s.upload(p.field)
+        solvers.append(s)
+    pool = ThreadPoolExecutor(max_workers=len(solvers))
+
+    def one_step() -> None:
+        list(pool.map(lambda s: s.run_resident(args.iterations), solvers))
+
+    for _ in range(args.warmup):
+        one_step()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        one_step()
+    elapsed = time.perf_counter() - t0
+    updates = sum(p.updates for p in probs) * args.steps
+    value = updates / elapsed
+    ms_per_step = 1e3 * elapsed / max(args.steps, 1)
+
+    # kernel durations of one solve, HIP events on the solve's stream
+    kt = solvers[0].run_resident(args.iterations, time_kernels=True)
+    npix = args.n * args.n
+    col_us = 1e3 * kt["col_ms"] / max(kt["col_launches"], 1)
+    row_us = 1e3 * kt["row_ms"] / max(kt["row_launches"], 1)
+    step_us = col_us + row_us
+    achieved = BYTES_PER_UPDATE * npix / (step_us * 1e-6) / 1e9
+    copy_bytes = npix * 16
+    copy_ms = _core.copy_probe(copy_bytes, 50)
+    roofline = {
+        "bound": "hbm",
+        "kernel": "k_col_step+k_row_step (one slab step)",
+        "achieved": round(achieved, 2),
+        "peak": HBM_PEAK_GBPS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBPS, 5),
+        "traffic": None,
+        "algorithmic_bytes_per_launch_pair": BYTES_PER_UPDATE * npix,
+        "k_col_step_us": round(col_us, 3),
+        "k_row_step_us": round(row_us, 3),
+        "copy_probe_GBps": round(2 * copy_bytes / (copy_ms * 1e-3) / 1e9, 1),
+    }
+    cpu = None
+    if not args.no_cpu_baseline:
+        threads = args.cpu_threads if args.cpu_threads > 0 else min(16, os.cpu_count() or 1)
+        cpu = cpu_baseline(args.n, args.cpu_nz, threads)
+    line = {
+        "metric": "grid-point·slab updates/s (3D vector 512²)",
+        "value": value,
+        "unit": "grid-point·slab updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c64" if args.precision.startswith("fp32") else "c128",
+        "data": "synthetic",
+        "config": {
+            "workload": f"tilted-interface 3D vector BWPM {args.n}x{args.n}x{args.nz} slabs, "
+                        f"{args.iterations} iterations, batch of {args.batch} concurrent solves",
+            "n": args.n, "nz": args.nz, "iterations": args.iterations, "batch": args.batch,
+            "precision": args.precision, "updates_per_solve": probs[0].updates,
+        },
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    for s in solvers:
+        s.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

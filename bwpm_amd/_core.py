@@ -63,6 +63,9 @@ _upload = _sig("bwpm_solver_upload", _p, _p)
 _run_resident = _sig("bwpm_solver_run_resident", _p, _int, _int, ctypes.POINTER(_d), ctypes.POINTER(_d),
                      ctypes.POINTER(_d), ctypes.POINTER(_i64), ctypes.POINTER(_i64))
 _dev_bytes = _sig("bwpm_solver_device_bytes", _p, ctypes.POINTER(_i64))
+_copy_probe = _sig("bwpm_copy_probe", _i64, _int, _int, ctypes.POINTER(_d))
+_fetch = _sig("bwpm_solver_fetch", _p, _int, ctypes.POINTER(_p))
+_batch = _sig("bwpm_batch_run", _p, _i64, _int, _int, ctypes.POINTER(_d))
 
 LIBRARY = str(_LIB_PATH)
 
@@ -80,6 +83,13 @@ def gpu_device_count() -> int:
     n = _int(0)
     _check(_device_count(ctypes.byref(n)))
     return int(n.value)
+
+
+def copy_probe(nbytes: int, reps: int = 20, device: int = 0) -> float:
+    """Average duration in ms of the plain copy kernel `k_copy_probe` over `nbytes`."""
+    ms = _d(0)
+    _check(_copy_probe(int(nbytes), int(reps), int(device), ctypes.byref(ms)))
+    return float(ms.value)
 
 
 HAS_GPU = True
@@ -187,6 +197,15 @@ class GpuSolver:
         kp = np.ascontiguousarray(np.asarray(keep_list, dtype=np.int64))
         res = _p(None)
         _check(_run(self._h, _ptr(e), int(iterations), _ptr(kp) if kp.size else None, kp.size, ctypes.byref(res)))
+        return self._collect(res, keep_list)
+
+    def fetch(self, iterations: int) -> dict[str, Any]:
+        """Results of the last resident or batched run (no kept planes)."""
+        res = _p(None)
+        _check(_fetch(self._h, int(iterations), ctypes.byref(res)))
+        return self._collect(res, [])
+
+    def _collect(self, res: Any, keep_list: list[int]) -> dict[str, Any]:
         try:
             shape = (self.ny, self.nx, 2)
 
@@ -249,3 +268,15 @@ class GpuSolver:
 
 
 CudaSolver = GpuSolver  # the reference's name for its GPU solver class
+
+
+def run_batch(solvers: list[GpuSolver], iterations: int, max_lane: int = 64) -> float:
+    """Run every solver (fields uploaded with `upload`) as one batch; returns the wall time in ms.
+
+    Solvers of one grid and precision share lanes of at most `max_lane` that step in lockstep
+    (one column and one row launch per slab step for the whole lane); collect results with
+    `fetch(iterations)`. Each result is bit-identical to the solver's run alone."""
+    arr = (_p * len(solvers))(*[s._h.value for s in solvers])
+    ms = _d(0)
+    _check(_batch(arr, len(solvers), int(iterations), int(max_lane), ctypes.byref(ms)))
+    return float(ms.value)

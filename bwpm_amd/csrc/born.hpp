@@ -9,16 +9,14 @@ namespace bwpm {
 template <typename T, bool COMP>
 class Born {
  public:
-  Born(const Spec&, const std::vector<char>&, const std::vector<c128>&, const double*, const double*,
-       const FftPlan&, const FftPlan&, hipStream_t) {
+  Born(const Spec&, const std::vector<char>&, const std::vector<c128>&, const double*, const double*, hipStream_t) {
     throw Error(kEinval, "index fluctuations are not supported by this build");
   }
-  const px2<T>* forward(const px2<T>*, long long, bool) { return nullptr; }
-  const px2<T>* backward(const px2<T>*, long long) { return nullptr; }
+  void forward(const px2<T>*, long long, bool) {}
+  void backward(const px2<T>*, long long) {}
+  const px2<T>* increment() const { return nullptr; }
   const px2<T>* fwd_plane(long long) const { return nullptr; }
   const px2<T>* bwd_plane(long long) const { return nullptr; }
-  void reset_iteration_planes() {}
-  void backward_done() {}
   long long iterations() const { return 0; }
   size_t device_bytes() const { return 0; }
 };

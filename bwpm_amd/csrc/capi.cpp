@@ -231,4 +231,37 @@ int bwpm_solver_device_bytes(const bwpm_solver* s, int64_t* bytes) {
   });
 }
 
+int bwpm_solver_fetch(bwpm_solver* s, int iterations, bwpm_result** out) {
+  return guarded([&] {
+    bwpm::require(s && out, "null argument");
+    bwpm::require(iterations >= 0, "iterations must be >= 0");
+    *out = nullptr;
+    auto r = std::make_unique<bwpm_result>();
+    s->engine->download(r->out, iterations, {});
+    r->npix = static_cast<long long>(r->out.transmitted.size() / 2);
+    *out = r.release();
+  });
+}
+
+int bwpm_batch_run(bwpm_solver* const* solvers, int64_t n, int iterations, int max_lane, double* wall_ms) {
+  return guarded([&] {
+    bwpm::require(n >= 0 && (n == 0 || solvers), "null solver array");
+    bwpm::require(iterations >= 0, "iterations must be >= 0");
+    std::vector<bwpm::EngineBase*> es;
+    for (int64_t i = 0; i < n; ++i) {
+      bwpm::require(solvers[i] != nullptr, "null solver in batch");
+      es.push_back(solvers[i]->engine.get());
+    }
+    const double ms = bwpm::run_batch(es, iterations, max_lane);
+    if (wall_ms) *wall_ms = ms;
+  });
+}
+
+int bwpm_copy_probe(int64_t bytes, int reps, int device, double* ms) {
+  return guarded([&] {
+    bwpm::require(ms != nullptr, "null argument");
+    *ms = bwpm::copy_probe(bytes, reps, device);
+  });
+}
+
 }  // extern "C"

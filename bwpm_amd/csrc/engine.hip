@@ -1,759 +1,107 @@
-// Device-resident bidirectional BWPM driver (reference semantics: Solver::run,
-// cpp/src/solver.cpp:444-528; the step order is listed in the comments of Engine::run_device).
-#include "engine.hpp"
-
+// Non-templated parts of the engine: precision dispatch, batched lanes, the exported FP64
+// interface functions and the HBM copy probe.
 #include <algorithm>
-#include <cstring>
-#include <unordered_map>
+#include <chrono>
+#include <map>
+#include <thread>
+#include <tuple>
 
-#include "born.hpp"
-#include "kernels.hpp"
+#include "engine.hpp"
+#include "interface_math.hpp"
 
 namespace bwpm {
 
 namespace {
 
-// ---- device memory -------------------------------------------------------------------------
-
-class DevBuf {
+class Buf {
  public:
-  DevBuf() = default;
-  explicit DevBuf(size_t bytes) { alloc(bytes); }
-  DevBuf(const DevBuf&) = delete;
-  DevBuf& operator=(const DevBuf&) = delete;
-  DevBuf(DevBuf&& o) noexcept : p_(o.p_), n_(o.n_) { o.p_ = nullptr; o.n_ = 0; }
-  DevBuf& operator=(DevBuf&& o) noexcept {
-    release();
-    p_ = o.p_; n_ = o.n_; o.p_ = nullptr; o.n_ = 0;
-    return *this;
-  }
-  ~DevBuf() { release(); }
-  void alloc(size_t bytes) {
-    release();
-    if (bytes == 0) return;
+  explicit Buf(size_t bytes) {
     const hipError_t e = hipMalloc(&p_, bytes);
-    if (e != hipSuccess) {
-      p_ = nullptr;
-      throw Error(kEnomem, "device allocation of " + std::to_string(bytes) + " bytes failed: " +
-                               hipGetErrorString(e));
-    }
-    n_ = bytes;
+    if (e != hipSuccess) throw Error(kEnomem, std::string("device allocation failed: ") + hipGetErrorString(e));
   }
-  void release() {
-    if (p_) (void)hipFree(p_);
-    p_ = nullptr;
-    n_ = 0;
-  }
+  ~Buf() { (void)hipFree(p_); }
+  Buf(const Buf&) = delete;
+  Buf& operator=(const Buf&) = delete;
   template <typename U>
   U* as() const { return static_cast<U*>(p_); }
-  size_t bytes() const { return n_; }
 
  private:
   void* p_ = nullptr;
-  size_t n_ = 0;
 };
-
-template <typename U>
-DevBuf upload_vec(const std::vector<U>& v, hipStream_t s) {
-  DevBuf b(v.size() * sizeof(U));
-  if (!v.empty()) HIP_CHECK(hipMemcpyAsync(b.as<void>(), v.data(), v.size() * sizeof(U),
-                                           hipMemcpyHostToDevice, s));
-  return b;
-}
-
-// ---- FFT plans ---------------------------------------------------------------------------------
-
-std::vector<int> factorize(long long n) {
-  std::vector<int> r;
-  long long m = n;
-  for (int R : {8, 4, 2, 3, 5, 7, 11, 13})
-    while (m % R == 0) {
-      r.push_back(R);
-      m /= R;
-    }
-  if (m != 1)
-    throw Error(kEinval, "grid size " + std::to_string(n) +
-                             " has a prime factor above 13; supported sizes are 2^a·3^b·5^c·7^d·11^e·13^f");
-  // a single radix-2 stage is cheaper after the radix-8 stages; merge 2·2 into 4 if produced
-  if (r.size() > kMaxStages) throw Error(kEinval, "grid size " + std::to_string(n) + " needs too many FFT stages");
-  return r;
-}
-
-template <typename T>
-struct Twiddles {
-  DevBuf hi, lo;
-  FftPlan plan{};
-  void build(long long n, hipStream_t s) {
-    plan.n = static_cast<int>(n);
-    const std::vector<int> r = n > 1 ? factorize(n) : std::vector<int>{};
-    plan.nst = static_cast<int>(r.size());
-    for (size_t i = 0; i < r.size(); ++i) plan.rad[i] = r[i];
-    std::vector<cx<T>> h(n), l(n);
-    for (long long t = 0; t < n; ++t) {
-      // exp(−2πi t/n) from the mathematics in extended precision, then rounded
-      const long double a = -2.0L * 3.141592653589793238462643383279502884L * static_cast<long double>(t) /
-                            static_cast<long double>(n);
-      const double c = static_cast<double>(cosl(a)), sn = static_cast<double>(sinl(a));
-      const T ch = static_cast<T>(c), sh = static_cast<T>(sn);
-      h[t] = cx<T>{ch, sh};
-      l[t] = cx<T>{static_cast<T>(c - static_cast<double>(ch)), static_cast<T>(sn - static_cast<double>(sh))};
-    }
-    hi = upload_vec(h, s);
-    lo = upload_vec(l, s);
-    plan.tw_hi = hi.as<void>();
-    plan.tw_lo = lo.as<void>();
-  }
-};
-
-// ---- per-boundary structure ------------------------------------------------------------------
-
-// Boundary q separates plane q (slab q − 1, or the medium before the volume for q = 0) from
-// plane q + 1 (slab q). Forward step q crosses it left to right, backward step q + 1 right to left.
-struct Boundary {
-  std::vector<std::pair<int32_t, int32_t>> pairs;  // (label left, label right), sorted
-  std::vector<int> mleft, mright;                   // medium ids of each pair
-  std::vector<int> rpair;                           // pair index of each R product
-  long long tsel_off = -1;                          // into the tsel byte array (pairs > 1)
-  long long sp_off = 0, sp_count = 0;               // sparse interface entries
-  long long rowptr_off = -1;                        // ny + 1 offsets (sp_count > 0)
-  int fwd_prod = 0, bwd_prod = 0;                   // offsets into the product descriptor array
-  int source = -1;                                  // boundary whose structure this one reuses
-};
-
-// Operator table of one (kind, medium pair): hi and lo parts.
-struct OpTable {
-  DevBuf hi, lo;
-};
-
-inline int pow2_floor(int v) {
-  int p = 1;
-  while (p * 2 <= v) p *= 2;
-  return p;
-}
 
 }  // namespace
 
-template <typename T, bool COMP>
-class Engine final : public EngineBase {
- public:
-  explicit Engine(const Spec& s) : sp_(s) {
-    HIP_CHECK(hipSetDevice(s.device));
-    HIP_CHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
-    try {
-      setup();
-    } catch (...) {
-      (void)hipStreamDestroy(stream_);
-      throw;
-    }
-  }
-  ~Engine() override {
-    (void)hipSetDevice(sp_.device);
-    (void)hipStreamSynchronize(stream_);
-    born_.reset();
-    (void)hipStreamDestroy(stream_);
-  }
+// Exported interface functions on arbitrary k arrays: full row-major 2×2 (which = 0: T, 1: R).
+__global__ void k_interface_eval(const double* kx, const double* ky, long long n, int which, zd n1,
+                                 zd n2, double k0, zd* out) {
+  const long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;
+  if (i >= n) return;
+  const Mat2 m = which == 0 ? transmission(kx[i], ky[i], n1, n2, k0) : reflection(kx[i], ky[i], n1, n2, k0);
+  out[4 * i] = m.m00;
+  out[4 * i + 1] = m.m01;
+  out[4 * i + 2] = m.m01;
+  out[4 * i + 3] = m.m11;
+}
 
-  hipStream_t stream() const override { return stream_; }
-  size_t device_bytes() const override { return bytes_; }
+__global__ void k_kz_eval(zd n, double k0, const double* kp2, long long cnt, zd* out) {
+  const long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;
+  if (i < cnt) out[i] = kz_of(n, k0, kp2[i]);
+}
 
-  void run(const c128* field, int iterations, const std::vector<long long>& keep, RunOutput& out) override {
-    HIP_CHECK(hipSetDevice(sp_.device));
-    upload(field);
-    keep_ = keep;
-    out.gpu_ms = run_resident_impl(iterations, false, nullptr);
-    download(out, iterations, keep);
-  }
-
-  void upload(const c128* field) override {
-    HIP_CHECK(hipSetDevice(sp_.device));
-    HIP_CHECK(hipMemcpyAsync(stage_.as<void>(), field, npix_ * 2 * sizeof(c128), hipMemcpyHostToDevice, stream_));
-    launch_1d(k_from_c128<T>, npix_, stage_.as<zd>(), field_in_.as<px2<T>>(), npix_);
-  }
-
-  double run_resident(int iterations, bool time_kernels, KernelTimes* kt) override {
-    HIP_CHECK(hipSetDevice(sp_.device));
-    keep_.clear();
-    return run_resident_impl(iterations, time_kernels, kt);
-  }
-
-  void download(RunOutput& out, int iterations, const std::vector<long long>& keep) override {
-    HIP_CHECK(hipSetDevice(sp_.device));
-    const int nit = std::max(iterations, 1);
-    auto fetch = [&](const px2<T>* src) {
-      std::vector<c128> v(npix_ * 2);
-      launch_1d(k_to_c128<T>, npix_, src, stage_.as<zd>(), npix_);
-      HIP_CHECK(hipMemcpyAsync(v.data(), stage_.as<void>(), npix_ * 2 * sizeof(c128), hipMemcpyDeviceToHost, stream_));
-      HIP_CHECK(hipStreamSynchronize(stream_));
-      return v;
-    };
-    out.transmitted = fetch(iter_t_.as<px2<T>>() + (nit - 1) * npix_);
-    if (iterations > 0) {
-      out.reflected = fetch(iter_r_.as<px2<T>>() + (nit - 1) * npix_);
-      out.transmitted_iter.clear();
-      out.reflected_iter.clear();
-      for (int it = 0; it < nit; ++it) {
-        out.transmitted_iter.push_back(fetch(iter_t_.as<px2<T>>() + it * npix_));
-        out.reflected_iter.push_back(fetch(iter_r_.as<px2<T>>() + it * npix_));
-      }
-    } else {
-      out.reflected.assign(npix_ * 2, c128(0.0, 0.0));
-    }
-    out.forward.clear();
-    out.backward.clear();
-    for (long long p : keep) {
-      const int slot = keep_slot(p);
-      if (slot < 0 || p < 0 || p > sp_.nz) continue;  // planes 0 … nz exist (solver.cpp:474,493)
-      out.forward[p] = fetch(keepf_.as<px2<T>>() + slot * npix_);
-      if (iterations > 0) out.backward[p] = fetch(keepb_.as<px2<T>>() + slot * npix_);
-    }
-    out.stored_values = iterations > 0 ? 2 * total_sparse_ : 0;
-    out.slab_steps = last_steps_;
-    out.gmres_iterations = born_ ? born_->iterations() : 0;
-  }
-
- private:
-  template <typename K, typename... A>
-  void launch_1d(K kern, long long n, A... args) {
-    if (n <= 0) return;
-    const int bs = 256;
-    const long long grid = (n + bs - 1) / bs;
-    hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>(grid)), dim3(bs), 0, stream_, args...);
-    HIP_CHECK(hipGetLastError());
-  }
-
-  int medium_of_label(int32_t l) const { return l < 0 ? before_id_ : label_medium_[l]; }
-  const int32_t* plane_labels(long long p) const {
-    return p == 0 ? before_.data() : sp_.labels.data() + (p - 1) * npix_;
-  }
-
-  void validate() {
-    const Spec& s = sp_;
-    require(s.ny >= 1 && s.nx >= 1, "grid must have ny, nx >= 1");
-    require(s.nz >= 1, "labels must have shape (nz, ny, nx) with nz >= 1");
-    require(static_cast<long long>(s.labels.size()) == s.nz * s.ny * s.nx, "labels must have shape (nz, ny, nx)");
-    require(s.mask.empty() || static_cast<long long>(s.mask.size()) == s.ny * s.nx, "mask must have shape (ny, nx)");
-    for (int32_t l : s.labels)
-      require(l >= 0 && static_cast<size_t>(l) < s.indices.size(), "labels must index into `indices`");
-    require(s.fluct.empty() || static_cast<long long>(s.fluct.size()) == s.nz * s.ny * s.nx,
-            "fluctuation must have the shape of labels (nz, ny, nx)");
-  }
-
-  void setup() {
-    validate();
-    const Spec& s = sp_;
-    npix_ = s.ny * s.nx;
-    // media: distinct complex index values
-    std::vector<c128> media;
-    auto id_of = [&](c128 v) {
-      for (size_t i = 0; i < media.size(); ++i)
-        if (media[i] == v) return static_cast<int>(i);
-      media.push_back(v);
-      return static_cast<int>(media.size() - 1);
-    };
-    before_id_ = id_of(s.index_before);
-    for (const c128& v : s.indices) label_medium_.push_back(id_of(v));
-    media_ = media;
-    before_.assign(npix_, -1);
-
-    // Born hosts per slab (validation identical to the reference constructor, solver.cpp:47-62)
-    has_host_.assign(s.nz, 0);
-    host_.assign(s.nz, c128(0.0, 0.0));
-    if (!s.fluct.empty()) {
-      for (long long j = 0; j < s.nz; ++j)
-        for (long long q = 0; q < npix_; ++q) {
-          const c128 d = s.fluct[j * npix_ + q];
-          if (d == c128(0.0, 0.0)) continue;
-          const c128 idx = s.indices[s.labels[j * npix_ + q]];
-          if (has_host_[j] && host_[j] != idx)
-            throw Error(kEinval, "slab " + std::to_string(j) + ": fluctuation must lie in one host index");
-          has_host_[j] = 1;
-          host_[j] = idx;
-        }
-    }
-
-    // k tables (host, rounded like 2π·numpy.fft.fftfreq)
-    std::vector<double> kx(s.nx), ky(s.ny);
-    for (long long i = 0; i < s.nx; ++i) kx[i] = fftfreq(i, s.nx, s.dx);
-    for (long long j = 0; j < s.ny; ++j) ky[j] = fftfreq(j, s.ny, s.dy);
-    kx_ = upload_vec(kx, stream_);
-    ky_ = upload_vec(ky, stream_);
-    twx_.build(s.nx, stream_);
-    twy_.build(s.ny, stream_);
-
-    build_boundaries();
-    build_products();
-
-    // launch geometry
-    const int elems = Budget<T>::elems;
-    auto ls_of = [](int lines) { return lines <= 2 ? lines : (lines | 1); };
-    // column kernel: widest power-of-two column block whose lines fit the element budget
-    auto col_cols = [&](int nbuf) {
-      int w = 16;
-      while (w > 1 && (s.ny * ls_of(2 * w) > elems || nbuf * s.ny * ls_of(2 * w) * (long long)sizeof(cx<T>) > kLdsMax))
-        w /= 2;
-      w = std::min<long long>(w, pow2_floor(static_cast<int>(std::max<long long>(1, s.nx))));
-      require(s.ny * ls_of(2 * w) <= elems && nbuf * s.ny * ls_of(2 * w) * (long long)sizeof(cx<T>) <= kLdsMax,
-              "grid size ny = " + std::to_string(s.ny) + " exceeds the column kernel's LDS budget");
-      return w;
-    };
-    cols1_ = col_cols(1);
-    cols2_ = col_cols(2);
-    int rw = static_cast<int>(std::min<long long>(s.ny, 256));
-    while (rw > 1 && (s.nx * ls_of(2 * rw) > elems || 2 * s.nx * ls_of(2 * rw) * (long long)sizeof(cx<T>) > kLdsMax)) rw /= 2;
-    require(s.nx * ls_of(2 * rw) <= elems && 2 * s.nx * ls_of(2 * rw) * (long long)sizeof(cx<T>) <= kLdsMax,
-            "grid size nx = " + std::to_string(s.nx) + " exceeds the row kernel's LDS budget");
-    rows_ = rw;
-    ls_col1_ = ls_of(2 * cols1_);
-    ls_col2_ = ls_of(2 * cols2_);
-    ls_row_ = ls_of(2 * rows_);
-    HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_col_step<T, COMP>),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, kLdsMax));
-    HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_row_step<T, COMP>),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, kLdsMax));
-
-    // buffers
-    const size_t pb = npix_ * sizeof(px2<T>);
-    stage_.alloc(npix_ * 2 * sizeof(c128));
-    field_in_.alloc(pb);
-    X_.alloc(pb);
-    prod_.alloc(pb * std::max(1, max_prod_));
-    e_.alloc(pb);
-    const size_t sp_bytes = std::max<long long>(1, total_sparse_) * sizeof(px2<T>);
-    fwd_vals_.alloc(sp_bytes);
-    bwd_vals_.alloc(sp_bytes);
-    if (!s.mask.empty()) {
-      std::vector<T> m(npix_);
-      for (long long q = 0; q < npix_; ++q) m[q] = static_cast<T>(s.mask[q]);
-      mask_ = upload_vec(m, stream_);
-    }
-    if (std::any_of(has_host_.begin(), has_host_.end(), [](char c) { return c != 0; }))
-      born_ = std::make_unique<Born<T, COMP>>(s, has_host_, host_, kx_.as<double>(), ky_.as<double>(),
-                                              twx_.plan, twy_.plan, stream_);
-    HIP_CHECK(hipStreamSynchronize(stream_));
-    bytes_ = stage_.bytes() + field_in_.bytes() + X_.bytes() + prod_.bytes() + e_.bytes() + fwd_vals_.bytes() +
-             bwd_vals_.bytes() + mask_.bytes() + ops_bytes_ + tsel_.bytes() + sp_pix_.bytes() + sp_rid_.bytes() +
-             rowptr_.bytes() + (born_ ? born_->device_bytes() : 0);
-  }
-
-  // Label pairs, tsel maps and sparse interface lists per boundary.
-  void build_boundaries() {
-    const Spec& s = sp_;
-    const long long nz = s.nz;
-    bnd_.resize(nz);
-    std::vector<uint8_t> tsel;
-    std::vector<int32_t> sp_pix, rowptr;
-    std::vector<uint8_t> sp_rid;
-    total_sparse_ = 0;
-    for (long long q = 0; q < nz; ++q) {
-      const int32_t* a = plane_labels(q);
-      const int32_t* b = plane_labels(q + 1);
-      Boundary& B = bnd_[q];
-      // reuse the previous boundary's structure when both label planes repeat
-      if (q >= 1 && std::memcmp(a, plane_labels(q - 1), npix_ * sizeof(int32_t)) == 0 &&
-          std::memcmp(b, plane_labels(q), npix_ * sizeof(int32_t)) == 0) {
-        const int src = bnd_[q - 1].source >= 0 ? bnd_[q - 1].source : static_cast<int>(q - 1);
-        B = bnd_[src];
-        B.source = src;
-        // sparse values are per boundary: give this boundary its own value slots
-        B.sp_off = total_sparse_;
-        total_sparse_ += B.sp_count;
-        continue;
-      }
-      // distinct pairs (sorted like the reference's std::set)
-      std::map<std::pair<int32_t, int32_t>, int> pid;
-      bool uniform = true;
-      for (long long i = 1; i < npix_ && uniform; ++i) uniform = a[i] == a[0] && b[i] == b[0];
-      if (uniform) {
-        pid[{a[0], b[0]}] = 0;
-      } else {
-        for (long long i = 0; i < npix_; ++i) pid.emplace(std::make_pair(a[i], b[i]), 0);
-      }
-      require(pid.size() <= 255, "boundary " + std::to_string(q) + " has more than 255 label pairs");
-      int k = 0;
-      for (auto& kv : pid) {
-        kv.second = k++;
-        B.pairs.push_back(kv.first);
-        B.mleft.push_back(medium_of_label(kv.first.first));
-        B.mright.push_back(medium_of_label(kv.first.second));
-      }
-      std::vector<int> rid_of_pair(B.pairs.size(), -1);
-      for (size_t i = 0; i < B.pairs.size(); ++i)
-        if (B.mleft[i] != B.mright[i]) {
-          rid_of_pair[i] = static_cast<int>(B.rpair.size());
-          B.rpair.push_back(static_cast<int>(i));
-        }
-      if (B.pairs.size() > 1) {
-        B.tsel_off = static_cast<long long>(tsel.size());
-        tsel.resize(tsel.size() + npix_);
-        for (long long i = 0; i < npix_; ++i) tsel[B.tsel_off + i] = static_cast<uint8_t>(pid[{a[i], b[i]}]);
-      }
-      if (!B.rpair.empty()) {
-        B.sp_off = total_sparse_;
-        B.rowptr_off = static_cast<long long>(rowptr.size());
-        long long cnt = 0;
-        for (long long y = 0; y < s.ny; ++y) {
-          rowptr.push_back(static_cast<int32_t>(cnt));
-          for (long long x = 0; x < s.nx; ++x) {
-            const long long i = y * s.nx + x;
-            const int pi = B.pairs.size() > 1 ? tsel[B.tsel_off + i] : 0;
-            if (rid_of_pair[pi] >= 0) {
-              sp_pix.push_back(static_cast<int32_t>(i));
-              sp_rid.push_back(static_cast<uint8_t>(rid_of_pair[pi]));
-              ++cnt;
-            }
-          }
-        }
-        rowptr.push_back(static_cast<int32_t>(cnt));
-        B.sp_count = cnt;
-        // the list entries of this structure start at this offset of sp_pix / sp_rid
-        list_off_[q] = static_cast<long long>(sp_pix.size()) - cnt;
-        total_sparse_ += cnt;
-      }
-    }
-    tsel_ = upload_vec(tsel, stream_);
-    sp_pix_ = upload_vec(sp_pix, stream_);
-    sp_rid_ = upload_vec(sp_rid, stream_);
-    rowptr_ = upload_vec(rowptr, stream_);
-  }
-
-  // Operator tables and per-step product descriptor lists.
-  void build_products() {
-    const Spec& s = sp_;
-    std::vector<Prod<T>> prods;
-    max_prod_ = 1;
-    for (long long q = 0; q < s.nz; ++q) {
-      Boundary& B = bnd_[q];
-      // forward step q: T kinds (FwdT) for every pair, then R kinds (FwdR)
-      B.fwd_prod = static_cast<int>(prods.size());
-      for (size_t i = 0; i < B.pairs.size(); ++i) prods.push_back(t_prod(kFwdT, B.mleft[i], B.mright[i]));
-      for (int pi : B.rpair) prods.push_back(matrix_prod(kFwdR, B.mleft[pi], B.mright[pi]));
-      // backward step q + 1: r = medium on the right (plane q + 1), s = medium on the left
-      B.bwd_prod = static_cast<int>(prods.size());
-      for (size_t i = 0; i < B.pairs.size(); ++i) prods.push_back(t_prod(kBwdT, B.mright[i], B.mleft[i]));
-      for (int pi : B.rpair) prods.push_back(matrix_prod(kBwdR, B.mright[pi], B.mleft[pi]));
-      max_prod_ = std::max<int>(max_prod_, static_cast<int>(B.pairs.size() + B.rpair.size()));
-    }
-    prods_ = upload_vec(prods, stream_);
-  }
-
-  Prod<T> t_prod(int kind, int mr, int ms) {
-    if (mr == ms) return phase_prod(mr);
-    return matrix_prod(kind, mr, ms);
-  }
-
-  Prod<T> phase_prod(int m) {
-    auto it = phase_ops_.find(m);
-    if (it == phase_ops_.end()) {
-      OpTable t;
-      t.hi.alloc(npix_ * sizeof(cx<T>));
-      t.lo.alloc(npix_ * sizeof(cx<T>));
-      const c128 n = media_[m];
-      launch_1d(k_build_phase<T>, npix_, kx_.as<double>(), ky_.as<double>(), static_cast<int>(sp_.ny),
-                static_cast<int>(sp_.nx), zd{n.real(), n.imag()}, sp_.k0, sp_.dz, 1.0 / static_cast<double>(npix_),
-                t.hi.as<cx<T>>(), t.lo.as<cx<T>>());
-      ops_bytes_ += 2 * npix_ * sizeof(cx<T>);
-      it = phase_ops_.emplace(m, std::move(t)).first;
-    }
-    return Prod<T>{0, it->second.hi.as<cx<T>>(), it->second.lo.as<cx<T>>()};
-  }
-
-  Prod<T> matrix_prod(int kind, int mr, int ms) {
-    const long long key = (static_cast<long long>(kind) << 40) | (static_cast<long long>(mr) << 20) | ms;
-    auto it = matrix_ops_.find(key);
-    if (it == matrix_ops_.end()) {
-      OpTable t;
-      t.hi.alloc(3 * npix_ * sizeof(cx<T>));
-      t.lo.alloc(3 * npix_ * sizeof(cx<T>));
-      const c128 nr = media_[mr], ns = media_[ms];
-      launch_1d(k_build_matrix<T>, npix_, kx_.as<double>(), ky_.as<double>(), static_cast<int>(sp_.ny),
-                static_cast<int>(sp_.nx), kind, zd{nr.real(), nr.imag()}, zd{ns.real(), ns.imag()}, sp_.k0, sp_.dz,
-                1.0 / static_cast<double>(npix_), t.hi.as<cx<T>>(), t.lo.as<cx<T>>());
-      ops_bytes_ += 6 * npix_ * sizeof(cx<T>);
-      it = matrix_ops_.emplace(key, std::move(t)).first;
-    }
-    return Prod<T>{1, it->second.hi.as<cx<T>>(), it->second.lo.as<cx<T>>()};
-  }
-
-  int keep_slot(long long p) const {
-    for (size_t i = 0; i < keep_.size(); ++i)
-      if (keep_[i] == p) return static_cast<int>(i);
-    return -1;
-  }
-
-  // Sparse list of boundary q (its structure may be shared with an earlier boundary).
-  const int* rowptr_of(long long q) const {
-    const Boundary& B = bnd_[q];
-    if (B.sp_count == 0) return nullptr;
-    return rowptr_.as<int>() + B.rowptr_off;
-  }
-  long long list_of(long long q) const {
-    const Boundary& B = bnd_[q];
-    return list_off_.at(B.source >= 0 ? B.source : q);
-  }
-
-  void col_step(const Boundary& B, bool forward, bool with_r) {
-    const int nt = static_cast<int>(B.pairs.size());
-    const int nprod = nt + (with_r ? static_cast<int>(B.rpair.size()) : 0);
-    ColStep<T> a{};
-    a.in = X_.as<px2<T>>();
-    a.out = prod_.as<px2<T>>();
-    a.plane = npix_;
-    a.ny = static_cast<int>(sp_.ny);
-    a.nx = static_cast<int>(sp_.nx);
-    a.cols = nprod > 1 ? cols2_ : cols1_;
-    a.ls = nprod > 1 ? ls_col2_ : ls_col1_;
-    a.nprod = nprod;
-    a.prods = prods_.as<Prod<T>>() + (forward ? B.fwd_prod : B.bwd_prod);
-    a.plan = twy_.plan;
-    const int nbuf = nprod > 1 ? 2 : 1;
-    const size_t lds = static_cast<size_t>(nbuf) * sp_.ny * a.ls * sizeof(cx<T>);
-    const unsigned grid = static_cast<unsigned>((sp_.nx + a.cols - 1) / a.cols);
-    timed(true, [&] { hipLaunchKernelGGL((k_col_step<T, COMP>), dim3(grid), dim3(kThreads), lds, stream_, a); });
-  }
-
-  struct RowOpts {
-    const Boundary* B = nullptr;
-    long long q = -1;        // boundary index (sparse list)
-    bool with_r = false;
-    bool forward = true;
-    const px2<T>* e_in = nullptr;
-    const px2<T>* inc = nullptr;
-    const px2<T>* sp_add = nullptr;
-    px2<T>* sp_store = nullptr;
-    const px2<T>* dense = nullptr;
-    px2<T>* e_out = nullptr;
-    px2<T>* e_out2 = nullptr;
-    bool next = true;
-  };
-
-  void row_step(const RowOpts& o) {
-    RowStep<T> a{};
-    a.ny = static_cast<int>(sp_.ny);
-    a.nx = static_cast<int>(sp_.nx);
-    a.rows = rows_;
-    a.ls = ls_row_;
-    a.plan = twx_.plan;
-    a.prod = prod_.as<px2<T>>();
-    a.plane = npix_;
-    if (o.B) {
-      a.nt = static_cast<int>(o.B->pairs.size());
-      a.nr = o.with_r ? static_cast<int>(o.B->rpair.size()) : 0;
-      a.tsel = o.B->tsel_off >= 0 ? tsel_.as<uint8_t>() + o.B->tsel_off : nullptr;
-      a.mask = mask_.as<T>();
-    }
-    a.e_in = o.e_in;
-    a.inc = o.inc;
-    if (o.q >= 0 && bnd_[o.q].sp_count > 0) {
-      const long long list = list_of(o.q);
-      a.rowptr = rowptr_of(o.q);
-      a.sp_pix = sp_pix_.as<int>() + list;
-      a.sp_rid = sp_rid_.as<uint8_t>() + list;
-      a.sp_add = o.sp_add;
-      a.sp_store = o.sp_store;
-    }
-    a.dense = o.dense;
-    a.e_out = o.e_out;
-    a.e_out2 = o.e_out2;
-    a.x_out = o.next ? X_.as<px2<T>>() : nullptr;
-    const int nbuf = (a.nt > 1 || a.nr > 0) ? 2 : 1;
-    const size_t lds = static_cast<size_t>(nbuf) * sp_.nx * a.ls * sizeof(cx<T>);
-    const unsigned grid = static_cast<unsigned>((sp_.ny + rows_ - 1) / rows_);
-    timed(false, [&] { hipLaunchKernelGGL((k_row_step<T, COMP>), dim3(grid), dim3(kThreads), lds, stream_, a); });
-  }
-
-  template <typename F>
-  void timed(bool col, F&& launch) {
-    if (!time_) {
-      launch();
-      HIP_CHECK(hipGetLastError());
-      return;
-    }
-    HIP_CHECK(hipEventRecord(ev0_, stream_));
-    launch();
-    HIP_CHECK(hipGetLastError());
-    HIP_CHECK(hipEventRecord(ev1_, stream_));
-    HIP_CHECK(hipEventSynchronize(ev1_));
-    float ms = 0;
-    HIP_CHECK(hipEventElapsedTime(&ms, ev0_, ev1_));
-    if (col) {
-      kt_.col_ms += ms;
-      ++kt_.col_launches;
-    } else {
-      kt_.row_ms += ms;
-      ++kt_.row_launches;
-    }
-  }
-
-  px2<T>* sp_vals(DevBuf& b, long long q) { return b.as<px2<T>>() + bnd_[q].sp_off; }
-
-  // The bidirectional iterations. Reference order (solver.cpp:469-525):
-  //  forward:  e = field; for p: [FwdR → src_bwd[p]] FwdT (+ mask·Born increment, R1 → born_bwd[p+1]);
-  //            e += src_fwd[p+1] + born_fwd[p+1]
-  //  backward: e = born_bwd[nz]; for p = nz…1: [R3 → born_fwd[p]; e += R2] [BwdR → src_fwd[p]] BwdT;
-  //            e += src_bwd[p−1] + born_bwd[p−1]
-  double run_resident_impl(int iterations, bool time_kernels, KernelTimes* kt) {
-    const long long nz = sp_.nz;
-    const int nit = std::max(iterations, 1);
-    const bool bidir = iterations > 0;
-    time_ = time_kernels;
-    kt_ = KernelTimes{};
-    if (time_) {
-      HIP_CHECK(hipEventCreate(&ev0_));
-      HIP_CHECK(hipEventCreate(&ev1_));
-    }
-    // per-iteration outputs and kept planes
-    const size_t pb = npix_ * sizeof(px2<T>);
-    if (iter_t_.bytes() < nit * pb) iter_t_.alloc(nit * pb);
-    if (bidir && iter_r_.bytes() < nit * pb) iter_r_.alloc(nit * pb);
-    if (!keep_.empty()) {
-      if (keepf_.bytes() < keep_.size() * pb) keepf_.alloc(keep_.size() * pb);
-      if (bidir && keepb_.bytes() < keep_.size() * pb) keepb_.alloc(keep_.size() * pb);
-    }
-    if (total_sparse_ > 0) launch_1d(k_fill_zero<T>, total_sparse_, fwd_vals_.as<px2<T>>(), total_sparse_);
-    if (born_) born_->reset_iteration_planes();
-
-    hipEvent_t t0, t1;
-    HIP_CHECK(hipEventCreate(&t0));
-    HIP_CHECK(hipEventCreate(&t1));
-    HIP_CHECK(hipEventRecord(t0, stream_));
-    long long steps = 0;
-    for (int it = 0; it < nit; ++it) {
-      const bool last = it == nit - 1;
-      auto kept = [&](DevBuf& buf, long long p) -> px2<T>* {
-        if (!last) return nullptr;
-        const int slot = keep_slot(p);
-        return slot < 0 ? nullptr : buf.as<px2<T>>() + slot * npix_;
-      };
-      px2<T>* trans = iter_t_.as<px2<T>>() + it * npix_;
-      // ---- forward pass
-      {
-        RowOpts o;
-        o.e_in = field_in_.as<px2<T>>();
-        o.e_out2 = kept(keepf_, 0);
-        o.e_out = (born_ && has_host_[0]) ? e_.as<px2<T>>() : nullptr;
-        row_step(o);
-      }
-      for (long long p = 0; p < nz; ++p) {
-        const Boundary& B = bnd_[p];
-        const bool with_r = bidir && !B.rpair.empty();
-        const px2<T>* inc = nullptr;
-        if (born_ && has_host_[p]) {
-          // e_ holds the field at plane p (input of the step)
-          inc = born_->forward(e_.as<px2<T>>(), p, bidir);
-        }
-        col_step(B, true, with_r);
-        RowOpts o;
-        o.B = &B;
-        o.q = p;
-        o.with_r = with_r;
-        o.inc = inc;
-        o.sp_add = (bidir && B.sp_count > 0) ? sp_vals(fwd_vals_, p) : nullptr;
-        o.sp_store = with_r ? sp_vals(bwd_vals_, p) : nullptr;
-        o.dense = born_ ? born_->fwd_plane(p + 1) : nullptr;
-        const bool final_plane = p + 1 == nz;
-        o.e_out = final_plane ? trans : ((born_ && has_host_[p + 1]) ? e_.as<px2<T>>() : nullptr);
-        o.e_out2 = kept(keepf_, p + 1);
-        o.next = !final_plane;
-        row_step(o);
-        ++steps;
-      }
-      if (!bidir) break;
-      // ---- backward pass
-      px2<T>* refl = iter_r_.as<px2<T>>() + it * npix_;
-      {
-        RowOpts o;
-        o.e_in = nullptr;
-        o.dense = born_ ? born_->bwd_plane(nz) : nullptr;
-        o.e_out2 = kept(keepb_, nz);
-        o.e_out = (born_ && has_host_[nz - 1]) ? e_.as<px2<T>>() : nullptr;
-        row_step(o);
-      }
-      for (long long p = nz; p >= 1; --p) {
-        const Boundary& B = bnd_[p - 1];
-        if (born_ && has_host_[p - 1]) {
-          // R3 from e at plane p, then e += R2 (no mask), then the row FFT of the sum
-          const px2<T>* inc = born_->backward(e_.as<px2<T>>(), p - 1);
-          RowOpts o;
-          o.e_in = e_.as<px2<T>>();
-          o.inc = inc;
-          row_step(o);
-        }
-        const bool with_r = !B.rpair.empty();
-        col_step(B, false, with_r);
-        RowOpts o;
-        o.B = &B;
-        o.q = p - 1;
-        o.with_r = with_r;
-        o.forward = false;
-        o.sp_add = B.sp_count > 0 ? sp_vals(bwd_vals_, p - 1) : nullptr;
-        o.sp_store = with_r ? sp_vals(fwd_vals_, p - 1) : nullptr;
-        o.dense = born_ ? born_->bwd_plane(p - 1) : nullptr;
-        const bool final_plane = p - 1 == 0;
-        o.e_out = final_plane ? refl : ((born_ && has_host_[p - 2]) ? e_.as<px2<T>>() : nullptr);
-        o.e_out2 = kept(keepb_, p - 1);
-        o.next = !final_plane;
-        row_step(o);
-        ++steps;
-      }
-      if (born_) born_->backward_done();
-    }
-    HIP_CHECK(hipEventRecord(t1, stream_));
-    HIP_CHECK(hipEventSynchronize(t1));
-    float ms = 0;
-    HIP_CHECK(hipEventElapsedTime(&ms, t0, t1));
-    (void)hipEventDestroy(t0);
-    (void)hipEventDestroy(t1);
-    if (time_) {
-      (void)hipEventDestroy(ev0_);
-      (void)hipEventDestroy(ev1_);
-      if (kt) *kt = kt_;
-    }
-    time_ = false;
-    last_steps_ = steps;
-    return ms;
-  }
-
-  static constexpr long long kLdsMax = 160 * 1024;
-
-  Spec sp_;
-  hipStream_t stream_{};
-  long long npix_ = 0;
-  std::vector<c128> media_;
-  std::vector<int> label_medium_;
-  int before_id_ = 0;
-  std::vector<int32_t> before_;
-  std::vector<char> has_host_;
-  std::vector<c128> host_;
-  DevBuf kx_, ky_;
-  Twiddles<T> twx_, twy_;
-  std::vector<Boundary> bnd_;
-  std::unordered_map<long long, long long> list_off_;
-  long long total_sparse_ = 0;
-  DevBuf tsel_, sp_pix_, sp_rid_, rowptr_;
-  std::map<int, OpTable> phase_ops_;
-  std::map<long long, OpTable> matrix_ops_;
-  size_t ops_bytes_ = 0;
-  DevBuf prods_;
-  int max_prod_ = 1;
-  int cols1_ = 1, cols2_ = 1, rows_ = 1, ls_col1_ = 3, ls_col2_ = 3, ls_row_ = 3;
-  DevBuf stage_, field_in_, X_, prod_, e_, fwd_vals_, bwd_vals_, mask_;
-  DevBuf iter_t_, iter_r_, keepf_, keepb_;
-  std::vector<long long> keep_;
-  std::unique_ptr<Born<T, COMP>> born_;
-  bool time_ = false;
-  KernelTimes kt_;
-  hipEvent_t ev0_{}, ev1_{};
-  long long last_steps_ = 0;
-  size_t bytes_ = 0;
-};
 
 std::unique_ptr<EngineBase> make_engine(const Spec& spec) {
   switch (spec.precision) {
-    case 64: return std::make_unique<Engine<double, false>>(spec);
-    case 32: return std::make_unique<Engine<float, true>>(spec);
-    case 31: return std::make_unique<Engine<float, false>>(spec);
+    case 64: return make_engine_f64(spec);
+    case 32: return make_engine_f32(spec, true);
+    case 31: return make_engine_f32(spec, false);
     default: throw Error(kEinval, "precision must be 64, 32 or 31");
   }
+}
+
+double run_batch(std::vector<EngineBase*>& engines, int iterations, int max_lane) {
+  require(max_lane >= 1, "lane size must be >= 1");
+  // group into lanes: equal precision, grid and device; solves with δn run one per lane
+  std::map<std::tuple<int, long long, long long, int>, std::vector<EngineBase*>> groups;
+  std::vector<std::vector<EngineBase*>> lanes;
+  for (EngineBase* e : engines) {
+    if (e->has_born()) {
+      lanes.push_back({e});
+      continue;
+    }
+    auto& g = groups[std::make_tuple(e->precision(), e->ny(), e->nx(), e->device())];
+    g.push_back(e);
+    if (static_cast<int>(g.size()) == max_lane) {
+      lanes.push_back(g);
+      g.clear();
+    }
+  }
+  for (auto& kv : groups)
+    if (!kv.second.empty()) lanes.push_back(kv.second);
+  // one host thread and one stream per lane (the stream of the lane's first engine)
+  const auto t0 = std::chrono::steady_clock::now();
+  std::vector<std::thread> threads;
+  std::vector<std::string> errors(lanes.size());
+  std::vector<int> codes(lanes.size(), kOk);
+  for (size_t i = 0; i < lanes.size(); ++i)
+    threads.emplace_back([&, i] {
+      try {
+        auto& lane = lanes[i];
+        HIP_CHECK(hipSetDevice(lane[0]->device()));
+        std::vector<std::vector<long long>> keeps(lane.size());
+        lane[0]->run_lane(lane, iterations, keeps, lane[0]->stream(), false, nullptr);
+      } catch (const Error& e) {
+        errors[i] = e.what();
+        codes[i] = e.code;
+      } catch (const std::exception& e) {
+        errors[i] = e.what();
+        codes[i] = kEinternal;
+      }
+    });
+  for (auto& t : threads) t.join();
+  for (size_t i = 0; i < lanes.size(); ++i)
+    if (codes[i] != kOk) throw Error(codes[i], errors[i]);
+  const auto t1 = std::chrono::steady_clock::now();
+  return std::chrono::duration<double, std::milli>(t1 - t0).count();
 }
 
 void interface_matrices(int which, const double* kx, const double* ky, long long n, c128 n_in, c128 n_out,
@@ -763,7 +111,7 @@ void interface_matrices(int which, const double* kx, const double* ky, long long
   hipStream_t s;
   HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
   try {
-    DevBuf dkx(n * sizeof(double)), dky(n * sizeof(double)), dout(4 * n * sizeof(zd));
+    Buf dkx(n * sizeof(double)), dky(n * sizeof(double)), dout(4 * n * sizeof(zd));
     HIP_CHECK(hipMemcpyAsync(dkx.as<void>(), kx, n * sizeof(double), hipMemcpyHostToDevice, s));
     HIP_CHECK(hipMemcpyAsync(dky.as<void>(), ky, n * sizeof(double), hipMemcpyHostToDevice, s));
     const unsigned grid = static_cast<unsigned>((n + 255) / 256);
@@ -779,13 +127,44 @@ void interface_matrices(int which, const double* kx, const double* ky, long long
   HIP_CHECK(hipStreamDestroy(s));
 }
 
+__global__ void k_copy_probe(const float4* __restrict__ src, float4* __restrict__ dst, long long n) {
+  for (long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; i < n;
+       i += static_cast<long long>(gridDim.x) * blockDim.x)
+    dst[i] = src[i];
+}
+
+double copy_probe(long long bytes, int reps, int device) {
+  HIP_CHECK(hipSetDevice(device));
+  require(bytes >= 16 && reps >= 1, "copy probe needs bytes >= 16 and reps >= 1");
+  const long long n = bytes / 16;
+  Buf a(n * 16), b(n * 16);
+  hipStream_t s;
+  HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  hipEvent_t e0, e1;
+  HIP_CHECK(hipEventCreate(&e0));
+  HIP_CHECK(hipEventCreate(&e1));
+  const unsigned grid = static_cast<unsigned>(std::min<long long>((n + 255) / 256, 256 * 64));
+  hipLaunchKernelGGL(k_copy_probe, dim3(grid), dim3(256), 0, s, a.as<float4>(), b.as<float4>(), n);
+  HIP_CHECK(hipEventRecord(e0, s));
+  for (int r = 0; r < reps; ++r)
+    hipLaunchKernelGGL(k_copy_probe, dim3(grid), dim3(256), 0, s, a.as<float4>(), b.as<float4>(), n);
+  HIP_CHECK(hipEventRecord(e1, s));
+  HIP_CHECK(hipEventSynchronize(e1));
+  float ms = 0;
+  HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipStreamDestroy(s);
+  return ms / reps;
+}
+
 void longitudinal_wavenumbers(c128 index, double k0, const double* kp2, long long n, c128* out, int device) {
   HIP_CHECK(hipSetDevice(device));
   if (n <= 0) return;
   hipStream_t s;
   HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
   try {
-    DevBuf dk(n * sizeof(double)), dout(n * sizeof(zd));
+    Buf dk(n * sizeof(double)), dout(n * sizeof(zd));
     HIP_CHECK(hipMemcpyAsync(dk.as<void>(), kp2, n * sizeof(double), hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(k_kz_eval, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, s,
                        zd{index.real(), index.imag()}, k0, dk.as<double>(), n, dout.as<zd>());

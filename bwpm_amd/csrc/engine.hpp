@@ -55,14 +55,36 @@ class EngineBase {
   virtual void download(RunOutput& out, int iterations, const std::vector<long long>& keep) = 0;
   virtual hipStream_t stream() const = 0;
   virtual size_t device_bytes() const = 0;
+  virtual int precision() const = 0;
+  virtual long long ny() const = 0;
+  virtual long long nx() const = 0;
+  virtual int device() const = 0;
+  virtual bool has_born() const = 0;
+  // Run the step programs of `members` (all of this engine's type, grid and device; this engine
+  // among them or not) in lockstep on `stream`: one column and one row launch per step serve every
+  // member. Fields must have been uploaded. Returns the device time in ms.
+  virtual double run_lane(std::vector<EngineBase*>& members, int iterations,
+                          const std::vector<std::vector<long long>>& keeps, hipStream_t stream,
+                          bool time_kernels, KernelTimes* kt) = 0;
 };
 
 std::unique_ptr<EngineBase> make_engine(const Spec& spec);
+std::unique_ptr<EngineBase> make_engine_f32(const Spec& spec, bool compensated);
+std::unique_ptr<EngineBase> make_engine_f64(const Spec& spec);
+
+// Batched solve of many engines: members are grouped into lanes of equal (precision, grid,
+// device, no δn); each lane runs in lockstep on its own stream, lanes run concurrently.
+// Results are left on the device (download per engine). Returns the wall time in ms.
+double run_batch(std::vector<EngineBase*>& engines, int iterations, int max_lane);
 
 // FP64 evaluation of the exported interface functions on the device.
 void interface_matrices(int which, const double* kx, const double* ky, long long n, c128 n_in,
                         c128 n_out, double k0, c128* out, int device);
 void longitudinal_wavenumbers(c128 index, double k0, const double* kp2, long long n, c128* out,
                               int device);
+
+// Average duration (ms) of a plain device-to-device copy kernel over `bytes` (HBM calibration of
+// the traffic counters and of the achievable bandwidth).
+double copy_probe(long long bytes, int reps, int device);
 
 }  // namespace bwpm

@@ -1,0 +1,895 @@
+// Device-resident bidirectional BWPM engine (reference semantics: Solver::run,
+// cpp/src/solver.cpp:444-528): per-solve set-up of label pairs, sparse interface lists and
+// operator tables; a static step program per run; a lane runner that executes the programs of
+// many solves of one grid in lockstep, one column and one row launch per step for all of them.
+#pragma once
+
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <unordered_map>
+
+#include "born.hpp"
+#include "engine.hpp"
+#include "kernels.hpp"
+
+namespace bwpm {
+
+namespace detail {
+
+// ---- device memory -------------------------------------------------------------------------
+
+class DevBuf {
+ public:
+  DevBuf() = default;
+  explicit DevBuf(size_t bytes) { alloc(bytes); }
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : p_(o.p_), n_(o.n_) { o.p_ = nullptr; o.n_ = 0; }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    release();
+    p_ = o.p_; n_ = o.n_; o.p_ = nullptr; o.n_ = 0;
+    return *this;
+  }
+  ~DevBuf() { release(); }
+  void alloc(size_t bytes) {
+    release();
+    if (bytes == 0) return;
+    const hipError_t e = hipMalloc(&p_, bytes);
+    if (e != hipSuccess) {
+      p_ = nullptr;
+      throw Error(kEnomem, "device allocation of " + std::to_string(bytes) + " bytes failed: " +
+                               hipGetErrorString(e));
+    }
+    n_ = bytes;
+  }
+  void release() {
+    if (p_) (void)hipFree(p_);
+    p_ = nullptr;
+    n_ = 0;
+  }
+  template <typename U>
+  U* as() const { return static_cast<U*>(p_); }
+  size_t bytes() const { return n_; }
+
+ private:
+  void* p_ = nullptr;
+  size_t n_ = 0;
+};
+
+template <typename U>
+inline DevBuf upload_vec(const std::vector<U>& v, hipStream_t s) {
+  DevBuf b(v.size() * sizeof(U));
+  if (!v.empty()) HIP_CHECK(hipMemcpyAsync(b.as<void>(), v.data(), v.size() * sizeof(U),
+                                           hipMemcpyHostToDevice, s));
+  return b;
+}
+
+// ---- FFT plans ---------------------------------------------------------------------------------
+
+inline std::vector<int> factorize(long long n) {
+  std::vector<int> r;
+  long long m = n;
+  for (int R : {8, 4, 2, 3, 5, 7, 11, 13})
+    while (m % R == 0) {
+      r.push_back(R);
+      m /= R;
+    }
+  if (m != 1)
+    throw Error(kEinval, "grid size " + std::to_string(n) +
+                             " has a prime factor above 13; supported sizes are 2^a·3^b·5^c·7^d·11^e·13^f");
+  // a single radix-2 stage is cheaper after the radix-8 stages; merge 2·2 into 4 if produced
+  if (r.size() > kMaxStages) throw Error(kEinval, "grid size " + std::to_string(n) + " needs too many FFT stages");
+  return r;
+}
+
+template <typename T>
+struct Twiddles {
+  DevBuf hi, lo;
+  FftPlan plan{};
+  void build(long long n, hipStream_t s) {
+    plan.n = static_cast<int>(n);
+    const std::vector<int> r = n > 1 ? factorize(n) : std::vector<int>{};
+    plan.nst = static_cast<int>(r.size());
+    for (size_t i = 0; i < r.size(); ++i) plan.rad[i] = r[i];
+    std::vector<cx<T>> h(n), l(n);
+    for (long long t = 0; t < n; ++t) {
+      // exp(−2πi t/n) from the mathematics in extended precision, then rounded
+      const long double a = -2.0L * 3.141592653589793238462643383279502884L * static_cast<long double>(t) /
+                            static_cast<long double>(n);
+      const double c = static_cast<double>(cosl(a)), sn = static_cast<double>(sinl(a));
+      const T ch = static_cast<T>(c), sh = static_cast<T>(sn);
+      h[t] = cx<T>{ch, sh};
+      l[t] = cx<T>{static_cast<T>(c - static_cast<double>(ch)), static_cast<T>(sn - static_cast<double>(sh))};
+    }
+    hi = upload_vec(h, s);
+    lo = upload_vec(l, s);
+    plan.tw_hi = hi.as<void>();
+    plan.tw_lo = lo.as<void>();
+  }
+};
+
+// ---- per-boundary structure ------------------------------------------------------------------
+
+// Boundary q separates plane q (slab q − 1, or the medium before the volume for q = 0) from
+// plane q + 1 (slab q). Forward step q crosses it left to right, backward step q + 1 right to left.
+struct Boundary {
+  std::vector<std::pair<int32_t, int32_t>> pairs;  // (label left, label right), sorted
+  std::vector<int> mleft, mright;                   // medium ids of each pair
+  std::vector<int> rpair;                           // pair index of each R product
+  long long tsel_off = -1;                          // into the tsel byte array (pairs > 1)
+  long long sp_off = 0, sp_count = 0;               // sparse interface entries
+  long long rowptr_off = -1;                        // ny + 1 offsets (sp_count > 0)
+  int fwd_prod = 0, bwd_prod = 0;                   // offsets into the product descriptor array
+  int source = -1;                                  // boundary whose structure this one reuses
+};
+
+// Operator table of one (kind, medium pair): hi and lo parts.
+struct OpTable {
+  DevBuf hi, lo;
+};
+
+inline int pow2_floor(int v) {
+  int p = 1;
+  while (p * 2 <= v) p *= 2;
+  return p;
+}
+
+// ---- kernel geometry ---------------------------------------------------------------------------
+
+inline bool fast_size(long long n) {
+  return n == 64 || n == 128 || n == 256 || n == 512 || n == 1024 || n == 2048 || n == 4096;
+}
+// columns per workgroup of the register column kernel (512 threads, ~74 KB LDS in FP32)
+template <typename T>
+constexpr int fast_w(int n) {
+  const int w = (sizeof(T) == 4 ? 512 : 256) / (n / 8);
+  return w > 16 ? 16 : (w < 1 ? 1 : w);
+}
+template <typename T>
+constexpr int fast_rw(int n) {
+  const int r = (sizeof(T) == 4 ? 512 : 256) / (n / 8);
+  return r < 1 ? 1 : r;
+}
+
+// Launch geometry of one grid size (shared by every solve of a lane).
+struct Geometry {
+  bool col_fast = false, row_fast = false;
+  int col_w = 1;                  // fast: columns per workgroup (fast_w or 1)
+  int cols1 = 1, cols2 = 1;       // generic: columns per workgroup for one / several products
+  int ls_col1 = 1, ls_col2 = 1;
+  int rw = 1;                     // fast: rows per workgroup (fast_rw or 1)
+  int rows = 1, ls_row = 1;       // generic
+};
+
+}  // namespace detail
+
+using namespace detail;
+
+// A lane step: one column launch and one row launch over every solve of the lane, or a Born
+// sub-step of a single-solve lane.
+enum OpKind : int { kOpPair = 0, kOpBornFwd = 1, kOpBornBwd = 2 };
+
+template <typename T>
+struct Op {
+  int kind = kOpPair;
+  ColDesc<T> col{};
+  RowDesc<T> row{};
+  long long slab = 0;
+  bool bidir = false;
+};
+
+template <typename T, bool COMP>
+class Engine final : public EngineBase {
+ public:
+  explicit Engine(const Spec& s) : sp_(s) {
+    HIP_CHECK(hipSetDevice(s.device));
+    HIP_CHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    try {
+      setup();
+    } catch (...) {
+      (void)hipStreamDestroy(stream_);
+      throw;
+    }
+  }
+  ~Engine() override {
+    (void)hipSetDevice(sp_.device);
+    (void)hipStreamSynchronize(stream_);
+    born_.reset();
+    (void)hipStreamDestroy(stream_);
+  }
+
+  hipStream_t stream() const override { return stream_; }
+  size_t device_bytes() const override { return bytes_; }
+  int precision() const override { return sp_.precision; }
+  long long ny() const override { return sp_.ny; }
+  long long nx() const override { return sp_.nx; }
+  int device() const override { return sp_.device; }
+  bool has_born() const override { return born_ != nullptr; }
+
+  void run(const c128* field, int iterations, const std::vector<long long>& keep, RunOutput& out) override {
+    HIP_CHECK(hipSetDevice(sp_.device));
+    upload(field);
+    std::vector<EngineBase*> one{this};
+    std::vector<std::vector<long long>> keeps{keep};
+    out.gpu_ms = run_lane(one, iterations, keeps, stream_, false, nullptr);
+    download(out, iterations, keep);
+  }
+
+  void upload(const c128* field) override {
+    HIP_CHECK(hipSetDevice(sp_.device));
+    HIP_CHECK(hipMemcpyAsync(stage_.as<void>(), field, npix_ * 2 * sizeof(c128), hipMemcpyHostToDevice, stream_));
+    launch_1d(k_from_c128<T>, npix_, stage_.as<zd>(), field_in_.as<px2<T>>(), npix_);
+    HIP_CHECK(hipStreamSynchronize(stream_));
+  }
+
+  double run_resident(int iterations, bool time_kernels, KernelTimes* kt) override {
+    HIP_CHECK(hipSetDevice(sp_.device));
+    std::vector<EngineBase*> one{this};
+    std::vector<std::vector<long long>> keeps{{}};
+    return run_lane(one, iterations, keeps, stream_, time_kernels, kt);
+  }
+
+  void download(RunOutput& out, int iterations, const std::vector<long long>& keep) override {
+    HIP_CHECK(hipSetDevice(sp_.device));
+    const int nit = std::max(iterations, 1);
+    auto fetch = [&](const px2<T>* src) {
+      std::vector<c128> v(npix_ * 2);
+      launch_1d(k_to_c128<T>, npix_, src, stage_.as<zd>(), npix_);
+      HIP_CHECK(hipMemcpyAsync(v.data(), stage_.as<void>(), npix_ * 2 * sizeof(c128), hipMemcpyDeviceToHost, stream_));
+      HIP_CHECK(hipStreamSynchronize(stream_));
+      return v;
+    };
+    out.transmitted = fetch(iter_t_.as<px2<T>>() + (nit - 1) * npix_);
+    out.transmitted_iter.clear();
+    out.reflected_iter.clear();
+    if (iterations > 0) {
+      out.reflected = fetch(iter_r_.as<px2<T>>() + (nit - 1) * npix_);
+      for (int it = 0; it < nit; ++it) {
+        out.transmitted_iter.push_back(fetch(iter_t_.as<px2<T>>() + it * npix_));
+        out.reflected_iter.push_back(fetch(iter_r_.as<px2<T>>() + it * npix_));
+      }
+    } else {
+      out.reflected.assign(npix_ * 2, c128(0.0, 0.0));
+    }
+    out.forward.clear();
+    out.backward.clear();
+    for (long long p : keep) {
+      const int slot = keep_slot(p);
+      if (slot < 0 || p < 0 || p > sp_.nz) continue;  // planes 0 … nz exist (solver.cpp:474,493)
+      out.forward[p] = fetch(keepf_.as<px2<T>>() + slot * npix_);
+      if (iterations > 0) out.backward[p] = fetch(keepb_.as<px2<T>>() + slot * npix_);
+    }
+    out.stored_values = iterations > 0 ? 2 * total_sparse_ : 0;
+    out.slab_steps = last_steps_;
+    out.gmres_iterations = born_ ? born_->iterations() : 0;
+  }
+
+  // Static step program of one run (reference order, solver.cpp:469-525):
+  //  forward:  e = field; for p: [FwdR → src_bwd[p]] FwdT (+ mask·Born increment, R1 → born_bwd[p+1]);
+  //            e += src_fwd[p+1] + born_fwd[p+1]
+  //  backward: e = born_bwd[nz]; for p = nz…1: [R3 → born_fwd[p]; e += R2] [BwdR → src_fwd[p]] BwdT;
+  //            e += src_bwd[p−1] + born_bwd[p−1]
+  std::vector<Op<T>> program(int iterations, const std::vector<long long>& keep) {
+    const long long nz = sp_.nz;
+    const int nit = std::max(iterations, 1);
+    const bool bidir = iterations > 0;
+    keep_ = keep;
+    const size_t pb = npix_ * sizeof(px2<T>);
+    if (iter_t_.bytes() < nit * pb) iter_t_.alloc(nit * pb);
+    if (bidir && iter_r_.bytes() < nit * pb) iter_r_.alloc(nit * pb);
+    if (!keep_.empty()) {
+      if (keepf_.bytes() < keep_.size() * pb) keepf_.alloc(keep_.size() * pb);
+      if (bidir && keepb_.bytes() < keep_.size() * pb) keepb_.alloc(keep_.size() * pb);
+    }
+    px2<T>* X = X_.as<px2<T>>();
+    px2<T>* E = e_.as<px2<T>>();
+    std::vector<Op<T>> ops;
+    long long steps = 0;
+    auto born_at = [&](long long slab) { return born_ && slab >= 0 && slab < nz && has_host_[slab]; };
+    for (int it = 0; it < nit; ++it) {
+      const bool last = it == nit - 1;
+      auto kept = [&](DevBuf& buf, long long p) -> px2<T>* {
+        if (!last) return nullptr;
+        const int slot = keep_slot(p);
+        return slot < 0 ? nullptr : buf.as<px2<T>>() + slot * npix_;
+      };
+      px2<T>* trans = iter_t_.as<px2<T>>() + it * npix_;
+      {  // forward start: e = field (+ src_fwd[0], never present)
+        Op<T> o;
+        o.row.active = 1;
+        o.row.e_in = field_in_.as<px2<T>>();
+        o.row.e_out2 = kept(keepf_, 0);
+        o.row.e_out = born_at(0) ? E : nullptr;
+        o.row.x_out = X;
+        ops.push_back(o);
+      }
+      for (long long p = 0; p < nz; ++p) {
+        const Boundary& B = bnd_[p];
+        const bool with_r = bidir && !B.rpair.empty();
+        if (born_at(p)) {
+          Op<T> b;
+          b.kind = kOpBornFwd;
+          b.slab = p;
+          b.bidir = bidir;
+          ops.push_back(b);
+        }
+        Op<T> o;
+        o.col = col_desc(B, true, with_r);
+        o.row = row_desc(B, p, with_r);
+        o.row.inc = born_at(p) ? born_->increment() : nullptr;
+        if (B.sp_count > 0) {
+          o.row.sp_add = bidir ? sp_vals(fwd_vals_, p) : nullptr;
+          o.row.sp_store = with_r ? sp_vals(bwd_vals_, p) : nullptr;
+        }
+        o.row.dense = (born_ && it > 0) ? born_->fwd_plane(p + 1) : nullptr;
+        const bool final_plane = p + 1 == nz;
+        o.row.e_out = final_plane ? trans : (born_at(p + 1) ? E : nullptr);
+        o.row.e_out2 = kept(keepf_, p + 1);
+        o.row.x_out = final_plane ? nullptr : X;
+        ops.push_back(o);
+        ++steps;
+      }
+      if (!bidir) break;
+      px2<T>* refl = iter_r_.as<px2<T>>() + it * npix_;
+      {  // backward start: e = 0 (+ src_bwd[nz], never present) + born_bwd[nz]
+        Op<T> o;
+        o.row.active = 1;
+        o.row.dense = born_ ? born_->bwd_plane(nz) : nullptr;
+        o.row.e_out2 = kept(keepb_, nz);
+        o.row.e_out = born_at(nz - 1) ? E : nullptr;
+        o.row.x_out = X;
+        ops.push_back(o);
+      }
+      for (long long p = nz; p >= 1; --p) {
+        const Boundary& B = bnd_[p - 1];
+        if (born_at(p - 1)) {
+          Op<T> b;  // R3 from e at plane p into born_fwd[p], then e += R2 (no mask) and the row FFT
+          b.kind = kOpBornBwd;
+          b.slab = p - 1;
+          b.bidir = true;
+          ops.push_back(b);
+          Op<T> o;
+          o.row.active = 1;
+          o.row.e_in = E;
+          o.row.inc = born_->increment();
+          o.row.x_out = X;
+          ops.push_back(o);
+        }
+        const bool with_r = !B.rpair.empty();
+        Op<T> o;
+        o.col = col_desc(B, false, with_r);
+        o.row = row_desc(B, p - 1, with_r);
+        if (B.sp_count > 0) {
+          o.row.sp_add = sp_vals(bwd_vals_, p - 1);
+          o.row.sp_store = with_r ? sp_vals(fwd_vals_, p - 1) : nullptr;
+        }
+        o.row.dense = born_ ? born_->bwd_plane(p - 1) : nullptr;
+        const bool final_plane = p - 1 == 0;
+        o.row.e_out = final_plane ? refl : (born_at(p - 2) ? E : nullptr);
+        o.row.e_out2 = kept(keepb_, p - 1);
+        o.row.x_out = final_plane ? nullptr : X;
+        ops.push_back(o);
+        ++steps;
+      }
+    }
+    last_steps_ = steps;
+    return ops;
+  }
+
+  // Run the programs of every member (same grid and precision) in lockstep on `stream`.
+  double run_lane(std::vector<EngineBase*>& members, int iterations, const std::vector<std::vector<long long>>& keeps,
+                  hipStream_t stream, bool time_kernels, KernelTimes* kt) override {
+    const int S = static_cast<int>(members.size());
+    std::vector<Engine*> es;
+    for (EngineBase* b : members) {
+      auto* e = dynamic_cast<Engine*>(b);
+      require(e && e->sp_.ny == sp_.ny && e->sp_.nx == sp_.nx && e->sp_.device == sp_.device,
+              "a batch lane needs solves of equal grid, precision and device");
+      es.push_back(e);
+    }
+    std::vector<std::vector<Op<T>>> progs(S);
+    size_t G = 0;
+    for (int i = 0; i < S; ++i) {
+      progs[i] = es[i]->program(iterations, keeps[i]);
+      G = std::max(G, progs[i].size());
+      if (S > 1)
+        for (const Op<T>& o : progs[i])
+          require(o.kind == kOpPair, "solves with index fluctuations run one per lane");
+    }
+    // every member's buffers (tables, kept planes) must be ready before the lane starts
+    for (Engine* e : es)
+      if (e->stream_ != stream) HIP_CHECK(hipStreamSynchronize(e->stream_));
+    std::vector<ColDesc<T>> hcol(G * S);
+    std::vector<RowDesc<T>> hrow(G * S);
+    std::vector<char> has_col(G, 0), has_row(G, 0);
+    for (size_t g = 0; g < G; ++g)
+      for (int i = 0; i < S; ++i) {
+        if (g >= progs[i].size() || progs[i][g].kind != kOpPair) continue;
+        hcol[g * S + i] = progs[i][g].col;
+        hrow[g * S + i] = progs[i][g].row;
+        has_col[g] |= progs[i][g].col.in != nullptr;
+        has_row[g] |= progs[i][g].row.active != 0;
+      }
+    if (lane_col_.bytes() < hcol.size() * sizeof(ColDesc<T>)) lane_col_.alloc(hcol.size() * sizeof(ColDesc<T>));
+    if (lane_row_.bytes() < hrow.size() * sizeof(RowDesc<T>)) lane_row_.alloc(hrow.size() * sizeof(RowDesc<T>));
+    HIP_CHECK(hipMemcpyAsync(lane_col_.as<void>(), hcol.data(), hcol.size() * sizeof(ColDesc<T>),
+                             hipMemcpyHostToDevice, stream));
+    HIP_CHECK(hipMemcpyAsync(lane_row_.as<void>(), hrow.data(), hrow.size() * sizeof(RowDesc<T>),
+                             hipMemcpyHostToDevice, stream));
+    if (total_sparse_ > 0 || S > 1)
+      for (Engine* e : es)
+        if (e->total_sparse_ > 0)
+          launch_1d_on(stream, k_fill_zero<T>, e->total_sparse_, e->fwd_vals_.as<px2<T>>(), e->total_sparse_);
+
+    KernelTimes times;
+    hipEvent_t t0, t1, k0 = nullptr, k1 = nullptr;
+    HIP_CHECK(hipEventCreate(&t0));
+    HIP_CHECK(hipEventCreate(&t1));
+    if (time_kernels) {
+      HIP_CHECK(hipEventCreate(&k0));
+      HIP_CHECK(hipEventCreate(&k1));
+    }
+    auto timed = [&](bool col, auto&& launch) {
+      if (!time_kernels) {
+        launch();
+        HIP_CHECK(hipGetLastError());
+        return;
+      }
+      HIP_CHECK(hipEventRecord(k0, stream));
+      launch();
+      HIP_CHECK(hipGetLastError());
+      HIP_CHECK(hipEventRecord(k1, stream));
+      HIP_CHECK(hipEventSynchronize(k1));
+      float ms = 0;
+      HIP_CHECK(hipEventElapsedTime(&ms, k0, k1));
+      (col ? times.col_ms : times.row_ms) += ms;
+      ++(col ? times.col_launches : times.row_launches);
+    };
+    HIP_CHECK(hipEventRecord(t0, stream));
+    for (size_t g = 0; g < G; ++g) {
+      if (S == 1 && progs[0][g].kind != kOpPair) {
+        const Op<T>& o = progs[0][g];
+        if (o.kind == kOpBornFwd)
+          born_->forward(e_.template as<px2<T>>(), o.slab, o.bidir);
+        else
+          born_->backward(e_.template as<px2<T>>(), o.slab);
+        continue;
+      }
+      const ColDesc<T>* dc = lane_col_.as<ColDesc<T>>() + g * S;
+      const RowDesc<T>* dr = lane_row_.as<RowDesc<T>>() + g * S;
+      if (has_col[g]) timed(true, [&] { launch_col(dc, S, stream); });
+      if (has_row[g]) timed(false, [&] { launch_row(dr, S, stream); });
+    }
+    HIP_CHECK(hipEventRecord(t1, stream));
+    HIP_CHECK(hipEventSynchronize(t1));
+    float ms = 0;
+    HIP_CHECK(hipEventElapsedTime(&ms, t0, t1));
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    if (time_kernels) {
+      (void)hipEventDestroy(k0);
+      (void)hipEventDestroy(k1);
+    }
+    if (kt) *kt = times;
+    return ms;
+  }
+
+ private:
+  template <typename K, typename... A>
+  void launch_1d(K kern, long long n, A... args) {
+    launch_1d_on(stream_, kern, n, args...);
+  }
+  template <typename K, typename... A>
+  static void launch_1d_on(hipStream_t s, K kern, long long n, A... args) {
+    if (n <= 0) return;
+    const int bs = 256;
+    const long long grid = (n + bs - 1) / bs;
+    hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>(grid)), dim3(bs), 0, s, args...);
+    HIP_CHECK(hipGetLastError());
+  }
+
+  // ---- launches over a lane of S solves
+  template <int N>
+  void col_fast_n(const ColDesc<T>* d, int S, hipStream_t s) {
+    constexpr int W = fast_w<T>(N);
+    const long long plane = npix_;
+    if (geo_.col_w == W) {
+      const dim3 grid(static_cast<unsigned>(sp_.nx / W), S);
+      hipLaunchKernelGGL((k_col_fast<N, W, T, COMP>), grid, dim3(N / 8 * W), 0, s, d, static_cast<int>(sp_.nx),
+                         plane, twy_.hi.template as<cx<T>>(), twy_.lo.template as<cx<T>>());
+    } else {
+      const dim3 grid(static_cast<unsigned>(sp_.nx), S);
+      hipLaunchKernelGGL((k_col_fast<N, 1, T, COMP>), grid, dim3(N / 8), 0, s, d, static_cast<int>(sp_.nx), plane,
+                         twy_.hi.template as<cx<T>>(), twy_.lo.template as<cx<T>>());
+    }
+  }
+  template <int N>
+  void row_fast_n(const RowDesc<T>* d, int S, hipStream_t s) {
+    constexpr int RW = fast_rw<T>(N);
+    const long long plane = npix_;
+    if (geo_.rw == RW) {
+      const dim3 grid(static_cast<unsigned>(sp_.ny / RW), S);
+      hipLaunchKernelGGL((k_row_fast<N, RW, T, COMP>), grid, dim3(N / 8 * RW), 0, s, d, plane,
+                         twx_.hi.template as<cx<T>>(), twx_.lo.template as<cx<T>>());
+    } else {
+      const dim3 grid(static_cast<unsigned>(sp_.ny), S);
+      hipLaunchKernelGGL((k_row_fast<N, 1, T, COMP>), grid, dim3(N / 8), 0, s, d, plane,
+                         twx_.hi.template as<cx<T>>(), twx_.lo.template as<cx<T>>());
+    }
+  }
+
+  void launch_col(const ColDesc<T>* d, int S, hipStream_t s) {
+    if (geo_.col_fast && COMP == (sizeof(T) == 4)) {
+      switch (sp_.ny) {
+        case 64: col_fast_n<64>(d, S, s); return;
+        case 128: col_fast_n<128>(d, S, s); return;
+        case 256: col_fast_n<256>(d, S, s); return;
+        case 512: col_fast_n<512>(d, S, s); return;
+        case 1024: col_fast_n<1024>(d, S, s); return;
+        case 2048: col_fast_n<2048>(d, S, s); return;
+        default: col_fast_n<4096>(d, S, s); return;
+      }
+    }
+    // generic: the column block must fit every product count of the lane; use the two-buffer width
+    ColGeom g{static_cast<int>(sp_.ny), static_cast<int>(sp_.nx), geo_.cols2, geo_.ls_col2, npix_, twy_.plan};
+    const size_t lds = 2 * static_cast<size_t>(sp_.ny) * g.ls * sizeof(cx<T>);
+    const dim3 grid(static_cast<unsigned>((sp_.nx + g.cols - 1) / g.cols), S);
+    hipLaunchKernelGGL((k_col_step<T, COMP>), grid, dim3(kThreads), lds, s, d, g);
+  }
+
+  void launch_row(const RowDesc<T>* d, int S, hipStream_t s) {
+    if (geo_.row_fast && COMP == (sizeof(T) == 4)) {
+      switch (sp_.nx) {
+        case 64: row_fast_n<64>(d, S, s); return;
+        case 128: row_fast_n<128>(d, S, s); return;
+        case 256: row_fast_n<256>(d, S, s); return;
+        case 512: row_fast_n<512>(d, S, s); return;
+        case 1024: row_fast_n<1024>(d, S, s); return;
+        case 2048: row_fast_n<2048>(d, S, s); return;
+        default: row_fast_n<4096>(d, S, s); return;
+      }
+    }
+    RowGeom g{static_cast<int>(sp_.ny), static_cast<int>(sp_.nx), geo_.rows, geo_.ls_row, npix_, twx_.plan};
+    const size_t lds = 2 * static_cast<size_t>(sp_.nx) * g.ls * sizeof(cx<T>);
+    const dim3 grid(static_cast<unsigned>((sp_.ny + g.rows - 1) / g.rows), S);
+    hipLaunchKernelGGL((k_row_step<T, COMP>), grid, dim3(kThreads), lds, s, d, g);
+  }
+
+  ColDesc<T> col_desc(const Boundary& B, bool forward, bool with_r) {
+    ColDesc<T> c{};
+    c.in = X_.as<px2<T>>();
+    c.out = prod_.as<px2<T>>();
+    c.prods = prods_.as<Prod<T>>() + (forward ? B.fwd_prod : B.bwd_prod);
+    c.nprod = static_cast<int>(B.pairs.size()) + (with_r ? static_cast<int>(B.rpair.size()) : 0);
+    return c;
+  }
+
+  RowDesc<T> row_desc(const Boundary& B, long long q, bool with_r) {
+    RowDesc<T> r{};
+    r.active = 1;
+    r.nt = static_cast<int>(B.pairs.size());
+    r.nr = with_r ? static_cast<int>(B.rpair.size()) : 0;
+    r.prod = prod_.as<px2<T>>();
+    r.tsel = B.tsel_off >= 0 ? tsel_.as<uint8_t>() + B.tsel_off : nullptr;
+    r.mask = mask_.as<T>();
+    if (B.sp_count > 0) {
+      const long long list = list_of(q);
+      r.rowptr = rowptr_.as<int>() + B.rowptr_off;
+      r.sp_pix = sp_pix_.as<int>() + list;
+      r.sp_rid = sp_rid_.as<uint8_t>() + list;
+    }
+    return r;
+  }
+
+  px2<T>* sp_vals(DevBuf& b, long long q) { return b.as<px2<T>>() + bnd_[q].sp_off; }
+
+  int keep_slot(long long p) const {
+    for (size_t i = 0; i < keep_.size(); ++i)
+      if (keep_[i] == p) return static_cast<int>(i);
+    return -1;
+  }
+
+  int medium_of_label(int32_t l) const { return l < 0 ? before_id_ : label_medium_[l]; }
+  const int32_t* plane_labels(long long p) const {
+    return p == 0 ? before_.data() : sp_.labels.data() + (p - 1) * npix_;
+  }
+
+  void validate() {
+    const Spec& s = sp_;
+    require(s.ny >= 1 && s.nx >= 1, "grid must have ny, nx >= 1");
+    require(s.nz >= 1, "labels must have shape (nz, ny, nx) with nz >= 1");
+    require(static_cast<long long>(s.labels.size()) == s.nz * s.ny * s.nx, "labels must have shape (nz, ny, nx)");
+    require(s.mask.empty() || static_cast<long long>(s.mask.size()) == s.ny * s.nx, "mask must have shape (ny, nx)");
+    for (int32_t l : s.labels)
+      require(l >= 0 && static_cast<size_t>(l) < s.indices.size(), "labels must index into `indices`");
+    require(s.fluct.empty() || static_cast<long long>(s.fluct.size()) == s.nz * s.ny * s.nx,
+            "fluctuation must have the shape of labels (nz, ny, nx)");
+  }
+
+  void setup() {
+    validate();
+    const Spec& s = sp_;
+    npix_ = s.ny * s.nx;
+    std::vector<c128> media;  // distinct complex index values
+    auto id_of = [&](c128 v) {
+      for (size_t i = 0; i < media.size(); ++i)
+        if (media[i] == v) return static_cast<int>(i);
+      media.push_back(v);
+      return static_cast<int>(media.size() - 1);
+    };
+    before_id_ = id_of(s.index_before);
+    for (const c128& v : s.indices) label_medium_.push_back(id_of(v));
+    media_ = media;
+    before_.assign(npix_, -1);
+
+    // Born hosts per slab (validation as the reference constructor, solver.cpp:47-62)
+    has_host_.assign(s.nz, 0);
+    host_.assign(s.nz, c128(0.0, 0.0));
+    if (!s.fluct.empty()) {
+      for (long long j = 0; j < s.nz; ++j)
+        for (long long q = 0; q < npix_; ++q) {
+          const c128 d = s.fluct[j * npix_ + q];
+          if (d == c128(0.0, 0.0)) continue;
+          const c128 idx = s.indices[s.labels[j * npix_ + q]];
+          if (has_host_[j] && host_[j] != idx)
+            throw Error(kEinval, "slab " + std::to_string(j) + ": fluctuation must lie in one host index");
+          has_host_[j] = 1;
+          host_[j] = idx;
+        }
+    }
+
+    // k tables (host, rounded like 2π·numpy.fft.fftfreq)
+    std::vector<double> kx(s.nx), ky(s.ny);
+    for (long long i = 0; i < s.nx; ++i) kx[i] = fftfreq(i, s.nx, s.dx);
+    for (long long j = 0; j < s.ny; ++j) ky[j] = fftfreq(j, s.ny, s.dy);
+    kx_ = upload_vec(kx, stream_);
+    ky_ = upload_vec(ky, stream_);
+    twx_.build(s.nx, stream_);
+    twy_.build(s.ny, stream_);
+
+    build_boundaries();
+    build_products();
+    choose_geometry();
+
+    const size_t pb = npix_ * sizeof(px2<T>);
+    stage_.alloc(npix_ * 2 * sizeof(c128));
+    field_in_.alloc(pb);
+    X_.alloc(pb);
+    prod_.alloc(pb * std::max(1, max_prod_));
+    e_.alloc(pb);
+    const size_t sp_bytes = std::max<long long>(1, total_sparse_) * sizeof(px2<T>);
+    fwd_vals_.alloc(sp_bytes);
+    bwd_vals_.alloc(sp_bytes);
+    if (!s.mask.empty()) {
+      std::vector<T> m(npix_);
+      for (long long q = 0; q < npix_; ++q) m[q] = static_cast<T>(s.mask[q]);
+      mask_ = upload_vec(m, stream_);
+    }
+    if (std::any_of(has_host_.begin(), has_host_.end(), [](char c) { return c != 0; }))
+      born_ = std::make_unique<Born<T, COMP>>(s, has_host_, host_, kx_.as<double>(), ky_.as<double>(), stream_);
+    HIP_CHECK(hipStreamSynchronize(stream_));
+    bytes_ = stage_.bytes() + field_in_.bytes() + X_.bytes() + prod_.bytes() + e_.bytes() + fwd_vals_.bytes() +
+             bwd_vals_.bytes() + mask_.bytes() + ops_bytes_ + tsel_.bytes() + sp_pix_.bytes() + sp_rid_.bytes() +
+             rowptr_.bytes() + (born_ ? born_->device_bytes() : 0);
+  }
+
+  void choose_geometry() {
+    const Spec& s = sp_;
+    Geometry& g = geo_;
+    const bool fast_prec = COMP == (sizeof(T) == 4);  // register kernels: FP32 compensated and FP64
+    g.col_fast = fast_prec && fast_size(s.ny);
+    g.row_fast = fast_prec && fast_size(s.nx);
+    if (g.col_fast) {
+      const int w = fast_w<T>(static_cast<int>(s.ny));
+      g.col_w = (s.nx % w == 0) ? w : 1;
+    }
+    if (g.row_fast) {
+      const int r = fast_rw<T>(static_cast<int>(s.nx));
+      g.rw = (s.ny % r == 0) ? r : 1;
+    }
+    const int elems = Budget<T>::elems;
+    auto ls_of = [](int lines) { return lines <= 2 ? lines : (lines | 1); };
+    if (!g.col_fast) {
+      int w = 16;
+      while (w > 1 && (s.ny * ls_of(2 * w) > elems || 2 * s.ny * ls_of(2 * w) * (long long)sizeof(cx<T>) > kLdsMax))
+        w /= 2;
+      w = std::min<long long>(w, pow2_floor(static_cast<int>(std::max<long long>(1, s.nx))));
+      require(s.ny * ls_of(2 * w) <= elems && 2 * s.ny * ls_of(2 * w) * (long long)sizeof(cx<T>) <= kLdsMax,
+              "grid size ny = " + std::to_string(s.ny) + " exceeds the column kernel's LDS budget");
+      g.cols2 = w;
+      g.ls_col2 = ls_of(2 * w);
+      HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_col_step<T, COMP>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, kLdsMax));
+    }
+    if (!g.row_fast) {
+      int rw = static_cast<int>(std::min<long long>(s.ny, 256));
+      while (rw > 1 && (s.nx * ls_of(2 * rw) > elems || 2 * s.nx * ls_of(2 * rw) * (long long)sizeof(cx<T>) > kLdsMax))
+        rw /= 2;
+      require(s.nx * ls_of(2 * rw) <= elems && 2 * s.nx * ls_of(2 * rw) * (long long)sizeof(cx<T>) <= kLdsMax,
+              "grid size nx = " + std::to_string(s.nx) + " exceeds the row kernel's LDS budget");
+      g.rows = rw;
+      g.ls_row = ls_of(2 * rw);
+      HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_row_step<T, COMP>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, kLdsMax));
+    }
+  }
+
+  // Label pairs, tsel maps and sparse interface lists per boundary.
+  void build_boundaries() {
+    const Spec& s = sp_;
+    const long long nz = s.nz;
+    bnd_.resize(nz);
+    std::vector<uint8_t> tsel;
+    std::vector<int32_t> sp_pix, rowptr;
+    std::vector<uint8_t> sp_rid;
+    total_sparse_ = 0;
+    for (long long q = 0; q < nz; ++q) {
+      const int32_t* a = plane_labels(q);
+      const int32_t* b = plane_labels(q + 1);
+      Boundary& B = bnd_[q];
+      // reuse the previous boundary's structure when both label planes repeat
+      if (q >= 1 && std::memcmp(a, plane_labels(q - 1), npix_ * sizeof(int32_t)) == 0 &&
+          std::memcmp(b, plane_labels(q), npix_ * sizeof(int32_t)) == 0) {
+        const int src = bnd_[q - 1].source >= 0 ? bnd_[q - 1].source : static_cast<int>(q - 1);
+        B = bnd_[src];
+        B.source = src;
+        // sparse values are per boundary: give this boundary its own value slots
+        B.sp_off = total_sparse_;
+        total_sparse_ += B.sp_count;
+        continue;
+      }
+      // distinct pairs (sorted like the reference's std::set)
+      std::map<std::pair<int32_t, int32_t>, int> pid;
+      bool uniform = true;
+      for (long long i = 1; i < npix_ && uniform; ++i) uniform = a[i] == a[0] && b[i] == b[0];
+      if (uniform) {
+        pid[{a[0], b[0]}] = 0;
+      } else {
+        for (long long i = 0; i < npix_; ++i) pid.emplace(std::make_pair(a[i], b[i]), 0);
+      }
+      require(pid.size() <= 255, "boundary " + std::to_string(q) + " has more than 255 label pairs");
+      int k = 0;
+      for (auto& kv : pid) {
+        kv.second = k++;
+        B.pairs.push_back(kv.first);
+        B.mleft.push_back(medium_of_label(kv.first.first));
+        B.mright.push_back(medium_of_label(kv.first.second));
+      }
+      std::vector<int> rid_of_pair(B.pairs.size(), -1);
+      for (size_t i = 0; i < B.pairs.size(); ++i)
+        if (B.mleft[i] != B.mright[i]) {
+          rid_of_pair[i] = static_cast<int>(B.rpair.size());
+          B.rpair.push_back(static_cast<int>(i));
+        }
+      if (B.pairs.size() > 1) {
+        B.tsel_off = static_cast<long long>(tsel.size());
+        tsel.resize(tsel.size() + npix_);
+        for (long long i = 0; i < npix_; ++i) tsel[B.tsel_off + i] = static_cast<uint8_t>(pid[{a[i], b[i]}]);
+      }
+      if (!B.rpair.empty()) {
+        B.sp_off = total_sparse_;
+        B.rowptr_off = static_cast<long long>(rowptr.size());
+        long long cnt = 0;
+        for (long long y = 0; y < s.ny; ++y) {
+          rowptr.push_back(static_cast<int32_t>(cnt));
+          for (long long x = 0; x < s.nx; ++x) {
+            const long long i = y * s.nx + x;
+            const int pi = B.pairs.size() > 1 ? tsel[B.tsel_off + i] : 0;
+            if (rid_of_pair[pi] >= 0) {
+              sp_pix.push_back(static_cast<int32_t>(i));
+              sp_rid.push_back(static_cast<uint8_t>(rid_of_pair[pi]));
+              ++cnt;
+            }
+          }
+        }
+        rowptr.push_back(static_cast<int32_t>(cnt));
+        B.sp_count = cnt;
+        // the list entries of this structure start at this offset of sp_pix / sp_rid
+        list_off_[q] = static_cast<long long>(sp_pix.size()) - cnt;
+        total_sparse_ += cnt;
+      }
+    }
+    tsel_ = upload_vec(tsel, stream_);
+    sp_pix_ = upload_vec(sp_pix, stream_);
+    sp_rid_ = upload_vec(sp_rid, stream_);
+    rowptr_ = upload_vec(rowptr, stream_);
+  }
+
+  // Operator tables and per-step product descriptor lists.
+  void build_products() {
+    const Spec& s = sp_;
+    std::vector<Prod<T>> prods;
+    max_prod_ = 1;
+    for (long long q = 0; q < s.nz; ++q) {
+      Boundary& B = bnd_[q];
+      // forward step q: T kinds (FwdT) for every pair, then R kinds (FwdR)
+      B.fwd_prod = static_cast<int>(prods.size());
+      for (size_t i = 0; i < B.pairs.size(); ++i) prods.push_back(t_prod(kFwdT, B.mleft[i], B.mright[i]));
+      for (int pi : B.rpair) prods.push_back(matrix_prod(kFwdR, B.mleft[pi], B.mright[pi]));
+      // backward step q + 1: r = medium on the right (plane q + 1), s = medium on the left
+      B.bwd_prod = static_cast<int>(prods.size());
+      for (size_t i = 0; i < B.pairs.size(); ++i) prods.push_back(t_prod(kBwdT, B.mright[i], B.mleft[i]));
+      for (int pi : B.rpair) prods.push_back(matrix_prod(kBwdR, B.mright[pi], B.mleft[pi]));
+      max_prod_ = std::max<int>(max_prod_, static_cast<int>(B.pairs.size() + B.rpair.size()));
+    }
+    prods_ = upload_vec(prods, stream_);
+  }
+
+  Prod<T> t_prod(int kind, int mr, int ms) {
+    if (mr == ms) return phase_prod(mr);
+    return matrix_prod(kind, mr, ms);
+  }
+
+  Prod<T> phase_prod(int m) {
+    auto it = phase_ops_.find(m);
+    if (it == phase_ops_.end()) {
+      OpTable t;
+      t.hi.alloc(npix_ * sizeof(cx<T>));
+      t.lo.alloc(npix_ * sizeof(cx<T>));
+      const c128 n = media_[m];
+      launch_1d(k_build_phase<T>, npix_, kx_.as<double>(), ky_.as<double>(), static_cast<int>(sp_.ny),
+                static_cast<int>(sp_.nx), zd{n.real(), n.imag()}, sp_.k0, sp_.dz, 1.0 / static_cast<double>(npix_),
+                t.hi.as<cx<T>>(), t.lo.as<cx<T>>());
+      ops_bytes_ += 2 * npix_ * sizeof(cx<T>);
+      it = phase_ops_.emplace(m, std::move(t)).first;
+    }
+    return Prod<T>{0, it->second.hi.as<cx<T>>(), it->second.lo.as<cx<T>>()};
+  }
+
+  Prod<T> matrix_prod(int kind, int mr, int ms) {
+    const long long key = (static_cast<long long>(kind) << 40) | (static_cast<long long>(mr) << 20) | ms;
+    auto it = matrix_ops_.find(key);
+    if (it == matrix_ops_.end()) {
+      OpTable t;
+      t.hi.alloc(3 * npix_ * sizeof(cx<T>));
+      t.lo.alloc(3 * npix_ * sizeof(cx<T>));
+      const c128 nr = media_[mr], ns = media_[ms];
+      launch_1d(k_build_matrix<T>, npix_, kx_.as<double>(), ky_.as<double>(), static_cast<int>(sp_.ny),
+                static_cast<int>(sp_.nx), kind, zd{nr.real(), nr.imag()}, zd{ns.real(), ns.imag()}, sp_.k0, sp_.dz,
+                1.0 / static_cast<double>(npix_), t.hi.as<cx<T>>(), t.lo.as<cx<T>>());
+      ops_bytes_ += 6 * npix_ * sizeof(cx<T>);
+      it = matrix_ops_.emplace(key, std::move(t)).first;
+    }
+    return Prod<T>{1, it->second.hi.as<cx<T>>(), it->second.lo.as<cx<T>>()};
+  }
+
+
+  long long list_of(long long q) const {
+    const Boundary& B = bnd_[q];
+    return list_off_.at(B.source >= 0 ? B.source : q);
+  }
+
+  static constexpr long long kLdsMax = 160 * 1024;
+
+  Spec sp_;
+  hipStream_t stream_{};
+  long long npix_ = 0;
+  std::vector<c128> media_;
+  std::vector<int> label_medium_;
+  int before_id_ = 0;
+  std::vector<int32_t> before_;
+  std::vector<char> has_host_;
+  std::vector<c128> host_;
+  DevBuf kx_, ky_;
+  Twiddles<T> twx_, twy_;
+  Geometry geo_;
+  std::vector<Boundary> bnd_;
+  std::unordered_map<long long, long long> list_off_;
+  long long total_sparse_ = 0;
+  DevBuf tsel_, sp_pix_, sp_rid_, rowptr_;
+  std::map<int, OpTable> phase_ops_;
+  std::map<long long, OpTable> matrix_ops_;
+  size_t ops_bytes_ = 0;
+  DevBuf prods_;
+  int max_prod_ = 1;
+  DevBuf stage_, field_in_, X_, prod_, e_, fwd_vals_, bwd_vals_, mask_;
+  DevBuf iter_t_, iter_r_, keepf_, keepb_;
+  DevBuf lane_col_, lane_row_;
+  std::vector<long long> keep_;
+  std::unique_ptr<Born<T, COMP>> born_;
+  long long last_steps_ = 0;
+  size_t bytes_ = 0;
+};
+
+}  // namespace bwpm

@@ -1,0 +1,112 @@
+// Register-resident radix-8 Stockham FFT of power-of-two lines (16 … 4096 points) for the hot
+// slab-step kernels.
+//
+// A line of N points is owned by T = N/8 threads. Thread t holds the eight pixel vectors
+// a[t + T·m] (m = 0 … 7, both field components), which is exactly the read pattern of every
+// Stockham pass of this plan: a pass of radix R lets each thread run 8/R butterflies
+// j = t + T·b on v[b + r·8/R], multiplies by W_N^{r·(j mod Ns)·N/(Ns·R)}, applies the size-R DFT
+// and writes y_r to LDS at (j − k)·R + k + r·Ns; the next pass reads a[t + T·m] back. The first
+// and the last pass are radix 8, so the input is taken in natural order from registers and the
+// last pass leaves A[t + T·r] in v[r] (natural order) without touching LDS: P − 1 LDS exchanges.
+//
+// LDS index i is padded to i + i/8 to spread the stride-8 writes of the first pass over banks.
+#pragma once
+
+#include "fft.hpp"
+
+namespace bwpm {
+
+template <int N>
+struct RegPlan;
+template <> struct RegPlan<16> { static constexpr int P = 2; static constexpr int rad[P] = {8, 2}; };
+template <> struct RegPlan<32> { static constexpr int P = 2; static constexpr int rad[P] = {8, 4}; };
+template <> struct RegPlan<64> { static constexpr int P = 2; static constexpr int rad[P] = {8, 8}; };
+template <> struct RegPlan<128> { static constexpr int P = 3; static constexpr int rad[P] = {8, 2, 8}; };
+template <> struct RegPlan<256> { static constexpr int P = 3; static constexpr int rad[P] = {8, 4, 8}; };
+template <> struct RegPlan<512> { static constexpr int P = 3; static constexpr int rad[P] = {8, 8, 8}; };
+template <> struct RegPlan<1024> { static constexpr int P = 4; static constexpr int rad[P] = {8, 2, 8, 8}; };
+template <> struct RegPlan<2048> { static constexpr int P = 4; static constexpr int rad[P] = {8, 4, 8, 8}; };
+template <> struct RegPlan<4096> { static constexpr int P = 4; static constexpr int rad[P] = {8, 8, 8, 8}; };
+
+// Note: the size-16 and size-32 plans end with a radix below 8; for them the last pass writes
+// through LDS like the others (see reg_pass).
+
+template <int N>
+constexpr int reg_lds_len() {
+  return N + N / 8;
+}
+__device__ __forceinline__ int lpad(int i) { return i + (i >> 3); }
+
+template <int R, bool COMP, bool INV, typename T>
+__device__ __forceinline__ void dft_r(cx<T>* x) {
+  if constexpr (R == 2) dft2<COMP, INV>(x);
+  else if constexpr (R == 4) dft4<COMP, INV>(x);
+  else dft8<COMP, INV>(x);
+}
+
+template <int N, int PASS, int NS, bool COMP, bool INV, typename T>
+__device__ __forceinline__ void reg_pass(px2<T> (&v)[8], px2<T>* lds, int t, const cx<T>* __restrict__ twh,
+                                         const cx<T>* __restrict__ twl) {
+  using PL = RegPlan<N>;
+  if constexpr (PASS < PL::P) {
+    constexpr int R = PL::rad[PASS];
+    constexpr int B = 8 / R;
+    constexpr int TT = N / 8;
+    constexpr bool last = PASS == PL::P - 1;
+    constexpr bool in_regs = last && R == 8;  // natural-order result stays in registers
+    px2<T> out[8];
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+      const int j = t + TT * b;
+      const int k = j & (NS - 1);
+      cx<T> xa[R], xb[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        xa[r] = v[b + r * B].a;
+        xb[r] = v[b + r * B].b;
+      }
+      if constexpr (NS > 1) {
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          const int e = r * k * (N / (NS * R));
+          cx<T> wh = twh[e], wl = twl[e];
+          if constexpr (INV) {
+            wh = conj(wh);
+            wl = conj(wl);
+          }
+          xa[r] = cmul2<COMP>(xa[r], wh, wl);
+          xb[r] = cmul2<COMP>(xb[r], wh, wl);
+        }
+      }
+      dft_r<R, COMP, INV>(xa);
+      dft_r<R, COMP, INV>(xb);
+      if constexpr (in_regs) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) out[b + r * B] = px2<T>{xa[r], xb[r]};
+      } else {
+        const int base = (j - k) * R + k;
+#pragma unroll
+        for (int r = 0; r < R; ++r) lds[lpad(base + r * NS)] = px2<T>{xa[r], xb[r]};
+      }
+    }
+    if constexpr (in_regs) {
+#pragma unroll
+      for (int m = 0; m < 8; ++m) v[m] = out[m];
+    } else {
+      __syncthreads();
+#pragma unroll
+      for (int m = 0; m < 8; ++m) v[m] = lds[lpad(t + TT * m)];
+      __syncthreads();
+    }
+    reg_pass<N, PASS + 1, NS * R, COMP, INV>(v, lds, t, twh, twl);
+  }
+}
+
+// Transform one line held as v[m] = a[t + T·m]; on return v[m] = A[t + T·m]. All threads of the
+// workgroup must call it together (it synchronises the workgroup between passes).
+template <int N, bool COMP, bool INV, typename T>
+__device__ __forceinline__ void reg_fft(px2<T> (&v)[8], px2<T>* lds, int t, const cx<T>* twh, const cx<T>* twl) {
+  reg_pass<N, 0, 1, COMP, INV>(v, lds, t, twh, twl);
+}
+
+}  // namespace bwpm

@@ -5,17 +5,21 @@
 // cpp/src/solver.cpp:384-417) is split over two kernels so that each moves the (E_x, E_y) plane
 // through HBM once:
 //
-//   k_col_step  X (rows already FFT'd along x)  → column FFT → for each product o (one per label
-//               pair, T kinds first, then R kinds): M_o·Ŝ or phase_o·Ŝ → column IFFT → plane P_o
-//   k_row_step  for each T product: row IFFT of P_o, keep the pixels of its pair; mask; Born
-//               increment; sparse reflection sources and dense Born planes added; the plane is
-//               written out only where needed (kept planes, pass ends, Born inputs); R products:
-//               row IFFT and gather of the interface pixels into the sparse store; finally the row
-//               FFT of the result for the next slab step (fused, so the field never makes an
-//               extra trip).
+//   column kernel  X (rows already FFT'd along x) → column FFT → for each product o (one per label
+//                  pair, T kinds first, then R kinds): M_o·Ŝ or phase_o·Ŝ → column IFFT → plane P_o
+//   row kernel     for each T product: row IFFT of P_o, keep the pixels of its pair; mask; Born
+//                  increment; sparse reflection sources and dense Born planes added; the plane is
+//                  written out only where needed (kept planes, pass ends, Born inputs); R products:
+//                  row IFFT and gather of the interface pixels into the sparse store; finally the
+//                  row FFT of the result for the next slab step (fused, so the field never makes
+//                  an extra trip).
+//
+// Every launch serves a batch of solves of equal grid: blockIdx.y selects the solve and its
+// per-step descriptor (a null descriptor is a no-op), so many planes share one launch.
 #pragma once
 
 #include "fft.hpp"
+#include "fft_reg.hpp"
 #include "interface_math.hpp"
 
 namespace bwpm {
@@ -27,8 +31,8 @@ __device__ __forceinline__ cx<T>* lds() {
   return reinterpret_cast<cx<T>*>(g_lds);
 }
 
-// Operator product descriptor: scalar phase table (type 0, one complex per k) or symmetric 2×2
-// matrix table (type 1, three complex per k: m00, m01, m11), each as hi and lo parts.
+// Operator product: scalar phase table (type 0, one complex per k) or symmetric 2×2 matrix table
+// (type 1, three complex per k: m00, m01, m11), each as hi and lo parts.
 template <typename T>
 struct Prod {
   int type;
@@ -36,83 +40,26 @@ struct Prod {
   const cx<T>* lo;
 };
 
-inline constexpr int kMaxProd = 64;
-
+// Column-kernel work of one solve for one step.
 template <typename T>
-struct ColStep {
-  const px2<T>* in;
-  px2<T>* out;          // nprod planes of ny·nx pixels
-  long long plane;      // pixels per plane
-  int ny, nx, cols, ls;
+struct ColDesc {
+  const px2<T>* in;      // X plane; nullptr: no column work for this solve at this step
+  px2<T>* out;           // product planes, `plane` pixels apart
+  const Prod<T>* prods;  // nprod descriptors (T kinds first)
   int nprod;
-  const Prod<T>* prods; // device array of nprod entries
-  FftPlan plan;         // length ny
 };
 
-template <typename T, bool COMP>
-__global__ __launch_bounds__(kThreads) void k_col_step(ColStep<T> a) {
-  cx<T>* A = lds<T>();
-  const int W = a.cols, ny = a.ny, nx = a.nx, ls = a.ls;
-  const int lines = 2 * W;
-  cx<T>* B = A + static_cast<long long>(ny) * ls;
-  const int x0 = blockIdx.x * W;
-  const int count = ny * W;
-  for (int i = threadIdx.x; i < count; i += kThreads) {
-    const int y = i / W, c = i - y * W, x = x0 + c;
-    px2<T> v{};
-    if (x < nx) v = a.in[static_cast<long long>(y) * nx + x];
-    A[y * ls + c] = v.a;
-    A[y * ls + W + c] = v.b;
-  }
-  __syncthreads();
-  fft_lines<COMP, false>(A, ls, lines, a.plan);
-  for (int o = 0; o < a.nprod; ++o) {
-    const Prod<T> pd = a.prods[o];
-    cx<T>* D = a.nprod == 1 ? A : B;
-    for (int i = threadIdx.x; i < count; i += kThreads) {
-      const int y = i / W, c = i - y * W, x = x0 + c;
-      if (x >= nx) continue;
-      const long long k = static_cast<long long>(y) * nx + x;
-      const cx<T> ea = A[y * ls + c], eb = A[y * ls + W + c];
-      cx<T> ra, rb;
-      if (pd.type == 0) {
-        const cx<T> ph = pd.hi[k], pl = pd.lo[k];
-        ra = cmul2<COMP>(ea, ph, pl);
-        rb = cmul2<COMP>(eb, ph, pl);
-      } else {
-        const cx<T>* h = pd.hi + 3 * k;
-        const cx<T>* l = pd.lo + 3 * k;
-        const cx<T> h00 = h[0], h01 = h[1], h11 = h[2];
-        const cx<T> l00 = l[0], l01 = l[1], l11 = l[2];
-        ra = cmul2<COMP>(ea, h00, l00) + cmul2<COMP>(eb, h01, l01);
-        rb = cmul2<COMP>(ea, h01, l01) + cmul2<COMP>(eb, h11, l11);
-      }
-      D[y * ls + c] = ra;
-      D[y * ls + W + c] = rb;
-    }
-    __syncthreads();
-    fft_lines<COMP, true>(D, ls, lines, a.plan);
-    px2<T>* out = a.out + o * a.plane;
-    for (int i = threadIdx.x; i < count; i += kThreads) {
-      const int y = i / W, c = i - y * W, x = x0 + c;
-      if (x < nx) out[static_cast<long long>(y) * nx + x] = px2<T>{D[y * ls + c], D[y * ls + W + c]};
-    }
-    __syncthreads();
-  }
-}
-
+// Row-kernel work of one solve for one step.
 template <typename T>
-struct RowStep {
-  int ny, nx, rows, ls;
-  FftPlan plan;               // length nx
-  const px2<T>* prod;         // nt + nr product planes
-  long long plane;
-  int nt, nr;
-  const uint8_t* tsel;        // T pair of each pixel (nt > 1), plane of ny·nx
+struct RowDesc {
+  int active;                 // 0: no row work at this step
+  int nt, nr;                 // T products (0: start a pass from e_in), R products
+  const px2<T>* prod;         // product planes
+  const uint8_t* tsel;        // T pair of each pixel (nt > 1)
   const T* mask;              // transmission mask or nullptr
   const px2<T>* e_in;         // nt == 0: start plane (nullptr: zero)
-  const px2<T>* inc;          // Born increment, multiplied by mask if mask != nullptr
-  const int* rowptr;          // ny + 1 offsets into the boundary's sparse list (nullptr: none)
+  const px2<T>* inc;          // Born increment, times mask if mask != nullptr
+  const int* rowptr;          // ny + 1 offsets into the sparse list (nullptr: none)
   const int* sp_pix;          // pixel index of each sparse entry
   const uint8_t* sp_rid;      // R product of each sparse entry
   const px2<T>* sp_add;       // source values added at the sparse pixels (nullptr: none)
@@ -123,9 +70,78 @@ struct RowStep {
   px2<T>* x_out;              // row FFT of the result for the next step (nullptr: none)
 };
 
+template <typename T, bool COMP>
+__device__ __forceinline__ px2<T> apply_prod(const Prod<T>& pd, long long k, px2<T> e) {
+  if (pd.type == 0) {
+    const cx<T> ph = pd.hi[k], pl = pd.lo[k];
+    return px2<T>{cmul2<COMP>(e.a, ph, pl), cmul2<COMP>(e.b, ph, pl)};
+  }
+  const cx<T>* h = pd.hi + 3 * k;
+  const cx<T>* l = pd.lo + 3 * k;
+  const cx<T> h00 = h[0], h01 = h[1], h11 = h[2];
+  const cx<T> l00 = l[0], l01 = l[1], l11 = l[2];
+  return px2<T>{cmul2<COMP>(e.a, h00, l00) + cmul2<COMP>(e.b, h01, l01),
+                cmul2<COMP>(e.a, h01, l01) + cmul2<COMP>(e.b, h11, l11)};
+}
+
+// ================================ generic sizes (LDS Stockham) ================================
+
+struct ColGeom {
+  int ny, nx, cols, ls;
+  long long plane;
+  FftPlan plan;  // length ny
+};
+
+template <typename T, bool COMP>
+__global__ __launch_bounds__(kThreads) void k_col_step(const ColDesc<T>* __restrict__ descs, ColGeom g) {
+  const ColDesc<T> d = descs[blockIdx.y];
+  if (!d.in) return;
+  cx<T>* A = lds<T>();
+  const int W = g.cols, ny = g.ny, nx = g.nx, ls = g.ls;
+  const int lines = 2 * W;
+  cx<T>* B = A + static_cast<long long>(ny) * ls;
+  const int x0 = blockIdx.x * W;
+  const int count = ny * W;
+  for (int i = threadIdx.x; i < count; i += kThreads) {
+    const int y = i / W, c = i - y * W, x = x0 + c;
+    px2<T> v{};
+    if (x < nx) v = d.in[static_cast<long long>(y) * nx + x];
+    A[y * ls + c] = v.a;
+    A[y * ls + W + c] = v.b;
+  }
+  __syncthreads();
+  fft_lines<COMP, false>(A, ls, lines, g.plan);
+  for (int o = 0; o < d.nprod; ++o) {
+    const Prod<T> pd = d.prods[o];
+    cx<T>* D = d.nprod == 1 ? A : B;
+    for (int i = threadIdx.x; i < count; i += kThreads) {
+      const int y = i / W, c = i - y * W, x = x0 + c;
+      if (x >= nx) continue;
+      const long long k = static_cast<long long>(y) * nx + x;
+      const px2<T> r = apply_prod<T, COMP>(pd, k, px2<T>{A[y * ls + c], A[y * ls + W + c]});
+      D[y * ls + c] = r.a;
+      D[y * ls + W + c] = r.b;
+    }
+    __syncthreads();
+    fft_lines<COMP, true>(D, ls, lines, g.plan);
+    px2<T>* out = d.out + o * g.plane;
+    for (int i = threadIdx.x; i < count; i += kThreads) {
+      const int y = i / W, c = i - y * W, x = x0 + c;
+      if (x < nx) out[static_cast<long long>(y) * nx + x] = px2<T>{D[y * ls + c], D[y * ls + W + c]};
+    }
+    __syncthreads();
+  }
+}
+
+struct RowGeom {
+  int ny, nx, rows, ls;
+  long long plane;
+  FftPlan plan;  // length nx
+};
+
 template <typename T>
-__device__ __forceinline__ void load_rows(cx<T>* buf, const px2<T>* src, int y0, int nrow, int nx,
-                                          int rows, int ls) {
+__device__ __forceinline__ void load_rows(cx<T>* buf, const px2<T>* src, int y0, int nrow, int nx, int rows,
+                                          int ls) {
   const int count = nrow * nx;
   for (int i = threadIdx.x; i < count; i += kThreads) {
     const int r = i / nx, x = i - r * nx;
@@ -136,10 +152,12 @@ __device__ __forceinline__ void load_rows(cx<T>* buf, const px2<T>* src, int y0,
 }
 
 template <typename T, bool COMP>
-__global__ __launch_bounds__(kThreads) void k_row_step(RowStep<T> a) {
-  const int nx = a.nx, rows = a.rows, ls = a.ls;
+__global__ __launch_bounds__(kThreads) void k_row_step(const RowDesc<T>* __restrict__ descs, RowGeom g) {
+  const RowDesc<T> a = descs[blockIdx.y];
+  if (!a.active) return;
+  const int nx = g.nx, rows = g.rows, ls = g.ls;
   const int y0 = blockIdx.x * rows;
-  const int nrow = min(rows, a.ny - y0);
+  const int nrow = min(rows, g.ny - y0);
   const int lines = 2 * rows;
   const int count = nrow * nx;
   cx<T>* acc = lds<T>();
@@ -160,12 +178,12 @@ __global__ __launch_bounds__(kThreads) void k_row_step(RowStep<T> a) {
   } else if (a.nt == 1) {
     load_rows(acc, a.prod, y0, nrow, nx, rows, ls);
     __syncthreads();
-    fft_lines<COMP, true>(acc, ls, lines, a.plan);
+    fft_lines<COMP, true>(acc, ls, lines, g.plan);
   } else {
     for (int o = 0; o < a.nt; ++o) {
-      load_rows(work, a.prod + o * a.plane, y0, nrow, nx, rows, ls);
+      load_rows(work, a.prod + o * g.plane, y0, nrow, nx, rows, ls);
       __syncthreads();
-      fft_lines<COMP, true>(work, ls, lines, a.plan);
+      fft_lines<COMP, true>(work, ls, lines, g.plan);
       for (int i = threadIdx.x; i < count; i += kThreads) {
         const int r = i / nx, x = i - r * nx;
         if (a.tsel[static_cast<long long>(y0 + r) * nx + x] == o) {
@@ -187,14 +205,14 @@ __global__ __launch_bounds__(kThreads) void k_row_step(RowStep<T> a) {
       ea = ea * m;
       eb = eb * m;
       if (a.inc) {
-        const px2<T> d = a.inc[q];
-        ea = ea + d.a * m;
-        eb = eb + d.b * m;
+        const px2<T> dd = a.inc[q];
+        ea = ea + dd.a * m;
+        eb = eb + dd.b * m;
       }
     } else if (a.inc) {
-      const px2<T> d = a.inc[q];
-      ea = ea + d.a;
-      eb = eb + d.b;
+      const px2<T> dd = a.inc[q];
+      ea = ea + dd.a;
+      eb = eb + dd.b;
     }
     acc[x * ls + r] = ea;
     acc[x * ls + rows + r] = eb;
@@ -214,9 +232,9 @@ __global__ __launch_bounds__(kThreads) void k_row_step(RowStep<T> a) {
   if (a.dense) {
     for (int i = threadIdx.x; i < count; i += kThreads) {
       const int r = i / nx, x = i - r * nx;
-      const px2<T> d = a.dense[static_cast<long long>(y0 + r) * nx + x];
-      acc[x * ls + r] = acc[x * ls + r] + d.a;
-      acc[x * ls + rows + r] = acc[x * ls + rows + r] + d.b;
+      const px2<T> dd = a.dense[static_cast<long long>(y0 + r) * nx + x];
+      acc[x * ls + r] = acc[x * ls + r] + dd.a;
+      acc[x * ls + rows + r] = acc[x * ls + rows + r] + dd.b;
     }
     __syncthreads();
   }
@@ -225,9 +243,9 @@ __global__ __launch_bounds__(kThreads) void k_row_step(RowStep<T> a) {
   if (a.nr > 0 && a.rowptr && a.sp_store) {
     const int e0 = a.rowptr[y0], e1 = a.rowptr[y0 + nrow];
     for (int o = 0; o < a.nr; ++o) {
-      load_rows(work, a.prod + (a.nt + o) * a.plane, y0, nrow, nx, rows, ls);
+      load_rows(work, a.prod + (a.nt + o) * g.plane, y0, nrow, nx, rows, ls);
       __syncthreads();
-      fft_lines<COMP, true>(work, ls, lines, a.plan);
+      fft_lines<COMP, true>(work, ls, lines, g.plan);
       for (int e = e0 + threadIdx.x; e < e1; e += kThreads) {
         if (a.sp_rid[e] != o) continue;
         const int pix = a.sp_pix[e];
@@ -251,11 +269,164 @@ __global__ __launch_bounds__(kThreads) void k_row_step(RowStep<T> a) {
 
   // 5. row FFT for the next slab step
   if (a.x_out) {
-    fft_lines<COMP, false>(acc, ls, lines, a.plan);
+    fft_lines<COMP, false>(acc, ls, lines, g.plan);
     for (int i = threadIdx.x; i < count; i += kThreads) {
       const int r = i / nx, x = i - r * nx;
       a.x_out[static_cast<long long>(y0 + r) * nx + x] = px2<T>{acc[x * ls + r], acc[x * ls + rows + r]};
     }
+  }
+}
+
+// ======================= power-of-two sizes (register-resident FFT) ==========================
+
+// Column kernel: W adjacent columns per workgroup, N/8 threads per column, each thread holding
+// eight (E_x, E_y) vectors of its column. Thread index = t·W + c, so a wavefront reads W·16-byte
+// row segments.
+template <int N, int W, typename T, bool COMP>
+__global__ __launch_bounds__(N / 8 * W) void k_col_fast(const ColDesc<T>* __restrict__ descs, int nx,
+                                                        long long plane, const cx<T>* __restrict__ twh,
+                                                        const cx<T>* __restrict__ twl) {
+  constexpr int TT = N / 8;
+  constexpr int LL = reg_lds_len<N>();
+  __shared__ px2<T> sh[W * LL];
+  const ColDesc<T> d = descs[blockIdx.y];
+  if (!d.in) return;
+  const int c = threadIdx.x % W, t = threadIdx.x / W;
+  const int x = blockIdx.x * W + c;
+  px2<T>* line = sh + c * LL;
+  px2<T> s[8];
+#pragma unroll
+  for (int m = 0; m < 8; ++m) s[m] = d.in[static_cast<long long>(t + TT * m) * nx + x];
+  reg_fft<N, COMP, false>(s, line, t, twh, twl);
+  for (int o = 0; o < d.nprod; ++o) {
+    const Prod<T> pd = d.prods[o];
+    px2<T> w[8];
+#pragma unroll
+    for (int m = 0; m < 8; ++m) w[m] = apply_prod<T, COMP>(pd, static_cast<long long>(t + TT * m) * nx + x, s[m]);
+    reg_fft<N, COMP, true>(w, line, t, twh, twl);
+    px2<T>* out = d.out + o * plane;
+#pragma unroll
+    for (int m = 0; m < 8; ++m) out[static_cast<long long>(t + TT * m) * nx + x] = w[m];
+  }
+}
+
+// Row kernel: RW rows per workgroup, N/8 threads per row (thread index = r·N/8 + t).
+template <int N, int RW, typename T, bool COMP>
+__global__ __launch_bounds__(N / 8 * RW) void k_row_fast(const RowDesc<T>* __restrict__ descs, long long plane,
+                                                         const cx<T>* __restrict__ twh,
+                                                         const cx<T>* __restrict__ twl) {
+  constexpr int TT = N / 8;
+  constexpr int LL = reg_lds_len<N>();
+  __shared__ px2<T> sh[RW * LL];
+  const RowDesc<T> a = descs[blockIdx.y];
+  if (!a.active) return;
+  const int r = threadIdx.x / TT, t = threadIdx.x % TT;
+  const int y = blockIdx.x * RW + r;
+  const long long row = static_cast<long long>(y) * N;
+  px2<T>* line = sh + r * LL;
+  px2<T> acc[8];
+
+  // 1. transmitted field (or the start plane)
+  if (a.nt == 0) {
+#pragma unroll
+    for (int m = 0; m < 8; ++m) acc[m] = a.e_in ? a.e_in[row + t + TT * m] : px2<T>{};
+  } else {
+#pragma unroll
+    for (int m = 0; m < 8; ++m) acc[m] = a.prod[row + t + TT * m];
+    reg_fft<N, COMP, true>(acc, line, t, twh, twl);
+    for (int o = 1; o < a.nt; ++o) {
+      px2<T> w[8];
+      const px2<T>* src = a.prod + o * plane;
+#pragma unroll
+      for (int m = 0; m < 8; ++m) w[m] = src[row + t + TT * m];
+      reg_fft<N, COMP, true>(w, line, t, twh, twl);
+#pragma unroll
+      for (int m = 0; m < 8; ++m)
+        if (a.tsel[row + t + TT * m] == o) acc[m] = w[m];
+    }
+  }
+
+  // 2. epilogue in the order of the reference: mask, Born increment, sparse sources, Born plane
+  if (a.mask) {
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+      const T mk = a.mask[row + t + TT * m];
+      acc[m].a = acc[m].a * mk;
+      acc[m].b = acc[m].b * mk;
+      if (a.inc) {
+        const px2<T> dd = a.inc[row + t + TT * m];
+        acc[m].a = acc[m].a + dd.a * mk;
+        acc[m].b = acc[m].b + dd.b * mk;
+      }
+    }
+  } else if (a.inc) {
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+      const px2<T> dd = a.inc[row + t + TT * m];
+      acc[m].a = acc[m].a + dd.a;
+      acc[m].b = acc[m].b + dd.b;
+    }
+  }
+  if (a.rowptr && a.sp_add) {
+#pragma unroll
+    for (int m = 0; m < 8; ++m) line[lpad(t + TT * m)] = acc[m];
+    __syncthreads();
+    const int e1 = a.rowptr[y + 1];
+    for (int e = a.rowptr[y] + t; e < e1; e += TT) {
+      const int x = a.sp_pix[e] - y * N;
+      const px2<T> v = a.sp_add[e];
+      px2<T> u = line[lpad(x)];
+      u.a = u.a + v.a;
+      u.b = u.b + v.b;
+      line[lpad(x)] = u;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < 8; ++m) acc[m] = line[lpad(t + TT * m)];
+    __syncthreads();
+  }
+  if (a.dense) {
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+      const px2<T> dd = a.dense[row + t + TT * m];
+      acc[m].a = acc[m].a + dd.a;
+      acc[m].b = acc[m].b + dd.b;
+    }
+  }
+
+  // 3. reflection products: row IFFT and gather of the interface pixels
+  if (a.nr > 0 && a.rowptr && a.sp_store) {
+    const int e0 = a.rowptr[y], e1 = a.rowptr[y + 1];
+    for (int o = 0; o < a.nr; ++o) {
+      px2<T> w[8];
+      const px2<T>* src = a.prod + (a.nt + o) * plane;
+#pragma unroll
+      for (int m = 0; m < 8; ++m) w[m] = src[row + t + TT * m];
+      reg_fft<N, COMP, true>(w, line, t, twh, twl);
+#pragma unroll
+      for (int m = 0; m < 8; ++m) line[lpad(t + TT * m)] = w[m];
+      __syncthreads();
+      for (int e = e0 + t; e < e1; e += TT)
+        if (a.sp_rid[e] == o) a.sp_store[e] = line[lpad(a.sp_pix[e] - y * N)];
+      __syncthreads();
+    }
+  }
+
+  // 4. real-space outputs
+  if (a.e_out) {
+#pragma unroll
+    for (int m = 0; m < 8; ++m) a.e_out[row + t + TT * m] = acc[m];
+  }
+  if (a.e_out2) {
+#pragma unroll
+    for (int m = 0; m < 8; ++m) a.e_out2[row + t + TT * m] = acc[m];
+  }
+
+  // 5. row FFT for the next slab step
+  if (a.x_out) {
+    reg_fft<N, COMP, false>(acc, line, t, twh, twl);
+#pragma unroll
+    for (int m = 0; m < 8; ++m) a.x_out[row + t + TT * m] = acc[m];
   }
 }
 
@@ -293,26 +464,8 @@ __global__ void k_build_matrix(const double* kx, const double* ky, int ny, int n
   split(zmul(m.m11, scale), hi + 3 * k + 2, lo + 3 * k + 2);
 }
 
-// Exported interface functions on arbitrary k arrays: full row-major 2×2 (which = 0: T, 1: R).
-__global__ void k_interface_eval(const double* kx, const double* ky, long long n, int which, zd n1,
-                                 zd n2, double k0, zd* out) {
-  const long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;
-  if (i >= n) return;
-  const Mat2 m = which == 0 ? transmission(kx[i], ky[i], n1, n2, k0) : reflection(kx[i], ky[i], n1, n2, k0);
-  out[4 * i] = m.m00;
-  out[4 * i + 1] = m.m01;
-  out[4 * i + 2] = m.m01;
-  out[4 * i + 3] = m.m11;
-}
-
-__global__ void k_kz_eval(zd n, double k0, const double* kp2, long long cnt, zd* out) {
-  const long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;
-  if (i < cnt) out[i] = kz_of(n, k0, kp2[i]);
-}
-
 // ---- small elementwise helpers ---------------------------------------------------------------
 
-// Convert interleaved complex128 (E_x, E_y) pixels to the working precision.
 template <typename T>
 __global__ void k_from_c128(const zd* src, px2<T>* dst, long long npix) {
   const long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;

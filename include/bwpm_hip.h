@@ -98,6 +98,22 @@ int bwpm_solver_run_resident(bwpm_solver* s, int iterations, int time_kernels, d
                              int64_t* row_launches);
 int bwpm_solver_device_bytes(const bwpm_solver* s, int64_t* bytes);
 
+/* Results of the last resident or batched run of `s` (no kept planes). */
+int bwpm_solver_fetch(bwpm_solver* s, int iterations, bwpm_result** out);
+
+/* Batched solve (no counterpart in the reference; the persistent multi-solve path): every solver
+ * must have its field uploaded. Solvers of equal precision, grid and device are grouped into
+ * lanes of at most `max_lane` that step in lockstep, one column and one row launch per slab step
+ * for the whole lane; lanes run concurrently on their own streams. Results stay on the device
+ * until bwpm_solver_fetch. Each result is bit-identical to the solver's run alone. */
+int bwpm_batch_run(bwpm_solver* const* solvers, int64_t n, int iterations, int max_lane,
+                   double* wall_ms);
+
+/* Average duration of a plain copy kernel (k_copy_probe) over `bytes` on `device`: the HBM
+ * calibration of the traffic counters and the achievable-bandwidth reference of the benchmark
+ * (no counterpart in the reference). */
+int bwpm_copy_probe(int64_t bytes, int reps, int device, double* ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,102 @@
+"""GPU parity of the register-resident slab kernels (power-of-two grids) and of batched lanes.
+
+Volumes mix a homogeneous run, a tilted glass interface, an absorbing layer and silver, with an
+absorbing edge mask and band-limited random fields (the construction of the reference's
+tests/test_core_solver.py:99-127), compared with the FP64 oracle; batched solves must equal the
+same solves run alone bit for bit.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+from helpers import SILVER, band_limited_field, k0_of, mask_2d
+from oracle import bwpm_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def layered_volume(ny: int, nx: int, nz: int, seed: int, mask: bool = True) -> dict:
+    rng = np.random.default_rng(seed)
+    lam = 0.6
+    dy, dx, dz = lam * 1e-3 / 6, lam * 1e-3 / 5, lam * 1e-3 / 4
+    k0 = k0_of(lam)
+    y = (np.arange(ny) - ny // 2) * dy
+    zc = (np.arange(nz) + 0.5) * dz
+    labels = np.zeros((nz, ny, nx), dtype=np.int32)
+    tilt = (zc[:, None] > 3 * dz + 0.3 * y[None, :])
+    labels[:] = np.where(tilt, 1, 0)[:, :, None]
+    labels[nz - 3] = 2
+    labels[nz - 2:] = 3
+    field = band_limited_field(rng, ny, nx, dy, dx, k0)
+    return dict(ny=ny, nx=nx, dy=dy, dx=dx, dz=dz, k0=k0, indices=[1.0, 1.5, 2.1 + 0.01j, SILVER],
+                labels=labels, index_before=1.0 + 0j, mask=mask_2d(ny, nx, dy, dx, 3 * dx) if mask else None,
+                field=field)
+
+
+def _solver(a: dict, precision: str):
+    from bwpm_amd import _core
+
+    return _core.GpuSolver(a["ny"], a["nx"], a["dy"], a["dx"], a["dz"], a["k0"], a["indices"], a["labels"],
+                           a["index_before"], a["mask"], precision=precision)
+
+
+@pytest.mark.parametrize("shape", [(64, 64), (128, 128), (256, 128), (512, 512), (512, 12), (12, 256)])
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_register_kernels_match_oracle(shape: tuple[int, int], precision: str) -> None:
+    ny, nx = shape
+    nz = 9
+    a = layered_volume(ny, nx, nz, seed=ny + nx)
+    keep = (0, 4, nz)
+    ref = O.Solver(a["ny"], a["nx"], a["dy"], a["dx"], a["dz"], a["k0"], a["indices"], a["labels"],
+                   a["index_before"], a["mask"]).run(a["field"], 2, keep)
+    out = _solver(a, precision).run(a["field"], 2, keep)
+    assert out["stored_values"] == ref["stored_values"]
+    pairs = [(out["transmitted"], ref["transmitted"]), (out["reflected"], ref["reflected"])]
+    for q in keep:
+        pairs += [(out["forward"][q], ref["forward"][q]), (out["backward"][q], ref["backward"][q])]
+    scale = float(np.max(np.abs(a["field"])))
+    for got, want in pairs:
+        if precision == "fp64":
+            assert np.max(np.abs(got - want)) < 1e-10 * scale
+        else:
+            assert rel_l2(got, want) < 1e-5
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_batch_equals_alone_bitwise(precision: str) -> None:
+    """Solves of two grid sizes, uneven depths and different geometries: the batched lanes give
+    each solve exactly its result alone."""
+    from bwpm_amd import _core
+
+    cases = [layered_volume(128, 128, nz, seed=s) for nz, s in ((9, 1), (12, 2), (7, 3), (9, 4))]
+    cases += [layered_volume(64, 64, 10, seed=5, mask=False), layered_volume(64, 64, 6, seed=6)]
+    its = 2
+    alone = []
+    for a in cases:
+        s = _solver(a, precision)
+        s.upload(a["field"])
+        s.run_resident(its)
+        alone.append(s.fetch(its))
+        s.close()
+    solvers = [_solver(a, precision) for a in cases]
+    for s, a in zip(solvers, cases):
+        s.upload(a["field"])
+    _core.run_batch(solvers, its, max_lane=8)
+    for s, a, ref in zip(solvers, cases, alone):
+        got = s.fetch(its)
+        for key in ("transmitted", "reflected"):
+            assert np.array_equal(got[key], ref[key]), key
+        for t_got, t_ref in zip(got["transmitted_iter"], ref["transmitted_iter"]):
+            assert np.array_equal(t_got, t_ref)
+        assert got["stored_values"] == ref["stored_values"]
+    # and the batch is right: one case against the oracle
+    a = cases[1]
+    want = O.Solver(a["ny"], a["nx"], a["dy"], a["dx"], a["dz"], a["k0"], a["indices"], a["labels"],
+                    a["index_before"], a["mask"]).run(a["field"], its)
+    got = solvers[1].fetch(its)
+    tol = 1e-10 if precision == "fp64" else 1e-5
+    assert rel_l2(got["transmitted"], want["transmitted"]) < tol
+    assert rel_l2(got["reflected"], want["reflected"]) < tol
