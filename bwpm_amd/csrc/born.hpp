@@ -1,24 +1,684 @@
-// Born scattering at weak index fluctuations δn (placeholder until the device Born path lands).
+// Born scattering at weak index fluctuations δn on the device (reference: Solver::born_sources,
+// born_operator, born_step, born_emission and gmres, cpp/src/solver.cpp:130-382).
+//
+// Three-component spectra (S_x, S_y, S_z) are carried as two (E_x, E_y)-shaped planes, (x, y) and
+// (z, ·); the two Gauss points of the sources share one (z₀, z₁) plane, so every transform is a
+// plain 2D FFT of such planes (fft2d.hpp), several planes per launch.
+//
+// Per host medium an FP64-built per-k table holds k_z, the propagating/non-grazing flag (as the
+// factor 1/k_z or 0), the half-step propagators P_½ and P_½⁻¹, the source phases e^{i k_z s_m}
+// and the emission factors ½e^{i d′k_z(dz−ζ_m)}·i dz/(2k_z) for both directions.
+//
+// The Cayley system (1 − G/2)y = G·P_½Ê is solved by restarted GMRES with the Krylov vectors
+// on the device, classical Gram–Schmidt with one re-orthogonalisation (two fused block-dot
+// reductions per inner step instead of j + 1 sequential ones), deterministic two-stage
+// reductions accumulated in FP64, and the Hessenberg matrix and Givens rotations on the host.
 #pragma once
 
+#include <cmath>
+#include <complex>
+#include <map>
+#include <vector>
+
 #include "engine.hpp"
-#include "fft.hpp"
+#include "fft2d.hpp"
+#include "interface_math.hpp"
 
 namespace bwpm {
+
+inline constexpr double kGauss0 = 0.21132486540518711775;  // (1 − 1/√3)/2
+inline constexpr double kGauss1 = 0.78867513459481288225;  // (1 + 1/√3)/2
+
+template <typename T>
+struct BornK {
+  cx<T> cz;          // propagating, non-grazing: 1/k_z; else 0
+  cx<T> kz;          // k_z
+  cx<T> green;       // i dz/(2k_z) (propagating) or 0
+  cx<T> half;        // P_½ = e^{i k_z dz/2} (propagating) or 0
+  cx<T> halfinv;     // 1/P_½ or 0
+  cx<T> ps[2][2];    // source phase e^{i k_z s_m}, [direction +1/−1][m]
+  cx<T> ge[2][2];    // emission factor, [direction′ +1/−1][m]
+};
+
+template <typename T>
+__global__ void k_born_table(const double* kx, const double* ky, int ny, int nx, zd host, double k0, double dz,
+                             BornK<T>* out) {
+  const long long k = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;
+  if (k >= static_cast<long long>(ny) * nx) return;
+  const int y = static_cast<int>(k / nx), x = static_cast<int>(k - static_cast<long long>(y) * nx);
+  const double kp2 = kx[x] * kx[x] + ky[y] * ky[y];
+  const zd kz = kz_of(host, k0, kp2);
+  const bool prop = born_propagating(host, k0, kp2);
+  auto c = [](zd v) { return cx<T>{static_cast<T>(v.x), static_cast<T>(v.y)}; };
+  const zd zero{0.0, 0.0};
+  BornK<T> b;
+  b.kz = c(kz);
+  b.cz = c(prop ? zdiv(zd{1.0, 0.0}, kz) : zero);
+  const zd green = zdiv(zd{0.0, dz}, zmul(kz, 2.0));  // i dz/(2k_z)
+  b.green = c(prop ? green : zero);
+  const zd half = prop ? propagation(kz, dz / 2.0, 1.0) : zero;
+  b.half = c(half);
+  b.halfinv = c(prop ? zdiv(zd{1.0, 0.0}, half) : zero);
+  const double g[2] = {kGauss0, kGauss1};
+  for (int m = 0; m < 2; ++m) {
+    b.ps[0][m] = c(prop ? propagation(kz, g[m] * dz, 1.0) : zero);          // d = +1: s = ζ_m
+    b.ps[1][m] = c(prop ? propagation(kz, (1.0 - g[m]) * dz, 1.0) : zero);  // d = −1: s = dz − ζ_m
+    for (int dd = 0; dd < 2; ++dd) {
+      const double sgn = dd == 0 ? 1.0 : -1.0;
+      const zd skz{sgn * kz.x, sgn * kz.y};
+      const zd e = propagation(skz, (1.0 - g[m]) * dz, 1.0);
+      b.ge[dd][m] = c(prop ? zmul(zmul(e, 0.5), green) : zero);
+    }
+  }
+  out[k] = b;
+}
+
+// G_d: (S_x, S_y) spectrum → (S_x, S_y) and (S_z, 0) planes on propagating waves.
+template <typename T>
+__global__ void k_born_prepare_op(const px2<T>* __restrict__ v, const BornK<T>* __restrict__ tb,
+                                  const T* __restrict__ kxT, const T* __restrict__ kyT, int nx, long long npix, T d,
+                                  px2<T>* xy, px2<T>* zz) {
+  const long long q = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;
+  if (q >= npix) return;
+  const T kx = kxT[q % nx], ky = kyT[q / nx];
+  const cx<T> cz = tb[q].cz;
+  const bool prop = cz.x != 0 || cz.y != 0;
+  const px2<T> s = v[q];
+  const cx<T> ex = prop ? s.a : cx<T>{0, 0}, ey = prop ? s.b : cx<T>{0, 0};
+  const cx<T> ez = (ex * kx + ey * ky) * cz * (-d);
+  xy[q] = px2<T>{ex, ey};
+  zz[q] = px2<T>{ez, cx<T>{0, 0}};
+}
+
+// out = i dz/(2k_z)·[P(k_d)·S]⊥ (mode 0) or v − ½ that (mode 1: the Cayley operator 1 − G/2).
+template <typename T>
+__global__ void k_born_finish_op(const px2<T>* __restrict__ xy, const px2<T>* __restrict__ zz,
+                                 const BornK<T>* __restrict__ tb, const T* __restrict__ kxT,
+                                 const T* __restrict__ kyT, int nx, long long npix, T d, cx<T> inv_k2,
+                                 const px2<T>* __restrict__ v, px2<T>* out, int mode) {
+  const long long q = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;
+  if (q >= npix) return;
+  const T kx = kxT[q % nx], ky = kyT[q / nx];
+  const BornK<T>& b = tb[q];
+  const px2<T> s = xy[q];
+  const cx<T> s2 = zz[q].a;
+  const cx<T> lon = (s.a * kx + s.b * ky + b.kz * d * s2) * inv_k2;
+  px2<T> g{b.green * (s.a - lon * kx), b.green * (s.b - lon * ky)};
+  if (mode == 1) {
+    const px2<T> w = v[q];
+    const T h = static_cast<T>(0.5);
+    g = px2<T>{w.a - g.a * h, w.b - g.b * h};
+  }
+  out[q] = g;
+}
+
+// x = P_½·Ê (which = 0), y·P_½ (which = 1, direction +1) or y·P_½⁻¹ (which = 2, direction −1).
+template <typename T>
+__global__ void k_born_half(const px2<T>* __restrict__ in, const BornK<T>* __restrict__ tb, long long npix,
+                            int which, px2<T>* out) {
+  const long long q = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;
+  if (q >= npix) return;
+  const cx<T> f = which == 2 ? tb[q].halfinv : tb[q].half;
+  const px2<T> v = in[q];
+  out[q] = px2<T>{v.a * f, v.b * f};
+}
+
+// Sources: (Ê_x, Ê_y, Ê_z)·e^{i k_z s_m} for both Gauss points → planes (x, y)₀, (x, y)₁, (z₀, z₁).
+template <typename T>
+__global__ void k_born_src_prepare(const px2<T>* __restrict__ e, const BornK<T>* __restrict__ tb,
+                                   const T* __restrict__ kxT, const T* __restrict__ kyT, int nx, long long npix,
+                                   T d, int dir, px2<T>* xy0, px2<T>* xy1, px2<T>* zz) {
+  const long long q = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;
+  if (q >= npix) return;
+  const T kx = kxT[q % nx], ky = kyT[q / nx];
+  const BornK<T>& b = tb[q];
+  const px2<T> s = e[q];
+  const cx<T> ez = (s.a * kx + s.b * ky) * b.cz * (-d);
+  const cx<T> p0 = b.ps[dir][0], p1 = b.ps[dir][1];
+  xy0[q] = px2<T>{s.a * p0, s.b * p0};
+  xy1[q] = px2<T>{s.a * p1, s.b * p1};
+  zz[q] = px2<T>{ez * p0, ez * p1};
+}
+
+// Emission in direction d′: Σ_m ge_m·[P(k_d′)·S_m]⊥.
+template <typename T>
+__global__ void k_born_emit(const px2<T>* __restrict__ xy0, const px2<T>* __restrict__ xy1,
+                            const px2<T>* __restrict__ zz, const BornK<T>* __restrict__ tb,
+                            const T* __restrict__ kxT, const T* __restrict__ kyT, int nx, long long npix, T dp,
+                            int dir, cx<T> inv_k2, px2<T>* out) {
+  const long long q = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;
+  if (q >= npix) return;
+  const T kx = kxT[q % nx], ky = kyT[q / nx];
+  const BornK<T>& b = tb[q];
+  const cx<T> skz = b.kz * dp;
+  const px2<T> z = zz[q];
+  px2<T> tot{};
+#pragma unroll
+  for (int m = 0; m < 2; ++m) {
+    const px2<T> s = m == 0 ? xy0[q] : xy1[q];
+    const cx<T> s2 = m == 0 ? z.a : z.b;
+    const cx<T> lon = (s.a * kx + s.b * ky + skz * s2) * inv_k2;
+    const cx<T> g = b.ge[dir][m];
+    tot.a = tot.a + g * (s.a - lon * kx);
+    tot.b = tot.b + g * (s.b - lon * ky);
+  }
+  out[q] = tot;
+}
+
+// ---- GMRES vector kernels (deterministic: fixed grid, fixed reduction order) ----------------------
+
+inline constexpr int kRedBlocks = 512;
+inline constexpr int kRedThreads = 256;
+
+// partial[i·kRedBlocks + block] = Σ conj(u_i)·v over the block's grid-stride share (FP64).
+template <typename T>
+__global__ __launch_bounds__(kRedThreads) void k_dot_partial(const px2<T>* const* __restrict__ us,
+                                                             const px2<T>* __restrict__ v, long long npix,
+                                                             double2* __restrict__ partial) {
+  __shared__ double sr[kRedThreads], si[kRedThreads];
+  const px2<T>* u = us[blockIdx.y];
+  double re = 0, im = 0;
+  for (long long q = blockIdx.x * static_cast<long long>(kRedThreads) + threadIdx.x; q < npix;
+       q += static_cast<long long>(kRedBlocks) * kRedThreads) {
+    const px2<T> a = u[q], b = v[q];
+    re += static_cast<double>(a.a.x) * b.a.x + static_cast<double>(a.a.y) * b.a.y +
+          static_cast<double>(a.b.x) * b.b.x + static_cast<double>(a.b.y) * b.b.y;
+    im += static_cast<double>(a.a.x) * b.a.y - static_cast<double>(a.a.y) * b.a.x +
+          static_cast<double>(a.b.x) * b.b.y - static_cast<double>(a.b.y) * b.b.x;
+  }
+  sr[threadIdx.x] = re;
+  si[threadIdx.x] = im;
+  __syncthreads();
+  for (int s = kRedThreads / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      sr[threadIdx.x] += sr[threadIdx.x + s];
+      si[threadIdx.x] += si[threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.y * kRedBlocks + blockIdx.x] = double2{sr[0], si[0]};
+}
+
+__global__ __launch_bounds__(kRedThreads) void k_dot_final(const double2* __restrict__ partial, double2* __restrict__ out);
+
+// w += Σ_i c_i·V_i (c in FP64, k ≤ 32).
+template <typename T>
+__global__ void k_axpy_multi(px2<T>* w, const px2<T>* const* __restrict__ vs, const double2* __restrict__ c, int k,
+                             long long npix) {
+  const long long q = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;
+  if (q >= npix) return;
+  px2<T> acc = w[q];
+  for (int i = 0; i < k; ++i) {
+    const cx<T> ci{static_cast<T>(c[i].x), static_cast<T>(c[i].y)};
+    const px2<T> v = vs[i][q];
+    acc.a = acc.a + ci * v.a;
+    acc.b = acc.b + ci * v.b;
+  }
+  w[q] = acc;
+}
+
+// out = a·x + b·y (real a, b).
+template <typename T>
+__global__ void k_axpby(const px2<T>* x, T a, const px2<T>* y, T b, px2<T>* out, long long npix) {
+  const long long q = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;
+  if (q >= npix) return;
+  const px2<T> u = x[q];
+  const px2<T> v = y ? y[q] : px2<T>{};
+  out[q] = px2<T>{u.a * a + v.a * b, u.b * a + v.b * b};
+}
 
 template <typename T, bool COMP>
 class Born {
  public:
-  Born(const Spec&, const std::vector<char>&, const std::vector<c128>&, const double*, const double*, hipStream_t) {
-    throw Error(kEinval, "index fluctuations are not supported by this build");
+  Born(const Spec& s, const std::vector<char>& has_host, const std::vector<c128>& host, const double* kx,
+       const double* ky, const FftPlan& px, const FftPlan& py, hipStream_t stream)
+      : ny_(s.ny), nx_(s.nx), nz_(s.nz), npix_(s.ny * s.nx), k0_(s.k0), dz_(s.dz), px_(px), py_(py), stream_(stream) {
+    tol_ = sizeof(T) == 8 ? 1e-12 : 1e-6;  // reference tolerance (solver.cpp:209); FP32 rounding floor
+    slab_v_.assign(nz_, -1);
+    slab_tab_.assign(nz_, -1);
+    std::vector<c128> hosts;
+    long long nborn = 0;
+    for (long long j = 0; j < nz_; ++j)
+      if (has_host[j]) {
+        slab_v_[j] = nborn++;
+        int t = -1;
+        for (size_t i = 0; i < hosts.size(); ++i)
+          if (hosts[i] == host[j]) t = static_cast<int>(i);
+        if (t < 0) {
+          hosts.push_back(host[j]);
+          t = static_cast<int>(hosts.size()) - 1;
+        }
+        slab_tab_[j] = t;
+      }
+    hosts_ = hosts;
+    // potential V = 2ñ₀k₀²δn per Born slab (solver.cpp:241,254)
+    V_.alloc(static_cast<size_t>(std::max<long long>(nborn, 1)) * npix_ * sizeof(cx<T>));
+    {
+      std::vector<cx<T>> hv(npix_);
+      for (long long j = 0; j < nz_; ++j) {
+        if (slab_v_[j] < 0) continue;
+        const c128 scale = 2.0 * host[j] * s.k0 * s.k0;
+        for (long long q = 0; q < npix_; ++q) {
+          const c128 v = scale * s.fluct[j * npix_ + q];
+          hv[q] = cx<T>{static_cast<T>(v.real()), static_cast<T>(v.imag())};
+        }
+        HIP_CHECK(hipMemcpy(V_.template as<cx<T>>() + slab_v_[j] * npix_, hv.data(), npix_ * sizeof(cx<T>),
+                            hipMemcpyHostToDevice));
+      }
+    }
+    tabs_.alloc(hosts.size() * npix_ * sizeof(BornK<T>));
+    for (size_t i = 0; i < hosts.size(); ++i)
+      launch(k_born_table<T>, kx, ky, static_cast<int>(ny_), static_cast<int>(nx_), zd{hosts[i].real(), hosts[i].imag()},
+             k0_, dz_, tabs_.template as<BornK<T>>() + i * npix_);
+    {
+      std::vector<T> a(nx_), b(ny_);
+      for (long long i = 0; i < nx_; ++i) a[i] = static_cast<T>(fftfreq(i, nx_, s.dx));
+      for (long long j = 0; j < ny_; ++j) b[j] = static_cast<T>(fftfreq(j, ny_, s.dy));
+      kxT_.alloc(nx_ * sizeof(T));
+      kyT_.alloc(ny_ * sizeof(T));
+      HIP_CHECK(hipMemcpy(kxT_.template as<void>(), a.data(), nx_ * sizeof(T), hipMemcpyHostToDevice));
+      HIP_CHECK(hipMemcpy(kyT_.template as<void>(), b.data(), ny_ * sizeof(T), hipMemcpyHostToDevice));
+    }
+    const size_t pb = npix_ * sizeof(px2<T>);
+    for (auto* b : {&ehat_, &xh_, &xy0_, &xy1_, &zz_, &rhs_, &x_, &w_, &tmp_, &inc_}) b->alloc(pb);
+    basis_.alloc((kRestart + 1) * pb);
+    // Born planes: born_bwd[p + 1] (R1) and born_fwd[p + 1] (R3) of each Born slab p
+    planes_.alloc(2 * static_cast<size_t>(std::max<long long>(nborn, 1)) * pb);
+    partial_.alloc(static_cast<size_t>(kRestart + 2) * kRedBlocks * sizeof(double2));
+    dres_.alloc((kRestart + 2) * sizeof(double2));
+    coef_.alloc((kRestart + 2) * sizeof(double2));
+    ptrs_.alloc(2 * (kRestart + 2) * sizeof(void*));
+    HIP_CHECK(hipHostMalloc(&hres_, (kRestart + 2) * sizeof(double2), hipHostMallocDefault));
+    choose_fft_geometry();
+    HIP_CHECK(hipStreamSynchronize(stream_));
+    bytes_ = V_.bytes() + tabs_.bytes() + 10 * pb + basis_.bytes() + planes_.bytes();
   }
-  void forward(const px2<T>*, long long, bool) {}
-  void backward(const px2<T>*, long long) {}
-  const px2<T>* increment() const { return nullptr; }
-  const px2<T>* fwd_plane(long long) const { return nullptr; }
-  const px2<T>* bwd_plane(long long) const { return nullptr; }
-  long long iterations() const { return 0; }
-  size_t device_bytes() const { return 0; }
+  ~Born() {
+    if (hres_) (void)hipHostFree(hres_);
+  }
+  Born(const Born&) = delete;
+  Born& operator=(const Born&) = delete;
+
+  const px2<T>* increment() const { return inc_.template as<px2<T>>(); }
+  // born_bwd[p]: the back scattering R1 of slab p − 1 (nullptr if that slab has no host)
+  const px2<T>* bwd_plane(long long p) const { return plane(p - 1, 0); }
+  // born_fwd[p]: the forward scattering R3 of slab p − 1
+  const px2<T>* fwd_plane(long long p) const { return plane(p - 1, 1); }
+  long long iterations() const { return iterations_; }
+  size_t device_bytes() const { return bytes_; }
+
+  // Forward step p: increment of slab p into increment(); with `bidir` also R1 → bwd_plane(p + 1).
+  void forward(const px2<T>* e, long long p, bool bidir) {
+    fft2(e, ehat_.template as<px2<T>>(), false);
+    step(+1, p);
+    if (bidir) emission(+1, -1, p, const_cast<px2<T>*>(plane(p, 0)));
+  }
+  // Backward step at slab: R3 → fwd_plane(slab + 1), then the R2 increment into increment().
+  void backward(const px2<T>* e, long long slab) {
+    fft2(e, ehat_.template as<px2<T>>(), false);
+    emission(-1, +1, slab, const_cast<px2<T>*>(plane(slab, 1)));
+    step(-1, slab);
+  }
+
+ private:
+  static constexpr int kRestart = 30;      // solver.cpp:341
+  static constexpr int kMaxRestarts = 100;
+
+  const px2<T>* plane(long long slab, int which) const {
+    if (slab < 0 || slab >= nz_ || slab_v_[slab] < 0) return nullptr;
+    return planes_.template as<px2<T>>() + (2 * slab_v_[slab] + which) * npix_;
+  }
+  const BornK<T>* table(long long slab) const { return tabs_.template as<BornK<T>>() + slab_tab_[slab] * npix_; }
+  const cx<T>* potential(long long slab) const { return V_.template as<cx<T>>() + slab_v_[slab] * npix_; }
+  cx<T> inv_k2(long long slab) const {
+    const c128 k = hosts_[slab_tab_[slab]] * k0_;
+    const c128 v = 1.0 / (k * k);
+    return cx<T>{static_cast<T>(v.real()), static_cast<T>(v.imag())};
+  }
+  px2<T>* B(int i) const { return basis_.template as<px2<T>>() + static_cast<long long>(i) * npix_; }
+
+  template <typename K, typename... A>
+  void launch(K kern, A... args) {
+    const unsigned grid = static_cast<unsigned>((npix_ + 255) / 256);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, stream_, args...);
+    HIP_CHECK(hipGetLastError());
+  }
+
+  // ---- 2D FFTs of up to kMaxPlanes planes -----------------------------------------------------------
+  void choose_fft_geometry() {
+    auto ls_of = [](int lines) { return lines <= 2 ? lines : (lines | 1); };
+    const int elems = Budget<T>::elems;
+    rows_fast_ = fast_n(nx_) && COMP == (sizeof(T) == 4);
+    cols_fast_ = fast_n(ny_) && COMP == (sizeof(T) == 4);
+    int rw = static_cast<int>(std::min<long long>(ny_, 64));
+    while (rw > 1 && nx_ * ls_of(2 * rw) > elems) rw /= 2;
+    rows_ = rw;
+    ls_row_ = ls_of(2 * rw);
+    int w = 16;
+    while (w > 1 && ny_ * ls_of(2 * w) > elems) w /= 2;
+    while (w > 1 && w > nx_) w /= 2;
+    cols_ = w;
+    ls_col_ = ls_of(2 * w);
+    require(nx_ * ls_row_ <= elems && ny_ * ls_col_ <= elems, "grid too large for the Born transforms");
+    HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fft2_rows<T, COMP, false>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+    HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fft2_rows<T, COMP, true>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+    HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fft2_cols<T, COMP, false>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+    HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fft2_cols<T, COMP, true>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+  }
+  static bool fast_n(long long n) {
+    return n == 64 || n == 128 || n == 256 || n == 512 || n == 1024 || n == 2048;
+  }
+
+  template <bool INV>
+  void rows_pass(const PlaneSet<T>& ps, int np) {
+    if (rows_fast_) {
+      switch (nx_) {
+        case 64: rows_fast_n<64, INV>(ps, np); return;
+        case 128: rows_fast_n<128, INV>(ps, np); return;
+        case 256: rows_fast_n<256, INV>(ps, np); return;
+        case 512: rows_fast_n<512, INV>(ps, np); return;
+        case 1024: rows_fast_n<1024, INV>(ps, np); return;
+        default: rows_fast_n<2048, INV>(ps, np); return;
+      }
+    }
+    const size_t lds = static_cast<size_t>(nx_) * ls_row_ * sizeof(cx<T>);
+    hipLaunchKernelGGL((k_fft2_rows<T, COMP, INV>), dim3(static_cast<unsigned>((ny_ + rows_ - 1) / rows_), np),
+                       dim3(kThreads), lds, stream_, ps, static_cast<int>(ny_), static_cast<int>(nx_), rows_, ls_row_, px_);
+    HIP_CHECK(hipGetLastError());
+  }
+  template <int N, bool INV>
+  void rows_fast_n(const PlaneSet<T>& ps, int np) {
+    constexpr int RW = (sizeof(T) == 4 ? 512 : 256) / (N / 8) > 0 ? (sizeof(T) == 4 ? 512 : 256) / (N / 8) : 1;
+    const auto* th = static_cast<const cx<T>*>(px_.tw_hi);
+    const auto* tl = static_cast<const cx<T>*>(px_.tw_lo);
+    if (ny_ % RW == 0)
+      hipLaunchKernelGGL((k_fft2_rows_fast<N, RW, T, COMP, INV>), dim3(static_cast<unsigned>(ny_ / RW), np),
+                         dim3(N / 8 * RW), 0, stream_, ps, th, tl);
+    else
+      hipLaunchKernelGGL((k_fft2_rows_fast<N, 1, T, COMP, INV>), dim3(static_cast<unsigned>(ny_), np), dim3(N / 8), 0,
+                         stream_, ps, th, tl);
+    HIP_CHECK(hipGetLastError());
+  }
+  template <bool INV>
+  void cols_pass(const PlaneSet<T>& ps, int np) {
+    if (cols_fast_) {
+      switch (ny_) {
+        case 64: cols_fast_n<64, INV>(ps, np); return;
+        case 128: cols_fast_n<128, INV>(ps, np); return;
+        case 256: cols_fast_n<256, INV>(ps, np); return;
+        case 512: cols_fast_n<512, INV>(ps, np); return;
+        case 1024: cols_fast_n<1024, INV>(ps, np); return;
+        default: cols_fast_n<2048, INV>(ps, np); return;
+      }
+    }
+    const size_t lds = static_cast<size_t>(ny_) * ls_col_ * sizeof(cx<T>);
+    hipLaunchKernelGGL((k_fft2_cols<T, COMP, INV>), dim3(static_cast<unsigned>((nx_ + cols_ - 1) / cols_), np),
+                       dim3(kThreads), lds, stream_, ps, static_cast<int>(ny_), static_cast<int>(nx_), cols_, ls_col_, py_);
+    HIP_CHECK(hipGetLastError());
+  }
+  template <int N, bool INV>
+  void cols_fast_n(const PlaneSet<T>& ps, int np) {
+    constexpr int W0 = (sizeof(T) == 4 ? 512 : 256) / (N / 8);
+    constexpr int W = W0 > 16 ? 16 : (W0 < 1 ? 1 : W0);
+    const auto* th = static_cast<const cx<T>*>(py_.tw_hi);
+    const auto* tl = static_cast<const cx<T>*>(py_.tw_lo);
+    if (nx_ % W == 0)
+      hipLaunchKernelGGL((k_fft2_cols_fast<N, W, T, COMP, INV>), dim3(static_cast<unsigned>(nx_ / W), np),
+                         dim3(N / 8 * W), 0, stream_, ps, static_cast<int>(nx_), th, tl);
+    else
+      hipLaunchKernelGGL((k_fft2_cols_fast<N, 1, T, COMP, INV>), dim3(static_cast<unsigned>(nx_), np), dim3(N / 8), 0,
+                         stream_, ps, static_cast<int>(nx_), th, tl);
+    HIP_CHECK(hipGetLastError());
+  }
+
+  // In-place 2D transform of np planes; inverse scaled by 1/N and optionally multiplied by `mul`.
+  void fft2_planes(px2<T>* const* planes, int np, bool inverse, const cx<T>* mul) {
+    PlaneSet<T> a{}, b{};
+    for (int i = 0; i < np; ++i) {
+      a.in[i] = b.in[i] = planes[i];
+      a.out[i] = b.out[i] = planes[i];
+      b.mul[i] = mul;
+    }
+    a.scale = 1;
+    b.scale = inverse ? static_cast<T>(1.0 / static_cast<double>(npix_)) : static_cast<T>(1);
+    if (inverse) {
+      rows_pass<true>(a, np);
+      cols_pass<true>(b, np);
+    } else {
+      rows_pass<false>(a, np);
+      cols_pass<false>(b, np);
+    }
+  }
+  void fft2(const px2<T>* in, px2<T>* out, bool inverse) {
+    PlaneSet<T> a{}, b{};
+    a.in[0] = in;
+    a.out[0] = out;
+    b.in[0] = out;
+    b.out[0] = out;
+    a.scale = 1;
+    b.scale = inverse ? static_cast<T>(1.0 / static_cast<double>(npix_)) : static_cast<T>(1);
+    if (inverse) {
+      rows_pass<true>(a, 1);
+      cols_pass<true>(b, 1);
+    } else {
+      rows_pass<false>(a, 1);
+      cols_pass<false>(b, 1);
+    }
+  }
+
+  // ---- the operators --------------------------------------------------------------------------------
+
+  // out = G_d·v (mode 0) or (1 − G_d/2)·v (mode 1) (solver.cpp:262-311).
+  void apply_g(const px2<T>* v, int d, long long slab, px2<T>* out, int mode) {
+    const BornK<T>* tb = table(slab);
+    px2<T>* xy = xy0_.template as<px2<T>>();
+    px2<T>* zz = zz_.template as<px2<T>>();
+    launch(k_born_prepare_op<T>, v, tb, kxT_.template as<T>(), kyT_.template as<T>(), static_cast<int>(nx_), npix_,
+           static_cast<T>(d), xy, zz);
+    px2<T>* planes[2] = {xy, zz};
+    fft2_planes(planes, 2, true, potential(slab));
+    fft2_planes(planes, 2, false, nullptr);
+    launch(k_born_finish_op<T>, xy, zz, tb, kxT_.template as<T>(), kyT_.template as<T>(), static_cast<int>(nx_), npix_,
+           static_cast<T>(d), inv_k2(slab), v, out, mode);
+  }
+
+  // R1 / R3: sources of direction d from Ê (in ehat_), emitted into direction dp, real space.
+  void emission(int d, int dp, long long slab, px2<T>* out) {
+    const BornK<T>* tb = table(slab);
+    px2<T>* x0 = xy0_.template as<px2<T>>();
+    px2<T>* x1 = xy1_.template as<px2<T>>();
+    px2<T>* zz = zz_.template as<px2<T>>();
+    launch(k_born_src_prepare<T>, ehat_.template as<px2<T>>(), tb, kxT_.template as<T>(), kyT_.template as<T>(),
+           static_cast<int>(nx_), npix_, static_cast<T>(d), d > 0 ? 0 : 1, x0, x1, zz);
+    px2<T>* planes[3] = {x0, x1, zz};
+    fft2_planes(planes, 3, true, potential(slab));
+    fft2_planes(planes, 3, false, nullptr);
+    px2<T>* tot = tmp_.template as<px2<T>>();
+    launch(k_born_emit<T>, x0, x1, zz, tb, kxT_.template as<T>(), kyT_.template as<T>(), static_cast<int>(nx_), npix_,
+           static_cast<T>(dp), dp > 0 ? 0 : 1, inv_k2(slab), tot);
+    fft2(tot, out, true);
+  }
+
+  // Cayley increment P_½(1 − G/2)⁻¹G·P_½Ê, real space, into inc_ (solver.cpp:313-350).
+  void step(int d, long long slab) {
+    const BornK<T>* tb = table(slab);
+    px2<T>* xh = xh_.template as<px2<T>>();
+    launch(k_born_half<T>, ehat_.template as<px2<T>>(), tb, npix_, 0, xh);
+    px2<T>* b = rhs_.template as<px2<T>>();
+    apply_g(xh, d, slab, b, 0);
+    px2<T>* x = x_.template as<px2<T>>();
+    HIP_CHECK(hipMemcpyAsync(x, b, npix_ * sizeof(px2<T>), hipMemcpyDeviceToDevice, stream_));
+    if (!gmres(d, slab))
+      throw Error(kEnoconv, "GMRES did not converge in the Born step");
+    px2<T>* y = tmp_.template as<px2<T>>();
+    launch(k_born_half<T>, x, tb, npix_, d > 0 ? 1 : 2, y);
+    fft2(y, inc_.template as<px2<T>>(), true);
+  }
+
+  // ---- GMRES ---------------------------------------------------------------------------------------
+
+  // dots[i] = Σ conj(us[i])·v for i < k (one fused reduction, one host synchronisation).
+  void dots(const px2<T>* const* us, int k, const px2<T>* v, std::complex<double>* res) {
+    const void** hp = host_ptrs_;
+    for (int i = 0; i < k; ++i) hp[i] = us[i];
+    HIP_CHECK(hipMemcpyAsync(ptrs_.template as<void>(), hp, k * sizeof(void*), hipMemcpyHostToDevice, stream_));
+    hipLaunchKernelGGL(k_dot_partial<T>, dim3(kRedBlocks, k), dim3(kRedThreads), 0, stream_,
+                       ptrs_.template as<const px2<T>* const>(), v, npix_, partial_.template as<double2>());
+    HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_dot_final, dim3(k), dim3(kRedThreads), 0, stream_, partial_.template as<double2>(),
+                       dres_.template as<double2>());
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipMemcpyAsync(hres_, dres_.template as<void>(), k * sizeof(double2), hipMemcpyDeviceToHost, stream_));
+    HIP_CHECK(hipStreamSynchronize(stream_));
+    for (int i = 0; i < k; ++i) res[i] = std::complex<double>(hres_[i].x, hres_[i].y);
+  }
+  double norm(const px2<T>* v) {
+    std::complex<double> r;
+    dots(&v, 1, v, &r);
+    return std::sqrt(r.real());
+  }
+  // w += Σ c_i·vs[i]
+  void axpy(px2<T>* w, const px2<T>* const* vs, const std::complex<double>* c, int k) {
+    const void** hp = host_ptrs_ + (kRestart + 2);
+    double2* hc = host_coef_;
+    for (int i = 0; i < k; ++i) {
+      hp[i] = vs[i];
+      hc[i] = double2{c[i].real(), c[i].imag()};
+    }
+    void** dptr = ptrs_.template as<void*>() + (kRestart + 2);
+    HIP_CHECK(hipMemcpyAsync(dptr, hp, k * sizeof(void*), hipMemcpyHostToDevice, stream_));
+    HIP_CHECK(hipMemcpyAsync(coef_.template as<void>(), hc, k * sizeof(double2), hipMemcpyHostToDevice, stream_));
+    launch(k_axpy_multi<T>, w, reinterpret_cast<const px2<T>* const*>(dptr), coef_.template as<double2>(), k, npix_);
+    // the host staging arrays are reused by the next call: wait for the copies to leave them
+    HIP_CHECK(hipStreamSynchronize(stream_));
+  }
+
+  // Restarted GMRES on (1 − G/2)x = b with b = rhs_, x = x_ (start value), following the structure
+  // of the reference's solver (solver.cpp:130-207) with block Gram–Schmidt.
+  bool gmres(int d, long long slab) {
+    const px2<T>* b = rhs_.template as<px2<T>>();
+    px2<T>* x = x_.template as<px2<T>>();
+    px2<T>* w = w_.template as<px2<T>>();
+    const double bnorm = norm(b);
+    if (bnorm == 0.0) {
+      launch(k_axpby<T>, static_cast<const px2<T>*>(x), static_cast<T>(0), static_cast<const px2<T>*>(nullptr),
+             static_cast<T>(0), x, npix_);
+      return true;
+    }
+    const int m = kRestart;
+    std::vector<std::complex<double>> h((m + 1) * m), g(m + 1), cs(m), sn(m), hv(m + 1), h2(m + 1);
+    for (int outer = 0; outer < kMaxRestarts; ++outer) {
+      apply_g(x, d, slab, w, 1);
+      launch(k_axpby<T>, b, static_cast<T>(1), static_cast<const px2<T>*>(w), static_cast<T>(-1), B(0), npix_);
+      const double beta = norm(B(0));
+      if (beta <= tol_ * bnorm) return true;
+      launch(k_axpby<T>, static_cast<const px2<T>*>(B(0)), static_cast<T>(1.0 / beta),
+             static_cast<const px2<T>*>(nullptr), static_cast<T>(0), B(0), npix_);
+      std::fill(g.begin(), g.end(), std::complex<double>(0.0, 0.0));
+      g[0] = beta;
+      int used = 0;
+      bool done = false;
+      std::vector<const px2<T>*> vs(m + 1);
+      for (int j = 0; j < m; ++j) {
+        apply_g(B(j), d, slab, w, 1);
+        ++iterations_;
+        for (int i = 0; i <= j; ++i) vs[i] = B(i);
+        // classical Gram–Schmidt, twice
+        dots(vs.data(), j + 1, w, hv.data());
+        for (int i = 0; i <= j; ++i) h2[i] = -hv[i];
+        axpy(w, vs.data(), h2.data(), j + 1);
+        dots(vs.data(), j + 1, w, h2.data());
+        for (int i = 0; i <= j; ++i) {
+          hv[i] += h2[i];
+          h2[i] = -h2[i];
+        }
+        axpy(w, vs.data(), h2.data(), j + 1);
+        for (int i = 0; i <= j; ++i) h[i * m + j] = hv[i];
+        const double hnext = norm(w);
+        if (hnext > 0.0)
+          launch(k_axpby<T>, static_cast<const px2<T>*>(w), static_cast<T>(1.0 / hnext),
+                 static_cast<const px2<T>*>(nullptr), static_cast<T>(0), B(j + 1), npix_);
+        for (int i = 0; i < j; ++i) {  // previous rotations on column j
+          const std::complex<double> a = h[i * m + j], c = h[(i + 1) * m + j];
+          h[i * m + j] = cs[i] * a + sn[i] * c;
+          h[(i + 1) * m + j] = -std::conj(sn[i]) * a + cs[i] * c;
+        }
+        const std::complex<double> a = h[j * m + j];
+        const double r = std::sqrt(std::norm(a) + hnext * hnext);
+        if (std::abs(a) == 0.0) {
+          cs[j] = 0.0;
+          sn[j] = 1.0;
+        } else {
+          cs[j] = std::abs(a) / r;
+          sn[j] = (a / std::abs(a)) * hnext / r;
+        }
+        h[j * m + j] = cs[j] * a + sn[j] * hnext;
+        g[j + 1] = -std::conj(sn[j]) * g[j];
+        g[j] = cs[j] * g[j];
+        used = j + 1;
+        if (std::abs(g[j + 1]) <= tol_ * bnorm || hnext == 0.0) {
+          done = true;
+          break;
+        }
+      }
+      std::vector<std::complex<double>> y(used);
+      for (int i = used - 1; i >= 0; --i) {
+        std::complex<double> s = g[i];
+        for (int k = i + 1; k < used; ++k) s -= h[i * m + k] * y[k];
+        y[i] = s / h[i * m + i];
+      }
+      for (int i = 0; i < used; ++i) vs[i] = B(i);
+      axpy(x, vs.data(), y.data(), used);
+      if (done) {
+        apply_g(x, d, slab, w, 1);
+        launch(k_axpby<T>, b, static_cast<T>(1), static_cast<const px2<T>*>(w), static_cast<T>(-1), w, npix_);
+        if (norm(w) <= tol_ * bnorm * 10.0) return true;
+      }
+    }
+    return false;
+  }
+
+  struct Dev {
+    void* p = nullptr;
+    size_t n = 0;
+    void alloc(size_t bytes) {
+      if (p) (void)hipFree(p);
+      p = nullptr;
+      n = 0;
+      const hipError_t e = hipMalloc(&p, bytes);
+      if (e != hipSuccess) throw Error(kEnomem, std::string("Born buffer allocation failed: ") + hipGetErrorString(e));
+      n = bytes;
+    }
+    ~Dev() {
+      if (p) (void)hipFree(p);
+    }
+    template <typename U>
+    U* as() const { return static_cast<U*>(p); }
+    size_t bytes() const { return n; }
+  };
+
+  long long ny_, nx_, nz_, npix_;
+  double k0_, dz_;
+  FftPlan px_, py_;
+  hipStream_t stream_;
+  double tol_ = 1e-12;
+  std::vector<long long> slab_v_;
+  std::vector<int> slab_tab_;
+  std::vector<c128> hosts_;
+  Dev V_, tabs_, kxT_, kyT_;
+  Dev ehat_, xh_, xy0_, xy1_, zz_, rhs_, x_, w_, tmp_, inc_, basis_, planes_;
+  Dev partial_, dres_, coef_, ptrs_;
+  double2* hres_ = nullptr;
+  const void* host_ptrs_[2 * (kRestart + 2)] = {};
+  double2 host_coef_[kRestart + 2] = {};
+  bool rows_fast_ = false, cols_fast_ = false;
+  int rows_ = 1, ls_row_ = 2, cols_ = 1, ls_col_ = 2;
+  long long iterations_ = 0;
+  size_t bytes_ = 0;
 };
 
 }  // namespace bwpm

@@ -667,7 +667,8 @@ class Engine final : public EngineBase {
       mask_ = upload_vec(m, stream_);
     }
     if (std::any_of(has_host_.begin(), has_host_.end(), [](char c) { return c != 0; }))
-      born_ = std::make_unique<Born<T, COMP>>(s, has_host_, host_, kx_.as<double>(), ky_.as<double>(), stream_);
+      born_ = std::make_unique<Born<T, COMP>>(s, has_host_, host_, kx_.as<double>(), ky_.as<double>(), twx_.plan,
+                                              twy_.plan, stream_);
     HIP_CHECK(hipStreamSynchronize(stream_));
     bytes_ = stage_.bytes() + field_in_.bytes() + X_.bytes() + prod_.bytes() + e_.bytes() + fwd_vals_.bytes() +
              bwd_vals_.bytes() + mask_.bytes() + ops_bytes_ + tsel_.bytes() + sp_pix_.bytes() + sp_rid_.bytes() +

@@ -188,3 +188,128 @@ def test_random_volumes_equal_oracle(seed: int, precision: str) -> None:
             assert np.max(np.abs(got - want)) < 1e-10 * scale
         else:
             assert rel_l2(got, want) < L2_FP32
+
+
+# ---- Born scattering at index fluctuations -------------------------------------------------------
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_fpwpm2d_fluctuation(precision: str) -> None:
+    d = load_golden("fpwpm2d_fluctuation")
+    a = fixture_2d_args(d)
+    nz = a["labels"].shape[0]
+    out = _solver(a, precision).run(a["field"], 0, list(range(nz + 1)))
+    planes = np.stack([out["forward"][p][:, 0] for p in range(nz + 1)])
+    if precision == "fp64":
+        _close64(planes, d["expected"]["planes"])
+    else:
+        assert rel_l2(planes, d["expected"]["planes"]) < L2_FP32
+    assert out["gmres_iterations"] > 0
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_bwpm3d_fluctuation(precision: str) -> None:
+    d = load_golden("bwpm3d_fluctuation")
+    a = fixture_3d_args(d)
+    nz = a["labels"].shape[0]
+    keep = (0, 8, nz)
+    out = _solver(a, precision).run(a["field"], 1, keep)
+    e = d["expected"]
+    pairs = [(out["transmitted"], e["transmitted"]), (out["reflected"], e["reflected"]),
+             (np.stack([out["forward"][q] for q in keep]), e["forward_planes"]),
+             (np.stack([out["backward"][q] for q in keep]), e["backward_planes"])]
+    for got, want in pairs:
+        if precision == "fp64":
+            _close64(got, want)
+        else:
+            assert rel_l2(got, want) < L2_FP32
+
+
+def _fluct_volume(seed: int, back: bool):
+    from helpers import k0_of, mask_2d
+
+    rng = np.random.default_rng(seed)
+    if back:  # tests/test_core_solver.py:224-257 construction
+        nx, ny, nz = (12, 10, 14) if seed % 2 else (8, 16, 11)
+        dx, dz, k0 = 0.05e-3, 0.04e-3, k0_of(0.8)
+        labels = np.zeros((nz, ny, nx), dtype=np.int32)
+        labels[nz - 3:] = 1
+        fl = np.zeros((nz, ny, nx), complex)
+        fl[1:nz - 4] = 0.02 * rng.normal(size=(nz - 5, ny, nx)) + 1e-3j
+        field = rng.normal(size=(ny, nx, 2)) + 1j * rng.normal(size=(ny, nx, 2))
+        mask = mask_2d(ny, nx, dx, dx, 2 * dx) if seed == 4 else None
+        return dict(ny=ny, nx=nx, dy=dx, dx=dx, dz=dz, k0=k0, indices=[1.38, 1.5 + 0.002j], labels=labels,
+                    index_before=1.38, mask=mask, fluctuation=fl, field=field), 3
+    # tests/test_core_solver.py:176-203 construction: a mixed slab without δn, grazing points
+    nx, ny, nz = (15, 12, 9) if seed % 2 else (8, 16, 11)
+    dx, dy, dz, k0 = 0.04e-3, 0.05e-3, 0.03e-3, k0_of(0.6)
+    labels = rng.integers(0, 2, size=(nz, 1, 1)) * np.ones((1, ny, nx), dtype=np.int64)
+    labels[nz // 2] = rng.integers(0, 3, size=(ny, nx))
+    fl = 0.03 * rng.normal(size=(nz, ny, nx)) + 0.002j * rng.normal(size=(nz, ny, nx))
+    fl[nz // 2] = 0.0
+    field = rng.normal(size=(ny, nx, 2)) + 1j * rng.normal(size=(ny, nx, 2))
+    mask = mask_2d(ny, nx, dy, dx, 2 * dx) if seed == 2 else None
+    return dict(ny=ny, nx=nx, dy=dy, dx=dx, dz=dz, k0=k0, indices=[1.33, 1.5, 2.1 + 0.01j], labels=labels,
+                index_before=1.33, mask=mask, fluctuation=fl, field=field), 1
+
+
+@pytest.mark.parametrize(("seed", "back"), [(1, False), (2, False), (3, True), (4, True)])
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_fluctuations_equal_oracle(seed: int, back: bool, precision: str) -> None:
+    a, its = _fluct_volume(seed, back)
+    nz = a["labels"].shape[0]
+    keep = (0, 5, nz)
+    ref = O.Solver(a["ny"], a["nx"], a["dy"], a["dx"], a["dz"], a["k0"], a["indices"], a["labels"],
+                   a["index_before"], a["mask"], fluctuation=a["fluctuation"]).run(a["field"], its, keep)
+    out = _solver(a, precision).run(a["field"], its, keep)
+    pairs = [(out["transmitted"], ref["transmitted"]), (out["reflected"], ref["reflected"])]
+    for q in keep:
+        pairs += [(out["forward"][q], ref["forward"][q]), (out["backward"][q], ref["backward"][q])]
+    scale = float(np.max(np.abs(a["field"])))
+    for got, want in pairs:
+        if precision == "fp64":
+            assert np.max(np.abs(got - want)) < 1e-10 * scale
+        else:
+            assert rel_l2(got, want) < L2_FP32
+
+
+def test_born_step_conserves_flux_and_absorbs_exactly() -> None:
+    """K20 invariants on the GPU (tests/test_stable_born.py:66-121): strong real δn conserves the
+    flux; uniform δn = iκ attenuates as e^{-2k0κL}."""
+    from helpers import k0_of
+
+    def flux(f, n0, k0, d):
+        n = f.shape[0]
+        kx = O.fftfreq(n, d)[None, :]
+        ky = O.fftfreq(n, d)[:, None]
+        k2 = (n0 * k0) ** 2
+        prop = kx ** 2 + ky ** 2 < k2 * (1 - 1e-12)
+        kz = np.sqrt(np.where(prop, k2 - kx ** 2 - ky ** 2, 1.0))
+        sp = np.fft.fft2(f, axes=(0, 1))
+        ez = -(kx * sp[..., 0] + ky * sp[..., 1]) / kz
+        return float(np.sum(np.where(prop, kz * (np.sum(np.abs(sp) ** 2, -1) + np.abs(ez) ** 2), 0.0)))
+
+    k0 = k0_of(0.8)
+    n, nz, h = 24, 12, 0.1e-3
+    f = np.zeros((n, n, 2), complex)
+    f[..., 0] = 1.0
+    delta = 0.1 * np.random.default_rng(7).normal(size=(nz, n, n))
+    s = _core().GpuSolver(n, n, h, h, h, k0, [1.38], np.zeros((nz, n, n), np.int32), 1.38, fluctuation=delta)
+    out = s.run(f, 0)["transmitted"]
+    assert abs(flux(out, 1.38, k0, h) / flux(f, 1.38, k0, h) - 1) < 1e-8
+    n, nz, h, kappa = 8, 100, 0.05e-3, 1e-3
+    f = np.zeros((n, n, 2), complex)
+    f[..., 0] = 1.0
+    s = _core().GpuSolver(n, n, h, h, h, k0, [1.38], np.zeros((nz, n, n), np.int32), 1.38,
+                          fluctuation=np.full((nz, n, n), 1j * kappa))
+    out = s.run(f, 0)["transmitted"]
+    expected = np.exp(-2 * k0 * kappa * nz * h)
+    assert abs(flux(out, 1.38, k0, h) / flux(f, 1.38, k0, h) / expected - 1) < 1e-6
+
+
+def test_two_hosts_in_a_slab_is_a_value_error() -> None:
+    labels = np.zeros((2, 4, 4), np.int32)
+    labels[0, :, :2] = 1
+    fl = np.zeros((2, 4, 4))
+    fl[0] = 0.01
+    with pytest.raises(ValueError, match="host"):
+        _core().GpuSolver(4, 4, 1e-4, 1e-4, 1e-4, 1e4, [1.0, 1.5], labels, 1.0, fluctuation=fl)
