@@ -5,7 +5,8 @@ Workload (default): the tilted-interface 3D vector geometry of the reference's p
 (λ = 1.55 µm, dx = dy = λ/6, dz = λ/4, interface 1.0 → 1.46 tilted 10°, pixel-centre labels,
 Gaussian w₀ = 8 µm, 4 bidirectional iterations) at 512² × 1024 slabs, FP32 bulk. One step is one
 batch of `--batch` independent solves (E_x and E_y polarised alternately) run concurrently on the
-GPU with fields and state resident in HBM; the timed region covers `--steps` such batches.
+GPU as one lane (every slab step is one column launch and one row launch for all solves of the
+batch) with fields and state resident in HBM; the timed region covers `--steps` such batches.
 
 Printed: one JSON line with grid-point·slab updates/s (whole job), the roofline of the slab-step
 kernels (HIP-event durations over a measured solve, algorithmic 32 B per update), and the
@@ -21,7 +22,6 @@ import json
 import os
 import sys
 import time
-from concurrent.futures import ThreadPoolExecutor
 from pathlib import Path
 
 import numpy as np
@@ -68,7 +68,7 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--batch", type=int, default=4)
+    ap.add_argument("--batch", type=int, default=8)
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--nz", type=int, default=1024)
     ap.add_argument("--iterations", type=int, default=4)
@@ -91,10 +91,10 @@ def main() -> int:
         s = _core.GpuSolver(*p.solver_args(), precision=args.precision)
         s.upload(p.field)
         solvers.append(s)
-    pool = ThreadPoolExecutor(max_workers=len(solvers))
 
     def one_step() -> None:
-        list(pool.map(lambda s: s.run_resident(args.iterations), solvers))
+        # one lane: every slab step is one column and one row launch for all solves of the batch
+        _core.run_batch(solvers, args.iterations, max_lane=args.batch)
 
     for _ in range(args.warmup):
         one_step()
@@ -106,15 +106,15 @@ def main() -> int:
     value = updates / elapsed
     ms_per_step = 1e3 * elapsed / max(args.steps, 1)
 
-    # kernel durations of one solve, HIP events on the solve's stream
-    kt = solvers[0].run_resident(args.iterations, time_kernels=True)
-    npix = args.n * args.n
+    # kernel durations inside the lane: HIP events around every launch on the lane's stream
+    kt = _core.run_batch(solvers, args.iterations, max_lane=args.batch, time_kernels=True)
+    npix = args.n * args.n * args.batch  # pixels one launch processes
     col_us = 1e3 * kt["col_ms"] / max(kt["col_launches"], 1)
     row_us = 1e3 * kt["row_ms"] / max(kt["row_launches"], 1)
     step_us = col_us + row_us
     achieved = BYTES_PER_UPDATE * npix / (step_us * 1e-6) / 1e9
-    copy_bytes = npix * 16
-    copy_ms = _core.copy_probe(copy_bytes, 50)
+    copy_bytes = 1 << 30
+    copy_ms = _core.copy_probe(copy_bytes, 10)
     roofline = {
         "bound": "hbm",
         "kernel": "k_col_step+k_row_step (one slab step)",
@@ -147,7 +147,7 @@ def main() -> int:
         "data": "synthetic",
         "config": {
             "workload": f"tilted-interface 3D vector BWPM {args.n}x{args.n}x{args.nz} slabs, "
-                        f"{args.iterations} iterations, batch of {args.batch} concurrent solves",
+                        f"{args.iterations} iterations, batch of {args.batch} solves in one lane",
             "n": args.n, "nz": args.nz, "iterations": args.iterations, "batch": args.batch,
             "precision": args.precision, "updates_per_solve": probs[0].updates,
         },

@@ -48,8 +48,23 @@ _device_count = _sig("bwpm_device_count", ctypes.POINTER(_int))
 _tmat = _sig("bwpm_transmission_matrix", _p, _p, _i64, _d, _d, _d, _d, _d, _p)
 _rmat = _sig("bwpm_reflection_matrix", _p, _p, _i64, _d, _d, _d, _d, _d, _p)
 _kz = _sig("bwpm_longitudinal_wavenumber", _d, _d, _d, _p, _i64, _p)
-_create = _sig("bwpm_solver_create", _i64, _i64, _d, _d, _d, _d, _p, _i64, _p, _i64, _d, _d, _p, _p, _int,
-               _int, ctypes.POINTER(_p))
+_create = _sig("bwpm_solver_create_ex", _i64, _i64, _d, _d, _d, _d, _p, _i64, _p, _i64, _d, _d, _p, _p, _p,
+               _int, _int, ctypes.POINTER(_p))
+_fresnel = _sig("bwpm_fresnel_coefficients", _p, _p, _i64, _d, _d, _d, _d, _d, _p)
+_flux = _sig("bwpm_power_flux", _p, _i64, _i64, _d, _d, _d, _d, _d, _int, _int, ctypes.POINTER(_d))
+
+
+def power_flux(field: Any, dy: float, dx: float, k0: float, index: complex, direction: int = 1,
+               device: int = 0) -> float:
+    """Poynting flux of a (ny, nx, 2) field in a homogeneous medium (device FP64 reduction)."""
+    f = np.ascontiguousarray(np.asarray(field, dtype=np.complex128))
+    if f.ndim != 3 or f.shape[2] != 2:
+        raise ValueError("field must have shape (ny, nx, 2)")
+    n = complex(index)
+    out = _d(0)
+    _check(_flux(_ptr(f), f.shape[0], f.shape[1], float(dy), float(dx), float(k0), n.real, n.imag, int(direction),
+                 int(device), ctypes.byref(out)))
+    return float(out.value)
 _destroy = _sig("bwpm_solver_destroy", _p)
 _run = _sig("bwpm_solver_run", _p, _p, _int, _p, _i64, ctypes.POINTER(_p))
 _res_field = _sig("bwpm_result_field", _p, _int, _p)
@@ -65,7 +80,8 @@ _run_resident = _sig("bwpm_solver_run_resident", _p, _int, _int, ctypes.POINTER(
 _dev_bytes = _sig("bwpm_solver_device_bytes", _p, ctypes.POINTER(_i64))
 _copy_probe = _sig("bwpm_copy_probe", _i64, _int, _int, ctypes.POINTER(_d))
 _fetch = _sig("bwpm_solver_fetch", _p, _int, ctypes.POINTER(_p))
-_batch = _sig("bwpm_batch_run", _p, _i64, _int, _int, ctypes.POINTER(_d))
+_batch = _sig("bwpm_batch_run", _p, _i64, _int, _int, _int, ctypes.POINTER(_d), ctypes.POINTER(_d),
+               ctypes.POINTER(_d), ctypes.POINTER(_i64), ctypes.POINTER(_i64))
 
 LIBRARY = str(_LIB_PATH)
 
@@ -125,6 +141,17 @@ def reflection_matrix(k_x: Any, k_y: Any, index_in: complex, index_out: complex,
     return _matrices(_rmat, k_x, k_y, index_in, index_out, k0)
 
 
+def fresnel_coefficients(k_x: Any, k_y: Any, index_in: complex, index_out: complex, k0: float) -> np.ndarray:
+    """(r_s, r_p, t_s, t_p, k_z,in, k_z,out) stacked on the last axis, Convention A."""
+    kx, ky = _f64(k_x), _f64(k_y)
+    if kx.shape != ky.shape:
+        raise ValueError("k_x and k_y must have the same shape")
+    out = np.empty(kx.shape + (6,), dtype=np.complex128)
+    a, b = complex(index_in), complex(index_out)
+    _check(_fresnel(_ptr(kx), _ptr(ky), kx.size, a.real, a.imag, b.real, b.imag, float(k0), _ptr(out)))
+    return out
+
+
 def longitudinal_wavenumber(index: complex, k0: float, k_perp_sq: Any) -> Any:
     """k_z = sqrt(ñ²k₀² − k⊥²) with Im ≥ 0 and the grazing rule; scalar in → complex out."""
     kp2 = _f64(k_perp_sq)
@@ -144,7 +171,7 @@ class GpuSolver:
 
     def __init__(self, ny: int, nx: int, dy: float, dx: float, dz: float, k0: float, indices: Any,
                  labels: Any, index_before: complex, mask: Any = None, device: int = 0,
-                 fluctuation: Any = None, precision: str = "fp64") -> None:
+                 fluctuation: Any = None, precision: str = "fp64", labels_before: Any = None) -> None:
         self._h = _p(None)
         if precision not in PRECISIONS:
             raise ValueError(f"precision must be one of {sorted(PRECISIONS)}")
@@ -163,13 +190,19 @@ class GpuSolver:
                 raise ValueError("fluctuation must have the shape of labels")
         if lab.shape[1:] != (int(ny), int(nx)):
             raise ValueError("labels must have shape (nz, ny, nx)")
+        lb = None
+        if labels_before is not None:
+            lb = np.ascontiguousarray(np.asarray(labels_before), dtype=np.int32)
+            if lb.size != int(ny) * int(nx):
+                raise ValueError("labels_before must have shape (ny, nx)")
         ib = complex(index_before)
         self.ny, self.nx, self.nz = int(ny), int(nx), int(lab.shape[0])
         self.precision = precision
         self.device = int(device)
         _check(_create(int(ny), int(nx), float(dy), float(dx), float(dz), float(k0), _ptr(idx), idx.size,
                        _ptr(lab), lab.shape[0], ib.real, ib.imag, None if m is None else _ptr(m),
-                       None if f is None else _ptr(f), int(device), PRECISIONS[precision], ctypes.byref(self._h)))
+                       None if f is None else _ptr(f), None if lb is None else _ptr(lb), int(device),
+                       PRECISIONS[precision], ctypes.byref(self._h)))
 
     def close(self) -> None:
         if self._h and self._h.value:
@@ -270,13 +303,19 @@ class GpuSolver:
 CudaSolver = GpuSolver  # the reference's name for its GPU solver class
 
 
-def run_batch(solvers: list[GpuSolver], iterations: int, max_lane: int = 64) -> float:
-    """Run every solver (fields uploaded with `upload`) as one batch; returns the wall time in ms.
+def run_batch(solvers: list[GpuSolver], iterations: int, max_lane: int = 64, time_kernels: bool = False) -> Any:
+    """Run every solver (fields uploaded with `upload`) as one batch; returns the wall time in ms
+    (with `time_kernels`, a dict that adds the summed per-launch HIP-event durations).
 
     Solvers of one grid and precision share lanes of at most `max_lane` that step in lockstep
     (one column and one row launch per slab step for the whole lane); collect results with
     `fetch(iterations)`. Each result is bit-identical to the solver's run alone."""
     arr = (_p * len(solvers))(*[s._h.value for s in solvers])
-    ms = _d(0)
-    _check(_batch(arr, len(solvers), int(iterations), int(max_lane), ctypes.byref(ms)))
-    return float(ms.value)
+    ms, cms, rms = _d(0), _d(0), _d(0)
+    cl, rl = _i64(0), _i64(0)
+    _check(_batch(arr, len(solvers), int(iterations), int(max_lane), 1 if time_kernels else 0, ctypes.byref(ms),
+                  ctypes.byref(cms), ctypes.byref(rms), ctypes.byref(cl), ctypes.byref(rl)))
+    if not time_kernels:
+        return float(ms.value)
+    return {"wall_ms": ms.value, "col_ms": cms.value, "row_ms": rms.value, "col_launches": cl.value,
+            "row_launches": rl.value}

@@ -92,9 +92,29 @@ int bwpm_longitudinal_wavenumber(double n_re, double n_im, double k0, const doub
   });
 }
 
+int bwpm_fresnel_coefficients(const double* kx, const double* ky, int64_t n, double n_in_re, double n_in_im,
+                              double n_out_re, double n_out_im, double k0, double* out) {
+  return guarded([&] {
+    bwpm::require(n >= 0, "n must be >= 0");
+    bwpm::require(n == 0 || (kx && ky && out), "null array");
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    bwpm::fresnel_coefficients(kx, ky, n, c128(n_in_re, n_in_im), c128(n_out_re, n_out_im), k0,
+                               reinterpret_cast<c128*>(out), dev);
+  });
+}
+
 int bwpm_solver_create(int64_t ny, int64_t nx, double dy, double dx, double dz, double k0, const double* indices,
                        int64_t m, const int32_t* labels, int64_t nz, double ib_re, double ib_im, const double* mask,
                        const double* fluct, int device, int precision, bwpm_solver** out) {
+  return bwpm_solver_create_ex(ny, nx, dy, dx, dz, k0, indices, m, labels, nz, ib_re, ib_im, mask, fluct, nullptr,
+                               device, precision, out);
+}
+
+int bwpm_solver_create_ex(int64_t ny, int64_t nx, double dy, double dx, double dz, double k0, const double* indices,
+                          int64_t m, const int32_t* labels, int64_t nz, double ib_re, double ib_im, const double* mask,
+                          const double* fluct, const int32_t* labels_before, int device, int precision,
+                          bwpm_solver** out) {
   return guarded([&] {
     bwpm::require(out != nullptr, "null output handle");
     *out = nullptr;
@@ -117,6 +137,7 @@ int bwpm_solver_create(int64_t ny, int64_t nx, double dy, double dx, double dz, 
       const c128* f = reinterpret_cast<const c128*>(fluct);
       s.fluct.assign(f, f + nz * ny * nx);
     }
+    if (labels_before) s.labels_before.assign(labels_before, labels_before + ny * nx);
     s.precision = precision;
     s.device = device;
     int count = 0;
@@ -243,7 +264,8 @@ int bwpm_solver_fetch(bwpm_solver* s, int iterations, bwpm_result** out) {
   });
 }
 
-int bwpm_batch_run(bwpm_solver* const* solvers, int64_t n, int iterations, int max_lane, double* wall_ms) {
+int bwpm_batch_run(bwpm_solver* const* solvers, int64_t n, int iterations, int max_lane, int time_kernels,
+                   double* wall_ms, double* col_ms, double* row_ms, int64_t* col_launches, int64_t* row_launches) {
   return guarded([&] {
     bwpm::require(n >= 0 && (n == 0 || solvers), "null solver array");
     bwpm::require(iterations >= 0, "iterations must be >= 0");
@@ -252,8 +274,13 @@ int bwpm_batch_run(bwpm_solver* const* solvers, int64_t n, int iterations, int m
       bwpm::require(solvers[i] != nullptr, "null solver in batch");
       es.push_back(solvers[i]->engine.get());
     }
-    const double ms = bwpm::run_batch(es, iterations, max_lane);
+    bwpm::KernelTimes kt;
+    const double ms = bwpm::run_batch(es, iterations, max_lane, time_kernels != 0, &kt);
     if (wall_ms) *wall_ms = ms;
+    if (col_ms) *col_ms = kt.col_ms;
+    if (row_ms) *row_ms = kt.row_ms;
+    if (col_launches) *col_launches = kt.col_launches;
+    if (row_launches) *row_launches = kt.row_launches;
   });
 }
 
@@ -265,3 +292,18 @@ int bwpm_copy_probe(int64_t bytes, int reps, int device, double* ms) {
 }
 
 }  // extern "C"
+
+namespace bwpm {
+double power_flux_device(const c128* field, long long ny, long long nx, double dy, double dx, double k0, c128 index,
+                         int direction, int device);
+}
+
+extern "C" int bwpm_power_flux(const double* field, int64_t ny, int64_t nx, double dy, double dx, double k0,
+                               double n_re, double n_im, int direction, int device, double* out) {
+  return guarded([&] {
+    bwpm::require(field && out, "null argument");
+    bwpm::require(direction == 1 || direction == -1, "direction must be +1 or -1");
+    *out = bwpm::power_flux_device(reinterpret_cast<const c128*>(field), ny, nx, dy, dx, k0, c128(n_re, n_im),
+                                   direction, device);
+  });
+}

@@ -43,6 +43,18 @@ __global__ void k_interface_eval(const double* kx, const double* ky, long long n
   out[4 * i + 3] = m.m11;
 }
 
+__global__ void k_fresnel_eval(const double* kx, const double* ky, long long n, zd n1, zd n2, double k0, zd* out) {
+  const long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;
+  if (i >= n) return;
+  const Fresnel f = fresnel(kx[i], ky[i], n1, n2, k0);
+  out[6 * i] = f.r_s;
+  out[6 * i + 1] = f.r_p;
+  out[6 * i + 2] = f.t_s;
+  out[6 * i + 3] = f.t_p;
+  out[6 * i + 4] = f.kz_in;
+  out[6 * i + 5] = f.kz_out;
+}
+
 __global__ void k_kz_eval(zd n, double k0, const double* kp2, long long cnt, zd* out) {
   const long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;
   if (i < cnt) out[i] = kz_of(n, k0, kp2[i]);
@@ -58,7 +70,8 @@ std::unique_ptr<EngineBase> make_engine(const Spec& spec) {
   }
 }
 
-double run_batch(std::vector<EngineBase*>& engines, int iterations, int max_lane) {
+double run_batch(std::vector<EngineBase*>& engines, int iterations, int max_lane, bool time_kernels,
+                 KernelTimes* kt) {
   require(max_lane >= 1, "lane size must be >= 1");
   // group into lanes: equal precision, grid and device; solves with δn run one per lane
   std::map<std::tuple<int, long long, long long, int>, std::vector<EngineBase*>> groups;
@@ -82,13 +95,14 @@ double run_batch(std::vector<EngineBase*>& engines, int iterations, int max_lane
   std::vector<std::thread> threads;
   std::vector<std::string> errors(lanes.size());
   std::vector<int> codes(lanes.size(), kOk);
+  std::vector<KernelTimes> times(lanes.size());
   for (size_t i = 0; i < lanes.size(); ++i)
     threads.emplace_back([&, i] {
       try {
         auto& lane = lanes[i];
         HIP_CHECK(hipSetDevice(lane[0]->device()));
         std::vector<std::vector<long long>> keeps(lane.size());
-        lane[0]->run_lane(lane, iterations, keeps, lane[0]->stream(), false, nullptr);
+        lane[0]->run_lane(lane, iterations, keeps, lane[0]->stream(), time_kernels, &times[i]);
       } catch (const Error& e) {
         errors[i] = e.what();
         codes[i] = e.code;
@@ -100,6 +114,15 @@ double run_batch(std::vector<EngineBase*>& engines, int iterations, int max_lane
   for (auto& t : threads) t.join();
   for (size_t i = 0; i < lanes.size(); ++i)
     if (codes[i] != kOk) throw Error(codes[i], errors[i]);
+  if (kt) {
+    *kt = KernelTimes{};
+    for (const KernelTimes& t : times) {
+      kt->col_ms += t.col_ms;
+      kt->row_ms += t.row_ms;
+      kt->col_launches += t.col_launches;
+      kt->row_launches += t.row_launches;
+    }
+  }
   const auto t1 = std::chrono::steady_clock::now();
   return std::chrono::duration<double, std::milli>(t1 - t0).count();
 }
@@ -156,6 +179,29 @@ double copy_probe(long long bytes, int reps, int device) {
   (void)hipEventDestroy(e1);
   (void)hipStreamDestroy(s);
   return ms / reps;
+}
+
+void fresnel_coefficients(const double* kx, const double* ky, long long n, c128 n_in, c128 n_out, double k0,
+                          c128* out, int device) {
+  HIP_CHECK(hipSetDevice(device));
+  if (n <= 0) return;
+  hipStream_t s;
+  HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  try {
+    Buf dkx(n * sizeof(double)), dky(n * sizeof(double)), dout(6 * n * sizeof(zd));
+    HIP_CHECK(hipMemcpyAsync(dkx.as<void>(), kx, n * sizeof(double), hipMemcpyHostToDevice, s));
+    HIP_CHECK(hipMemcpyAsync(dky.as<void>(), ky, n * sizeof(double), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_fresnel_eval, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, s, dkx.as<double>(),
+                       dky.as<double>(), n, zd{n_in.real(), n_in.imag()}, zd{n_out.real(), n_out.imag()}, k0,
+                       dout.as<zd>());
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipMemcpyAsync(out, dout.as<void>(), 6 * n * sizeof(zd), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+  } catch (...) {
+    (void)hipStreamDestroy(s);
+    throw;
+  }
+  HIP_CHECK(hipStreamDestroy(s));
 }
 
 void longitudinal_wavenumbers(c128 index, double k0, const double* kp2, long long n, c128* out, int device) {

@@ -21,6 +21,7 @@ struct Spec {
   c128 index_before{1.0, 0.0};
   std::vector<double> mask;     // (ny, nx) or empty
   std::vector<c128> fluct;      // (nz, ny, nx) or empty
+  std::vector<int32_t> labels_before;  // labels of plane 0 (ny, nx); empty: index_before everywhere
   int precision = 32;           // 64: FP64; 32: FP32 compensated; 31: FP32 plain
   int device = 0;
 };
@@ -75,11 +76,15 @@ std::unique_ptr<EngineBase> make_engine_f64(const Spec& spec);
 // Batched solve of many engines: members are grouped into lanes of equal (precision, grid,
 // device, no δn); each lane runs in lockstep on its own stream, lanes run concurrently.
 // Results are left on the device (download per engine). Returns the wall time in ms.
-double run_batch(std::vector<EngineBase*>& engines, int iterations, int max_lane);
+double run_batch(std::vector<EngineBase*>& engines, int iterations, int max_lane, bool time_kernels = false,
+                 KernelTimes* kt = nullptr);
 
 // FP64 evaluation of the exported interface functions on the device.
 void interface_matrices(int which, const double* kx, const double* ky, long long n, c128 n_in,
                         c128 n_out, double k0, c128* out, int device);
+// Fresnel coefficients r_s, r_p, t_s, t_p, k_z,in, k_z,out per point (6 complex each).
+void fresnel_coefficients(const double* kx, const double* ky, long long n, c128 n_in, c128 n_out, double k0,
+                          c128* out, int device);
 void longitudinal_wavenumbers(c128 index, double k0, const double* kp2, long long n, c128* out,
                               int device);
 

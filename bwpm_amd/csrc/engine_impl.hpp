@@ -12,6 +12,7 @@
 #include "born.hpp"
 #include "engine.hpp"
 #include "kernels.hpp"
+#include "kernels_fast.hpp"
 
 namespace bwpm {
 
@@ -137,8 +138,10 @@ inline int pow2_floor(int v) {
 
 // ---- kernel geometry ---------------------------------------------------------------------------
 
+template <typename T>
 inline bool fast_size(long long n) {
-  return n == 64 || n == 128 || n == 256 || n == 512 || n == 1024 || n == 2048 || n == 4096;
+  // FP64 lines of 4096 would exceed the 160 KB of LDS a workgroup may use
+  return n == 64 || n == 128 || n == 256 || n == 512 || n == 1024 || n == 2048 || (n == 4096 && sizeof(T) == 4);
 }
 // columns per workgroup of the register column kernel (512 threads, ~74 KB LDS in FP32)
 template <typename T>
@@ -528,7 +531,9 @@ class Engine final : public EngineBase {
         case 512: col_fast_n<512>(d, S, s); return;
         case 1024: col_fast_n<1024>(d, S, s); return;
         case 2048: col_fast_n<2048>(d, S, s); return;
-        default: col_fast_n<4096>(d, S, s); return;
+        default:
+          if constexpr (sizeof(T) == 4) col_fast_n<4096>(d, S, s);
+          return;
       }
     }
     // generic: the column block must fit every product count of the lane; use the two-buffer width
@@ -547,7 +552,9 @@ class Engine final : public EngineBase {
         case 512: row_fast_n<512>(d, S, s); return;
         case 1024: row_fast_n<1024>(d, S, s); return;
         case 2048: row_fast_n<2048>(d, S, s); return;
-        default: row_fast_n<4096>(d, S, s); return;
+        default:
+          if constexpr (sizeof(T) == 4) row_fast_n<4096>(d, S, s);
+          return;
       }
     }
     RowGeom g{static_cast<int>(sp_.ny), static_cast<int>(sp_.nx), geo_.rows, geo_.ls_row, npix_, twx_.plan};
@@ -605,6 +612,10 @@ class Engine final : public EngineBase {
       require(l >= 0 && static_cast<size_t>(l) < s.indices.size(), "labels must index into `indices`");
     require(s.fluct.empty() || static_cast<long long>(s.fluct.size()) == s.nz * s.ny * s.nx,
             "fluctuation must have the shape of labels (nz, ny, nx)");
+    require(s.labels_before.empty() || static_cast<long long>(s.labels_before.size()) == s.ny * s.nx,
+            "labels_before must have shape (ny, nx)");
+    for (int32_t l : s.labels_before)
+      require(l >= -1 && l < static_cast<long long>(s.indices.size()), "labels must index into `indices`");
   }
 
   void setup() {
@@ -621,7 +632,10 @@ class Engine final : public EngineBase {
     before_id_ = id_of(s.index_before);
     for (const c128& v : s.indices) label_medium_.push_back(id_of(v));
     media_ = media;
-    before_.assign(npix_, -1);
+    if (s.labels_before.empty())
+      before_.assign(npix_, -1);
+    else
+      before_ = s.labels_before;
 
     // Born hosts per slab (validation as the reference constructor, solver.cpp:47-62)
     has_host_.assign(s.nz, 0);
@@ -679,8 +693,8 @@ class Engine final : public EngineBase {
     const Spec& s = sp_;
     Geometry& g = geo_;
     const bool fast_prec = COMP == (sizeof(T) == 4);  // register kernels: FP32 compensated and FP64
-    g.col_fast = fast_prec && fast_size(s.ny);
-    g.row_fast = fast_prec && fast_size(s.nx);
+    g.col_fast = fast_prec && fast_size<T>(s.ny);
+    g.row_fast = fast_prec && fast_size<T>(s.nx);
     if (g.col_fast) {
       const int w = fast_w<T>(static_cast<int>(s.ny));
       g.col_w = (s.nx % w == 0) ? w : 1;

@@ -9,6 +9,8 @@
 // and the last pass are radix 8, so the input is taken in natural order from registers and the
 // last pass leaves A[t + T·r] in v[r] (natural order) without touching LDS: P − 1 LDS exchanges.
 //
+// The two field components are transformed one after the other inside each butterfly, which
+// halves the live temporaries. Twiddles are read from a table the kernel stages in LDS once.
 // LDS index i is padded to i + i/8 to spread the stride-8 writes of the first pass over banks.
 #pragma once
 
@@ -28,14 +30,30 @@ template <> struct RegPlan<1024> { static constexpr int P = 4; static constexpr 
 template <> struct RegPlan<2048> { static constexpr int P = 4; static constexpr int rad[P] = {8, 4, 8, 8}; };
 template <> struct RegPlan<4096> { static constexpr int P = 4; static constexpr int rad[P] = {8, 8, 8, 8}; };
 
-// Note: the size-16 and size-32 plans end with a radix below 8; for them the last pass writes
-// through LDS like the others (see reg_pass).
+// The size-16 and size-32 plans end with a radix below 8; their last pass writes through LDS too.
 
 template <int N>
 constexpr int reg_lds_len() {
   return N + N / 8;
 }
 __device__ __forceinline__ int lpad(int i) { return i + (i >> 3); }
+
+// LDS twiddle table of a workgroup: hi (and lo for the compensated FP32 path), N entries each.
+template <typename T, bool COMP>
+struct TwLds {
+  cx<T>* hi;
+  cx<T>* lo;
+};
+
+template <int N, bool COMP, typename T>
+__device__ __forceinline__ void stage_twiddles(cx<T>* sh_hi, cx<T>* sh_lo, const cx<T>* __restrict__ g_hi,
+                                               const cx<T>* __restrict__ g_lo) {
+  for (int i = threadIdx.x; i < N; i += blockDim.x) {
+    sh_hi[i] = g_hi[i];
+    if constexpr (COMP) sh_lo[i] = g_lo[i];
+  }
+  __syncthreads();
+}
 
 template <int R, bool COMP, bool INV, typename T>
 __device__ __forceinline__ void dft_r(cx<T>* x) {
@@ -45,8 +63,7 @@ __device__ __forceinline__ void dft_r(cx<T>* x) {
 }
 
 template <int N, int PASS, int NS, bool COMP, bool INV, typename T>
-__device__ __forceinline__ void reg_pass(px2<T> (&v)[8], px2<T>* lds, int t, const cx<T>* __restrict__ twh,
-                                         const cx<T>* __restrict__ twl) {
+__device__ __forceinline__ void reg_pass(px2<T> (&v)[8], px2<T>* lds, int t, const cx<T>* twh, const cx<T>* twl) {
   using PL = RegPlan<N>;
   if constexpr (PASS < PL::P) {
     constexpr int R = PL::rad[PASS];
@@ -54,45 +71,46 @@ __device__ __forceinline__ void reg_pass(px2<T> (&v)[8], px2<T>* lds, int t, con
     constexpr int TT = N / 8;
     constexpr bool last = PASS == PL::P - 1;
     constexpr bool in_regs = last && R == 8;  // natural-order result stays in registers
-    px2<T> out[8];
 #pragma unroll
     for (int b = 0; b < B; ++b) {
       const int j = t + TT * b;
       const int k = j & (NS - 1);
-      cx<T> xa[R], xb[R];
-#pragma unroll
-      for (int r = 0; r < R; ++r) {
-        xa[r] = v[b + r * B].a;
-        xb[r] = v[b + r * B].b;
-      }
+      cx<T> wh[R], wl[R];
       if constexpr (NS > 1) {
 #pragma unroll
         for (int r = 1; r < R; ++r) {
           const int e = r * k * (N / (NS * R));
-          cx<T> wh = twh[e], wl = twl[e];
-          if constexpr (INV) {
-            wh = conj(wh);
-            wl = conj(wl);
-          }
-          xa[r] = cmul2<COMP>(xa[r], wh, wl);
-          xb[r] = cmul2<COMP>(xb[r], wh, wl);
+          wh[r] = INV ? conj(twh[e]) : twh[e];
+          if constexpr (COMP) wl[r] = INV ? conj(twl[e]) : twl[e];
         }
       }
-      dft_r<R, COMP, INV>(xa);
-      dft_r<R, COMP, INV>(xb);
-      if constexpr (in_regs) {
+      const int base = (j - k) * R + k;
 #pragma unroll
-        for (int r = 0; r < R; ++r) out[b + r * B] = px2<T>{xa[r], xb[r]};
-      } else {
-        const int base = (j - k) * R + k;
+      for (int comp = 0; comp < 2; ++comp) {
+        cx<T> x[R];
 #pragma unroll
-        for (int r = 0; r < R; ++r) lds[lpad(base + r * NS)] = px2<T>{xa[r], xb[r]};
+        for (int r = 0; r < R; ++r) x[r] = comp == 0 ? v[b + r * B].a : v[b + r * B].b;
+        if constexpr (NS > 1) {
+#pragma unroll
+          for (int r = 1; r < R; ++r) x[r] = cmul2<COMP>(x[r], wh[r], wl[r]);
+        }
+        dft_r<R, COMP, INV>(x);
+        if constexpr (in_regs) {
+#pragma unroll
+          for (int r = 0; r < R; ++r) {
+            if (comp == 0) v[r].a = x[r];
+            else v[r].b = x[r];
+          }
+        } else {
+#pragma unroll
+          for (int r = 0; r < R; ++r) {
+            cx<T>* dst = reinterpret_cast<cx<T>*>(lds + lpad(base + r * NS)) + comp;
+            *dst = x[r];
+          }
+        }
       }
     }
-    if constexpr (in_regs) {
-#pragma unroll
-      for (int m = 0; m < 8; ++m) v[m] = out[m];
-    } else {
+    if constexpr (!in_regs) {
       __syncthreads();
 #pragma unroll
       for (int m = 0; m < 8; ++m) v[m] = lds[lpad(t + TT * m)];
@@ -103,7 +121,8 @@ __device__ __forceinline__ void reg_pass(px2<T> (&v)[8], px2<T>* lds, int t, con
 }
 
 // Transform one line held as v[m] = a[t + T·m]; on return v[m] = A[t + T·m]. All threads of the
-// workgroup must call it together (it synchronises the workgroup between passes).
+// workgroup must call it together (it synchronises the workgroup between passes). twh/twl point
+// to the twiddle table W_N^e (LDS or global).
 template <int N, bool COMP, bool INV, typename T>
 __device__ __forceinline__ void reg_fft(px2<T> (&v)[8], px2<T>* lds, int t, const cx<T>* twh, const cx<T>* twl) {
   reg_pass<N, 0, 1, COMP, INV>(v, lds, t, twh, twl);

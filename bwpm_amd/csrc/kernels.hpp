@@ -277,159 +277,6 @@ __global__ __launch_bounds__(kThreads) void k_row_step(const RowDesc<T>* __restr
   }
 }
 
-// ======================= power-of-two sizes (register-resident FFT) ==========================
-
-// Column kernel: W adjacent columns per workgroup, N/8 threads per column, each thread holding
-// eight (E_x, E_y) vectors of its column. Thread index = t·W + c, so a wavefront reads W·16-byte
-// row segments.
-template <int N, int W, typename T, bool COMP>
-__global__ __launch_bounds__(N / 8 * W) void k_col_fast(const ColDesc<T>* __restrict__ descs, int nx,
-                                                        long long plane, const cx<T>* __restrict__ twh,
-                                                        const cx<T>* __restrict__ twl) {
-  constexpr int TT = N / 8;
-  constexpr int LL = reg_lds_len<N>();
-  __shared__ px2<T> sh[W * LL];
-  const ColDesc<T> d = descs[blockIdx.y];
-  if (!d.in) return;
-  const int c = threadIdx.x % W, t = threadIdx.x / W;
-  const int x = blockIdx.x * W + c;
-  px2<T>* line = sh + c * LL;
-  px2<T> s[8];
-#pragma unroll
-  for (int m = 0; m < 8; ++m) s[m] = d.in[static_cast<long long>(t + TT * m) * nx + x];
-  reg_fft<N, COMP, false>(s, line, t, twh, twl);
-  for (int o = 0; o < d.nprod; ++o) {
-    const Prod<T> pd = d.prods[o];
-    px2<T> w[8];
-#pragma unroll
-    for (int m = 0; m < 8; ++m) w[m] = apply_prod<T, COMP>(pd, static_cast<long long>(t + TT * m) * nx + x, s[m]);
-    reg_fft<N, COMP, true>(w, line, t, twh, twl);
-    px2<T>* out = d.out + o * plane;
-#pragma unroll
-    for (int m = 0; m < 8; ++m) out[static_cast<long long>(t + TT * m) * nx + x] = w[m];
-  }
-}
-
-// Row kernel: RW rows per workgroup, N/8 threads per row (thread index = r·N/8 + t).
-template <int N, int RW, typename T, bool COMP>
-__global__ __launch_bounds__(N / 8 * RW) void k_row_fast(const RowDesc<T>* __restrict__ descs, long long plane,
-                                                         const cx<T>* __restrict__ twh,
-                                                         const cx<T>* __restrict__ twl) {
-  constexpr int TT = N / 8;
-  constexpr int LL = reg_lds_len<N>();
-  __shared__ px2<T> sh[RW * LL];
-  const RowDesc<T> a = descs[blockIdx.y];
-  if (!a.active) return;
-  const int r = threadIdx.x / TT, t = threadIdx.x % TT;
-  const int y = blockIdx.x * RW + r;
-  const long long row = static_cast<long long>(y) * N;
-  px2<T>* line = sh + r * LL;
-  px2<T> acc[8];
-
-  // 1. transmitted field (or the start plane)
-  if (a.nt == 0) {
-#pragma unroll
-    for (int m = 0; m < 8; ++m) acc[m] = a.e_in ? a.e_in[row + t + TT * m] : px2<T>{};
-  } else {
-#pragma unroll
-    for (int m = 0; m < 8; ++m) acc[m] = a.prod[row + t + TT * m];
-    reg_fft<N, COMP, true>(acc, line, t, twh, twl);
-    for (int o = 1; o < a.nt; ++o) {
-      px2<T> w[8];
-      const px2<T>* src = a.prod + o * plane;
-#pragma unroll
-      for (int m = 0; m < 8; ++m) w[m] = src[row + t + TT * m];
-      reg_fft<N, COMP, true>(w, line, t, twh, twl);
-#pragma unroll
-      for (int m = 0; m < 8; ++m)
-        if (a.tsel[row + t + TT * m] == o) acc[m] = w[m];
-    }
-  }
-
-  // 2. epilogue in the order of the reference: mask, Born increment, sparse sources, Born plane
-  if (a.mask) {
-#pragma unroll
-    for (int m = 0; m < 8; ++m) {
-      const T mk = a.mask[row + t + TT * m];
-      acc[m].a = acc[m].a * mk;
-      acc[m].b = acc[m].b * mk;
-      if (a.inc) {
-        const px2<T> dd = a.inc[row + t + TT * m];
-        acc[m].a = acc[m].a + dd.a * mk;
-        acc[m].b = acc[m].b + dd.b * mk;
-      }
-    }
-  } else if (a.inc) {
-#pragma unroll
-    for (int m = 0; m < 8; ++m) {
-      const px2<T> dd = a.inc[row + t + TT * m];
-      acc[m].a = acc[m].a + dd.a;
-      acc[m].b = acc[m].b + dd.b;
-    }
-  }
-  if (a.rowptr && a.sp_add) {
-#pragma unroll
-    for (int m = 0; m < 8; ++m) line[lpad(t + TT * m)] = acc[m];
-    __syncthreads();
-    const int e1 = a.rowptr[y + 1];
-    for (int e = a.rowptr[y] + t; e < e1; e += TT) {
-      const int x = a.sp_pix[e] - y * N;
-      const px2<T> v = a.sp_add[e];
-      px2<T> u = line[lpad(x)];
-      u.a = u.a + v.a;
-      u.b = u.b + v.b;
-      line[lpad(x)] = u;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int m = 0; m < 8; ++m) acc[m] = line[lpad(t + TT * m)];
-    __syncthreads();
-  }
-  if (a.dense) {
-#pragma unroll
-    for (int m = 0; m < 8; ++m) {
-      const px2<T> dd = a.dense[row + t + TT * m];
-      acc[m].a = acc[m].a + dd.a;
-      acc[m].b = acc[m].b + dd.b;
-    }
-  }
-
-  // 3. reflection products: row IFFT and gather of the interface pixels
-  if (a.nr > 0 && a.rowptr && a.sp_store) {
-    const int e0 = a.rowptr[y], e1 = a.rowptr[y + 1];
-    for (int o = 0; o < a.nr; ++o) {
-      px2<T> w[8];
-      const px2<T>* src = a.prod + (a.nt + o) * plane;
-#pragma unroll
-      for (int m = 0; m < 8; ++m) w[m] = src[row + t + TT * m];
-      reg_fft<N, COMP, true>(w, line, t, twh, twl);
-#pragma unroll
-      for (int m = 0; m < 8; ++m) line[lpad(t + TT * m)] = w[m];
-      __syncthreads();
-      for (int e = e0 + t; e < e1; e += TT)
-        if (a.sp_rid[e] == o) a.sp_store[e] = line[lpad(a.sp_pix[e] - y * N)];
-      __syncthreads();
-    }
-  }
-
-  // 4. real-space outputs
-  if (a.e_out) {
-#pragma unroll
-    for (int m = 0; m < 8; ++m) a.e_out[row + t + TT * m] = acc[m];
-  }
-  if (a.e_out2) {
-#pragma unroll
-    for (int m = 0; m < 8; ++m) a.e_out2[row + t + TT * m] = acc[m];
-  }
-
-  // 5. row FFT for the next slab step
-  if (a.x_out) {
-    reg_fft<N, COMP, false>(acc, line, t, twh, twl);
-#pragma unroll
-    for (int m = 0; m < 8; ++m) a.x_out[row + t + TT * m] = acc[m];
-  }
-}
-
 // ---- operator tables (FP64 math, stored as T hi + lo) ---------------------------------------
 
 template <typename T>

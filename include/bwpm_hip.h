@@ -60,6 +60,20 @@ int bwpm_reflection_matrix(const double* kx, const double* ky, int64_t n, double
 int bwpm_longitudinal_wavenumber(double n_re, double n_im, double k0, const double* kp2, int64_t n,
                                  double* out);
 
+/* Fresnel coefficients (Convention A) at n points: r_s, r_p, t_s, t_p, k_z,in, k_z,out (12n doubles).
+ * Replaces `bwpm.interface.fresnel_coefficients` (src/bwpm/interface.py:68-90, cpp/src/interface.cpp:20-28). */
+int bwpm_fresnel_coefficients(const double* kx, const double* ky, int64_t n, double n_in_re,
+                              double n_in_im, double n_out_re, double n_out_im, double k0,
+                              double* out);
+
+/* Poynting flux Σ_k Re(k_z)(|Ê_x|² + |Ê_y|² + |Ê_z|²)·dx·dy/N of a field (ny, nx, 2) in a
+ * homogeneous medium, E_z from transversality for waves of `direction` (±1), FP64 on the device
+ * with a deterministic reduction. The 2D reference's power_flux is the case nx = 1, dx = 1.
+ * Replaces `bwpm.fpwpm.power_flux` and `bwpm.adapter.port_power` (src/bwpm/fpwpm.py:205-216,
+ * src/bwpm/adapter.py:267-284), the basis of the energy residual (src/bwpm/fast.py:148-155). */
+int bwpm_power_flux(const double* field, int64_t ny, int64_t nx, double dy, double dx, double k0,
+                    double n_re, double n_im, int direction, int device, double* out);
+
 /* Solver over a labelled volume. labels: (nz, ny, nx) int32 into `indices` (m complex);
  * mask: (ny, nx) or NULL; fluct: δn (nz, ny, nx) complex or NULL; precision: BWPM_FP64/FP32.
  * Replaces `_core.CudaSolver(...)` / `_core.Solver(...)` (cpp/python/bindings.cpp:139-151,158-171,
@@ -68,6 +82,15 @@ int bwpm_solver_create(int64_t ny, int64_t nx, double dy, double dx, double dz, 
                        const double* indices, int64_t m, const int32_t* labels, int64_t nz,
                        double index_before_re, double index_before_im, const double* mask,
                        const double* fluct, int device, int precision, bwpm_solver** out);
+
+/* As bwpm_solver_create with the labels of plane 0 given per pixel (labels_before: (ny, nx) int32,
+ * -1 = index_before; NULL = -1 everywhere). Serves the single step of `Propagator2D.step`
+ * (src/bwpm/fpwpm.py:134-161), whose incoming plane may carry any labels. */
+int bwpm_solver_create_ex(int64_t ny, int64_t nx, double dy, double dx, double dz, double k0,
+                          const double* indices, int64_t m, const int32_t* labels, int64_t nz,
+                          double index_before_re, double index_before_im, const double* mask,
+                          const double* fluct, const int32_t* labels_before, int device,
+                          int precision, bwpm_solver** out);
 
 int bwpm_solver_destroy(bwpm_solver* s);
 
@@ -105,9 +128,12 @@ int bwpm_solver_fetch(bwpm_solver* s, int iterations, bwpm_result** out);
  * must have its field uploaded. Solvers of equal precision, grid and device are grouped into
  * lanes of at most `max_lane` that step in lockstep, one column and one row launch per slab step
  * for the whole lane; lanes run concurrently on their own streams. Results stay on the device
- * until bwpm_solver_fetch. Each result is bit-identical to the solver's run alone. */
+ * until bwpm_solver_fetch. Each result is bit-identical to the solver's run alone. With
+ * time_kernels != 0 every slab-step launch is bracketed by HIP events on its lane's stream and the
+ * summed durations and launch counts are returned (any output pointer may be NULL). */
 int bwpm_batch_run(bwpm_solver* const* solvers, int64_t n, int iterations, int max_lane,
-                   double* wall_ms);
+                   int time_kernels, double* wall_ms, double* col_ms, double* row_ms,
+                   int64_t* col_launches, int64_t* row_launches);
 
 /* Average duration of a plain copy kernel (k_copy_probe) over `bytes` on `device`: the HBM
  * calibration of the traffic counters and the achievable-bandwidth reference of the benchmark

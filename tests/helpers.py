@@ -70,3 +70,25 @@ def fixture_3d_args(d: dict):
                 k0=k0_of(p["wavelength_um"]), indices=i["indices"], labels=i["labels"],
                 index_before=i["index_before"], mask=i.get("mask"), fluctuation=i.get("fluctuation"),
                 field=i["field"])
+
+
+def kz_branch(n: complex, k0: float, kp2: float) -> complex:
+    q = complex(np.sqrt(complex(n * n * k0 * k0 - kp2)))
+    return -q if (q.imag < 0 or (q.imag == 0 and q.real < 0)) else q
+
+
+def transfer_rt(media, before, kp2, k0, dz, pol):
+    """(r, t) of the tangential field of a planar stack of slabs of thickness dz, from the layer
+    transfer matrices (independent of the slab-stepping method)."""
+    layers = [before, *media]
+    kzs = [kz_branch(n, k0, kp2) for n in layers]
+    eta = [k if pol == "s" else n * n * k0 * k0 / k for n, k in zip(layers, kzs)]
+    ap, am = 1.0 + 0j, 0.0 + 0j
+    for j in range(len(layers) - 1, 0, -1):
+        et, ht = ap + am, eta[j] * (ap - am)
+        lp, lm = 0.5 * (et + ht / eta[j - 1]), 0.5 * (et - ht / eta[j - 1])
+        if j == 1:
+            ap, am = lp, lm
+        else:
+            ap, am = lp * np.exp(-1j * kzs[j - 1] * dz), lm * np.exp(1j * kzs[j - 1] * dz)
+    return complex(am / ap), complex(np.exp(1j * kzs[-1] * dz) / ap)
