@@ -3,16 +3,20 @@
 
 Workload (default): the tilted-interface 3D vector geometry of the reference's plain config
 (λ = 1.55 µm, dx = dy = λ/6, dz = λ/4, interface 1.0 → 1.46 tilted 10°, pixel-centre labels,
-Gaussian w₀ = 8 µm, 4 bidirectional iterations) at 512² × 1024 slabs, FP32 bulk. One step is one
-batch of `--batch` independent solves (E_x and E_y polarised alternately) run concurrently on the
-GPU as one lane (every slab step is one column launch and one row launch for all solves of the
-batch) with fields and state resident in HBM; the timed region covers `--steps` such batches.
+Gaussian w₀ = 8 µm, 4 bidirectional iterations) at 512² × 1024 slabs, FP64 (complex128) bulk by
+default. One step is one batch of `--batch` independent solves (E_x and E_y polarised alternately)
+run on the GPU as one lane (every slab step is one column launch and one row launch for all solves
+of the batch) with fields and state resident in HBM; the timed region covers `--steps` such batches.
 
-Printed: one JSON line with grid-point·slab updates/s (whole job), the roofline of the slab-step
-kernels (HIP-event durations over a measured solve, algorithmic 32 B per update), and the
-reference's CPU core timed on this host over a bounded sample of the same geometry.
+Printed: one JSON line with grid-point·slab updates/s (whole job), the roofline of the slab step
+(HIP-event durations of the column and the row kernel on the lane's stream; algorithmic bytes per
+update = one read and one write of the (E_x, E_y) plane: 64 B in complex128, 32 B in complex64),
+the plain one-solve-per-launch rate of the same solve on the same GPU, the accuracy of the run
+(batch result against the solve alone; FP32 against the FP64 device path), and the reference's
+CPU core timed on this host over a bounded sample of the same geometry.
 
     python bench.py [--steps K] [--warmup W] [--batch B] [--n 512] [--nz 1024] [--iterations 4]
+                    [--precision fp64|fp32|fp32_plain]
 """
 
 from __future__ import annotations
@@ -20,6 +24,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import re
 import sys
 import time
 from pathlib import Path
@@ -30,7 +35,25 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBPS = 8000.0  # MI355X HBM3E
-BYTES_PER_UPDATE = 32   # read + write of the complex64 (E_x, E_y) plane per slab step
+# algorithmic bytes per grid-point·slab update: read + write of the (E_x, E_y) plane per slab step
+BYTES_PER_UPDATE = {"fp64": 64, "fp32": 32, "fp32_plain": 32}
+
+
+def profiled_traffic(precision: str, n: int, batch: int) -> tuple[int | None, str | None]:
+    """HBM bytes per launch pair from the committed rocprofv3 summary of the same configuration
+    (FETCH_SIZE and WRITE_SIZE per dispatch, scaled by the copy-kernel calibration)."""
+    tag = f"config: precision={precision} n={n} batch={batch}"
+    for f in sorted((ROOT / "profiles").glob("*.md")):
+        text = f.read_text()
+        m = re.search(r"^traffic_bytes_per_launch_pair: (\d+)$", text, re.M)
+        if m and tag in text:
+            return int(m.group(1)), f.name
+    return None, None
+
+
+def rel_l2(a: np.ndarray, b: np.ndarray) -> float:
+    nb = float(np.linalg.norm(b))
+    return float(np.linalg.norm(a - b)) / nb if nb > 0 else float(np.linalg.norm(a))
 
 
 def cpu_baseline(n: int, nz_sample: int, threads: int) -> dict:
@@ -72,10 +95,11 @@ def main() -> int:
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--nz", type=int, default=1024)
     ap.add_argument("--iterations", type=int, default=4)
-    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64", "fp32_plain"])
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32", "fp32_plain"])
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "0") or 0))
     ap.add_argument("--cpu-nz", type=int, default=48)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-checks", action="store_true", help="skip the plain-path and accuracy runs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU figures are not produced",
@@ -108,26 +132,61 @@ def main() -> int:
 
     # kernel durations inside the lane: HIP events around every launch on the lane's stream
     kt = _core.run_batch(solvers, args.iterations, max_lane=args.batch, time_kernels=True)
+    bpu = BYTES_PER_UPDATE[args.precision]
     npix = args.n * args.n * args.batch  # pixels one launch processes
     col_us = 1e3 * kt["col_ms"] / max(kt["col_launches"], 1)
     row_us = 1e3 * kt["row_ms"] / max(kt["row_launches"], 1)
     step_us = col_us + row_us
-    achieved = BYTES_PER_UPDATE * npix / (step_us * 1e-6) / 1e9
+    achieved = bpu * npix / (step_us * 1e-6) / 1e9
+    traffic, traffic_src = profiled_traffic(args.precision, args.n, args.batch)
     copy_bytes = 1 << 30
     copy_ms = _core.copy_probe(copy_bytes, 10)
     roofline = {
         "bound": "hbm",
-        "kernel": "k_col_step+k_row_step (one slab step)",
+        "kernel": "k_col_fast + k_row_fast (one slab step of the lane)",
         "achieved": round(achieved, 2),
         "peak": HBM_PEAK_GBPS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBPS, 5),
-        "traffic": None,
-        "algorithmic_bytes_per_launch_pair": BYTES_PER_UPDATE * npix,
-        "k_col_step_us": round(col_us, 3),
-        "k_row_step_us": round(row_us, 3),
+        "traffic": traffic,
+        "traffic_source": traffic_src,
+        "algorithmic_bytes_per_launch_pair": bpu * npix,
+        "k_col_us": round(col_us, 3),
+        "k_row_us": round(row_us, 3),
+        "launch_pairs": kt["col_launches"],
         "copy_probe_GBps": round(2 * copy_bytes / (copy_ms * 1e-3) / 1e9, 1),
     }
+
+    checks = None
+    if not args.no_checks:
+        batch_res = solvers[0].fetch(args.iterations)
+        # the plain path: the same solve alone, one solve per launch, synchronised per solve
+        solvers[0].run_resident(args.iterations)
+        t0 = time.perf_counter()
+        solvers[0].run_resident(args.iterations)
+        plain = probs[0].updates / (time.perf_counter() - t0)
+        alone = solvers[0].fetch(args.iterations)
+        # the same solve through host buffers (field copied in, results copied out over PCIe)
+        t0 = time.perf_counter()
+        solvers[0].run(probs[0].field, args.iterations)
+        host_rate = probs[0].updates / (time.perf_counter() - t0)
+        solvers[0].upload(probs[0].field)
+        checks = {
+            "plain_one_solve_per_launch": plain,
+            "one_solve_with_host_buffers": host_rate,
+            "batched_over_plain": value / plain,
+            "batch_vs_alone_bitwise": bool(np.array_equal(batch_res["transmitted"], alone["transmitted"])
+                                           and np.array_equal(batch_res["reflected"], alone["reflected"])),
+        }
+        if args.precision != "fp64":
+            ref = _core.GpuSolver(*probs[0].solver_args(), precision="fp64")
+            ref.upload(probs[0].field)
+            ref.run_resident(args.iterations)
+            r64 = ref.fetch(args.iterations)
+            ref.close()
+            checks["rel_l2_vs_fp64_device"] = {"transmitted": rel_l2(alone["transmitted"], r64["transmitted"]),
+                                               "reflected": rel_l2(alone["reflected"], r64["reflected"])}
+
     cpu = None
     if not args.no_cpu_baseline:
         threads = args.cpu_threads if args.cpu_threads > 0 else min(16, os.cpu_count() or 1)
@@ -152,6 +211,7 @@ def main() -> int:
             "precision": args.precision, "updates_per_solve": probs[0].updates,
         },
         "roofline": roofline,
+        "checks": checks,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
