@@ -93,6 +93,64 @@ BWPM_HD cx<T> rmul2(cx<T> a, T hi, T lo) {
   }
 }
 
+// ---- device global-memory access ------------------------------------------------------------
+// Loads and stores through pointers in the global address space compile to global_load/store
+// rather than flat ones, so the LDS waits of the FFT passes do not also wait for HBM traffic.
+#define BWPM_GLOBAL __attribute__((address_space(1)))
+template <typename T>
+struct Vec2;
+template <>
+struct Vec2<float> {
+  typedef float type __attribute__((ext_vector_type(2)));
+};
+template <>
+struct Vec2<double> {
+  typedef double type __attribute__((ext_vector_type(2)));
+};
+
+// Element i of an array whose byte size is below 4 GiB: the byte offset is formed in 32 bits, so
+// the access uses the scalar base + 32-bit vector offset form when the base is uniform.
+template <typename U, typename E>
+__device__ __forceinline__ const BWPM_GLOBAL E* gaddr(const U* p, unsigned i) {
+  return (const BWPM_GLOBAL E*)((const BWPM_GLOBAL char*)p + i * static_cast<unsigned>(sizeof(E)));
+}
+template <typename U, typename E>
+__device__ __forceinline__ BWPM_GLOBAL E* gaddr(U* p, unsigned i) {
+  return (BWPM_GLOBAL E*)((BWPM_GLOBAL char*)p + i * static_cast<unsigned>(sizeof(E)));
+}
+
+template <typename T>
+__device__ __forceinline__ cx<T> gload(const cx<T>* p, unsigned i) {
+  using V = typename Vec2<T>::type;
+  const V v = *gaddr<cx<T>, V>(p, i);
+  return cx<T>{v.x, v.y};
+}
+template <typename T>
+__device__ __forceinline__ void gstore(cx<T>* p, unsigned i, cx<T> z) {
+  using V = typename Vec2<T>::type;
+  V v;
+  v.x = z.x;
+  v.y = z.y;
+  *gaddr<cx<T>, V>(p, i) = v;
+}
+template <typename U>
+__device__ __forceinline__ U gload1(const U* p, unsigned i) {
+  return *gaddr<U, U>(p, i);
+}
+
+// Values every lane of a wavefront holds equally, moved to scalar registers (so branches on them
+// are scalar and addresses derived from them use the scalar base of global loads).
+__device__ __forceinline__ int uniform(int v) {
+  return __builtin_amdgcn_readfirstlane(v);
+}
+template <typename P>
+__device__ __forceinline__ P* uniform(P* p) {
+  const unsigned long long u = reinterpret_cast<unsigned long long>(p);
+  const unsigned lo = static_cast<unsigned>(__builtin_amdgcn_readfirstlane(static_cast<int>(u & 0xffffffffULL)));
+  const unsigned hi = static_cast<unsigned>(__builtin_amdgcn_readfirstlane(static_cast<int>(u >> 32)));
+  return reinterpret_cast<P*>((static_cast<unsigned long long>(hi) << 32) | lo);
+}
+
 // Status codes of the C ABI (include/bwpm_hip.h).
 enum Status : int { kOk = 0, kEinval = 1, kEnoconv = 2, kEdevice = 3, kEnomem = 4, kEinternal = 5 };
 

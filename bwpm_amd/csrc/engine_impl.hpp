@@ -7,6 +7,9 @@
 #include <algorithm>
 #include <cstring>
 #include <map>
+#include <memory>
+#include <mutex>
+#include <tuple>
 #include <unordered_map>
 
 #include "born.hpp"
@@ -130,6 +133,44 @@ struct OpTable {
   DevBuf hi, lo;
 };
 
+// Operator tables are shared by every engine of the process with the same device, precision,
+// grid, wavelength, slab thickness, kind and media: solves of one structure (different incident
+// fields, polarisations or δn realisations) then read one table, which stays in the caches. The
+// reference keeps the same cache per solver object (Solver::op, cpp/src/solver.cpp:73-106).
+struct TableKey {
+  int device = 0, precision = 0, kind = 0;  // kind −1: phase table of a homogeneous pair
+  long long ny = 0, nx = 0;
+  double dy = 0, dx = 0, dz = 0, k0 = 0, nr_re = 0, nr_im = 0, ns_re = 0, ns_im = 0;
+  bool operator<(const TableKey& o) const {
+    return std::tie(device, precision, kind, ny, nx, dy, dx, dz, k0, nr_re, nr_im, ns_re, ns_im) <
+           std::tie(o.device, o.precision, o.kind, o.ny, o.nx, o.dy, o.dx, o.dz, o.k0, o.nr_re, o.nr_im, o.ns_re,
+                    o.ns_im);
+  }
+};
+
+inline std::mutex& table_mutex() {
+  static std::mutex m;
+  return m;
+}
+inline std::map<TableKey, std::weak_ptr<OpTable>>& table_cache() {
+  static std::map<TableKey, std::weak_ptr<OpTable>> c;
+  return c;
+}
+
+// The live table of `key`, or a new one from make() (which returns it complete on the device).
+template <typename F>
+std::shared_ptr<OpTable> shared_table(const TableKey& key, F&& make) {
+  std::lock_guard<std::mutex> lock(table_mutex());
+  auto& cache = table_cache();
+  for (auto it = cache.begin(); it != cache.end();) it = it->second.expired() ? cache.erase(it) : std::next(it);
+  auto it = cache.find(key);
+  if (it != cache.end())
+    if (auto live = it->second.lock()) return live;
+  std::shared_ptr<OpTable> t = make();
+  cache[key] = t;
+  return t;
+}
+
 inline int pow2_floor(int v) {
   int p = 1;
   while (p * 2 <= v) p *= 2;
@@ -143,16 +184,14 @@ inline bool fast_size(long long n) {
   // FP64 lines of 4096 would exceed the 160 KB of LDS a workgroup may use
   return n == 64 || n == 128 || n == 256 || n == 512 || n == 1024 || n == 2048 || (n == 4096 && sizeof(T) == 4);
 }
-// columns per workgroup of the register column kernel (512 threads, ~74 KB LDS in FP32)
-template <typename T>
+// columns / rows per workgroup of the register kernels (kernels_fast.hpp)
+template <typename T, bool COMP>
 constexpr int fast_w(int n) {
-  const int w = (sizeof(T) == 4 ? 512 : 256) / (n / 8);
-  return w > 16 ? 16 : (w < 1 ? 1 : w);
+  return fast_lines(n, kColSkew, static_cast<int>(sizeof(cx<T>)), COMP);
 }
-template <typename T>
+template <typename T, bool COMP>
 constexpr int fast_rw(int n) {
-  const int r = (sizeof(T) == 4 ? 512 : 256) / (n / 8);
-  return r < 1 ? 1 : r;
+  return fast_lines(n, kRowSkew, static_cast<int>(sizeof(cx<T>)), COMP);
 }
 
 // Launch geometry of one grid size (shared by every solve of a lane).
@@ -495,29 +534,29 @@ class Engine final : public EngineBase {
   // ---- launches over a lane of S solves
   template <int N>
   void col_fast_n(const ColDesc<T>* d, int S, hipStream_t s) {
-    constexpr int W = fast_w<T>(N);
+    constexpr int W = fast_w<T, COMP>(N);
     const long long plane = npix_;
     if (geo_.col_w == W) {
       const dim3 grid(static_cast<unsigned>(sp_.nx / W), S);
-      hipLaunchKernelGGL((k_col_fast<N, W, T, COMP>), grid, dim3(N / 8 * W), 0, s, d, static_cast<int>(sp_.nx),
+      hipLaunchKernelGGL((k_col_fast<N, W, T, COMP>), grid, dim3(N / 4 * W), 0, s, d, static_cast<int>(sp_.nx),
                          plane, twy_.hi.template as<cx<T>>(), twy_.lo.template as<cx<T>>());
     } else {
       const dim3 grid(static_cast<unsigned>(sp_.nx), S);
-      hipLaunchKernelGGL((k_col_fast<N, 1, T, COMP>), grid, dim3(N / 8), 0, s, d, static_cast<int>(sp_.nx), plane,
+      hipLaunchKernelGGL((k_col_fast<N, 1, T, COMP>), grid, dim3(N / 4), 0, s, d, static_cast<int>(sp_.nx), plane,
                          twy_.hi.template as<cx<T>>(), twy_.lo.template as<cx<T>>());
     }
   }
   template <int N>
   void row_fast_n(const RowDesc<T>* d, int S, hipStream_t s) {
-    constexpr int RW = fast_rw<T>(N);
+    constexpr int RW = fast_rw<T, COMP>(N);
     const long long plane = npix_;
     if (geo_.rw == RW) {
       const dim3 grid(static_cast<unsigned>(sp_.ny / RW), S);
-      hipLaunchKernelGGL((k_row_fast<N, RW, T, COMP>), grid, dim3(N / 8 * RW), 0, s, d, plane,
+      hipLaunchKernelGGL((k_row_fast<N, RW, T, COMP>), grid, dim3(N / 4 * RW), 0, s, d, plane,
                          twx_.hi.template as<cx<T>>(), twx_.lo.template as<cx<T>>());
     } else {
       const dim3 grid(static_cast<unsigned>(sp_.ny), S);
-      hipLaunchKernelGGL((k_row_fast<N, 1, T, COMP>), grid, dim3(N / 8), 0, s, d, plane,
+      hipLaunchKernelGGL((k_row_fast<N, 1, T, COMP>), grid, dim3(N / 4), 0, s, d, plane,
                          twx_.hi.template as<cx<T>>(), twx_.lo.template as<cx<T>>());
     }
   }
@@ -696,11 +735,11 @@ class Engine final : public EngineBase {
     g.col_fast = fast_prec && fast_size<T>(s.ny);
     g.row_fast = fast_prec && fast_size<T>(s.nx);
     if (g.col_fast) {
-      const int w = fast_w<T>(static_cast<int>(s.ny));
+      const int w = fast_w<T, COMP>(static_cast<int>(s.ny));
       g.col_w = (s.nx % w == 0) ? w : 1;
     }
     if (g.row_fast) {
-      const int r = fast_rw<T>(static_cast<int>(s.nx));
+      const int r = fast_rw<T, COMP>(static_cast<int>(s.nx));
       g.rw = (s.ny % r == 0) ? r : 1;
     }
     const int elems = Budget<T>::elems;
@@ -836,39 +875,64 @@ class Engine final : public EngineBase {
     return matrix_prod(kind, mr, ms);
   }
 
+  TableKey table_key(int kind, int mr, int ms) const {
+    TableKey k;
+    k.device = sp_.device;
+    k.precision = COMP ? 32 : (sizeof(T) == 8 ? 64 : 31);
+    k.kind = kind;
+    k.ny = sp_.ny;
+    k.nx = sp_.nx;
+    k.dy = sp_.dy;
+    k.dx = sp_.dx;
+    k.dz = sp_.dz;
+    k.k0 = sp_.k0;
+    k.nr_re = media_[mr].real();
+    k.nr_im = media_[mr].imag();
+    k.ns_re = media_[ms].real();
+    k.ns_im = media_[ms].imag();
+    return k;
+  }
+
   Prod<T> phase_prod(int m) {
     auto it = phase_ops_.find(m);
     if (it == phase_ops_.end()) {
-      OpTable t;
-      t.hi.alloc(npix_ * sizeof(cx<T>));
-      t.lo.alloc(npix_ * sizeof(cx<T>));
-      const c128 n = media_[m];
-      launch_1d(k_build_phase<T>, npix_, kx_.as<double>(), ky_.as<double>(), static_cast<int>(sp_.ny),
-                static_cast<int>(sp_.nx), zd{n.real(), n.imag()}, sp_.k0, sp_.dz, 1.0 / static_cast<double>(npix_),
-                t.hi.as<cx<T>>(), t.lo.as<cx<T>>());
+      auto t = shared_table(table_key(-1, m, m), [&] {
+        auto nt = std::make_shared<OpTable>();
+        nt->hi.alloc(npix_ * sizeof(cx<T>));
+        nt->lo.alloc(npix_ * sizeof(cx<T>));
+        const c128 n = media_[m];
+        launch_1d(k_build_phase<T>, npix_, kx_.as<double>(), ky_.as<double>(), static_cast<int>(sp_.ny),
+                  static_cast<int>(sp_.nx), zd{n.real(), n.imag()}, sp_.k0, sp_.dz,
+                  1.0 / static_cast<double>(npix_), nt->hi.as<cx<T>>(), nt->lo.as<cx<T>>());
+        HIP_CHECK(hipStreamSynchronize(stream_));
+        return nt;
+      });
       ops_bytes_ += 2 * npix_ * sizeof(cx<T>);
       it = phase_ops_.emplace(m, std::move(t)).first;
     }
-    return Prod<T>{0, it->second.hi.as<cx<T>>(), it->second.lo.as<cx<T>>()};
+    return Prod<T>{0, it->second->hi.as<cx<T>>(), it->second->lo.as<cx<T>>()};
   }
 
   Prod<T> matrix_prod(int kind, int mr, int ms) {
     const long long key = (static_cast<long long>(kind) << 40) | (static_cast<long long>(mr) << 20) | ms;
     auto it = matrix_ops_.find(key);
     if (it == matrix_ops_.end()) {
-      OpTable t;
-      t.hi.alloc(3 * npix_ * sizeof(cx<T>));
-      t.lo.alloc(3 * npix_ * sizeof(cx<T>));
-      const c128 nr = media_[mr], ns = media_[ms];
-      launch_1d(k_build_matrix<T>, npix_, kx_.as<double>(), ky_.as<double>(), static_cast<int>(sp_.ny),
-                static_cast<int>(sp_.nx), kind, zd{nr.real(), nr.imag()}, zd{ns.real(), ns.imag()}, sp_.k0, sp_.dz,
-                1.0 / static_cast<double>(npix_), t.hi.as<cx<T>>(), t.lo.as<cx<T>>());
+      auto t = shared_table(table_key(kind, mr, ms), [&] {
+        auto nt = std::make_shared<OpTable>();
+        nt->hi.alloc(3 * npix_ * sizeof(cx<T>));
+        nt->lo.alloc(3 * npix_ * sizeof(cx<T>));
+        const c128 nr = media_[mr], ns = media_[ms];
+        launch_1d(k_build_matrix<T>, npix_, kx_.as<double>(), ky_.as<double>(), static_cast<int>(sp_.ny),
+                  static_cast<int>(sp_.nx), kind, zd{nr.real(), nr.imag()}, zd{ns.real(), ns.imag()}, sp_.k0,
+                  sp_.dz, 1.0 / static_cast<double>(npix_), nt->hi.as<cx<T>>(), nt->lo.as<cx<T>>());
+        HIP_CHECK(hipStreamSynchronize(stream_));
+        return nt;
+      });
       ops_bytes_ += 6 * npix_ * sizeof(cx<T>);
       it = matrix_ops_.emplace(key, std::move(t)).first;
     }
-    return Prod<T>{1, it->second.hi.as<cx<T>>(), it->second.lo.as<cx<T>>()};
+    return Prod<T>{1, it->second->hi.as<cx<T>>(), it->second->lo.as<cx<T>>()};
   }
-
 
   long long list_of(long long q) const {
     const Boundary& B = bnd_[q];
@@ -893,8 +957,8 @@ class Engine final : public EngineBase {
   std::unordered_map<long long, long long> list_off_;
   long long total_sparse_ = 0;
   DevBuf tsel_, sp_pix_, sp_rid_, rowptr_;
-  std::map<int, OpTable> phase_ops_;
-  std::map<long long, OpTable> matrix_ops_;
+  std::map<int, std::shared_ptr<OpTable>> phase_ops_;
+  std::map<long long, std::shared_ptr<OpTable>> matrix_ops_;
   size_t ops_bytes_ = 0;
   DevBuf prods_;
   int max_prod_ = 1;

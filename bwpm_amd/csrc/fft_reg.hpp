@@ -9,8 +9,9 @@
 // and the last pass are radix 8, so the input is taken in natural order from registers and the
 // last pass leaves A[t + T·r] in v[r] (natural order) without touching LDS: P − 1 LDS exchanges.
 //
-// The two field components are transformed one after the other inside each butterfly, which
-// halves the live temporaries. Twiddles are read from a table the kernel stages in LDS once.
+// In the px2 form the two field components are transformed one after the other inside each
+// butterfly, which halves the live temporaries; the cx form transforms a single component. Twiddles
+// are read from a table the kernel stages in LDS once (or from global memory).
 // LDS index i is padded to i + i/8 to spread the stride-8 writes of the first pass over banks.
 #pragma once
 
@@ -38,17 +39,16 @@ constexpr int reg_lds_len() {
 }
 __device__ __forceinline__ int lpad(int i) { return i + (i >> 3); }
 
-// LDS twiddle table of a workgroup: hi (and lo for the compensated FP32 path), N entries each.
-template <typename T, bool COMP>
-struct TwLds {
-  cx<T>* hi;
-  cx<T>* lo;
-};
+// Twiddle exponents e = r·k·N/(Ns·R) of every pass stay below (R − 1)·N/R ≤ 7N/8.
+template <int N>
+constexpr int reg_tw_len() {
+  return N < 8 ? N : 7 * N / 8;
+}
 
 template <int N, bool COMP, typename T>
 __device__ __forceinline__ void stage_twiddles(cx<T>* sh_hi, cx<T>* sh_lo, const cx<T>* __restrict__ g_hi,
                                                const cx<T>* __restrict__ g_lo) {
-  for (int i = threadIdx.x; i < N; i += blockDim.x) {
+  for (int i = threadIdx.x; i < reg_tw_len<N>(); i += blockDim.x) {
     sh_hi[i] = g_hi[i];
     if constexpr (COMP) sh_lo[i] = g_lo[i];
   }
@@ -120,12 +120,65 @@ __device__ __forceinline__ void reg_pass(px2<T> (&v)[8], px2<T>* lds, int t, con
   }
 }
 
+// One field component per thread: the same passes on v[m] = a[t + T·m] of a single complex line
+// (the slab-step kernels give the two components of a pixel to two adjacent lanes).
+template <int N, int PASS, int NS, bool COMP, bool INV, typename T>
+__device__ __forceinline__ void reg_pass1(cx<T> (&v)[8], cx<T>* lds, int t, const cx<T>* twh, const cx<T>* twl) {
+  using PL = RegPlan<N>;
+  if constexpr (PASS < PL::P) {
+    constexpr int R = PL::rad[PASS];
+    constexpr int B = 8 / R;
+    constexpr int TT = N / 8;
+    constexpr bool last = PASS == PL::P - 1;
+    constexpr bool in_regs = last && R == 8;
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+      const int j = t + TT * b;
+      const int k = j & (NS - 1);
+      cx<T> x[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) x[r] = v[b + r * B];
+      if constexpr (NS > 1) {
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          const int e = r * k * (N / (NS * R));
+          const cx<T> wh = INV ? conj(twh[e]) : twh[e];
+          cx<T> wl{};
+          if constexpr (COMP) wl = INV ? conj(twl[e]) : twl[e];
+          x[r] = cmul2<COMP>(x[r], wh, wl);
+        }
+      }
+      dft_r<R, COMP, INV>(x);
+      if constexpr (in_regs) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[r] = x[r];
+      } else {
+        const int base = (j - k) * R + k;
+#pragma unroll
+        for (int r = 0; r < R; ++r) lds[lpad(base + r * NS)] = x[r];
+      }
+    }
+    if constexpr (!in_regs) {
+      __syncthreads();
+#pragma unroll
+      for (int m = 0; m < 8; ++m) v[m] = lds[lpad(t + TT * m)];
+      __syncthreads();
+    }
+    reg_pass1<N, PASS + 1, NS * R, COMP, INV>(v, lds, t, twh, twl);
+  }
+}
+
 // Transform one line held as v[m] = a[t + T·m]; on return v[m] = A[t + T·m]. All threads of the
 // workgroup must call it together (it synchronises the workgroup between passes). twh/twl point
-// to the twiddle table W_N^e (LDS or global).
+// to the twiddle table W_N^e (LDS or global). The px2 form transforms both components of a pixel
+// vector in one thread, the cx form one component.
 template <int N, bool COMP, bool INV, typename T>
 __device__ __forceinline__ void reg_fft(px2<T> (&v)[8], px2<T>* lds, int t, const cx<T>* twh, const cx<T>* twl) {
   reg_pass<N, 0, 1, COMP, INV>(v, lds, t, twh, twl);
+}
+template <int N, bool COMP, bool INV, typename T>
+__device__ __forceinline__ void reg_fft(cx<T> (&v)[8], cx<T>* lds, int t, const cx<T>* twh, const cx<T>* twl) {
+  reg_pass1<N, 0, 1, COMP, INV>(v, lds, t, twh, twl);
 }
 
 }  // namespace bwpm

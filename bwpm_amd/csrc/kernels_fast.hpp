@@ -1,11 +1,15 @@
-// Slab-step kernels for power-of-two grids (register-resident FFT; see kernels.hpp for the split
-// of one slab step over the column and the row kernel, and fft_reg.hpp for the FFT).
+// Slab-step kernels for power-of-two grids: register-resident FFT (fft_reg.hpp) with one field
+// component per thread. See kernels.hpp for how one slab step is split over the column and the
+// row kernel.
 //
-// Register economy: each thread keeps one array of eight pixel vectors. A column step with several
-// products stores the column spectrum back over its own input positions of X and reloads it per
-// product; a row step with several transmitted products merges the selected pixels of each product
-// into the positions of product plane 0 (each thread touches only the pixels it owns); reflection
-// products are handled before the transmitted ones. Twiddles are staged in LDS once per workgroup.
+// Lanes 2i and 2i + 1 hold E_x and E_y of the same pixels, so a wavefront reads and writes whole
+// (E_x, E_y) pixel vectors contiguously. Each lane owns a complex line in LDS for the FFT
+// exchanges. The 2×2 interface operator needs the other component of the pixel, and gets it
+// from the partner lane with a DPP quad permutation. Register economy: a thread keeps eight
+// complex values. A column step with several products stores the column spectrum back over its
+// own input positions of X and reloads it per product. A row step with several transmitted
+// products merges the selected pixels of each product into the positions of product plane 0 (each
+// thread touches only what it owns). Reflection products are handled before the transmitted ones.
 #pragma once
 
 #include "fft_reg.hpp"
@@ -13,139 +17,219 @@
 
 namespace bwpm {
 
-// Column kernel: W adjacent columns per workgroup, N/8 threads per column (thread index t·W + c,
-// so a wavefront reads W·sizeof(px2) contiguous bytes per row).
+// ---- geometry shared by the kernels and their launches --------------------------------------
+
+// LDS line stride (complex entries) of a component line: the padded line plus a skew that spreads
+// the lines of one wavefront's lanes over the banks.
+template <int N, int SKEW>
+constexpr int fast_line_stride() {
+  return reg_lds_len<N>() + SKEW;
+}
+constexpr int kColSkew = 1;
+constexpr int kRowSkew = 4;
+
+// Static LDS of a kernel with `lines` component lines of n points (complex size csize).
+constexpr long long fast_lds_bytes(int n, int lines, int skew, int csize, bool comp) {
+  return static_cast<long long>(lines) * (n + n / 8 + skew) * csize +
+         static_cast<long long>(7 * n / 8) * csize * (comp ? 2 : 1);
+}
+
+// Lines (columns or rows) per workgroup: 256 threads (n/4 per line) where the line allows, and at
+// most 52 KB of LDS, so that three workgroups share a CU and hide each other's memory and barrier
+// waits.
+constexpr int fast_lines(int n, int skew, int csize, bool comp) {
+  int w = 1024 / n < 1 ? 1 : 1024 / n;
+  while (w > 1 && fast_lds_bytes(n, 2 * w, skew, csize, comp) > 53248) w /= 2;
+  return w;
+}
+
+// Workgroups are dispatched to the eight XCDs round-robin. Column groups that share cache lines
+// (W·sizeof(px2) < 128 B) are mapped so that neighbouring groups run on the same XCD and its L2:
+// XCD j takes the j-th contiguous eighth of the groups.
+__device__ __forceinline__ int xcd_group(int b, int n) {
+  return (n & 7) ? b : (b & 7) * (n >> 3) + (b >> 3);
+}
+
+// ---- lane-pair exchange (lanes 2i ↔ 2i + 1): DPP quad permutation [1, 0, 3, 2] ---------------
+
+__device__ __forceinline__ int pair_swap(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xF, 0xF, false);
+}
+__device__ __forceinline__ float pair_swap(float v) {
+  return __int_as_float(pair_swap(__float_as_int(v)));
+}
+__device__ __forceinline__ double pair_swap(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = pair_swap(static_cast<int>(b & 0xffffffffLL));
+  const int hi = pair_swap(static_cast<int>(b >> 32));
+  return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
+}
+template <typename T>
+__device__ __forceinline__ cx<T> pair_swap(cx<T> z) {
+  return cx<T>{pair_swap(z.x), pair_swap(z.y)};
+}
+
+// Operator product on one component (comp 0: E_x, 1: E_y) of spectrum entry k. The phase type is
+// the same for both components; the symmetric matrix type takes the partner lane's component:
+// E_x' = m00·E_x + m01·E_y, E_y' = m01·E_x + m11·E_y (terms in this order). Every lane of the
+// wavefront must call it with the same product (the swap reads the partner's register).
+template <typename T, bool COMP>
+__device__ __forceinline__ cx<T> apply_prod1(const Prod<T>& pd, unsigned k, int comp, cx<T> e) {
+  if (pd.type == 0) {
+    cx<T> pl{};
+    if constexpr (COMP) pl = gload(pd.lo, k);
+    return cmul2<COMP>(e, gload(pd.hi, k), pl);
+  }
+  const cx<T> o = pair_swap(e);
+  const cx<T> ea = comp ? o : e;
+  const cx<T> eb = comp ? e : o;
+  const unsigned i = 3u * k + static_cast<unsigned>(comp);
+  cx<T> l0{}, l1{};
+  if constexpr (COMP) {
+    l0 = gload(pd.lo, i);
+    l1 = gload(pd.lo, i + 1);
+  }
+  return cmul2<COMP>(ea, gload(pd.hi, i), l0) + cmul2<COMP>(eb, gload(pd.hi, i + 1), l1);
+}
+
+// Column kernel: W adjacent columns per workgroup, N/8 threads per column and component (thread
+// index (t·W + c)·2 + comp, so a wavefront reads W·sizeof(px2) contiguous bytes per row).
 template <int N, int W, typename T, bool COMP>
-__global__ __launch_bounds__(N / 8 * W) void k_col_fast(const ColDesc<T>* __restrict__ descs, int nx,
+__global__ __launch_bounds__(N / 4 * W) void k_col_fast(const ColDesc<T>* __restrict__ descs, int nx,
                                                         long long plane, const cx<T>* __restrict__ g_twh,
                                                         const cx<T>* __restrict__ g_twl) {
   constexpr int TT = N / 8;
-  constexpr int LL = reg_lds_len<N>();
-  __shared__ px2<T> sh[W * LL];
-  __shared__ cx<T> tw_hi[N];
-  __shared__ cx<T> tw_lo[COMP ? N : 1];
+  constexpr int LS = fast_line_stride<N, kColSkew>();
+  __shared__ cx<T> sh[2 * W * LS];
+  __shared__ cx<T> tw_hi[reg_tw_len<N>()];
+  __shared__ cx<T> tw_lo[COMP ? reg_tw_len<N>() : 1];
   const ColDesc<T> d = descs[blockIdx.y];
   if (!d.in) return;
   stage_twiddles<N, COMP>(tw_hi, tw_lo, g_twh, g_twl);
-  const int c = threadIdx.x % W, t = threadIdx.x / W;
-  const int x = blockIdx.x * W + c;
-  px2<T>* line = sh + c * LL;
-  px2<T>* xin = const_cast<px2<T>*>(d.in);
-  px2<T> s[8];
+  const int comp = threadIdx.x & 1;
+  const int q = threadIdx.x >> 1;
+  const int c = q % W, t = q / W;
+  const unsigned x = xcd_group(blockIdx.x, gridDim.x) * W + c;
+  cx<T>* line = sh + (2 * c + comp) * LS;
+  cx<T>* xio = reinterpret_cast<cx<T>*>(const_cast<px2<T>*>(d.in));
+  // pixel index of row t + T·m and complex index of this lane's component (planes < 2^31 values)
+  auto pix = [&](int m) { return static_cast<unsigned>(t + TT * m) * static_cast<unsigned>(nx) + x; };
+  auto at = [&](int m) { return 2u * pix(m) + static_cast<unsigned>(comp); };
+  const int nprod = uniform(d.nprod);
+  cx<T> s[8];
 #pragma unroll
-  for (int m = 0; m < 8; ++m) s[m] = d.in[static_cast<long long>(t + TT * m) * nx + x];
+  for (int m = 0; m < 8; ++m) s[m] = gload(xio, at(m));
   reg_fft<N, COMP, false>(s, line, t, tw_hi, tw_lo);
-  if (d.nprod > 1) {  // keep the spectrum in the thread's own positions of X
+  if (nprod > 1) {  // keep the spectrum in the thread's own positions of X
 #pragma unroll
-    for (int m = 0; m < 8; ++m) xin[static_cast<long long>(t + TT * m) * nx + x] = s[m];
+    for (int m = 0; m < 8; ++m) gstore(xio, at(m), s[m]);
   }
-  for (int o = 0; o < d.nprod; ++o) {
+  for (int o = 0; o < nprod; ++o) {
     if (o > 0) {
 #pragma unroll
-      for (int m = 0; m < 8; ++m) s[m] = xin[static_cast<long long>(t + TT * m) * nx + x];
+      for (int m = 0; m < 8; ++m) s[m] = gload(xio, at(m));
     }
-    const Prod<T> pd = d.prods[o];
+    const Prod<T>* pp = d.prods + o;
+    const Prod<T> pd{uniform(gload1(&pp->type, 0)), uniform(gload1(&pp->hi, 0)), uniform(gload1(&pp->lo, 0))};
 #pragma unroll
-    for (int m = 0; m < 8; ++m) s[m] = apply_prod<T, COMP>(pd, static_cast<long long>(t + TT * m) * nx + x, s[m]);
+    for (int m = 0; m < 8; ++m) s[m] = apply_prod1<T, COMP>(pd, pix(m), comp, s[m]);
     reg_fft<N, COMP, true>(s, line, t, tw_hi, tw_lo);
-    px2<T>* out = d.out + o * plane;
+    cx<T>* out = reinterpret_cast<cx<T>*>(d.out + o * plane);
 #pragma unroll
-    for (int m = 0; m < 8; ++m) out[static_cast<long long>(t + TT * m) * nx + x] = s[m];
+    for (int m = 0; m < 8; ++m) gstore(out, at(m), s[m]);
   }
 }
 
-// Row kernel: RW rows per workgroup, N/8 threads per row (thread index r·N/8 + t).
+// Row kernel: RW rows per workgroup, N/8 threads per row and component (thread index
+// (r·N/8 + t)·2 + comp).
 template <int N, int RW, typename T, bool COMP>
-__global__ __launch_bounds__(N / 8 * RW) void k_row_fast(const RowDesc<T>* __restrict__ descs, long long plane,
+__global__ __launch_bounds__(N / 4 * RW) void k_row_fast(const RowDesc<T>* __restrict__ descs, long long plane,
                                                          const cx<T>* __restrict__ g_twh,
                                                          const cx<T>* __restrict__ g_twl) {
   constexpr int TT = N / 8;
-  constexpr int LL = reg_lds_len<N>();
-  __shared__ px2<T> sh[RW * LL];
-  __shared__ cx<T> tw_hi[N];
-  __shared__ cx<T> tw_lo[COMP ? N : 1];
+  constexpr int LS = fast_line_stride<N, kRowSkew>();
+  __shared__ cx<T> sh[2 * RW * LS];
+  __shared__ cx<T> tw_hi[reg_tw_len<N>()];
+  __shared__ cx<T> tw_lo[COMP ? reg_tw_len<N>() : 1];
   const RowDesc<T> a = descs[blockIdx.y];
   if (!a.active) return;
   stage_twiddles<N, COMP>(tw_hi, tw_lo, g_twh, g_twl);
-  const int r = threadIdx.x / TT, t = threadIdx.x % TT;
+  const int comp = threadIdx.x & 1;
+  const int q = threadIdx.x >> 1;
+  const int r = q / TT, t = q % TT;
   const int y = blockIdx.x * RW + r;
-  const long long row = static_cast<long long>(y) * N;
-  px2<T>* line = sh + r * LL;
-  px2<T> acc[8];
+  const unsigned row = static_cast<unsigned>(y) * N;
+  cx<T>* line = sh + (2 * r + comp) * LS;
+  // complex index of (pixel, component) of this lane (planes < 2^31 values)
+  auto at = [&](int m) { return 2u * (row + t + TT * m) + static_cast<unsigned>(comp); };
+  auto cplane = [](const px2<T>* p) { return reinterpret_cast<const cx<T>*>(p); };
+  cx<T> acc[8];
 
   // 1. reflection products: row IFFT and gather of the interface pixels
   if (a.nr > 0 && a.rowptr && a.sp_store) {
-    const int e0 = a.rowptr[y], e1 = a.rowptr[y + 1];
+    const int e0 = gload1(a.rowptr, y), e1 = gload1(a.rowptr, y + 1);
+    cx<T>* store = reinterpret_cast<cx<T>*>(a.sp_store);
     for (int o = 0; o < a.nr; ++o) {
-      const px2<T>* src = a.prod + (a.nt + o) * plane;
+      const cx<T>* src = cplane(a.prod + (a.nt + o) * plane);
 #pragma unroll
-      for (int m = 0; m < 8; ++m) acc[m] = src[row + t + TT * m];
+      for (int m = 0; m < 8; ++m) acc[m] = gload(src, at(m));
       reg_fft<N, COMP, true>(acc, line, t, tw_hi, tw_lo);
 #pragma unroll
       for (int m = 0; m < 8; ++m) line[lpad(t + TT * m)] = acc[m];
       __syncthreads();
       for (int e = e0 + t; e < e1; e += TT)
-        if (a.sp_rid[e] == o) a.sp_store[e] = line[lpad(a.sp_pix[e] - y * N)];
+        if (gload1(a.sp_rid, e) == o) gstore(store, 2u * e + comp, line[lpad(gload1(a.sp_pix, e) - y * N)]);
       __syncthreads();
     }
   }
 
   // 2. transmitted field (or the start plane)
   if (a.nt == 0) {
+    const cx<T>* ein = cplane(a.e_in);
 #pragma unroll
-    for (int m = 0; m < 8; ++m) acc[m] = a.e_in ? a.e_in[row + t + TT * m] : px2<T>{};
+    for (int m = 0; m < 8; ++m) acc[m] = ein ? gload(ein, at(m)) : cx<T>{};
   } else if (a.nt == 1) {
+    const cx<T>* src = cplane(a.prod);
 #pragma unroll
-    for (int m = 0; m < 8; ++m) acc[m] = a.prod[row + t + TT * m];
+    for (int m = 0; m < 8; ++m) acc[m] = gload(src, at(m));
     reg_fft<N, COMP, true>(acc, line, t, tw_hi, tw_lo);
   } else {
-    px2<T>* merged = const_cast<px2<T>*>(a.prod);  // product plane 0 collects the selected pixels
+    cx<T>* merged = reinterpret_cast<cx<T>*>(const_cast<px2<T>*>(a.prod));  // product plane 0 collects
     for (int o = 0; o < a.nt; ++o) {
-      const px2<T>* src = a.prod + o * plane;
+      const cx<T>* src = cplane(a.prod + o * plane);
 #pragma unroll
-      for (int m = 0; m < 8; ++m) acc[m] = src[row + t + TT * m];
+      for (int m = 0; m < 8; ++m) acc[m] = gload(src, at(m));
       reg_fft<N, COMP, true>(acc, line, t, tw_hi, tw_lo);
 #pragma unroll
       for (int m = 0; m < 8; ++m)
-        if (a.tsel[row + t + TT * m] == o) merged[row + t + TT * m] = acc[m];
+        if (gload1(a.tsel, row + t + TT * m) == o) gstore(merged, at(m), acc[m]);
     }
 #pragma unroll
-    for (int m = 0; m < 8; ++m) acc[m] = merged[row + t + TT * m];
+    for (int m = 0; m < 8; ++m) acc[m] = gload(merged, at(m));
   }
 
   // 3. epilogue in the order of the reference: mask, Born increment, sparse sources, Born plane
+  const cx<T>* inc = cplane(a.inc);
   if (a.mask) {
 #pragma unroll
     for (int m = 0; m < 8; ++m) {
-      const T mk = a.mask[row + t + TT * m];
-      acc[m].a = acc[m].a * mk;
-      acc[m].b = acc[m].b * mk;
-      if (a.inc) {
-        const px2<T> dd = a.inc[row + t + TT * m];
-        acc[m].a = acc[m].a + dd.a * mk;
-        acc[m].b = acc[m].b + dd.b * mk;
-      }
+      const T mk = gload1(a.mask, row + t + TT * m);
+      acc[m] = acc[m] * mk;
+      if (inc) acc[m] = acc[m] + gload(inc, at(m)) * mk;
     }
-  } else if (a.inc) {
+  } else if (inc) {
 #pragma unroll
-    for (int m = 0; m < 8; ++m) {
-      const px2<T> dd = a.inc[row + t + TT * m];
-      acc[m].a = acc[m].a + dd.a;
-      acc[m].b = acc[m].b + dd.b;
-    }
+    for (int m = 0; m < 8; ++m) acc[m] = acc[m] + gload(inc, at(m));
   }
   if (a.rowptr && a.sp_add) {
+    const cx<T>* add = cplane(a.sp_add);
 #pragma unroll
     for (int m = 0; m < 8; ++m) line[lpad(t + TT * m)] = acc[m];
     __syncthreads();
-    const int e1 = a.rowptr[y + 1];
-    for (int e = a.rowptr[y] + t; e < e1; e += TT) {
-      const int x = a.sp_pix[e] - y * N;
-      const px2<T> v = a.sp_add[e];
-      px2<T> u = line[lpad(x)];
-      u.a = u.a + v.a;
-      u.b = u.b + v.b;
-      line[lpad(x)] = u;
+    const int e1 = gload1(a.rowptr, y + 1);
+    for (int e = gload1(a.rowptr, y) + t; e < e1; e += TT) {
+      const int xx = lpad(gload1(a.sp_pix, e) - y * N);
+      line[xx] = line[xx] + gload(add, 2u * e + comp);
     }
     __syncthreads();
 #pragma unroll
@@ -153,29 +237,29 @@ __global__ __launch_bounds__(N / 8 * RW) void k_row_fast(const RowDesc<T>* __res
     __syncthreads();
   }
   if (a.dense) {
+    const cx<T>* dense = cplane(a.dense);
 #pragma unroll
-    for (int m = 0; m < 8; ++m) {
-      const px2<T> dd = a.dense[row + t + TT * m];
-      acc[m].a = acc[m].a + dd.a;
-      acc[m].b = acc[m].b + dd.b;
-    }
+    for (int m = 0; m < 8; ++m) acc[m] = acc[m] + gload(dense, at(m));
   }
 
   // 4. real-space outputs
   if (a.e_out) {
+    cx<T>* o = reinterpret_cast<cx<T>*>(a.e_out);
 #pragma unroll
-    for (int m = 0; m < 8; ++m) a.e_out[row + t + TT * m] = acc[m];
+    for (int m = 0; m < 8; ++m) gstore(o, at(m), acc[m]);
   }
   if (a.e_out2) {
+    cx<T>* o = reinterpret_cast<cx<T>*>(a.e_out2);
 #pragma unroll
-    for (int m = 0; m < 8; ++m) a.e_out2[row + t + TT * m] = acc[m];
+    for (int m = 0; m < 8; ++m) gstore(o, at(m), acc[m]);
   }
 
   // 5. row FFT for the next slab step
   if (a.x_out) {
     reg_fft<N, COMP, false>(acc, line, t, tw_hi, tw_lo);
+    cx<T>* o = reinterpret_cast<cx<T>*>(a.x_out);
 #pragma unroll
-    for (int m = 0; m < 8; ++m) a.x_out[row + t + TT * m] = acc[m];
+    for (int m = 0; m < 8; ++m) gstore(o, at(m), acc[m]);
   }
 }
 
