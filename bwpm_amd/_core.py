@@ -80,6 +80,14 @@ _run_resident = _sig("bwpm_solver_run_resident", _p, _int, _int, ctypes.POINTER(
 _dev_bytes = _sig("bwpm_solver_device_bytes", _p, ctypes.POINTER(_i64))
 _copy_probe = _sig("bwpm_copy_probe", _i64, _int, _int, ctypes.POINTER(_d))
 _fetch = _sig("bwpm_solver_fetch", _p, _int, ctypes.POINTER(_p))
+_q_create = _sig("bwpm_queue_create", _int, _d, _int, _int, ctypes.POINTER(_p))
+_q_destroy = _sig("bwpm_queue_destroy", _p)
+_q_submit = _sig("bwpm_queue_submit", _p, _i64, _i64, _d, _d, _d, _d, _p, _i64, _p, _i64, _d, _d, _p, _p, _p, _int,
+                 _p, _int, ctypes.POINTER(_i64))
+_q_state = _sig("bwpm_queue_state", _p, _i64, ctypes.POINTER(_int))
+_q_wait = _sig("bwpm_queue_wait", _p, _i64, _d, ctypes.POINTER(_int))
+_q_result = _sig("bwpm_queue_result", _p, _i64, ctypes.POINTER(_p))
+_q_stats = _sig("bwpm_queue_stats", _p, _p)
 _batch = _sig("bwpm_batch_run", _p, _i64, _int, _int, _int, ctypes.POINTER(_d), ctypes.POINTER(_d),
                ctypes.POINTER(_d), ctypes.POINTER(_i64), ctypes.POINTER(_i64))
 
@@ -161,6 +169,79 @@ def longitudinal_wavenumber(index: complex, k0: float, k_perp_sq: Any) -> Any:
     return complex(out) if out.ndim == 0 else out
 
 
+def _solver_arrays(ny: int, nx: int, indices: Any, labels: Any, mask: Any, fluctuation: Any, labels_before: Any,
+                   precision: str) -> tuple:
+    """Checked, contiguous copies of a solve's arrays (shapes as cpp/python/bindings.cpp:100-115)."""
+    if precision not in PRECISIONS:
+        raise ValueError(f"precision must be one of {sorted(PRECISIONS)}")
+    lab = np.asarray(labels)
+    if lab.ndim != 3:
+        raise ValueError("labels must have shape (nz, ny, nx)")
+    lab = np.ascontiguousarray(lab, dtype=np.int32)
+    idx = np.ascontiguousarray(np.asarray(indices, dtype=np.complex128).ravel())
+    m = None if mask is None else _f64(mask)
+    if m is not None and m.size != int(ny) * int(nx):
+        raise ValueError("mask must have shape (ny, nx)")
+    f = None
+    if fluctuation is not None:
+        f = np.ascontiguousarray(np.asarray(fluctuation, dtype=np.complex128))
+        if f.shape != lab.shape:
+            raise ValueError("fluctuation must have the shape of labels")
+    if lab.shape[1:] != (int(ny), int(nx)):
+        raise ValueError("labels must have shape (nz, ny, nx)")
+    lb = None
+    if labels_before is not None:
+        lb = np.ascontiguousarray(np.asarray(labels_before), dtype=np.int32)
+        if lb.size != int(ny) * int(nx):
+            raise ValueError("labels_before must have shape (ny, nx)")
+    return idx, lab, m, f, lb
+
+
+def _collect(res: Any, ny: int, nx: int, keep_list: list[int]) -> dict[str, Any]:
+    """Result dict of a bwpm_result (freed here): the keys of cpp/python/bindings.cpp:71-85 plus stats."""
+    try:
+        shape = (ny, nx, 2)
+
+        def field_of(which: int) -> np.ndarray:
+            out = np.empty(shape, dtype=np.complex128)
+            _check(_res_field(res, which, _ptr(out)))
+            return out
+
+        out: dict[str, Any] = {"transmitted": field_of(0), "reflected": field_of(1)}
+        n = _i64(0)
+        _check(_res_iters(res, ctypes.byref(n)))
+        ti, ri = [], []
+        for it in range(n.value):
+            for which, lst in ((0, ti), (1, ri)):
+                a = np.empty(shape, dtype=np.complex128)
+                _check(_res_iter_field(res, which, it, _ptr(a)))
+                lst.append(a)
+        out["transmitted_iter"] = ti
+        out["reflected_iter"] = ri
+        fw: dict[int, np.ndarray] = {}
+        bw: dict[int, np.ndarray] = {}
+        present = _int(0)
+        for p in sorted(set(keep_list)):
+            for which, dct in ((0, fw), (1, bw)):
+                a = np.empty(shape, dtype=np.complex128)
+                _check(_res_plane(res, which, p, _ptr(a), ctypes.byref(present)))
+                if present.value:
+                    dct[p] = a
+        out["forward"] = fw
+        out["backward"] = bw
+        sv = _i64(0)
+        _check(_res_stored(res, ctypes.byref(sv)))
+        out["stored_values"] = int(sv.value)
+        ms, steps, gm = _d(0), _i64(0), _i64(0)
+        _check(_res_stats(res, ctypes.byref(ms), ctypes.byref(steps), ctypes.byref(gm)))
+        out["gpu_ms"] = float(ms.value)
+        out["slab_steps"] = int(steps.value)
+        out["gmres_iterations"] = int(gm.value)
+        return out
+    finally:
+        _res_free(res)
+
+
 class GpuSolver:
     """FPWPM/BWPM on one MI355X: `GpuSolver(ny, nx, dy, dx, dz, k0, indices, labels, index_before,
     mask, device=0, fluctuation=None, precision="fp64")` with `.run(field, iterations, keep=[])`.
@@ -173,28 +254,7 @@ class GpuSolver:
                  labels: Any, index_before: complex, mask: Any = None, device: int = 0,
                  fluctuation: Any = None, precision: str = "fp64", labels_before: Any = None) -> None:
         self._h = _p(None)
-        if precision not in PRECISIONS:
-            raise ValueError(f"precision must be one of {sorted(PRECISIONS)}")
-        lab = np.asarray(labels)
-        if lab.ndim != 3:
-            raise ValueError("labels must have shape (nz, ny, nx)")
-        lab = np.ascontiguousarray(lab, dtype=np.int32)
-        idx = np.ascontiguousarray(np.asarray(indices, dtype=np.complex128).ravel())
-        m = None if mask is None else _f64(mask)
-        if m is not None and m.size != int(ny) * int(nx):
-            raise ValueError("mask must have shape (ny, nx)")
-        f = None
-        if fluctuation is not None:
-            f = np.ascontiguousarray(np.asarray(fluctuation, dtype=np.complex128))
-            if f.shape != lab.shape:
-                raise ValueError("fluctuation must have the shape of labels")
-        if lab.shape[1:] != (int(ny), int(nx)):
-            raise ValueError("labels must have shape (nz, ny, nx)")
-        lb = None
-        if labels_before is not None:
-            lb = np.ascontiguousarray(np.asarray(labels_before), dtype=np.int32)
-            if lb.size != int(ny) * int(nx):
-                raise ValueError("labels_before must have shape (ny, nx)")
+        idx, lab, m, f, lb = _solver_arrays(ny, nx, indices, labels, mask, fluctuation, labels_before, precision)
         ib = complex(index_before)
         self.ny, self.nx, self.nz = int(ny), int(nx), int(lab.shape[0])
         self.precision = precision
@@ -239,47 +299,7 @@ class GpuSolver:
         return self._collect(res, [])
 
     def _collect(self, res: Any, keep_list: list[int]) -> dict[str, Any]:
-        try:
-            shape = (self.ny, self.nx, 2)
-
-            def field_of(which: int) -> np.ndarray:
-                out = np.empty(shape, dtype=np.complex128)
-                _check(_res_field(res, which, _ptr(out)))
-                return out
-
-            out: dict[str, Any] = {"transmitted": field_of(0), "reflected": field_of(1)}
-            n = _i64(0)
-            _check(_res_iters(res, ctypes.byref(n)))
-            ti, ri = [], []
-            for it in range(n.value):
-                for which, lst in ((0, ti), (1, ri)):
-                    a = np.empty(shape, dtype=np.complex128)
-                    _check(_res_iter_field(res, which, it, _ptr(a)))
-                    lst.append(a)
-            out["transmitted_iter"] = ti
-            out["reflected_iter"] = ri
-            fw: dict[int, np.ndarray] = {}
-            bw: dict[int, np.ndarray] = {}
-            present = _int(0)
-            for p in sorted(set(keep_list)):
-                for which, dct in ((0, fw), (1, bw)):
-                    a = np.empty(shape, dtype=np.complex128)
-                    _check(_res_plane(res, which, p, _ptr(a), ctypes.byref(present)))
-                    if present.value:
-                        dct[p] = a
-            out["forward"] = fw
-            out["backward"] = bw
-            sv = _i64(0)
-            _check(_res_stored(res, ctypes.byref(sv)))
-            out["stored_values"] = int(sv.value)
-            ms, steps, gm = _d(0), _i64(0), _i64(0)
-            _check(_res_stats(res, ctypes.byref(ms), ctypes.byref(steps), ctypes.byref(gm)))
-            out["gpu_ms"] = float(ms.value)
-            out["slab_steps"] = int(steps.value)
-            out["gmres_iterations"] = int(gm.value)
-            return out
-        finally:
-            _res_free(res)
+        return _collect(res, self.ny, self.nx, keep_list)
 
     # device-resident timing path (fields stay in HBM)
     def upload(self, field: Any) -> None:
@@ -319,3 +339,73 @@ def run_batch(solvers: list[GpuSolver], iterations: int, max_lane: int = 64, tim
         return float(ms.value)
     return {"wall_ms": ms.value, "col_ms": cms.value, "row_ms": rms.value, "col_launches": cl.value,
             "row_launches": rl.value}
+
+
+TICKET_STATES = ("pending", "admitted", "done", "failed")
+QUEUE_STATS = ("p50_ms", "p99_ms", "mean_ms", "updates_per_s", "completed", "failed", "pending", "running",
+               "used_bytes", "budget_bytes", "peak_bytes")
+
+
+class GpuQueue:
+    """Request queue with continuous batching on one device (the bwpm_queue_* entry points).
+
+    `submit(...)` takes the arguments of `GpuSolver` plus the input field and iterations and returns
+    a ticket at once; admission by HBM footprint, lane scheduling and the result copy run on the
+    library's own threads. `result(ticket)` returns the dict of `GpuSolver.run` (no kept planes)."""
+
+    def __init__(self, device: int = 0, hbm_budget: float | None = None, slots: int = 16, chunk: int = 64) -> None:
+        self._h = _p(None)
+        self.device = int(device)
+        _check(_q_create(int(device), float(hbm_budget or 0.0), int(slots), int(chunk), ctypes.byref(self._h)))
+        self._shape: dict[int, tuple[int, int]] = {}
+
+    def close(self) -> None:
+        if self._h and self._h.value:
+            _q_destroy(self._h)
+            self._h = _p(None)
+
+    def __del__(self) -> None:
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, ny: int, nx: int, dy: float, dx: float, dz: float, k0: float, indices: Any, labels: Any,
+               index_before: complex, mask: Any, field: Any, iterations: int, fluctuation: Any = None,
+               precision: str = "fp64", labels_before: Any = None) -> int:
+        idx, lab, m, f, lb = _solver_arrays(ny, nx, indices, labels, mask, fluctuation, labels_before, precision)
+        e = np.asarray(field)
+        if e.shape != (int(ny), int(nx), 2):
+            raise ValueError("field must have shape (ny, nx, 2)")
+        e = np.ascontiguousarray(e, dtype=np.complex128)
+        if int(iterations) < 0:
+            raise ValueError("iterations must be >= 0")
+        ib = complex(index_before)
+        t = _i64(0)
+        _check(_q_submit(self._h, int(ny), int(nx), float(dy), float(dx), float(dz), float(k0), _ptr(idx), idx.size,
+                         _ptr(lab), lab.shape[0], ib.real, ib.imag, None if m is None else _ptr(m),
+                         None if f is None else _ptr(f), None if lb is None else _ptr(lb), PRECISIONS[precision],
+                         _ptr(e), int(iterations), ctypes.byref(t)))
+        self._shape[int(t.value)] = (int(ny), int(nx))
+        return int(t.value)
+
+    def state(self, ticket: int) -> str:
+        st = _int(0)
+        _check(_q_state(self._h, int(ticket), ctypes.byref(st)))
+        return TICKET_STATES[st.value]
+
+    def wait(self, ticket: int, timeout: float | None = None) -> str:
+        st = _int(0)
+        _check(_q_wait(self._h, int(ticket), -1.0 if timeout is None else float(timeout), ctypes.byref(st)))
+        return TICKET_STATES[st.value]
+
+    def result(self, ticket: int) -> dict[str, Any]:
+        ny, nx = self._shape.pop(int(ticket))
+        res = _p(None)
+        _check(_q_result(self._h, int(ticket), ctypes.byref(res)))
+        return _collect(res, ny, nx, [])
+
+    def stats(self) -> dict[str, float]:
+        out = np.zeros(len(QUEUE_STATS), dtype=np.float64)
+        _check(_q_stats(self._h, _ptr(out)))
+        return dict(zip(QUEUE_STATS, (float(v) for v in out)))

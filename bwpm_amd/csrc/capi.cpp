@@ -4,6 +4,7 @@
 
 #include "../../include/bwpm_hip.h"
 #include "engine.hpp"
+#include "serve.hpp"
 
 using bwpm::c128;
 
@@ -15,6 +16,10 @@ struct bwpm_solver {
 struct bwpm_result {
   bwpm::RunOutput out;
   long long npix = 0;
+};
+
+struct bwpm_queue {
+  std::unique_ptr<bwpm::Queue> q;
 };
 
 namespace {
@@ -55,6 +60,37 @@ int interface_call(int which, const double* kx, const double* ky, int64_t n, dou
     (void)hipGetDevice(&dev);
     bwpm::interface_matrices(which, kx, ky, n, c128(a, b), c128(c, d), k0, reinterpret_cast<c128*>(out), dev);
   });
+}
+
+
+bwpm::Spec make_spec(int64_t ny, int64_t nx, double dy, double dx, double dz, double k0, const double* indices,
+                   int64_t m, const int32_t* labels, int64_t nz, double ib_re, double ib_im, const double* mask,
+                   const double* fluct, const int32_t* labels_before, int device, int precision) {
+  bwpm::require(ny >= 1 && nx >= 1 && nz >= 1, "labels must have shape (nz, ny, nx)");
+  bwpm::require(m >= 0 && (m == 0 || indices), "indices missing");
+  bwpm::require(labels != nullptr, "labels missing");
+  bwpm::require(precision == BWPM_FP64 || precision == BWPM_FP32 || precision == BWPM_FP32_PLAIN,
+                "precision must be BWPM_FP64, BWPM_FP32 or BWPM_FP32_PLAIN");
+  bwpm::Spec s;
+  s.ny = ny;
+  s.nx = nx;
+  s.nz = nz;
+  s.dy = dy;
+  s.dx = dx;
+  s.dz = dz;
+  s.k0 = k0;
+  s.indices.assign(reinterpret_cast<const c128*>(indices), reinterpret_cast<const c128*>(indices) + m);
+  s.labels.assign(labels, labels + nz * ny * nx);
+  s.index_before = c128(ib_re, ib_im);
+  if (mask) s.mask.assign(mask, mask + ny * nx);
+  if (fluct) {
+    const c128* f = reinterpret_cast<const c128*>(fluct);
+    s.fluct.assign(f, f + nz * ny * nx);
+  }
+  if (labels_before) s.labels_before.assign(labels_before, labels_before + ny * nx);
+  s.precision = precision;
+  s.device = device;
+  return s;
 }
 
 }  // namespace
@@ -118,34 +154,14 @@ int bwpm_solver_create_ex(int64_t ny, int64_t nx, double dy, double dx, double d
   return guarded([&] {
     bwpm::require(out != nullptr, "null output handle");
     *out = nullptr;
-    bwpm::require(ny >= 1 && nx >= 1 && nz >= 1, "labels must have shape (nz, ny, nx)");
-    bwpm::require(m >= 0 && (m == 0 || indices), "indices missing");
-    bwpm::require(labels != nullptr, "labels missing");
-    bwpm::Spec s;
-    s.ny = ny;
-    s.nx = nx;
-    s.nz = nz;
-    s.dy = dy;
-    s.dx = dx;
-    s.dz = dz;
-    s.k0 = k0;
-    s.indices.assign(reinterpret_cast<const c128*>(indices), reinterpret_cast<const c128*>(indices) + m);
-    s.labels.assign(labels, labels + nz * ny * nx);
-    s.index_before = c128(ib_re, ib_im);
-    if (mask) s.mask.assign(mask, mask + ny * nx);
-    if (fluct) {
-      const c128* f = reinterpret_cast<const c128*>(fluct);
-      s.fluct.assign(f, f + nz * ny * nx);
-    }
-    if (labels_before) s.labels_before.assign(labels_before, labels_before + ny * nx);
-    s.precision = precision;
-    s.device = device;
+    bwpm::Spec s = make_spec(ny, nx, dy, dx, dz, k0, indices, m, labels, nz, ib_re, ib_im, mask, fluct,
+                             labels_before, device, precision);
     int count = 0;
     if (hipGetDeviceCount(&count) != hipSuccess || count == 0)
       throw bwpm::Error(bwpm::kEdevice, "no GPU device available");
     bwpm::require(device >= 0 && device < count, "device index out of range");
     auto h = std::make_unique<bwpm_solver>();
-    h->engine = bwpm::make_engine(s);
+    h->engine = bwpm::make_engine(std::move(s));
     h->device = device;
     *out = h.release();
   });
@@ -288,6 +304,77 @@ int bwpm_copy_probe(int64_t bytes, int reps, int device, double* ms) {
   return guarded([&] {
     bwpm::require(ms != nullptr, "null argument");
     *ms = bwpm::copy_probe(bytes, reps, device);
+  });
+}
+
+int bwpm_queue_create(int device, double hbm_budget_bytes, int slots, int chunk, bwpm_queue** out) {
+  return guarded([&] {
+    bwpm::require(out != nullptr, "null output handle");
+    *out = nullptr;
+    auto h = std::make_unique<bwpm_queue>();
+    h->q = std::make_unique<bwpm::Queue>(device, hbm_budget_bytes, slots, chunk);
+    *out = h.release();
+  });
+}
+
+int bwpm_queue_destroy(bwpm_queue* q) {
+  return guarded([&] { delete q; });
+}
+
+int bwpm_queue_submit(bwpm_queue* q, int64_t ny, int64_t nx, double dy, double dx, double dz, double k0,
+                      const double* indices, int64_t m, const int32_t* labels, int64_t nz, double ib_re, double ib_im,
+                      const double* mask, const double* fluct, const int32_t* labels_before, int precision,
+                      const double* field, int iterations, int64_t* ticket) {
+  return guarded([&] {
+    bwpm::require(q && field && ticket, "null argument");
+    bwpm::Spec s = make_spec(ny, nx, dy, dx, dz, k0, indices, m, labels, nz, ib_re, ib_im, mask, fluct,
+                             labels_before, 0, precision);
+    const c128* f = reinterpret_cast<const c128*>(field);
+    *ticket = q->q->submit(std::move(s), std::vector<c128>(f, f + 2 * ny * nx), iterations);
+  });
+}
+
+int bwpm_queue_state(bwpm_queue* q, int64_t ticket, int* state) {
+  return guarded([&] {
+    bwpm::require(q && state, "null argument");
+    *state = q->q->state(ticket);
+  });
+}
+
+int bwpm_queue_wait(bwpm_queue* q, int64_t ticket, double timeout_s, int* state) {
+  return guarded([&] {
+    bwpm::require(q && state, "null argument");
+    *state = q->q->wait(ticket, timeout_s);
+  });
+}
+
+int bwpm_queue_result(bwpm_queue* q, int64_t ticket, bwpm_result** out) {
+  return guarded([&] {
+    bwpm::require(q && out, "null argument");
+    *out = nullptr;
+    auto r = std::make_unique<bwpm_result>();
+    r->out = q->q->take(ticket);
+    r->npix = static_cast<long long>(r->out.transmitted.size() / 2);
+    *out = r.release();
+  });
+}
+
+int bwpm_queue_stats(bwpm_queue* q, double* out) {
+  return guarded([&] {
+    bwpm::require(q && out, "null argument");
+    const bwpm::QueueStats st = q->q->stats();
+    const double v[BWPM_QUEUE_NSTATS] = {st.p50_ms,
+                                         st.p99_ms,
+                                         st.mean_ms,
+                                         st.updates_per_s,
+                                         static_cast<double>(st.completed),
+                                         static_cast<double>(st.failed),
+                                         static_cast<double>(st.pending),
+                                         static_cast<double>(st.running),
+                                         st.used_bytes,
+                                         st.budget_bytes,
+                                         st.peak_bytes};
+    std::memcpy(out, v, sizeof(v));
   });
 }
 

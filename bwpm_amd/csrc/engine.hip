@@ -61,13 +61,42 @@ __global__ void k_kz_eval(zd n, double k0, const double* kp2, long long cnt, zd*
 }
 
 
-std::unique_ptr<EngineBase> make_engine(const Spec& spec) {
+std::unique_ptr<EngineBase> make_engine(Spec spec) {
   switch (spec.precision) {
-    case 64: return make_engine_f64(spec);
-    case 32: return make_engine_f32(spec, true);
-    case 31: return make_engine_f32(spec, false);
+    case 64: return make_engine_f64(std::move(spec));
+    case 32: return make_engine_f32(std::move(spec), true);
+    case 31: return make_engine_f32(std::move(spec), false);
     default: throw Error(kEinval, "precision must be 64, 32 or 31");
   }
+}
+
+void run_continuous_lane(int precision, LaneFeed& feed, int slots, int chunk, hipStream_t stream) {
+  if (precision == 64)
+    run_continuous_lane_f64(feed, slots, chunk, stream);
+  else
+    run_continuous_lane_f32(feed, slots, chunk, stream, precision == 32);
+}
+
+size_t estimate_device_bytes(const Spec& s, int iterations) {
+  const size_t elem = s.precision == 64 ? 16 : 8;  // complex
+  const size_t npix = static_cast<size_t>(s.ny) * static_cast<size_t>(s.nx);
+  const size_t plane = 2 * npix * elem;            // (E_x, E_y) plane
+  const size_t nit = static_cast<size_t>(std::max(iterations, 1));
+  std::vector<c128> media;
+  for (const c128& v : s.indices)
+    if (std::find(media.begin(), media.end(), v) == media.end()) media.push_back(v);
+  if (std::find(media.begin(), media.end(), s.index_before) == media.end()) media.push_back(s.index_before);
+  const size_t m = media.size();
+  // staging (complex128), input, X, e, up to four product planes, per-iteration outputs
+  size_t bytes = 2 * npix * 16 + plane * (3 + 4 + 2 * nit);
+  // phase tables per medium and 2×2 tables per ordered medium pair and kind (hi and lo parts)
+  bytes += m * 2 * npix * elem + (m > 1 ? m * (m - 1) * 4 * 6 * npix * elem : 0);
+  // sparse interface values (two sets) and lists, bounded by two planes
+  bytes += 2 * plane + 2 * npix * 8;
+  // Born: potential and emitted planes per slab, per-host tables, Krylov basis and work planes
+  if (!s.fluct.empty())
+    bytes += static_cast<size_t>(s.nz) * (npix * elem + 2 * plane) + m * npix * 16 * elem + (31 + 12) * plane;
+  return bytes;
 }
 
 double run_batch(std::vector<EngineBase*>& engines, int iterations, int max_lane, bool time_kernels,

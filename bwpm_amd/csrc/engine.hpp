@@ -69,15 +69,49 @@ class EngineBase {
                           bool time_kernels, KernelTimes* kt) = 0;
 };
 
-std::unique_ptr<EngineBase> make_engine(const Spec& spec);
-std::unique_ptr<EngineBase> make_engine_f32(const Spec& spec, bool compensated);
-std::unique_ptr<EngineBase> make_engine_f64(const Spec& spec);
+// The engine takes the spec over (the label volume is moved, not copied).
+std::unique_ptr<EngineBase> make_engine(Spec spec);
+std::unique_ptr<EngineBase> make_engine_f32(Spec spec, bool compensated);
+std::unique_ptr<EngineBase> make_engine_f64(Spec spec);
 
 // Batched solve of many engines: members are grouped into lanes of equal (precision, grid,
 // device, no δn); each lane runs in lockstep on its own stream, lanes run concurrently.
 // Results are left on the device (download per engine). Returns the wall time in ms.
 double run_batch(std::vector<EngineBase*>& engines, int iterations, int max_lane, bool time_kernels = false,
                  KernelTimes* kt = nullptr);
+
+// ---- continuous lanes (serving) ---------------------------------------------------------------
+
+// A solve admitted into a lane: its engine (field uploaded), its iterations and the caller's tag.
+struct LaneMember {
+  EngineBase* engine = nullptr;
+  int iterations = 0;
+  long long tag = 0;
+};
+
+// What a continuous lane asks of its owner.
+class LaneFeed {
+ public:
+  virtual ~LaneFeed() = default;
+  // A prepared solve for a free slot, or a member with engine == nullptr; must not block.
+  virtual LaneMember next() = 0;
+  // The member's last slab step is on the lane stream before `done` (the callee owns the event).
+  virtual void finished(const LaneMember& m, hipEvent_t done) = 0;
+  // Called while the lane is empty: blocks until work may be there; false ends the lane.
+  virtual bool wait() = 0;
+};
+
+// Runs a lane of up to `slots` solves of one grid and precision on `stream`: members join free
+// slots between chunks of `chunk` slab steps and leave when their program ends. Every member's
+// result is bit-identical to its solve alone (each slot's work depends on its own descriptors
+// only). Solves with index fluctuations are not accepted (they run one per lane).
+void run_continuous_lane(int precision, LaneFeed& feed, int slots, int chunk, hipStream_t stream);
+void run_continuous_lane_f32(LaneFeed& feed, int slots, int chunk, hipStream_t stream, bool compensated);
+void run_continuous_lane_f64(LaneFeed& feed, int slots, int chunk, hipStream_t stream);
+
+// Upper bound of the device bytes of one solve of `spec` with `iterations` (admission by HBM
+// footprint before the engine exists).
+size_t estimate_device_bytes(const Spec& spec, int iterations);
 
 // FP64 evaluation of the exported interface functions on the device.
 void interface_matrices(int which, const double* kx, const double* ky, long long n, c128 n_in,

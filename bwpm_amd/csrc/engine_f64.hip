@@ -5,7 +5,13 @@ namespace bwpm {
 
 template class Engine<double, false>;
 
-std::unique_ptr<EngineBase> make_engine_f64(const Spec& spec) { return std::make_unique<Engine<double, false>>(spec); }
+std::unique_ptr<EngineBase> make_engine_f64(Spec spec) {
+  return std::make_unique<Engine<double, false>>(std::move(spec));
+}
+
+void run_continuous_lane_f64(LaneFeed& feed, int slots, int chunk, hipStream_t stream) {
+  Engine<double, false>::continuous(feed, slots, chunk, stream);
+}
 
 }  // namespace bwpm
 

@@ -61,6 +61,31 @@ class DevBuf {
   size_t n_ = 0;
 };
 
+// Page-locked host memory (allocated pinned; existing host memory is never registered).
+class PinnedBuf {
+ public:
+  explicit PinnedBuf(size_t bytes) {
+    if (bytes == 0) return;
+    const hipError_t e = hipHostMalloc(&p_, bytes, hipHostMallocDefault);
+    if (e != hipSuccess) {
+      p_ = nullptr;
+      throw Error(kEnomem, "pinned host allocation of " + std::to_string(bytes) + " bytes failed: " +
+                               hipGetErrorString(e));
+    }
+  }
+  PinnedBuf(const PinnedBuf&) = delete;
+  PinnedBuf& operator=(const PinnedBuf&) = delete;
+  PinnedBuf(PinnedBuf&& o) noexcept : p_(o.p_) { o.p_ = nullptr; }
+  ~PinnedBuf() {
+    if (p_) (void)hipHostFree(p_);
+  }
+  template <typename U>
+  U* as() const { return static_cast<U*>(p_); }
+
+ private:
+  void* p_ = nullptr;
+};
+
 template <typename U>
 inline DevBuf upload_vec(const std::vector<U>& v, hipStream_t s) {
   DevBuf b(v.size() * sizeof(U));
@@ -181,8 +206,7 @@ inline int pow2_floor(int v) {
 
 template <typename T>
 inline bool fast_size(long long n) {
-  // FP64 lines of 4096 would exceed the 160 KB of LDS a workgroup may use
-  return n == 64 || n == 128 || n == 256 || n == 512 || n == 1024 || n == 2048 || (n == 4096 && sizeof(T) == 4);
+  return n == 64 || n == 128 || n == 256 || n == 512 || n == 1024 || n == 2048 || n == 4096;
 }
 // columns / rows per workgroup of the register kernels (kernels_fast.hpp)
 template <typename T, bool COMP>
@@ -208,6 +232,81 @@ struct Geometry {
 
 using namespace detail;
 
+// Launches of one slab step over a lane of S solves of one grid: the kernel geometry and the
+// twiddle tables (shared, so a lane runner may outlive the solve it took them from).
+template <typename T, bool COMP>
+struct Launcher {
+  long long ny = 0, nx = 0, npix = 0;
+  Geometry geo;
+  std::shared_ptr<Twiddles<T>> twx, twy;
+
+  void col(const ColDesc<T>* d, int S, hipStream_t s) const {
+    if (geo.col_fast && COMP == (sizeof(T) == 4)) {
+      switch (ny) {
+        case 64: col_fast_n<64>(d, S, s); return;
+        case 128: col_fast_n<128>(d, S, s); return;
+        case 256: col_fast_n<256>(d, S, s); return;
+        case 512: col_fast_n<512>(d, S, s); return;
+        case 1024: col_fast_n<1024>(d, S, s); return;
+        case 2048: col_fast_n<2048>(d, S, s); return;
+        default: col_fast_n<4096>(d, S, s); return;
+      }
+    }
+    // generic: one LDS buffer of ny × ls per block (products reload the spectrum from X)
+    ColGeom g{static_cast<int>(ny), static_cast<int>(nx), geo.cols2, geo.ls_col2, npix, twy->plan};
+    const size_t lds = static_cast<size_t>(ny) * g.ls * sizeof(cx<T>);
+    const dim3 grid(static_cast<unsigned>((nx + g.cols - 1) / g.cols), S);
+    hipLaunchKernelGGL((k_col_step<T, COMP>), grid, dim3(kThreads), lds, s, d, g);
+  }
+
+  void row(const RowDesc<T>* d, int S, hipStream_t s) const {
+    if (geo.row_fast && COMP == (sizeof(T) == 4)) {
+      switch (nx) {
+        case 64: row_fast_n<64>(d, S, s); return;
+        case 128: row_fast_n<128>(d, S, s); return;
+        case 256: row_fast_n<256>(d, S, s); return;
+        case 512: row_fast_n<512>(d, S, s); return;
+        case 1024: row_fast_n<1024>(d, S, s); return;
+        case 2048: row_fast_n<2048>(d, S, s); return;
+        default: row_fast_n<4096>(d, S, s); return;
+      }
+    }
+    RowGeom g{static_cast<int>(ny), static_cast<int>(nx), geo.rows, geo.ls_row, npix, twx->plan};
+    const size_t lds = 2 * static_cast<size_t>(nx) * g.ls * sizeof(cx<T>);
+    const dim3 grid(static_cast<unsigned>((ny + g.rows - 1) / g.rows), S);
+    hipLaunchKernelGGL((k_row_step<T, COMP>), grid, dim3(kThreads), lds, s, d, g);
+  }
+
+ private:
+  template <int N>
+  void col_fast_n(const ColDesc<T>* d, int S, hipStream_t s) const {
+    constexpr int W = fast_w<T, COMP>(N);
+    const cx<T>* th = twy->hi.template as<cx<T>>();
+    const cx<T>* tl = twy->lo.template as<cx<T>>();
+    if (geo.col_w == W) {
+      const dim3 grid(static_cast<unsigned>(nx / W), S);
+      hipLaunchKernelGGL((k_col_fast<N, W, T, COMP>), grid, dim3(N / 4 * W), 0, s, d, static_cast<int>(nx), npix, th,
+                         tl);
+    } else {
+      const dim3 grid(static_cast<unsigned>(nx), S);
+      hipLaunchKernelGGL((k_col_fast<N, 1, T, COMP>), grid, dim3(N / 4), 0, s, d, static_cast<int>(nx), npix, th, tl);
+    }
+  }
+  template <int N>
+  void row_fast_n(const RowDesc<T>* d, int S, hipStream_t s) const {
+    constexpr int RW = fast_rw<T, COMP>(N);
+    const cx<T>* th = twx->hi.template as<cx<T>>();
+    const cx<T>* tl = twx->lo.template as<cx<T>>();
+    if (geo.rw == RW) {
+      const dim3 grid(static_cast<unsigned>(ny / RW), S);
+      hipLaunchKernelGGL((k_row_fast<N, RW, T, COMP>), grid, dim3(N / 4 * RW), 0, s, d, npix, th, tl);
+    } else {
+      const dim3 grid(static_cast<unsigned>(ny), S);
+      hipLaunchKernelGGL((k_row_fast<N, 1, T, COMP>), grid, dim3(N / 4), 0, s, d, npix, th, tl);
+    }
+  }
+};
+
 // A lane step: one column launch and one row launch over every solve of the lane, or a Born
 // sub-step of a single-solve lane.
 enum OpKind : int { kOpPair = 0, kOpBornFwd = 1, kOpBornBwd = 2 };
@@ -224,8 +323,8 @@ struct Op {
 template <typename T, bool COMP>
 class Engine final : public EngineBase {
  public:
-  explicit Engine(const Spec& s) : sp_(s) {
-    HIP_CHECK(hipSetDevice(s.device));
+  explicit Engine(Spec s) : sp_(std::move(s)) {
+    HIP_CHECK(hipSetDevice(sp_.device));
     HIP_CHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
     try {
       setup();
@@ -500,8 +599,8 @@ class Engine final : public EngineBase {
       }
       const ColDesc<T>* dc = lane_col_.as<ColDesc<T>>() + g * S;
       const RowDesc<T>* dr = lane_row_.as<RowDesc<T>>() + g * S;
-      if (has_col[g]) timed(true, [&] { launch_col(dc, S, stream); });
-      if (has_row[g]) timed(false, [&] { launch_row(dr, S, stream); });
+      if (has_col[g]) timed(true, [&] { launch_.col(dc, S, stream); });
+      if (has_row[g]) timed(false, [&] { launch_.row(dr, S, stream); });
     }
     HIP_CHECK(hipEventRecord(t1, stream));
     HIP_CHECK(hipEventSynchronize(t1));
@@ -517,6 +616,122 @@ class Engine final : public EngineBase {
     return ms;
   }
 
+  // Continuous lane (see run_continuous_lane): members join between chunks of slab steps.
+  static void continuous(LaneFeed& feed, int S, int chunk, hipStream_t stream) {
+    require(S >= 1 && chunk >= 1, "lane slots and chunk must be >= 1");
+    std::vector<Engine*> slot(S, nullptr);
+    std::vector<LaneMember> mem(S);
+    std::vector<std::vector<Op<T>>> prog(S);
+    std::vector<size_t> pc(S, 0);
+    Launcher<T, COMP> L;
+    bool have_launcher = false;
+    const size_t n = static_cast<size_t>(chunk) * S;
+    DevBuf dcol(n * sizeof(ColDesc<T>)), drow(n * sizeof(RowDesc<T>));
+    // descriptor staging in pinned memory, two sets: a set is rewritten only after the chunk that
+    // copied it has completed (at most two chunks are in flight)
+    PinnedBuf hcol[2] = {PinnedBuf(n * sizeof(ColDesc<T>)), PinnedBuf(n * sizeof(ColDesc<T>))};
+    PinnedBuf hrow[2] = {PinnedBuf(n * sizeof(RowDesc<T>)), PinnedBuf(n * sizeof(RowDesc<T>))};
+    std::vector<char> has_col(chunk), has_row(chunk);
+    hipEvent_t prev = nullptr;
+    auto settle = [&] {
+      if (!prev) return;
+      HIP_CHECK(hipEventSynchronize(prev));
+      (void)hipEventDestroy(prev);
+      prev = nullptr;
+    };
+    int set = 0;
+    try {
+      for (;;) {
+        bool any = false;
+        for (int i = 0; i < S; ++i) {
+          if (!slot[i]) {
+            const LaneMember m = feed.next();
+            if (m.engine) {
+              auto* e = dynamic_cast<Engine*>(m.engine);
+              require(e != nullptr, "a lane needs solves of equal grid, precision and device");
+              if (have_launcher)
+                require(e->sp_.ny == L.ny && e->sp_.nx == L.nx, "a lane needs solves of equal grid");
+              prog[i] = e->program(m.iterations, {});
+              for (const Op<T>& o : prog[i])
+                require(o.kind == kOpPair, "solves with index fluctuations run one per lane");
+              HIP_CHECK(hipStreamSynchronize(e->stream_));
+              if (e->total_sparse_ > 0)
+                launch_1d_on(stream, k_fill_zero<T>, e->total_sparse_, e->fwd_vals_.template as<px2<T>>(),
+                             e->total_sparse_);
+              if (!have_launcher) {
+                L = e->launch_;
+                have_launcher = true;
+              }
+              slot[i] = e;
+              mem[i] = m;
+              pc[i] = 0;
+            }
+          }
+          any |= slot[i] != nullptr;
+        }
+        if (!any) {
+          settle();
+          if (!feed.wait()) break;
+          continue;
+        }
+        size_t G = 0;
+        for (int i = 0; i < S; ++i)
+          if (slot[i]) G = std::max(G, std::min<size_t>(chunk, prog[i].size() - pc[i]));
+        auto* hc = hcol[set].template as<ColDesc<T>>();
+        auto* hr = hrow[set].template as<RowDesc<T>>();
+        std::fill(hc, hc + G * S, ColDesc<T>{});
+        std::fill(hr, hr + G * S, RowDesc<T>{});
+        std::fill(has_col.begin(), has_col.end(), 0);
+        std::fill(has_row.begin(), has_row.end(), 0);
+        for (size_t g = 0; g < G; ++g)
+          for (int i = 0; i < S; ++i) {
+            if (!slot[i] || pc[i] + g >= prog[i].size()) continue;
+            const Op<T>& o = prog[i][pc[i] + g];
+            hc[g * S + i] = o.col;
+            hr[g * S + i] = o.row;
+            has_col[g] |= o.col.in != nullptr;
+            has_row[g] |= o.row.active != 0;
+          }
+        HIP_CHECK(hipMemcpyAsync(dcol.as<void>(), hc, G * S * sizeof(ColDesc<T>), hipMemcpyHostToDevice, stream));
+        HIP_CHECK(hipMemcpyAsync(drow.as<void>(), hr, G * S * sizeof(RowDesc<T>), hipMemcpyHostToDevice, stream));
+        for (size_t g = 0; g < G; ++g) {
+          if (has_col[g]) {
+            L.col(dcol.as<ColDesc<T>>() + g * S, S, stream);
+            HIP_CHECK(hipGetLastError());
+          }
+          if (has_row[g]) {
+            L.row(drow.as<RowDesc<T>>() + g * S, S, stream);
+            HIP_CHECK(hipGetLastError());
+          }
+        }
+        for (int i = 0; i < S; ++i) {
+          if (!slot[i]) continue;
+          if (pc[i] + G >= prog[i].size()) {
+            hipEvent_t done;
+            HIP_CHECK(hipEventCreateWithFlags(&done, hipEventDisableTiming));
+            HIP_CHECK(hipEventRecord(done, stream));
+            feed.finished(mem[i], done);
+            slot[i] = nullptr;
+          } else {
+            pc[i] += G;
+          }
+        }
+        hipEvent_t cur;
+        HIP_CHECK(hipEventCreateWithFlags(&cur, hipEventDisableTiming));
+        HIP_CHECK(hipEventRecord(cur, stream));
+        settle();
+        prev = cur;
+        set ^= 1;
+      }
+    } catch (...) {
+      if (prev) {
+        (void)hipEventSynchronize(prev);
+        (void)hipEventDestroy(prev);
+      }
+      throw;
+    }
+  }
+
  private:
   template <typename K, typename... A>
   void launch_1d(K kern, long long n, A... args) {
@@ -529,77 +744,6 @@ class Engine final : public EngineBase {
     const long long grid = (n + bs - 1) / bs;
     hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>(grid)), dim3(bs), 0, s, args...);
     HIP_CHECK(hipGetLastError());
-  }
-
-  // ---- launches over a lane of S solves
-  template <int N>
-  void col_fast_n(const ColDesc<T>* d, int S, hipStream_t s) {
-    constexpr int W = fast_w<T, COMP>(N);
-    const long long plane = npix_;
-    if (geo_.col_w == W) {
-      const dim3 grid(static_cast<unsigned>(sp_.nx / W), S);
-      hipLaunchKernelGGL((k_col_fast<N, W, T, COMP>), grid, dim3(N / 4 * W), 0, s, d, static_cast<int>(sp_.nx),
-                         plane, twy_.hi.template as<cx<T>>(), twy_.lo.template as<cx<T>>());
-    } else {
-      const dim3 grid(static_cast<unsigned>(sp_.nx), S);
-      hipLaunchKernelGGL((k_col_fast<N, 1, T, COMP>), grid, dim3(N / 4), 0, s, d, static_cast<int>(sp_.nx), plane,
-                         twy_.hi.template as<cx<T>>(), twy_.lo.template as<cx<T>>());
-    }
-  }
-  template <int N>
-  void row_fast_n(const RowDesc<T>* d, int S, hipStream_t s) {
-    constexpr int RW = fast_rw<T, COMP>(N);
-    const long long plane = npix_;
-    if (geo_.rw == RW) {
-      const dim3 grid(static_cast<unsigned>(sp_.ny / RW), S);
-      hipLaunchKernelGGL((k_row_fast<N, RW, T, COMP>), grid, dim3(N / 4 * RW), 0, s, d, plane,
-                         twx_.hi.template as<cx<T>>(), twx_.lo.template as<cx<T>>());
-    } else {
-      const dim3 grid(static_cast<unsigned>(sp_.ny), S);
-      hipLaunchKernelGGL((k_row_fast<N, 1, T, COMP>), grid, dim3(N / 4), 0, s, d, plane,
-                         twx_.hi.template as<cx<T>>(), twx_.lo.template as<cx<T>>());
-    }
-  }
-
-  void launch_col(const ColDesc<T>* d, int S, hipStream_t s) {
-    if (geo_.col_fast && COMP == (sizeof(T) == 4)) {
-      switch (sp_.ny) {
-        case 64: col_fast_n<64>(d, S, s); return;
-        case 128: col_fast_n<128>(d, S, s); return;
-        case 256: col_fast_n<256>(d, S, s); return;
-        case 512: col_fast_n<512>(d, S, s); return;
-        case 1024: col_fast_n<1024>(d, S, s); return;
-        case 2048: col_fast_n<2048>(d, S, s); return;
-        default:
-          if constexpr (sizeof(T) == 4) col_fast_n<4096>(d, S, s);
-          return;
-      }
-    }
-    // generic: the column block must fit every product count of the lane; use the two-buffer width
-    ColGeom g{static_cast<int>(sp_.ny), static_cast<int>(sp_.nx), geo_.cols2, geo_.ls_col2, npix_, twy_.plan};
-    const size_t lds = 2 * static_cast<size_t>(sp_.ny) * g.ls * sizeof(cx<T>);
-    const dim3 grid(static_cast<unsigned>((sp_.nx + g.cols - 1) / g.cols), S);
-    hipLaunchKernelGGL((k_col_step<T, COMP>), grid, dim3(kThreads), lds, s, d, g);
-  }
-
-  void launch_row(const RowDesc<T>* d, int S, hipStream_t s) {
-    if (geo_.row_fast && COMP == (sizeof(T) == 4)) {
-      switch (sp_.nx) {
-        case 64: row_fast_n<64>(d, S, s); return;
-        case 128: row_fast_n<128>(d, S, s); return;
-        case 256: row_fast_n<256>(d, S, s); return;
-        case 512: row_fast_n<512>(d, S, s); return;
-        case 1024: row_fast_n<1024>(d, S, s); return;
-        case 2048: row_fast_n<2048>(d, S, s); return;
-        default:
-          if constexpr (sizeof(T) == 4) row_fast_n<4096>(d, S, s);
-          return;
-      }
-    }
-    RowGeom g{static_cast<int>(sp_.ny), static_cast<int>(sp_.nx), geo_.rows, geo_.ls_row, npix_, twx_.plan};
-    const size_t lds = 2 * static_cast<size_t>(sp_.nx) * g.ls * sizeof(cx<T>);
-    const dim3 grid(static_cast<unsigned>((sp_.ny + g.rows - 1) / g.rows), S);
-    hipLaunchKernelGGL((k_row_step<T, COMP>), grid, dim3(kThreads), lds, s, d, g);
   }
 
   ColDesc<T> col_desc(const Boundary& B, bool forward, bool with_r) {
@@ -647,14 +791,22 @@ class Engine final : public EngineBase {
     require(s.nz >= 1, "labels must have shape (nz, ny, nx) with nz >= 1");
     require(static_cast<long long>(s.labels.size()) == s.nz * s.ny * s.nx, "labels must have shape (nz, ny, nx)");
     require(s.mask.empty() || static_cast<long long>(s.mask.size()) == s.ny * s.nx, "mask must have shape (ny, nx)");
-    for (int32_t l : s.labels)
-      require(l >= 0 && static_cast<size_t>(l) < s.indices.size(), "labels must index into `indices`");
+    int32_t lo = 0, hi = 0;
+    if (!s.labels.empty()) {
+      const auto mm = std::minmax_element(s.labels.begin(), s.labels.end());
+      lo = *mm.first;
+      hi = *mm.second;
+    }
+    require(lo >= 0 && static_cast<size_t>(hi) < s.indices.size(), "labels must index into `indices`");
     require(s.fluct.empty() || static_cast<long long>(s.fluct.size()) == s.nz * s.ny * s.nx,
             "fluctuation must have the shape of labels (nz, ny, nx)");
     require(s.labels_before.empty() || static_cast<long long>(s.labels_before.size()) == s.ny * s.nx,
             "labels_before must have shape (ny, nx)");
-    for (int32_t l : s.labels_before)
-      require(l >= -1 && l < static_cast<long long>(s.indices.size()), "labels must index into `indices`");
+    if (!s.labels_before.empty()) {
+      const auto mm = std::minmax_element(s.labels_before.begin(), s.labels_before.end());
+      require(*mm.first >= -1 && *mm.second < static_cast<long long>(s.indices.size()),
+              "labels must index into `indices`");
+    }
   }
 
   void setup() {
@@ -698,12 +850,20 @@ class Engine final : public EngineBase {
     for (long long j = 0; j < s.ny; ++j) ky[j] = fftfreq(j, s.ny, s.dy);
     kx_ = upload_vec(kx, stream_);
     ky_ = upload_vec(ky, stream_);
-    twx_.build(s.nx, stream_);
-    twy_.build(s.ny, stream_);
+    twx_ = std::make_shared<Twiddles<T>>();
+    twy_ = std::make_shared<Twiddles<T>>();
+    twx_->build(s.nx, stream_);
+    twy_->build(s.ny, stream_);
 
     build_boundaries();
     build_products();
     choose_geometry();
+    launch_.ny = s.ny;
+    launch_.nx = s.nx;
+    launch_.npix = npix_;
+    launch_.geo = geo_;
+    launch_.twx = twx_;
+    launch_.twy = twy_;
 
     const size_t pb = npix_ * sizeof(px2<T>);
     stage_.alloc(npix_ * 2 * sizeof(c128));
@@ -720,12 +880,15 @@ class Engine final : public EngineBase {
       mask_ = upload_vec(m, stream_);
     }
     if (std::any_of(has_host_.begin(), has_host_.end(), [](char c) { return c != 0; }))
-      born_ = std::make_unique<Born<T, COMP>>(s, has_host_, host_, kx_.as<double>(), ky_.as<double>(), twx_.plan,
-                                              twy_.plan, stream_);
+      born_ = std::make_unique<Born<T, COMP>>(s, has_host_, host_, kx_.as<double>(), ky_.as<double>(), twx_->plan,
+                                              twy_->plan, stream_);
     HIP_CHECK(hipStreamSynchronize(stream_));
     bytes_ = stage_.bytes() + field_in_.bytes() + X_.bytes() + prod_.bytes() + e_.bytes() + fwd_vals_.bytes() +
              bwd_vals_.bytes() + mask_.bytes() + ops_bytes_ + tsel_.bytes() + sp_pix_.bytes() + sp_rid_.bytes() +
              rowptr_.bytes() + (born_ ? born_->device_bytes() : 0);
+    // the label volume and δn now live in the boundary structures and device tables
+    sp_.labels = std::vector<int32_t>();
+    sp_.fluct = std::vector<c128>();
   }
 
   void choose_geometry() {
@@ -746,10 +909,10 @@ class Engine final : public EngineBase {
     auto ls_of = [](int lines) { return lines <= 2 ? lines : (lines | 1); };
     if (!g.col_fast) {
       int w = 16;
-      while (w > 1 && (s.ny * ls_of(2 * w) > elems || 2 * s.ny * ls_of(2 * w) * (long long)sizeof(cx<T>) > kLdsMax))
+      while (w > 1 && (s.ny * ls_of(2 * w) > elems || s.ny * ls_of(2 * w) * (long long)sizeof(cx<T>) > kLdsMax))
         w /= 2;
       w = std::min<long long>(w, pow2_floor(static_cast<int>(std::max<long long>(1, s.nx))));
-      require(s.ny * ls_of(2 * w) <= elems && 2 * s.ny * ls_of(2 * w) * (long long)sizeof(cx<T>) <= kLdsMax,
+      require(s.ny * ls_of(2 * w) <= elems && s.ny * ls_of(2 * w) * (long long)sizeof(cx<T>) <= kLdsMax,
               "grid size ny = " + std::to_string(s.ny) + " exceeds the column kernel's LDS budget");
       g.cols2 = w;
       g.ls_col2 = ls_of(2 * w);
@@ -778,13 +941,16 @@ class Engine final : public EngineBase {
     std::vector<int32_t> sp_pix, rowptr;
     std::vector<uint8_t> sp_rid;
     total_sparse_ = 0;
+    // same[p]: label plane p equals plane p − 1 (one comparison per plane)
+    std::vector<char> same(nz + 1, 0);
+    for (long long p = 1; p <= nz; ++p)
+      same[p] = std::memcmp(plane_labels(p), plane_labels(p - 1), npix_ * sizeof(int32_t)) == 0;
     for (long long q = 0; q < nz; ++q) {
       const int32_t* a = plane_labels(q);
       const int32_t* b = plane_labels(q + 1);
       Boundary& B = bnd_[q];
       // reuse the previous boundary's structure when both label planes repeat
-      if (q >= 1 && std::memcmp(a, plane_labels(q - 1), npix_ * sizeof(int32_t)) == 0 &&
-          std::memcmp(b, plane_labels(q), npix_ * sizeof(int32_t)) == 0) {
+      if (q >= 1 && same[q] && same[q + 1]) {
         const int src = bnd_[q - 1].source >= 0 ? bnd_[q - 1].source : static_cast<int>(q - 1);
         B = bnd_[src];
         B.source = src;
@@ -793,14 +959,18 @@ class Engine final : public EngineBase {
         total_sparse_ += B.sp_count;
         continue;
       }
-      // distinct pairs (sorted like the reference's std::set)
+      // distinct pairs (sorted like the reference's std::set); neighbouring pixels mostly share
+      // their pair, so the map is consulted only where the pair changes
       std::map<std::pair<int32_t, int32_t>, int> pid;
-      bool uniform = true;
-      for (long long i = 1; i < npix_ && uniform; ++i) uniform = a[i] == a[0] && b[i] == b[0];
-      if (uniform) {
-        pid[{a[0], b[0]}] = 0;
-      } else {
-        for (long long i = 0; i < npix_; ++i) pid.emplace(std::make_pair(a[i], b[i]), 0);
+      {
+        int32_t la = a[0], lb = b[0];
+        pid.emplace(std::make_pair(la, lb), 0);
+        for (long long i = 1; i < npix_; ++i)
+          if (a[i] != la || b[i] != lb) {
+            la = a[i];
+            lb = b[i];
+            pid.emplace(std::make_pair(la, lb), 0);
+          }
       }
       require(pid.size() <= 255, "boundary " + std::to_string(q) + " has more than 255 label pairs");
       int k = 0;
@@ -819,7 +989,16 @@ class Engine final : public EngineBase {
       if (B.pairs.size() > 1) {
         B.tsel_off = static_cast<long long>(tsel.size());
         tsel.resize(tsel.size() + npix_);
-        for (long long i = 0; i < npix_; ++i) tsel[B.tsel_off + i] = static_cast<uint8_t>(pid[{a[i], b[i]}]);
+        int32_t la = a[0], lb = b[0];
+        uint8_t id = static_cast<uint8_t>(pid.at({la, lb}));
+        for (long long i = 0; i < npix_; ++i) {
+          if (a[i] != la || b[i] != lb) {
+            la = a[i];
+            lb = b[i];
+            id = static_cast<uint8_t>(pid.at({la, lb}));
+          }
+          tsel[B.tsel_off + i] = id;
+        }
       }
       if (!B.rpair.empty()) {
         B.sp_off = total_sparse_;
@@ -951,7 +1130,8 @@ class Engine final : public EngineBase {
   std::vector<char> has_host_;
   std::vector<c128> host_;
   DevBuf kx_, ky_;
-  Twiddles<T> twx_, twy_;
+  std::shared_ptr<Twiddles<T>> twx_, twy_;
+  Launcher<T, COMP> launch_;
   Geometry geo_;
   std::vector<Boundary> bnd_;
   std::unordered_map<long long, long long> list_off_;

@@ -164,7 +164,7 @@ __device__ __forceinline__ void dft(cx<T>* x, const cx<T>* twh, const cx<T>* twl
 // Elements (line length × line stride) one workgroup may hold in one LDS buffer.
 template <typename T>
 struct Budget {
-  static constexpr int elems = sizeof(T) == 4 ? 8192 : 4096;
+  static constexpr int elems = 8192;
 };
 
 // One Stockham stage over `lines` interleaved lines of length n: element j of line l lives at

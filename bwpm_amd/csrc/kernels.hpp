@@ -99,7 +99,6 @@ __global__ __launch_bounds__(kThreads) void k_col_step(const ColDesc<T>* __restr
   cx<T>* A = lds<T>();
   const int W = g.cols, ny = g.ny, nx = g.nx, ls = g.ls;
   const int lines = 2 * W;
-  cx<T>* B = A + static_cast<long long>(ny) * ls;
   const int x0 = blockIdx.x * W;
   const int count = ny * W;
   for (int i = threadIdx.x; i < count; i += kThreads) {
@@ -111,23 +110,33 @@ __global__ __launch_bounds__(kThreads) void k_col_step(const ColDesc<T>* __restr
   }
   __syncthreads();
   fft_lines<COMP, false>(A, ls, lines, g.plan);
+  // several products: keep the spectrum in this block's own positions of X and reload it per
+  // product, so one LDS buffer suffices
+  px2<T>* xio = const_cast<px2<T>*>(d.in);
+  if (d.nprod > 1) {
+    for (int i = threadIdx.x; i < count; i += kThreads) {
+      const int y = i / W, c = i - y * W, x = x0 + c;
+      if (x < nx) xio[static_cast<long long>(y) * nx + x] = px2<T>{A[y * ls + c], A[y * ls + W + c]};
+    }
+    __syncthreads();
+  }
   for (int o = 0; o < d.nprod; ++o) {
     const Prod<T> pd = d.prods[o];
-    cx<T>* D = d.nprod == 1 ? A : B;
     for (int i = threadIdx.x; i < count; i += kThreads) {
       const int y = i / W, c = i - y * W, x = x0 + c;
       if (x >= nx) continue;
       const long long k = static_cast<long long>(y) * nx + x;
-      const px2<T> r = apply_prod<T, COMP>(pd, k, px2<T>{A[y * ls + c], A[y * ls + W + c]});
-      D[y * ls + c] = r.a;
-      D[y * ls + W + c] = r.b;
+      const px2<T> e = o == 0 ? px2<T>{A[y * ls + c], A[y * ls + W + c]} : xio[k];
+      const px2<T> r = apply_prod<T, COMP>(pd, k, e);
+      A[y * ls + c] = r.a;
+      A[y * ls + W + c] = r.b;
     }
     __syncthreads();
-    fft_lines<COMP, true>(D, ls, lines, g.plan);
+    fft_lines<COMP, true>(A, ls, lines, g.plan);
     px2<T>* out = d.out + o * g.plane;
     for (int i = threadIdx.x; i < count; i += kThreads) {
       const int y = i / W, c = i - y * W, x = x0 + c;
-      if (x < nx) out[static_cast<long long>(y) * nx + x] = px2<T>{D[y * ls + c], D[y * ls + W + c]};
+      if (x < nx) out[static_cast<long long>(y) * nx + x] = px2<T>{A[y * ls + c], A[y * ls + W + c]};
     }
     __syncthreads();
   }

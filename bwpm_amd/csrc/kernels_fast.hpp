@@ -43,6 +43,13 @@ constexpr int fast_lines(int n, int skew, int csize, bool comp) {
   return w;
 }
 
+// Twiddles are staged in LDS when the line buffers leave room for them (160 KB per workgroup);
+// the 4096-point FP64 lines read them from global memory.
+template <int N, int LINES, int SKEW, typename T, bool COMP>
+constexpr bool fast_tw_in_lds() {
+  return fast_lds_bytes(N, LINES, SKEW, static_cast<int>(sizeof(cx<T>)), COMP) <= 163840;
+}
+
 // Workgroups are dispatched to the eight XCDs round-robin. Column groups that share cache lines
 // (W·sizeof(px2) < 128 B) are mapped so that neighbouring groups run on the same XCD and its L2:
 // XCD j takes the j-th contiguous eighth of the groups.
@@ -100,12 +107,15 @@ __global__ __launch_bounds__(N / 4 * W) void k_col_fast(const ColDesc<T>* __rest
                                                         const cx<T>* __restrict__ g_twl) {
   constexpr int TT = N / 8;
   constexpr int LS = fast_line_stride<N, kColSkew>();
+  constexpr bool TWL = fast_tw_in_lds<N, 2 * W, kColSkew, T, COMP>();
   __shared__ cx<T> sh[2 * W * LS];
-  __shared__ cx<T> tw_hi[reg_tw_len<N>()];
-  __shared__ cx<T> tw_lo[COMP ? reg_tw_len<N>() : 1];
+  __shared__ cx<T> tw_hi_s[TWL ? reg_tw_len<N>() : 1];
+  __shared__ cx<T> tw_lo_s[TWL && COMP ? reg_tw_len<N>() : 1];
   const ColDesc<T> d = descs[blockIdx.y];
   if (!d.in) return;
-  stage_twiddles<N, COMP>(tw_hi, tw_lo, g_twh, g_twl);
+  if constexpr (TWL) stage_twiddles<N, COMP>(tw_hi_s, tw_lo_s, g_twh, g_twl);
+  const cx<T>* tw_hi = TWL ? tw_hi_s : g_twh;
+  const cx<T>* tw_lo = TWL ? tw_lo_s : g_twl;
   const int comp = threadIdx.x & 1;
   const int q = threadIdx.x >> 1;
   const int c = q % W, t = q / W;
@@ -148,12 +158,15 @@ __global__ __launch_bounds__(N / 4 * RW) void k_row_fast(const RowDesc<T>* __res
                                                          const cx<T>* __restrict__ g_twl) {
   constexpr int TT = N / 8;
   constexpr int LS = fast_line_stride<N, kRowSkew>();
+  constexpr bool TWL = fast_tw_in_lds<N, 2 * RW, kRowSkew, T, COMP>();
   __shared__ cx<T> sh[2 * RW * LS];
-  __shared__ cx<T> tw_hi[reg_tw_len<N>()];
-  __shared__ cx<T> tw_lo[COMP ? reg_tw_len<N>() : 1];
+  __shared__ cx<T> tw_hi_s[TWL ? reg_tw_len<N>() : 1];
+  __shared__ cx<T> tw_lo_s[TWL && COMP ? reg_tw_len<N>() : 1];
   const RowDesc<T> a = descs[blockIdx.y];
   if (!a.active) return;
-  stage_twiddles<N, COMP>(tw_hi, tw_lo, g_twh, g_twl);
+  if constexpr (TWL) stage_twiddles<N, COMP>(tw_hi_s, tw_lo_s, g_twh, g_twl);
+  const cx<T>* tw_hi = TWL ? tw_hi_s : g_twh;
+  const cx<T>* tw_lo = TWL ? tw_lo_s : g_twl;
   const int comp = threadIdx.x & 1;
   const int q = threadIdx.x >> 1;
   const int r = q / TT, t = q % TT;

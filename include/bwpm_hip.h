@@ -140,6 +140,43 @@ int bwpm_batch_run(bwpm_solver* const* solvers, int64_t n, int iterations, int m
  * (no counterpart in the reference). */
 int bwpm_copy_probe(int64_t bytes, int reps, int device, double* ms);
 
+/* ---- serving: request queue with continuous batching (no counterpart in the reference, which
+ * runs one solve per call, src/bwpm/fast.py:165-200) --------------------------------------------
+ *
+ * Requests are admitted in arrival order while their device footprint fits the HBM budget;
+ * admitted solves of one (precision, grid) share a continuous lane in which a request joins a free
+ * slot between chunks of slab steps and leaves when its program ends, so requests that arrive
+ * while others are on the GPU overlap with them. Each result is bit-identical to the same solve
+ * run alone. Solves with index fluctuations run one at a time. All calls are thread-safe. */
+typedef struct bwpm_queue bwpm_queue;
+
+#define BWPM_TICKET_PENDING 0  /* waiting for admission */
+#define BWPM_TICKET_ADMITTED 1 /* on the device */
+#define BWPM_TICKET_DONE 2     /* result ready (bwpm_queue_result) */
+#define BWPM_TICKET_FAILED 3   /* bwpm_queue_result returns its error */
+
+/* hbm_budget_bytes <= 0: 90 % of the device memory free now; slots: solves per lane (1 … 1024);
+ * chunk: slab steps between admissions into a running lane. */
+int bwpm_queue_create(int device, double hbm_budget_bytes, int slots, int chunk, bwpm_queue** out);
+/* Finishes the admitted requests, fails the pending ones, frees the queue. */
+int bwpm_queue_destroy(bwpm_queue* q);
+/* Arguments as bwpm_solver_create_ex (device: the queue's) plus the input field and iterations. */
+int bwpm_queue_submit(bwpm_queue* q, int64_t ny, int64_t nx, double dy, double dx, double dz, double k0,
+                      const double* indices, int64_t m, const int32_t* labels, int64_t nz, double ib_re,
+                      double ib_im, const double* mask, const double* fluct, const int32_t* labels_before,
+                      int precision, const double* field, int iterations, int64_t* ticket);
+int bwpm_queue_state(bwpm_queue* q, int64_t ticket, int* state);
+/* Waits until the ticket is done or failed, or timeout_s passes (timeout_s < 0: no limit). */
+int bwpm_queue_wait(bwpm_queue* q, int64_t ticket, double timeout_s, int* state);
+/* Result of a done ticket (as bwpm_solver_run; kept planes are not served); the ticket is then
+ * forgotten. A failed ticket returns its error code and message. */
+int bwpm_queue_result(bwpm_queue* q, int64_t ticket, bwpm_result** out);
+/* out[BWPM_QUEUE_NSTATS]: p50, p99 and mean latency (ms, submit → result on the host), completed
+ * grid-point·slab updates per second (since the first submit), completed, failed, pending and
+ * admitted requests, device bytes in use, the budget and the peak in use. */
+#define BWPM_QUEUE_NSTATS 11
+int bwpm_queue_stats(bwpm_queue* q, double* out);
+
 #ifdef __cplusplus
 }
 #endif

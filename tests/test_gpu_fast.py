@@ -43,7 +43,8 @@ def _solver(a: dict, precision: str):
                            a["index_before"], a["mask"], precision=precision)
 
 
-@pytest.mark.parametrize("shape", [(64, 64), (128, 128), (256, 128), (512, 512), (512, 12), (12, 256)])
+@pytest.mark.parametrize("shape", [(64, 64), (128, 128), (256, 128), (512, 512), (512, 12), (12, 256), (4096, 1),
+                                   (2048, 2), (3072, 1), (384, 96)])
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
 def test_register_kernels_match_oracle(shape: tuple[int, int], precision: str) -> None:
     ny, nx = shape
@@ -61,6 +62,10 @@ def test_register_kernels_match_oracle(shape: tuple[int, int], precision: str) -
     for got, want in pairs:
         if precision == "fp64":
             assert np.max(np.abs(got - want)) < 1e-10 * scale
+        elif np.max(np.abs(want)) < 1e-6 * scale:
+            # the field is extinguished (behind the silver layer at some shapes it is ~1e-64 of the
+            # input, below FP32's range): the FP32 result must be as small
+            assert np.max(np.abs(got)) < 1e-6 * scale
         else:
             assert rel_l2(got, want) < 1e-5
 
