@@ -12,8 +12,9 @@ Printed: one JSON line with grid-point·slab updates/s (whole job), the roofline
 (HIP-event durations of the column and the row kernel on the lane's stream; algorithmic bytes per
 update = one read and one write of the (E_x, E_y) plane: 64 B in complex128, 32 B in complex64),
 the plain one-solve-per-launch rate of the same solve on the same GPU, the accuracy of the run
-(batch result against the solve alone; FP32 against the FP64 device path), and the reference's
-CPU core timed on this host over a bounded sample of the same geometry.
+(batch result against the solve alone; FP32 against the FP64 device path), the served mix on the
+request queue (capacity, then p50/p99 latency of a Poisson stream at 70 % of it), and the
+reference's CPU core timed on this host over a bounded sample of the same geometry.
 
     python bench.py [--steps K] [--warmup W] [--batch B] [--n 512] [--nz 1024] [--iterations 4]
                     [--precision fp64|fp32|fp32_plain]
@@ -86,12 +87,48 @@ def cpu_baseline(n: int, nz_sample: int, threads: int) -> dict:
             "sample": f"tilted_3d {n}x{n}x{nz_sample} slabs, 1 iteration ({prob.updates} updates) in {dt:.2f} s"}
 
 
+def served(count: int, precision: str) -> dict:
+    """The served mix on the request queue: a burst of `count` requests measures capacity, then a
+    Poisson stream of `count` other requests at 70 % of it gives latency under load."""
+    from bwpm_amd import serve
+
+    burst = [serve.served_request(1000 + i) for i in range(count)]
+    with serve.Server(slots=16, chunk=64, precision=precision) as srv:
+        t0 = time.perf_counter()
+        tickets = [srv.submit(p) for p in burst]
+        for t in tickets:
+            srv.result(t)
+        span = time.perf_counter() - t0
+        cap = srv.stats()
+    rate = 0.7 * count / span
+    with serve.Server(slots=16, chunk=64, precision=precision) as srv:
+        rep = serve.run_stream(srv, (serve.served_request(2000 + i) for i in range(count)), rate_hz=rate, seed=1)
+    st = rep.stats
+    return {
+        "workload": "served mix: 2D 512-4096 points and 3D vector 128^2-512^2, 200-1024 slabs, planar / tilted / "
+                    "silver-layer / index-fluctuation geometries, 1-3 iterations (seeded)",
+        "requests": count,
+        "capacity_req_per_s": round(count / span, 3),
+        "capacity_updates_per_s": float(sum(p.updates for p in burst) / span),
+        "burst_p50_ms": round(cap["p50_ms"], 1),
+        "burst_p99_ms": round(cap["p99_ms"], 1),
+        "offered_rate_hz": round(rate, 3),
+        "p50_ms": round(st["p50_ms"], 1),
+        "p99_ms": round(st["p99_ms"], 1),
+        "mean_ms": round(st["mean_ms"], 1),
+        "updates_per_s": st["updates_per_s"],
+        "completed": int(st["completed"]),
+        "failed": int(st["failed"]) + len(rep.failures),
+        "peak_hbm_gb": round(st["peak_bytes"] / 1e9, 2),
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--batch", type=int, default=8)
+    ap.add_argument("--batch", type=int, default=16)
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--nz", type=int, default=1024)
     ap.add_argument("--iterations", type=int, default=4)
@@ -100,6 +137,8 @@ def main() -> int:
     ap.add_argument("--cpu-nz", type=int, default=48)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-checks", action="store_true", help="skip the plain-path and accuracy runs")
+    ap.add_argument("--serve-requests", type=int, default=24,
+                    help="requests of the served mix (0: no serving run)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU figures are not produced",
@@ -187,6 +226,10 @@ def main() -> int:
             checks["rel_l2_vs_fp64_device"] = {"transmitted": rel_l2(alone["transmitted"], r64["transmitted"]),
                                                "reflected": rel_l2(alone["reflected"], r64["reflected"])}
 
+    serving = None
+    if args.serve_requests > 0:
+        serving = served(args.serve_requests, args.precision)
+
     cpu = None
     if not args.no_cpu_baseline:
         threads = args.cpu_threads if args.cpu_threads > 0 else min(16, os.cpu_count() or 1)
@@ -212,6 +255,7 @@ def main() -> int:
         },
         "roofline": roofline,
         "checks": checks,
+        "serving": serving,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

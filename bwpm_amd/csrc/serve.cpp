@@ -48,7 +48,7 @@ Queue::Queue(int device, double budget_bytes, int slots, int chunk)
   size_t free_b = 0, total_b = 0;
   HIP_CHECK(hipMemGetInfo(&free_b, &total_b));
   budget_ = budget_bytes > 0 ? std::min(budget_bytes, static_cast<double>(total_b)) : 0.9 * static_cast<double>(free_b);
-  admit_thread_ = std::thread([this] { admit_loop(); });
+  for (int i = 0; i < kAdmitWorkers; ++i) admit_threads_.emplace_back([this] { admit_loop(); });
   complete_thread_ = std::thread([this] { complete_loop(); });
   for (int i = 0; i < kBornWorkers; ++i) born_threads_.emplace_back([this] { born_loop(); });
 }
@@ -61,7 +61,8 @@ Queue::~Queue() {
   }
   cv_admit_.notify_all();
   cv_born_.notify_all();
-  if (admit_thread_.joinable()) admit_thread_.join();
+  for (auto& t : admit_threads_)
+    if (t.joinable()) t.join();
   for (auto& kv : lanes_)
     if (kv.second->thread.joinable()) kv.second->thread.join();
   for (auto& t : born_threads_)

@@ -1,8 +1,9 @@
 // Request queue with continuous batching on one device.
 //
-// Requests (a solve spec, its input field and iteration count) arrive at any time. An admission
-// thread admits them in arrival order while their device footprint fits the HBM budget: it builds
-// the engine and uploads the field. Admitted solves of one (precision, grid) join a continuous
+// Requests (a solve spec, its input field and iteration count) arrive at any time. Admission
+// workers take them in arrival order while their device footprint fits the HBM budget, build the
+// engine (host set-up of boundaries and tables) and upload the field; several workers build
+// engines concurrently. Admitted solves of one (precision, grid) join a continuous
 // lane (run_continuous_lane) that steps all its members together, one column and one row launch
 // per slab step; a member joins a free slot between chunks of slab steps and leaves when its
 // program ends, so requests that arrive while others are on the GPU overlap with them. Lanes of
@@ -100,9 +101,10 @@ class Queue {
   Clock::time_point first_submit_{}, last_done_{};
   bool any_submit_ = false;
 
+  static constexpr int kAdmitWorkers = 4;
   static constexpr int kBornWorkers = 4;
-  std::thread admit_thread_, complete_thread_;
-  std::vector<std::thread> born_threads_;
+  std::thread complete_thread_;
+  std::vector<std::thread> admit_threads_, born_threads_;
 };
 
 }  // namespace bwpm
