@@ -104,7 +104,9 @@ class LaneFeed {
 // Runs a lane of up to `slots` solves of one grid and precision on `stream`: members join free
 // slots between chunks of `chunk` slab steps and leave when their program ends. Every member's
 // result is bit-identical to its solve alone (each slot's work depends on its own descriptors
-// only). Solves with index fluctuations are not accepted (they run one per lane).
+// only). Solves with index fluctuations are not accepted (they run one per lane). Chunks whose
+// steps are all plain launches replay one captured graph per shape for the lane's lifetime, so a
+// long-lived lane of small grids is not bound by launch overhead.
 void run_continuous_lane(int precision, LaneFeed& feed, int slots, int chunk, hipStream_t stream);
 void run_continuous_lane_f32(LaneFeed& feed, int slots, int chunk, hipStream_t stream, bool compensated);
 void run_continuous_lane_f64(LaneFeed& feed, int slots, int chunk, hipStream_t stream);

@@ -228,6 +228,69 @@ struct Geometry {
   int rows = 1, ls_row = 1;       // generic
 };
 
+// ---- lines-per-workgroup autotuning ----------------------------------------------------------
+
+// The register kernels can run with W columns (rows) per workgroup or with one. Which is faster
+// depends on the grid, the number of solves per launch and the device, so lanes measure it: the
+// first uniform stretches of slab steps of a new (device, precision, grid, slots) key run each
+// candidate once, timed with events, and later launches use the fastest. Every candidate computes
+// the same values per line, so tuning does not change results.
+struct TuneKey {
+  int device = 0, precision = 0, slots = 0;
+  long long ny = 0, nx = 0;
+  bool operator<(const TuneKey& o) const {
+    return std::tie(device, precision, slots, ny, nx) < std::tie(o.device, o.precision, o.slots, o.ny, o.nx);
+  }
+};
+struct TuneState {
+  std::vector<std::pair<int, int>> cand;  // (columns, rows) per workgroup
+  std::vector<double> us;                 // measured time per solve-step, < 0: not yet
+  int best = -1;
+};
+inline std::mutex& tune_mutex() {
+  static std::mutex m;
+  return m;
+}
+inline std::map<TuneKey, TuneState>& tune_cache() {
+  static std::map<TuneKey, TuneState> c;
+  return c;
+}
+
+// Candidate to run next for `key` (given the kernel geometry `g`): the best one when tuning is
+// done (trial = −1), else the next untimed candidate (trial = its index).
+inline std::pair<int, int> tune_pick(const TuneKey& key, const Geometry& g, int& trial) {
+  std::lock_guard<std::mutex> lock(tune_mutex());
+  TuneState& st = tune_cache()[key];
+  if (st.cand.empty()) {
+    for (int cw : {g.col_w, 1})
+      for (int rw : {g.rw, 1}) {
+        if (!g.col_fast && cw != g.col_w) continue;
+        if (!g.row_fast && rw != g.rw) continue;
+        const std::pair<int, int> c{cw, rw};
+        if (std::find(st.cand.begin(), st.cand.end(), c) == st.cand.end()) st.cand.push_back(c);
+      }
+    st.us.assign(st.cand.size(), -1.0);
+    if (st.cand.size() == 1) st.best = 0;
+  }
+  trial = -1;
+  if (st.best >= 0) return st.cand[st.best];
+  for (size_t i = 0; i < st.cand.size(); ++i)
+    if (st.us[i] < 0) {
+      trial = static_cast<int>(i);
+      return st.cand[i];
+    }
+  return st.cand[0];
+}
+
+inline void tune_report(const TuneKey& key, int trial, double us_per_solve_step) {
+  std::lock_guard<std::mutex> lock(tune_mutex());
+  TuneState& st = tune_cache()[key];
+  if (trial < 0 || trial >= static_cast<int>(st.us.size()) || st.best >= 0) return;
+  st.us[trial] = us_per_solve_step;
+  if (std::all_of(st.us.begin(), st.us.end(), [](double v) { return v >= 0; }))
+    st.best = static_cast<int>(std::min_element(st.us.begin(), st.us.end()) - st.us.begin());
+}
+
 }  // namespace detail
 
 using namespace detail;
@@ -240,16 +303,18 @@ struct Launcher {
   Geometry geo;
   std::shared_ptr<Twiddles<T>> twx, twy;
 
-  void col(const ColDesc<T>* d, int S, hipStream_t s) const {
+  // multi: some solve of the launch has several products at this step (register kernels keep
+  // their spectrum in LDS); the result bits do not depend on it.
+  void col(const ColDesc<T>* d, int S, hipStream_t s, bool multi) const {
     if (geo.col_fast && COMP == (sizeof(T) == 4)) {
       switch (ny) {
-        case 64: col_fast_n<64>(d, S, s); return;
-        case 128: col_fast_n<128>(d, S, s); return;
-        case 256: col_fast_n<256>(d, S, s); return;
-        case 512: col_fast_n<512>(d, S, s); return;
-        case 1024: col_fast_n<1024>(d, S, s); return;
-        case 2048: col_fast_n<2048>(d, S, s); return;
-        default: col_fast_n<4096>(d, S, s); return;
+        case 64: col_fast_n<64>(d, S, s, multi); return;
+        case 128: col_fast_n<128>(d, S, s, multi); return;
+        case 256: col_fast_n<256>(d, S, s, multi); return;
+        case 512: col_fast_n<512>(d, S, s, multi); return;
+        case 1024: col_fast_n<1024>(d, S, s, multi); return;
+        case 2048: col_fast_n<2048>(d, S, s, multi); return;
+        default: col_fast_n<4096>(d, S, s, multi); return;
       }
     }
     // generic: one LDS buffer of ny × ls per block (products reload the spectrum from X)
@@ -259,16 +324,16 @@ struct Launcher {
     hipLaunchKernelGGL((k_col_step<T, COMP>), grid, dim3(kThreads), lds, s, d, g);
   }
 
-  void row(const RowDesc<T>* d, int S, hipStream_t s) const {
+  void row(const RowDesc<T>* d, int S, hipStream_t s, bool multi) const {
     if (geo.row_fast && COMP == (sizeof(T) == 4)) {
       switch (nx) {
-        case 64: row_fast_n<64>(d, S, s); return;
-        case 128: row_fast_n<128>(d, S, s); return;
-        case 256: row_fast_n<256>(d, S, s); return;
-        case 512: row_fast_n<512>(d, S, s); return;
-        case 1024: row_fast_n<1024>(d, S, s); return;
-        case 2048: row_fast_n<2048>(d, S, s); return;
-        default: row_fast_n<4096>(d, S, s); return;
+        case 64: row_fast_n<64>(d, S, s, multi); return;
+        case 128: row_fast_n<128>(d, S, s, multi); return;
+        case 256: row_fast_n<256>(d, S, s, multi); return;
+        case 512: row_fast_n<512>(d, S, s, multi); return;
+        case 1024: row_fast_n<1024>(d, S, s, multi); return;
+        case 2048: row_fast_n<2048>(d, S, s, multi); return;
+        default: row_fast_n<4096>(d, S, s, multi); return;
       }
     }
     RowGeom g{static_cast<int>(ny), static_cast<int>(nx), geo.rows, geo.ls_row, npix, twx->plan};
@@ -278,32 +343,54 @@ struct Launcher {
   }
 
  private:
+  // The LDS-stash form is instantiated where its extra LDS fits a workgroup.
+  template <int N, int W>
+  static constexpr bool stash_fits() {
+    return fast_lds_bytes(N, 2 * W, kColSkew, static_cast<int>(sizeof(cx<T>)), COMP) +
+               2LL * W * (N + 1) * static_cast<long long>(sizeof(cx<T>)) <=
+           163840;
+  }
+  template <int N, int W>
+  void col_launch(const ColDesc<T>* d, int S, hipStream_t s, bool multi, const cx<T>* th, const cx<T>* tl) const {
+    const dim3 grid(static_cast<unsigned>(nx / W), S);
+    if constexpr (stash_fits<N, W>())
+      if (multi) {
+        hipLaunchKernelGGL((k_col_fast<N, W, T, COMP, true>), grid, dim3(N / 4 * W), 0, s, d, static_cast<int>(nx),
+                           npix, th, tl);
+        return;
+      }
+    hipLaunchKernelGGL((k_col_fast<N, W, T, COMP, false>), grid, dim3(N / 4 * W), 0, s, d, static_cast<int>(nx), npix,
+                       th, tl);
+  }
+  template <int N, int RW>
+  void row_launch(const RowDesc<T>* d, int S, hipStream_t s, bool multi, const cx<T>* th, const cx<T>* tl) const {
+    const dim3 grid(static_cast<unsigned>(ny / RW), S);
+    if constexpr (stash_fits<N, RW>())
+      if (multi) {
+        hipLaunchKernelGGL((k_row_fast<N, RW, T, COMP, true>), grid, dim3(N / 4 * RW), 0, s, d, npix, th, tl);
+        return;
+      }
+    hipLaunchKernelGGL((k_row_fast<N, RW, T, COMP, false>), grid, dim3(N / 4 * RW), 0, s, d, npix, th, tl);
+  }
   template <int N>
-  void col_fast_n(const ColDesc<T>* d, int S, hipStream_t s) const {
+  void col_fast_n(const ColDesc<T>* d, int S, hipStream_t s, bool multi) const {
     constexpr int W = fast_w<T, COMP>(N);
     const cx<T>* th = twy->hi.template as<cx<T>>();
     const cx<T>* tl = twy->lo.template as<cx<T>>();
-    if (geo.col_w == W) {
-      const dim3 grid(static_cast<unsigned>(nx / W), S);
-      hipLaunchKernelGGL((k_col_fast<N, W, T, COMP>), grid, dim3(N / 4 * W), 0, s, d, static_cast<int>(nx), npix, th,
-                         tl);
-    } else {
-      const dim3 grid(static_cast<unsigned>(nx), S);
-      hipLaunchKernelGGL((k_col_fast<N, 1, T, COMP>), grid, dim3(N / 4), 0, s, d, static_cast<int>(nx), npix, th, tl);
-    }
+    if (geo.col_w == W)
+      col_launch<N, W>(d, S, s, multi, th, tl);
+    else
+      col_launch<N, 1>(d, S, s, multi, th, tl);
   }
   template <int N>
-  void row_fast_n(const RowDesc<T>* d, int S, hipStream_t s) const {
+  void row_fast_n(const RowDesc<T>* d, int S, hipStream_t s, bool multi) const {
     constexpr int RW = fast_rw<T, COMP>(N);
     const cx<T>* th = twx->hi.template as<cx<T>>();
     const cx<T>* tl = twx->lo.template as<cx<T>>();
-    if (geo.rw == RW) {
-      const dim3 grid(static_cast<unsigned>(ny / RW), S);
-      hipLaunchKernelGGL((k_row_fast<N, RW, T, COMP>), grid, dim3(N / 4 * RW), 0, s, d, npix, th, tl);
-    } else {
-      const dim3 grid(static_cast<unsigned>(ny), S);
-      hipLaunchKernelGGL((k_row_fast<N, 1, T, COMP>), grid, dim3(N / 4), 0, s, d, npix, th, tl);
-    }
+    if (geo.rw == RW)
+      row_launch<N, RW>(d, S, s, multi, th, tl);
+    else
+      row_launch<N, 1>(d, S, s, multi, th, tl);
   }
 };
 
@@ -543,14 +630,17 @@ class Engine final : public EngineBase {
       if (e->stream_ != stream) HIP_CHECK(hipStreamSynchronize(e->stream_));
     std::vector<ColDesc<T>> hcol(G * S);
     std::vector<RowDesc<T>> hrow(G * S);
-    std::vector<char> has_col(G, 0), has_row(G, 0);
+    std::vector<char> has_col(G, 0), has_row(G, 0), multi_col(G, 0), multi_row(G, 0);
     for (size_t g = 0; g < G; ++g)
       for (int i = 0; i < S; ++i) {
         if (g >= progs[i].size() || progs[i][g].kind != kOpPair) continue;
-        hcol[g * S + i] = progs[i][g].col;
-        hrow[g * S + i] = progs[i][g].row;
-        has_col[g] |= progs[i][g].col.in != nullptr;
-        has_row[g] |= progs[i][g].row.active != 0;
+        const Op<T>& o = progs[i][g];
+        hcol[g * S + i] = o.col;
+        hrow[g * S + i] = o.row;
+        has_col[g] |= o.col.in != nullptr;
+        has_row[g] |= o.row.active != 0;
+        multi_col[g] |= o.col.in != nullptr && o.col.nprod > 1;
+        multi_row[g] |= o.row.active != 0 && o.row.nt > 1;
       }
     if (lane_col_.bytes() < hcol.size() * sizeof(ColDesc<T>)) lane_col_.alloc(hcol.size() * sizeof(ColDesc<T>));
     if (lane_row_.bytes() < hrow.size() * sizeof(RowDesc<T>)) lane_row_.alloc(hrow.size() * sizeof(RowDesc<T>));
@@ -588,19 +678,49 @@ class Engine final : public EngineBase {
       ++(col ? times.col_launches : times.row_launches);
     };
     HIP_CHECK(hipEventRecord(t0, stream));
-    for (size_t g = 0; g < G; ++g) {
-      if (S == 1 && progs[0][g].kind != kOpPair) {
-        const Op<T>& o = progs[0][g];
-        if (o.kind == kOpBornFwd)
-          born_->forward(e_.template as<px2<T>>(), o.slab, o.bidir);
-        else
-          born_->backward(e_.template as<px2<T>>(), o.slab);
-        continue;
+    // lines per workgroup: the tuned choice for this lane shape; untuned shapes time each
+    // candidate on a uniform stretch of kTuneSteps slab steps first (not while timing kernels)
+    constexpr size_t kTuneSteps = 32;
+    const TuneKey tk{sp_.device, precision_code(), S, sp_.ny, sp_.nx};
+    Launcher<T, COMP> Lc = launch_;
+    for (size_t g = 0; g < G;) {
+      int trial = -1;
+      const std::pair<int, int> choice = tune_pick(tk, launch_.geo, trial);
+      Lc.geo.col_w = choice.first;
+      Lc.geo.rw = choice.second;
+      bool uniform = !time_kernels && trial >= 0 && g + kTuneSteps <= G;
+      for (size_t q = g; uniform && q < g + kTuneSteps; ++q)
+        uniform = (S > 1 || progs[0][q].kind == kOpPair) && has_col[q] && has_row[q] && !multi_col[q] && !multi_row[q];
+      const size_t end = uniform ? g + kTuneSteps : std::min(G, g + kTuneSteps);
+      hipEvent_t a = nullptr, b = nullptr;
+      if (uniform) {
+        HIP_CHECK(hipEventCreate(&a));
+        HIP_CHECK(hipEventCreate(&b));
+        HIP_CHECK(hipEventRecord(a, stream));
       }
-      const ColDesc<T>* dc = lane_col_.as<ColDesc<T>>() + g * S;
-      const RowDesc<T>* dr = lane_row_.as<RowDesc<T>>() + g * S;
-      if (has_col[g]) timed(true, [&] { launch_.col(dc, S, stream); });
-      if (has_row[g]) timed(false, [&] { launch_.row(dr, S, stream); });
+      for (; g < end; ++g) {
+        if (S == 1 && progs[0][g].kind != kOpPair) {
+          const Op<T>& o = progs[0][g];
+          if (o.kind == kOpBornFwd)
+            born_->forward(e_.template as<px2<T>>(), o.slab, o.bidir);
+          else
+            born_->backward(e_.template as<px2<T>>(), o.slab);
+          continue;
+        }
+        const ColDesc<T>* dc = lane_col_.as<ColDesc<T>>() + g * S;
+        const RowDesc<T>* dr = lane_row_.as<RowDesc<T>>() + g * S;
+        if (has_col[g]) timed(true, [&] { Lc.col(dc, S, stream, multi_col[g] != 0); });
+        if (has_row[g]) timed(false, [&] { Lc.row(dr, S, stream, multi_row[g] != 0); });
+      }
+      if (uniform) {
+        HIP_CHECK(hipEventRecord(b, stream));
+        HIP_CHECK(hipEventSynchronize(b));
+        float ms = 0;
+        HIP_CHECK(hipEventElapsedTime(&ms, a, b));
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        tune_report(tk, trial, 1e3 * ms / (static_cast<double>(kTuneSteps) * S));
+      }
     }
     HIP_CHECK(hipEventRecord(t1, stream));
     HIP_CHECK(hipEventSynchronize(t1));
@@ -631,7 +751,13 @@ class Engine final : public EngineBase {
     // copied it has completed (at most two chunks are in flight)
     PinnedBuf hcol[2] = {PinnedBuf(n * sizeof(ColDesc<T>)), PinnedBuf(n * sizeof(ColDesc<T>))};
     PinnedBuf hrow[2] = {PinnedBuf(n * sizeof(RowDesc<T>)), PinnedBuf(n * sizeof(RowDesc<T>))};
-    std::vector<char> has_col(chunk), has_row(chunk);
+    std::vector<char> has_col(chunk), has_row(chunk), multi_col(chunk), multi_row(chunk);
+    std::map<std::tuple<size_t, int, int>, hipGraphExec_t> graphs;
+    bool graphs_ok = true;
+    auto drop_graphs = [&] {
+      for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
+      graphs.clear();
+    };
     hipEvent_t prev = nullptr;
     auto settle = [&] {
       if (!prev) return;
@@ -683,6 +809,8 @@ class Engine final : public EngineBase {
         std::fill(hr, hr + G * S, RowDesc<T>{});
         std::fill(has_col.begin(), has_col.end(), 0);
         std::fill(has_row.begin(), has_row.end(), 0);
+        std::fill(multi_col.begin(), multi_col.end(), 0);
+        std::fill(multi_row.begin(), multi_row.end(), 0);
         for (size_t g = 0; g < G; ++g)
           for (int i = 0; i < S; ++i) {
             if (!slot[i] || pc[i] + g >= prog[i].size()) continue;
@@ -691,18 +819,78 @@ class Engine final : public EngineBase {
             hr[g * S + i] = o.row;
             has_col[g] |= o.col.in != nullptr;
             has_row[g] |= o.row.active != 0;
+            multi_col[g] |= o.col.in != nullptr && o.col.nprod > 1;
+            multi_row[g] |= o.row.active != 0 && o.row.nt > 1;
           }
         HIP_CHECK(hipMemcpyAsync(dcol.as<void>(), hc, G * S * sizeof(ColDesc<T>), hipMemcpyHostToDevice, stream));
         HIP_CHECK(hipMemcpyAsync(drow.as<void>(), hr, G * S * sizeof(RowDesc<T>), hipMemcpyHostToDevice, stream));
-        for (size_t g = 0; g < G; ++g) {
-          if (has_col[g]) {
-            L.col(dcol.as<ColDesc<T>>() + g * S, S, stream);
-            HIP_CHECK(hipGetLastError());
+        // uniform chunk: every step is one plain column and one plain row launch
+        bool uniform = true;
+        for (size_t g = 0; g < G; ++g)
+          uniform = uniform && has_col[g] && has_row[g] && !multi_col[g] && !multi_row[g];
+        int active = 0;
+        for (int i = 0; i < S; ++i) active += slot[i] != nullptr;
+        const TuneKey tk{slot_device(slot), precision_code(), S, L.ny, L.nx};
+        int trial = -1;
+        const std::pair<int, int> choice = tune_pick(tk, L.geo, trial);
+        Launcher<T, COMP> Lc = L;
+        Lc.geo.col_w = choice.first;
+        Lc.geo.rw = choice.second;
+        auto launch_chunk = [&](bool check) {
+          for (size_t g = 0; g < G; ++g) {
+            if (has_col[g]) {
+              Lc.col(dcol.as<ColDesc<T>>() + g * S, S, stream, multi_col[g] != 0);
+              if (check) HIP_CHECK(hipGetLastError());
+            }
+            if (has_row[g]) {
+              Lc.row(drow.as<RowDesc<T>>() + g * S, S, stream, multi_row[g] != 0);
+              if (check) HIP_CHECK(hipGetLastError());
+            }
           }
-          if (has_row[g]) {
-            L.row(drow.as<RowDesc<T>>() + g * S, S, stream);
-            HIP_CHECK(hipGetLastError());
+        };
+        if (uniform && trial >= 0) {
+          // a timed trial of a tuning candidate
+          hipEvent_t a, b;
+          HIP_CHECK(hipEventCreate(&a));
+          HIP_CHECK(hipEventCreate(&b));
+          HIP_CHECK(hipEventRecord(a, stream));
+          launch_chunk(true);
+          HIP_CHECK(hipEventRecord(b, stream));
+          HIP_CHECK(hipEventSynchronize(b));
+          float ms = 0;
+          HIP_CHECK(hipEventElapsedTime(&ms, a, b));
+          (void)hipEventDestroy(a);
+          (void)hipEventDestroy(b);
+          tune_report(tk, trial, 1e3 * ms / (static_cast<double>(G) * active));
+        } else if (uniform && graphs_ok) {
+          // uniform chunks replay one captured graph per shape (descriptors live in dcol/drow)
+          const auto gk = std::make_tuple(G, Lc.geo.col_w, Lc.geo.rw);
+          auto it = graphs.find(gk);
+          if (it == graphs.end()) {
+            // relaxed capture: other threads of the process keep allocating and copying meanwhile
+            hipGraph_t graph = nullptr;
+            hipGraphExec_t exec = nullptr;
+            bool ok = hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed) == hipSuccess;
+            if (ok) {
+              launch_chunk(false);
+              ok = hipStreamEndCapture(stream, &graph) == hipSuccess && graph != nullptr;
+              ok = ok && hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) == hipSuccess;
+              if (graph) (void)hipGraphDestroy(graph);
+            }
+            if (!ok) {
+              // the capture did not hold: clear its error, run this chunk directly and stop
+              // capturing on this lane
+              (void)hipGetLastError();
+              if (exec) (void)hipGraphExecDestroy(exec);
+              graphs_ok = false;
+              launch_chunk(true);
+            } else {
+              it = graphs.emplace(gk, exec).first;
+            }
           }
+          if (it != graphs.end()) HIP_CHECK(hipGraphLaunch(it->second, stream));
+        } else {
+          launch_chunk(true);
         }
         for (int i = 0; i < S; ++i) {
           if (!slot[i]) continue;
@@ -728,8 +916,19 @@ class Engine final : public EngineBase {
         (void)hipEventSynchronize(prev);
         (void)hipEventDestroy(prev);
       }
+      (void)hipStreamSynchronize(stream);
+      drop_graphs();
       throw;
     }
+    (void)hipStreamSynchronize(stream);
+    drop_graphs();
+  }
+
+  static int precision_code() { return COMP ? 32 : (sizeof(T) == 8 ? 64 : 31); }
+  static int slot_device(const std::vector<Engine*>& slot) {
+    for (const Engine* e : slot)
+      if (e) return e->sp_.device;
+    return 0;
   }
 
  private:

@@ -101,7 +101,10 @@ __device__ __forceinline__ cx<T> apply_prod1(const Prod<T>& pd, unsigned k, int 
 
 // Column kernel: W adjacent columns per workgroup, N/8 threads per column and component (thread
 // index (t·W + c)·2 + comp, so a wavefront reads W·sizeof(px2) contiguous bytes per row).
-template <int N, int W, typename T, bool COMP>
+// MULTI (launched for steps with several products): each thread keeps its eight spectrum values in
+// an LDS stash across the products instead of writing them back to X and reloading them. The
+// arithmetic is the same, so both forms give the same bits.
+template <int N, int W, typename T, bool COMP, bool MULTI>
 __global__ __launch_bounds__(N / 4 * W) void k_col_fast(const ColDesc<T>* __restrict__ descs, int nx,
                                                         long long plane, const cx<T>* __restrict__ g_twh,
                                                         const cx<T>* __restrict__ g_twl) {
@@ -111,6 +114,7 @@ __global__ __launch_bounds__(N / 4 * W) void k_col_fast(const ColDesc<T>* __rest
   __shared__ cx<T> sh[2 * W * LS];
   __shared__ cx<T> tw_hi_s[TWL ? reg_tw_len<N>() : 1];
   __shared__ cx<T> tw_lo_s[TWL && COMP ? reg_tw_len<N>() : 1];
+  __shared__ cx<T> stash[MULTI ? 2 * W * (N + 1) : 1];
   const ColDesc<T> d = descs[blockIdx.y];
   if (!d.in) return;
   if constexpr (TWL) stage_twiddles<N, COMP>(tw_hi_s, tw_lo_s, g_twh, g_twl);
@@ -130,14 +134,28 @@ __global__ __launch_bounds__(N / 4 * W) void k_col_fast(const ColDesc<T>* __rest
 #pragma unroll
   for (int m = 0; m < 8; ++m) s[m] = gload(xio, at(m));
   reg_fft<N, COMP, false>(s, line, t, tw_hi, tw_lo);
-  if (nprod > 1) {  // keep the spectrum in the thread's own positions of X
+  // several products: keep the spectrum (in the LDS stash, or in the thread's own positions of X);
+  // stash entry m of this lane is own[m·T], its component line is N + 1 entries long
+  cx<T>* own = stash;
+  if constexpr (MULTI) own = stash + (2 * c + comp) * (N + 1) + t;
+  if (nprod > 1) {
 #pragma unroll
-    for (int m = 0; m < 8; ++m) gstore(xio, at(m), s[m]);
+    for (int m = 0; m < 8; ++m) {
+      if constexpr (MULTI)
+        own[m * TT] = s[m];
+      else
+        gstore(xio, at(m), s[m]);
+    }
   }
   for (int o = 0; o < nprod; ++o) {
     if (o > 0) {
 #pragma unroll
-      for (int m = 0; m < 8; ++m) s[m] = gload(xio, at(m));
+      for (int m = 0; m < 8; ++m) {
+        if constexpr (MULTI)
+          s[m] = own[m * TT];
+        else
+          s[m] = gload(xio, at(m));
+      }
     }
     const Prod<T>* pp = d.prods + o;
     const Prod<T> pd{uniform(gload1(&pp->type, 0)), uniform(gload1(&pp->hi, 0)), uniform(gload1(&pp->lo, 0))};
@@ -151,8 +169,9 @@ __global__ __launch_bounds__(N / 4 * W) void k_col_fast(const ColDesc<T>* __rest
 }
 
 // Row kernel: RW rows per workgroup, N/8 threads per row and component (thread index
-// (r·N/8 + t)·2 + comp).
-template <int N, int RW, typename T, bool COMP>
+// (r·N/8 + t)·2 + comp). MULTI (launched for steps with several transmitted products): the
+// selected pixels of each product are merged in an LDS stash instead of product plane 0.
+template <int N, int RW, typename T, bool COMP, bool MULTI>
 __global__ __launch_bounds__(N / 4 * RW) void k_row_fast(const RowDesc<T>* __restrict__ descs, long long plane,
                                                          const cx<T>* __restrict__ g_twh,
                                                          const cx<T>* __restrict__ g_twl) {
@@ -162,6 +181,7 @@ __global__ __launch_bounds__(N / 4 * RW) void k_row_fast(const RowDesc<T>* __res
   __shared__ cx<T> sh[2 * RW * LS];
   __shared__ cx<T> tw_hi_s[TWL ? reg_tw_len<N>() : 1];
   __shared__ cx<T> tw_lo_s[TWL && COMP ? reg_tw_len<N>() : 1];
+  __shared__ cx<T> stash[MULTI ? 2 * RW * (N + 1) : 1];
   const RowDesc<T> a = descs[blockIdx.y];
   if (!a.active) return;
   if constexpr (TWL) stage_twiddles<N, COMP>(tw_hi_s, tw_lo_s, g_twh, g_twl);
@@ -207,7 +227,10 @@ __global__ __launch_bounds__(N / 4 * RW) void k_row_fast(const RowDesc<T>* __res
     for (int m = 0; m < 8; ++m) acc[m] = gload(src, at(m));
     reg_fft<N, COMP, true>(acc, line, t, tw_hi, tw_lo);
   } else {
-    cx<T>* merged = reinterpret_cast<cx<T>*>(const_cast<px2<T>*>(a.prod));  // product plane 0 collects
+    // the selected pixels are collected in the LDS stash or in product plane 0
+    cx<T>* merged = reinterpret_cast<cx<T>*>(const_cast<px2<T>*>(a.prod));
+    cx<T>* own = stash;  // stash entry m of this lane is own[m·T]
+    if constexpr (MULTI) own = stash + (2 * r + comp) * (N + 1) + t;
     for (int o = 0; o < a.nt; ++o) {
       const cx<T>* src = cplane(a.prod + o * plane);
 #pragma unroll
@@ -215,10 +238,20 @@ __global__ __launch_bounds__(N / 4 * RW) void k_row_fast(const RowDesc<T>* __res
       reg_fft<N, COMP, true>(acc, line, t, tw_hi, tw_lo);
 #pragma unroll
       for (int m = 0; m < 8; ++m)
-        if (gload1(a.tsel, row + t + TT * m) == o) gstore(merged, at(m), acc[m]);
+        if (gload1(a.tsel, row + t + TT * m) == o) {
+          if constexpr (MULTI)
+            own[m * TT] = acc[m];
+          else
+            gstore(merged, at(m), acc[m]);
+        }
     }
 #pragma unroll
-    for (int m = 0; m < 8; ++m) acc[m] = gload(merged, at(m));
+    for (int m = 0; m < 8; ++m) {
+      if constexpr (MULTI)
+        acc[m] = own[m * TT];
+      else
+        acc[m] = gload(merged, at(m));
+    }
   }
 
   // 3. epilogue in the order of the reference: mask, Born increment, sparse sources, Born plane

@@ -114,3 +114,23 @@ def test_poisson_stream_of_the_served_mix() -> None:
     assert len(seen) == len(reqs)
     for r in seen.values():
         assert np.all(np.isfinite(r["transmitted"]))
+
+
+def test_replayed_lane_graphs_match_eager_solves() -> None:
+    # slab steps in a homogeneous medium make uniform chunks: the lane captures one graph and
+    # replays it for every chunk of every request, with eager work of other requests in between
+    reqs = []
+    for i in range(6):
+        p = workloads.tilted_3d(n=64, nz=48 + 8 * i, iterations=1 + i % 2, polarisation="xy"[i % 2])
+        p.labels = np.zeros_like(p.labels)
+        p.indices = [1.33 + 0j]
+        p.index_before = 1.33 + 0j
+        reqs.append(p)
+    other = workloads.tilted_3d(n=96, nz=40, iterations=1)
+    with serve.Server(slots=2, chunk=16) as srv:
+        tickets = [srv.submit(p) for p in reqs]
+        t_other = srv.submit(other)
+        results = [srv.result(t, timeout=120) for t in tickets]
+        _same(srv.result(t_other, timeout=120), _alone(other, "fp64"))
+    for p, res in zip(reqs, results):
+        _same(res, _alone(p, "fp64"))
