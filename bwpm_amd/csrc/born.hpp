@@ -217,6 +217,65 @@ __global__ void k_axpy_multi(px2<T>* w, const px2<T>* const* __restrict__ vs, co
   w[q] = acc;
 }
 
+// Partial dots <B_i, v> of the basis vectors B_i = basis + i·stride (blockIdx.y = i); the same
+// summation order as k_dot_partial.
+template <typename T>
+__global__ __launch_bounds__(kRedThreads) void k_dot_basis(const px2<T>* __restrict__ basis, long long stride,
+                                                           const px2<T>* __restrict__ v, long long npix,
+                                                           double2* __restrict__ partial) {
+  __shared__ double sr[kRedThreads], si[kRedThreads];
+  const px2<T>* u = basis + blockIdx.y * stride;
+  double re = 0, im = 0;
+  for (long long q = blockIdx.x * static_cast<long long>(kRedThreads) + threadIdx.x; q < npix;
+       q += static_cast<long long>(kRedBlocks) * kRedThreads) {
+    const px2<T> a = u[q], b = v[q];
+    re += static_cast<double>(a.a.x) * b.a.x + static_cast<double>(a.a.y) * b.a.y +
+          static_cast<double>(a.b.x) * b.b.x + static_cast<double>(a.b.y) * b.b.y;
+    im += static_cast<double>(a.a.x) * b.a.y - static_cast<double>(a.a.y) * b.a.x +
+          static_cast<double>(a.b.x) * b.b.y - static_cast<double>(a.b.y) * b.b.x;
+  }
+  sr[threadIdx.x] = re;
+  si[threadIdx.x] = im;
+  __syncthreads();
+  for (int s = kRedThreads / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      sr[threadIdx.x] += sr[threadIdx.x + s];
+      si[threadIdx.x] += si[threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.y * kRedBlocks + blockIdx.x] = double2{sr[0], si[0]};
+}
+
+// w −= Σ_i c_i·B_i with the coefficients c (FP64) read from device memory (no host round trip).
+template <typename T>
+__global__ void k_sub_basis(px2<T>* w, const px2<T>* __restrict__ basis, long long stride,
+                            const double2* __restrict__ c, int k, long long npix) {
+  const long long q = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;
+  if (q >= npix) return;
+  px2<T> acc = w[q];
+  for (int i = 0; i < k; ++i) {
+    const cx<T> ci{static_cast<T>(-c[i].x), static_cast<T>(-c[i].y)};
+    const px2<T> v = basis[i * stride + q];
+    acc.a = acc.a + ci * v.a;
+    acc.b = acc.b + ci * v.b;
+  }
+  w[q] = acc;
+}
+
+// out = w / sqrt(n2.x), with n2 (a squared norm) read from device memory; a zero norm leaves out.
+template <typename T>
+__global__ void k_normalize(const px2<T>* __restrict__ w, const double2* __restrict__ n2, px2<T>* out,
+                            long long npix) {
+  const long long q = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;
+  if (q >= npix) return;
+  const double nrm = sqrt(n2->x);
+  if (!(nrm > 0.0)) return;
+  const T s = static_cast<T>(1.0 / nrm);
+  const px2<T> u = w[q];
+  out[q] = px2<T>{u.a * s, u.b * s};
+}
+
 // out = a·x + b·y (real a, b).
 template <typename T>
 __global__ void k_axpby(const px2<T>* x, T a, const px2<T>* y, T b, px2<T>* out, long long npix) {
@@ -285,10 +344,10 @@ class Born {
     // Born planes: born_bwd[p + 1] (R1) and born_fwd[p + 1] (R3) of each Born slab p
     planes_.alloc(2 * static_cast<size_t>(std::max<long long>(nborn, 1)) * pb);
     partial_.alloc(static_cast<size_t>(kRestart + 2) * kRedBlocks * sizeof(double2));
-    dres_.alloc((kRestart + 2) * sizeof(double2));
+    dres_.alloc((2 * (kRestart + 2) + 1) * sizeof(double2));
     coef_.alloc((kRestart + 2) * sizeof(double2));
     ptrs_.alloc(2 * (kRestart + 2) * sizeof(void*));
-    HIP_CHECK(hipHostMalloc(&hres_, (kRestart + 2) * sizeof(double2), hipHostMallocDefault));
+    HIP_CHECK(hipHostMalloc(&hres_, (2 * (kRestart + 2) + 1) * sizeof(double2), hipHostMallocDefault));
     choose_fft_geometry();
     HIP_CHECK(hipStreamSynchronize(stream_));
     bytes_ = V_.bytes() + tabs_.bytes() + 10 * pb + basis_.bytes() + planes_.bytes();
@@ -557,6 +616,36 @@ class Born {
     HIP_CHECK(hipStreamSynchronize(stream_));
   }
 
+  // One Arnoldi step after w = G·B_j: classical Gram–Schmidt twice against B_0 … B_j and the new
+  // basis vector B_{j+1} = w/‖w‖, all on the device; the host receives the two projections and
+  // ‖w‖² in one copy (hv = h1 + h2, the Hessenberg column) and returns ‖w‖.
+  double arnoldi(int j, px2<T>* w, std::complex<double>* hv) {
+    const int k = j + 1;
+    const px2<T>* basis = basis_.template as<px2<T>>();
+    double2* part = partial_.template as<double2>();
+    double2* res = dres_.template as<double2>();
+    constexpr int kH2 = kRestart + 2, kN2 = 2 * (kRestart + 2);  // offsets of h2 and ‖w‖² in res
+    auto dot = [&](const px2<T>* base, int n, const px2<T>* v, double2* out) {
+      hipLaunchKernelGGL(k_dot_basis<T>, dim3(kRedBlocks, n), dim3(kRedThreads), 0, stream_, base,
+                         static_cast<long long>(npix_), v, npix_, part);
+      HIP_CHECK(hipGetLastError());
+      hipLaunchKernelGGL(k_dot_final, dim3(n), dim3(kRedThreads), 0, stream_, part, out);
+      HIP_CHECK(hipGetLastError());
+    };
+    dot(basis, k, w, res);
+    launch(k_sub_basis<T>, w, basis, static_cast<long long>(npix_), static_cast<const double2*>(res), k, npix_);
+    dot(basis, k, w, res + kH2);
+    launch(k_sub_basis<T>, w, basis, static_cast<long long>(npix_), static_cast<const double2*>(res + kH2), k,
+           npix_);
+    dot(w, 1, w, res + kN2);
+    launch(k_normalize<T>, static_cast<const px2<T>*>(w), static_cast<const double2*>(res + kN2), B(j + 1), npix_);
+    HIP_CHECK(hipMemcpyAsync(hres_, res, (kN2 + 1) * sizeof(double2), hipMemcpyDeviceToHost, stream_));
+    HIP_CHECK(hipStreamSynchronize(stream_));
+    for (int i = 0; i < k; ++i)
+      hv[i] = std::complex<double>(hres_[i].x, hres_[i].y) + std::complex<double>(hres_[kH2 + i].x, hres_[kH2 + i].y);
+    return std::sqrt(hres_[kN2].x);
+  }
+
   // Restarted GMRES on (1 − G/2)x = b with b = rhs_, x = x_ (start value), following the structure
   // of the reference's solver (solver.cpp:130-207) with block Gram–Schmidt.
   bool gmres(int d, long long slab) {
@@ -570,7 +659,7 @@ class Born {
       return true;
     }
     const int m = kRestart;
-    std::vector<std::complex<double>> h((m + 1) * m), g(m + 1), cs(m), sn(m), hv(m + 1), h2(m + 1);
+    std::vector<std::complex<double>> h((m + 1) * m), g(m + 1), cs(m), sn(m), hv(m + 1);
     for (int outer = 0; outer < kMaxRestarts; ++outer) {
       apply_g(x, d, slab, w, 1);
       launch(k_axpby<T>, b, static_cast<T>(1), static_cast<const px2<T>*>(w), static_cast<T>(-1), B(0), npix_);
@@ -586,22 +675,9 @@ class Born {
       for (int j = 0; j < m; ++j) {
         apply_g(B(j), d, slab, w, 1);
         ++iterations_;
-        for (int i = 0; i <= j; ++i) vs[i] = B(i);
-        // classical Gram–Schmidt, twice
-        dots(vs.data(), j + 1, w, hv.data());
-        for (int i = 0; i <= j; ++i) h2[i] = -hv[i];
-        axpy(w, vs.data(), h2.data(), j + 1);
-        dots(vs.data(), j + 1, w, h2.data());
-        for (int i = 0; i <= j; ++i) {
-          hv[i] += h2[i];
-          h2[i] = -h2[i];
-        }
-        axpy(w, vs.data(), h2.data(), j + 1);
+        // classical Gram–Schmidt, twice, and the next basis vector (device)
+        const double hnext = arnoldi(j, w, hv.data());
         for (int i = 0; i <= j; ++i) h[i * m + j] = hv[i];
-        const double hnext = norm(w);
-        if (hnext > 0.0)
-          launch(k_axpby<T>, static_cast<const px2<T>*>(w), static_cast<T>(1.0 / hnext),
-                 static_cast<const px2<T>*>(nullptr), static_cast<T>(0), B(j + 1), npix_);
         for (int i = 0; i < j; ++i) {  // previous rotations on column j
           const std::complex<double> a = h[i * m + j], c = h[(i + 1) * m + j];
           h[i * m + j] = cs[i] * a + sn[i] * c;
