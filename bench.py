@@ -12,8 +12,9 @@ Printed: one JSON line with grid-point·slab updates/s (whole job), the roofline
 (HIP-event durations of the column and the row kernel on the lane's stream; algorithmic bytes per
 update = one read and one write of the (E_x, E_y) plane: 64 B in complex128, 32 B in complex64),
 the plain one-solve-per-launch rate of the same solve on the same GPU, the accuracy of the run
-(batch result against the solve alone; FP32 against the FP64 device path), the served mix on the
-request queue (capacity, then p50/p99 latency of a Poisson stream at 70 % of it), and the
+(batch result against the solve alone; FP32 against the FP64 device path), the reference's other
+configurations as single solves (2D, 3D 256² tilted, 3D 512² Born volume at reduced depth), the
+served mix on the request queue (capacity, then p50/p99 latency of a Poisson stream at 70 % of it), and the
 reference's CPU core timed on this host over a bounded sample of the same geometry.
 
     python bench.py [--steps K] [--warmup W] [--batch B] [--n 512] [--nz 1024] [--iterations 4]
@@ -87,6 +88,54 @@ def cpu_baseline(n: int, nz_sample: int, threads: int) -> dict:
             "sample": f"tilted_3d {n}x{n}x{nz_sample} slabs, 1 iteration ({prob.updates} updates) in {dt:.2f} s"}
 
 
+def run_solve(p, precision: str) -> tuple[dict, float]:
+    """One solve resident on the device: result and the updates/s of its timed run (after a warm run)."""
+    from bwpm_amd import _core
+
+    s = _core.GpuSolver(*p.solver_args(), fluctuation=p.fluctuation, precision=precision)
+    try:
+        s.upload(p.field)
+        s.run_resident(p.iterations)
+        t0 = time.perf_counter()
+        s.run_resident(p.iterations)
+        dt = time.perf_counter() - t0
+        return s.fetch(p.iterations), p.updates / dt
+    finally:
+        s.close()
+
+
+def configs(c3_nz: int) -> dict:
+    """The reference's other configurations, each as one solve on the device: config 1 (2D, 1024
+    points, 200 slabs, 3 iterations), config 2 (3D 256² × 512 tilted interface, 4 iterations, E_x
+    and E_y) and config 3 (3D 512² Born volume, 6 iterations, at `c3_nz` of its 1024 slabs). FP32
+    errors are relative L2 against the FP64 device result (itself within 1e-10 of the oracle)."""
+    from bwpm_amd import workloads
+
+    out: dict = {}
+    p1 = workloads.plain_2d(ny=1024, nz=200, iterations=3)
+    _, r1 = run_solve(p1, "fp64")
+    out["c1_2d_1024x200_it3"] = {"fp64_updates_per_s": r1}
+    c2 = {}
+    for pol in ("x", "y"):
+        p2 = workloads.tilted_3d(n=256, nz=512, iterations=4, polarisation=pol)
+        a64, r64 = run_solve(p2, "fp64")
+        a32, r32 = run_solve(p2, "fp32")
+        c2[f"E{pol}"] = {"fp64_updates_per_s": r64, "fp32_updates_per_s": r32,
+                         "fp32_rel_l2_transmitted": rel_l2(a32["transmitted"], a64["transmitted"]),
+                         "fp32_rel_l2_reflected": rel_l2(a32["reflected"], a64["reflected"])}
+    out["c2_tilted_256x256x512_it4"] = c2
+    p3 = workloads.born_3d(n=512, nz=c3_nz, iterations=6)
+    b64, q64 = run_solve(p3, "fp64")
+    b32, q32 = run_solve(p3, "fp32")
+    out[f"c3_born_512x512x{c3_nz}_it6"] = {
+        "fp64_updates_per_s": q64, "fp32_updates_per_s": q32,
+        "fp64_gmres_iterations": b64["gmres_iterations"],
+        "fp32_rel_l2_transmitted": rel_l2(b32["transmitted"], b64["transmitted"]),
+        "fp32_rel_l2_reflected": rel_l2(b32["reflected"], b64["reflected"]),
+        "note": f"{c3_nz} of the configuration's 1024 slabs (the δn volume and run time scale with nz)"}
+    return out
+
+
 def served(count: int, precision: str) -> dict:
     """The served mix on the request queue: a burst of `count` requests measures capacity, then a
     Poisson stream of `count` other requests at 70 % of it gives latency under load."""
@@ -139,6 +188,7 @@ def main() -> int:
     ap.add_argument("--no-checks", action="store_true", help="skip the plain-path and accuracy runs")
     ap.add_argument("--serve-requests", type=int, default=24,
                     help="requests of the served mix (0: no serving run)")
+    ap.add_argument("--c3-nz", type=int, default=128, help="slabs of the Born configuration (0: skip configs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU figures are not produced",
@@ -226,6 +276,10 @@ def main() -> int:
             checks["rel_l2_vs_fp64_device"] = {"transmitted": rel_l2(alone["transmitted"], r64["transmitted"]),
                                                "reflected": rel_l2(alone["reflected"], r64["reflected"])}
 
+    for s in solvers:
+        s.close()
+    solvers = []
+    other = configs(args.c3_nz) if args.c3_nz > 0 else None
     serving = None
     if args.serve_requests > 0:
         serving = served(args.serve_requests, args.precision)
@@ -255,6 +309,7 @@ def main() -> int:
         },
         "roofline": roofline,
         "checks": checks,
+        "configs": other,
         "serving": serving,
         "cpu_baseline": cpu,
     }
