@@ -19,6 +19,8 @@ from typing import Any
 
 import numpy as np
 
+from bwpm_amd.specs import FluctuationSynthesis, Uniform
+
 _LIB_PATH = Path(os.environ.get("BWPM_HIP_LIB", Path(__file__).resolve().parent / "lib" / "libbwpm_hip.so"))
 if not _LIB_PATH.exists():
     raise ImportError(
@@ -84,6 +86,11 @@ _q_create = _sig("bwpm_queue_create", _int, _d, _int, _int, ctypes.POINTER(_p))
 _q_destroy = _sig("bwpm_queue_destroy", _p)
 _q_submit = _sig("bwpm_queue_submit", _p, _i64, _i64, _d, _d, _d, _d, _p, _i64, _p, _i64, _d, _d, _p, _p, _p, _int,
                  _p, _int, ctypes.POINTER(_i64))
+_q_submit_synth = _sig("bwpm_queue_submit_synth", _p, _i64, _i64, _d, _d, _d, _d, _p, _i64, _p, _int, _i64, _d, _d,
+                       _p, _p, _d, _d, ctypes.c_uint64, _int, _p, _int, ctypes.POINTER(_i64))
+_create_synth = _sig("bwpm_solver_create_synth", _i64, _i64, _d, _d, _d, _d, _p, _i64, _p, _int, _i64, _d, _d, _p,
+                     _p, _d, _d, ctypes.c_uint64, _int, _int, ctypes.POINTER(_p))
+_fluct = _sig("bwpm_solver_fluctuation", _p, _p)
 _q_state = _sig("bwpm_queue_state", _p, _i64, ctypes.POINTER(_int))
 _q_wait = _sig("bwpm_queue_wait", _p, _i64, _d, ctypes.POINTER(_int))
 _q_result = _sig("bwpm_queue_result", _p, _i64, ctypes.POINTER(_p))
@@ -174,20 +181,34 @@ def _solver_arrays(ny: int, nx: int, indices: Any, labels: Any, mask: Any, fluct
     """Checked, contiguous copies of a solve's arrays (shapes as cpp/python/bindings.cpp:100-115)."""
     if precision not in PRECISIONS:
         raise ValueError(f"precision must be one of {sorted(PRECISIONS)}")
-    lab = np.asarray(labels)
-    if lab.ndim != 3:
-        raise ValueError("labels must have shape (nz, ny, nx)")
-    lab = np.ascontiguousarray(lab, dtype=np.int32)
+    if isinstance(labels, Uniform):
+        if int(labels.nz) < 1:
+            raise ValueError("a uniform volume needs nz >= 1")
+        lab = labels
+    else:
+        lab = np.asarray(labels)
+        if lab.ndim != 3:
+            raise ValueError("labels must have shape (nz, ny, nx)")
+        lab = np.ascontiguousarray(lab, dtype=np.int32)
     idx = np.ascontiguousarray(np.asarray(indices, dtype=np.complex128).ravel())
     m = None if mask is None else _f64(mask)
     if m is not None and m.size != int(ny) * int(nx):
         raise ValueError("mask must have shape (ny, nx)")
     f = None
-    if fluctuation is not None:
+    if isinstance(fluctuation, FluctuationSynthesis):
+        if fluctuation.sigma < 0 or fluctuation.corr_mm < 0:
+            raise ValueError("sigma and corr_mm must be >= 0")
+        f = fluctuation
+    elif fluctuation is not None:
+        if isinstance(lab, Uniform):
+            raise ValueError("a fluctuation array needs a label array")
         f = np.ascontiguousarray(np.asarray(fluctuation, dtype=np.complex128))
         if f.shape != lab.shape:
             raise ValueError("fluctuation must have the shape of labels")
-    if lab.shape[1:] != (int(ny), int(nx)):
+    if isinstance(lab, Uniform):
+        if not isinstance(f, FluctuationSynthesis):
+            raise ValueError("a uniform label volume is given with a synthesised fluctuation")
+    elif lab.shape[1:] != (int(ny), int(nx)):
         raise ValueError("labels must have shape (nz, ny, nx)")
     lb = None
     if labels_before is not None:
@@ -256,9 +277,18 @@ class GpuSolver:
         self._h = _p(None)
         idx, lab, m, f, lb = _solver_arrays(ny, nx, indices, labels, mask, fluctuation, labels_before, precision)
         ib = complex(index_before)
-        self.ny, self.nx, self.nz = int(ny), int(nx), int(lab.shape[0])
+        nz = int(lab.nz) if isinstance(lab, Uniform) else int(lab.shape[0])
+        self.ny, self.nx, self.nz = int(ny), int(nx), nz
         self.precision = precision
         self.device = int(device)
+        if isinstance(f, FluctuationSynthesis):
+            arr = None if isinstance(lab, Uniform) else _ptr(lab)
+            ul = int(lab.label) if isinstance(lab, Uniform) else -1
+            _check(_create_synth(int(ny), int(nx), float(dy), float(dx), float(dz), float(k0), _ptr(idx), idx.size,
+                                 arr, ul, nz, ib.real, ib.imag, None if m is None else _ptr(m),
+                                 None if lb is None else _ptr(lb), float(f.sigma), float(f.corr_mm), int(f.seed),
+                                 int(device), PRECISIONS[precision], ctypes.byref(self._h)))
+            return
         _check(_create(int(ny), int(nx), float(dy), float(dx), float(dz), float(k0), _ptr(idx), idx.size,
                        _ptr(lab), lab.shape[0], ib.real, ib.imag, None if m is None else _ptr(m),
                        None if f is None else _ptr(f), None if lb is None else _ptr(lb), int(device),
@@ -313,6 +343,12 @@ class GpuSolver:
                              ctypes.byref(cms), ctypes.byref(rms), ctypes.byref(cl), ctypes.byref(rl)))
         return {"gpu_ms": ms.value, "col_ms": cms.value, "row_ms": rms.value, "col_launches": cl.value,
                 "row_launches": rl.value}
+
+    def fluctuation(self) -> np.ndarray:
+        """The δn the solver holds, (nz, ny, nx) complex128 (generated or given; zeros without)."""
+        out = np.empty((self.nz, self.ny, self.nx), dtype=np.complex128)
+        _check(_fluct(self._h, _ptr(out)))
+        return out
 
     def device_bytes(self) -> int:
         b = _i64(0)
@@ -382,10 +418,19 @@ class GpuQueue:
             raise ValueError("iterations must be >= 0")
         ib = complex(index_before)
         t = _i64(0)
-        _check(_q_submit(self._h, int(ny), int(nx), float(dy), float(dx), float(dz), float(k0), _ptr(idx), idx.size,
-                         _ptr(lab), lab.shape[0], ib.real, ib.imag, None if m is None else _ptr(m),
-                         None if f is None else _ptr(f), None if lb is None else _ptr(lb), PRECISIONS[precision],
-                         _ptr(e), int(iterations), ctypes.byref(t)))
+        if isinstance(f, FluctuationSynthesis):
+            uni = isinstance(lab, Uniform)
+            _check(_q_submit_synth(self._h, int(ny), int(nx), float(dy), float(dx), float(dz), float(k0), _ptr(idx),
+                                   idx.size, None if uni else _ptr(lab), int(lab.label) if uni else -1,
+                                   int(lab.nz) if uni else lab.shape[0], ib.real, ib.imag,
+                                   None if m is None else _ptr(m), None if lb is None else _ptr(lb), float(f.sigma),
+                                   float(f.corr_mm), int(f.seed), PRECISIONS[precision], _ptr(e), int(iterations),
+                                   ctypes.byref(t)))
+        else:
+            _check(_q_submit(self._h, int(ny), int(nx), float(dy), float(dx), float(dz), float(k0), _ptr(idx),
+                             idx.size, _ptr(lab), lab.shape[0], ib.real, ib.imag, None if m is None else _ptr(m),
+                             None if f is None else _ptr(f), None if lb is None else _ptr(lb), PRECISIONS[precision],
+                             _ptr(e), int(iterations), ctypes.byref(t)))
         self._shape[int(t.value)] = (int(ny), int(nx))
         return int(t.value)
 

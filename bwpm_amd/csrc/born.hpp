@@ -286,6 +286,89 @@ __global__ void k_axpby(const px2<T>* x, T a, const px2<T>* y, T b, px2<T>* out,
   out[q] = px2<T>{u.a * a + v.a * b, u.b * a + v.b * b};
 }
 
+// ---- δn generated on the device ------------------------------------------------------------
+
+// splitmix64 (S. Vigna's public-domain generator, finaliser form): element i of stream `seed`.
+__device__ __forceinline__ unsigned long long synth_mix(unsigned long long z) {
+  z += 0x9E3779B97F4A7C15ULL;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+
+// Standard normal value i of stream `seed` (Box–Muller on two independent uniforms).
+__device__ __forceinline__ double synth_normal(unsigned long long seed, unsigned long long i) {
+  const unsigned long long a = synth_mix(seed ^ synth_mix(2 * i));
+  const unsigned long long b = synth_mix(seed ^ synth_mix(2 * i + 1));
+  const double u1 = (static_cast<double>(a >> 11) + 1.0) * 0x1.0p-53;  // (0, 1]
+  const double u2 = static_cast<double>(b >> 11) * 0x1.0p-53;          // [0, 1)
+  return sqrt(-2.0 * log(u1)) * cos(6.283185307179586476925286766559 * u2);
+}
+
+template <typename T>
+__global__ void k_synth_noise(cx<T>* v, long long n, unsigned long long seed) {
+  const long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;
+  if (i >= n) return;
+  v[i] = cx<T>{static_cast<T>(synth_normal(seed, static_cast<unsigned long long>(i))), static_cast<T>(0)};
+}
+
+// Periodic convolution along one axis (coordinate c = (i / stride) mod len) with the symmetric
+// weights w[0 … r]: reads the real parts and writes the imaginary parts (from_real), or back.
+template <typename T>
+__global__ void k_synth_conv(cx<T>* v, long long n, long long stride, long long len, int r,
+                             const double* __restrict__ w, int from_real) {
+  const long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;
+  if (i >= n) return;
+  const long long c = (i / stride) % len;
+  const long long base = i - c * stride;
+  double acc = 0;
+  for (int d = -r; d <= r; ++d) {
+    long long cc = (c + d) % len;
+    if (cc < 0) cc += len;
+    const cx<T> u = v[base + cc * stride];
+    acc += w[d < 0 ? -d : d] * static_cast<double>(from_real ? u.x : u.y);
+  }
+  if (from_real)
+    v[i].y = static_cast<T>(acc);
+  else
+    v[i].x = static_cast<T>(acc);
+}
+
+// Partial sums (Σ v, Σ v²) of the imaginary parts, kRedBlocks per launch (deterministic order).
+template <typename T>
+__global__ __launch_bounds__(kRedThreads) void k_synth_moments(const cx<T>* __restrict__ v, long long n,
+                                                               double2* __restrict__ partial) {
+  __shared__ double s1[kRedThreads], s2[kRedThreads];
+  double a = 0, b = 0;
+  for (long long q = blockIdx.x * static_cast<long long>(kRedThreads) + threadIdx.x; q < n;
+       q += static_cast<long long>(kRedBlocks) * kRedThreads) {
+    const double x = static_cast<double>(v[q].y);
+    a += x;
+    b += x * x;
+  }
+  s1[threadIdx.x] = a;
+  s2[threadIdx.x] = b;
+  __syncthreads();
+  for (int s = kRedThreads / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      s1[threadIdx.x] += s1[threadIdx.x + s];
+      s2[threadIdx.x] += s2[threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = double2{s1[0], s2[0]};
+}
+
+// V = scale_slab · f · v (v: the filtered noise in the imaginary parts; f = σ / std).
+template <typename T>
+__global__ void k_synth_scale(cx<T>* v, long long n, long long npix, double f, const zd* __restrict__ scale) {
+  const long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;
+  if (i >= n) return;
+  const double d = f * static_cast<double>(v[i].y);
+  const zd sc = scale[i / npix];
+  v[i] = cx<T>{static_cast<T>(sc.x * d), static_cast<T>(sc.y * d)};
+}
+
 template <typename T, bool COMP>
 class Born {
  public:
@@ -312,7 +395,7 @@ class Born {
     hosts_ = hosts;
     // potential V = 2ñ₀k₀²δn per Born slab (solver.cpp:241,254)
     V_.alloc(static_cast<size_t>(std::max<long long>(nborn, 1)) * npix_ * sizeof(cx<T>));
-    {
+    if (!s.synth.on) {
       std::vector<cx<T>> hv(npix_);
       for (long long j = 0; j < nz_; ++j) {
         if (slab_v_[j] < 0) continue;
@@ -348,6 +431,7 @@ class Born {
     coef_.alloc((kRestart + 2) * sizeof(double2));
     ptrs_.alloc(2 * (kRestart + 2) * sizeof(void*));
     HIP_CHECK(hipHostMalloc(&hres_, (2 * (kRestart + 2) + 1) * sizeof(double2), hipHostMallocDefault));
+    if (s.synth.on) synthesise(s, host);
     choose_fft_geometry();
     HIP_CHECK(hipStreamSynchronize(stream_));
     bytes_ = V_.bytes() + tabs_.bytes() + 10 * pb + basis_.bytes() + planes_.bytes();
@@ -365,6 +449,21 @@ class Born {
   const px2<T>* fwd_plane(long long p) const { return plane(p - 1, 1); }
   long long iterations() const { return iterations_; }
   size_t device_bytes() const { return bytes_; }
+
+  // δn of every slab, (nz, ny, nx) complex, from the potential V = 2ñ₀k₀²δn held on the device.
+  std::vector<c128> fluctuation() const {
+    std::vector<c128> out(static_cast<size_t>(nz_) * npix_, c128(0.0, 0.0));
+    std::vector<cx<T>> hv(npix_);
+    for (long long j = 0; j < nz_; ++j) {
+      if (slab_v_[j] < 0) continue;
+      HIP_CHECK(hipMemcpy(hv.data(), V_.template as<cx<T>>() + slab_v_[j] * npix_, npix_ * sizeof(cx<T>),
+                          hipMemcpyDeviceToHost));
+      const c128 scale = 2.0 * hosts_[slab_tab_[j]] * k0_ * k0_;
+      for (long long q = 0; q < npix_; ++q)
+        out[j * npix_ + q] = c128(static_cast<double>(hv[q].x), static_cast<double>(hv[q].y)) / scale;
+    }
+    return out;
+  }
 
   // Forward step p: increment of slab p into increment(); with `bidir` also R1 → bwd_plane(p + 1).
   void forward(const px2<T>* e, long long p, bool bidir) {
@@ -399,6 +498,12 @@ class Born {
   template <typename K, typename... A>
   void launch(K kern, A... args) {
     const unsigned grid = static_cast<unsigned>((npix_ + 255) / 256);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, stream_, args...);
+    HIP_CHECK(hipGetLastError());
+  }
+  template <typename K, typename... A>
+  void launch_n(K kern, long long n, A... args) {
+    const unsigned grid = static_cast<unsigned>((n + 255) / 256);
     hipLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, stream_, args...);
     HIP_CHECK(hipGetLastError());
   }
@@ -613,6 +718,64 @@ class Born {
     HIP_CHECK(hipMemcpyAsync(coef_.template as<void>(), hc, k * sizeof(double2), hipMemcpyHostToDevice, stream_));
     launch(k_axpy_multi<T>, w, reinterpret_cast<const px2<T>* const*>(dptr), coef_.template as<double2>(), k, npix_);
     // the host staging arrays are reused by the next call: wait for the copies to leave them
+    HIP_CHECK(hipStreamSynchronize(stream_));
+  }
+
+  // Gaussian-correlated δn generated in V_ (every slab is a Born slab): normal noise in the real
+  // parts, periodic Gaussian convolutions along x, y and z (weights exp(−(d·Δ)²/corr²), the
+  // real-space form of the filter exp(−|q|²corr²/4), truncated at 4·corr), then δn = σ·v/std(v)
+  // (no mean removed, as the reference's construction in tests/test_fluctuation_3d.py:34-41)
+  // and V = 2ñ₀k₀²·δn per slab.
+  void synthesise(const Spec& s, const std::vector<c128>& host) {
+    const long long n = nz_ * npix_;
+    cx<T>* v = V_.template as<cx<T>>();
+    launch_n(k_synth_noise<T>, n, v, n, static_cast<unsigned long long>(s.synth.seed));
+    const double spacing[3] = {s.dx, s.dy, s.dz};
+    const long long stride[3] = {1, nx_, npix_};
+    const long long len[3] = {nx_, ny_, nz_};
+    Dev w;
+    int from_real = 1;
+    for (int axis = 0; axis < 3; ++axis) {
+      int r = 0;
+      if (s.synth.corr > 0) r = static_cast<int>(std::ceil(4.0 * s.synth.corr / spacing[axis]));
+      r = static_cast<int>(std::min<long long>(r, (len[axis] - 1) / 2));
+      std::vector<double> hw(r + 1);
+      for (int d = 0; d <= r; ++d) {
+        const double x = d * spacing[axis] / (s.synth.corr > 0 ? s.synth.corr : 1.0);
+        hw[d] = std::exp(-x * x);
+      }
+      w.alloc(hw.size() * sizeof(double));
+      HIP_CHECK(hipMemcpyAsync(w.template as<void>(), hw.data(), hw.size() * sizeof(double), hipMemcpyHostToDevice,
+                               stream_));
+      launch_n(k_synth_conv<T>, n, v, n, stride[axis], len[axis], r, w.template as<const double>(), from_real);
+      HIP_CHECK(hipStreamSynchronize(stream_));  // the weights buffer is replaced next
+      from_real ^= 1;
+    }
+    if (from_real)  // three passes leave the result in the imaginary parts; keep that convention
+      launch_n(k_synth_conv<T>, n, v, n, 1, 1, 0, w.template as<const double>(), 0);
+    // moments of the filtered field
+    hipLaunchKernelGGL(k_synth_moments<T>, dim3(kRedBlocks), dim3(kRedThreads), 0, stream_,
+                       static_cast<const cx<T>*>(v), n, partial_.template as<double2>());
+    HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(kRedThreads), 0, stream_, partial_.template as<double2>(),
+                       dres_.template as<double2>());
+    HIP_CHECK(hipGetLastError());
+    double2 mom{};
+    HIP_CHECK(hipMemcpyAsync(&mom, dres_.template as<void>(), sizeof(double2), hipMemcpyDeviceToHost, stream_));
+    HIP_CHECK(hipStreamSynchronize(stream_));
+    const double mean = mom.x / static_cast<double>(n);
+    const double var = std::max(0.0, mom.y / static_cast<double>(n) - mean * mean);
+    const double f = var > 0 ? s.synth.sigma / std::sqrt(var) : 0.0;
+    std::vector<zd> sc(nz_);
+    for (long long j = 0; j < nz_; ++j) {
+      const c128 a = 2.0 * host[j] * s.k0 * s.k0;
+      sc[j] = zd{a.real(), a.imag()};
+    }
+    Dev dsc;
+    dsc.alloc(sc.size() * sizeof(zd));
+    HIP_CHECK(hipMemcpyAsync(dsc.template as<void>(), sc.data(), sc.size() * sizeof(zd), hipMemcpyHostToDevice,
+                             stream_));
+    launch_n(k_synth_scale<T>, n, v, n, npix_, f, dsc.template as<const zd>());
     HIP_CHECK(hipStreamSynchronize(stream_));
   }
 

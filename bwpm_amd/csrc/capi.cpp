@@ -65,10 +65,11 @@ int interface_call(int which, const double* kx, const double* ky, int64_t n, dou
 
 bwpm::Spec make_spec(int64_t ny, int64_t nx, double dy, double dx, double dz, double k0, const double* indices,
                    int64_t m, const int32_t* labels, int64_t nz, double ib_re, double ib_im, const double* mask,
-                   const double* fluct, const int32_t* labels_before, int device, int precision) {
+                   const double* fluct, const int32_t* labels_before, int device, int precision,
+                   int uniform_label = -1) {
   bwpm::require(ny >= 1 && nx >= 1 && nz >= 1, "labels must have shape (nz, ny, nx)");
   bwpm::require(m >= 0 && (m == 0 || indices), "indices missing");
-  bwpm::require(labels != nullptr, "labels missing");
+  bwpm::require(labels != nullptr || uniform_label >= 0, "labels missing");
   bwpm::require(precision == BWPM_FP64 || precision == BWPM_FP32 || precision == BWPM_FP32_PLAIN,
                 "precision must be BWPM_FP64, BWPM_FP32 or BWPM_FP32_PLAIN");
   bwpm::Spec s;
@@ -80,7 +81,10 @@ bwpm::Spec make_spec(int64_t ny, int64_t nx, double dy, double dx, double dz, do
   s.dz = dz;
   s.k0 = k0;
   s.indices.assign(reinterpret_cast<const c128*>(indices), reinterpret_cast<const c128*>(indices) + m);
-  s.labels.assign(labels, labels + nz * ny * nx);
+  if (labels)
+    s.labels.assign(labels, labels + nz * ny * nx);
+  else
+    s.uniform_label = uniform_label;
   s.index_before = c128(ib_re, ib_im);
   if (mask) s.mask.assign(mask, mask + ny * nx);
   if (fluct) {
@@ -164,6 +168,38 @@ int bwpm_solver_create_ex(int64_t ny, int64_t nx, double dy, double dx, double d
     h->engine = bwpm::make_engine(std::move(s));
     h->device = device;
     *out = h.release();
+  });
+}
+
+int bwpm_solver_create_synth(int64_t ny, int64_t nx, double dy, double dx, double dz, double k0,
+                             const double* indices, int64_t m, const int32_t* labels, int uniform_label, int64_t nz,
+                             double ib_re, double ib_im, const double* mask, const int32_t* labels_before,
+                             double sigma, double corr, uint64_t seed, int device, int precision, bwpm_solver** out) {
+  return guarded([&] {
+    bwpm::require(out != nullptr, "null output handle");
+    *out = nullptr;
+    bwpm::Spec s = make_spec(ny, nx, dy, dx, dz, k0, indices, m, labels, nz, ib_re, ib_im, mask, nullptr,
+                             labels_before, device, precision, labels ? -1 : uniform_label);
+    s.synth.on = true;
+    s.synth.sigma = sigma;
+    s.synth.corr = corr;
+    s.synth.seed = seed;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0)
+      throw bwpm::Error(bwpm::kEdevice, "no GPU device available");
+    bwpm::require(device >= 0 && device < count, "device index out of range");
+    auto h = std::make_unique<bwpm_solver>();
+    h->engine = bwpm::make_engine(std::move(s));
+    h->device = device;
+    *out = h.release();
+  });
+}
+
+int bwpm_solver_fluctuation(const bwpm_solver* s, double* out) {
+  return guarded([&] {
+    bwpm::require(s && out, "null argument");
+    const std::vector<c128> v = s->engine->fluctuation();
+    std::memcpy(out, v.data(), v.size() * sizeof(c128));
   });
 }
 
@@ -329,6 +365,24 @@ int bwpm_queue_submit(bwpm_queue* q, int64_t ny, int64_t nx, double dy, double d
     bwpm::require(q && field && ticket, "null argument");
     bwpm::Spec s = make_spec(ny, nx, dy, dx, dz, k0, indices, m, labels, nz, ib_re, ib_im, mask, fluct,
                              labels_before, 0, precision);
+    const c128* f = reinterpret_cast<const c128*>(field);
+    *ticket = q->q->submit(std::move(s), std::vector<c128>(f, f + 2 * ny * nx), iterations);
+  });
+}
+
+int bwpm_queue_submit_synth(bwpm_queue* q, int64_t ny, int64_t nx, double dy, double dx, double dz, double k0,
+                            const double* indices, int64_t m, const int32_t* labels, int uniform_label, int64_t nz,
+                            double ib_re, double ib_im, const double* mask, const int32_t* labels_before,
+                            double sigma, double corr, uint64_t seed, int precision, const double* field,
+                            int iterations, int64_t* ticket) {
+  return guarded([&] {
+    bwpm::require(q && field && ticket, "null argument");
+    bwpm::Spec s = make_spec(ny, nx, dy, dx, dz, k0, indices, m, labels, nz, ib_re, ib_im, mask, nullptr,
+                             labels_before, 0, precision, labels ? -1 : uniform_label);
+    s.synth.on = true;
+    s.synth.sigma = sigma;
+    s.synth.corr = corr;
+    s.synth.seed = seed;
     const c128* f = reinterpret_cast<const c128*>(field);
     *ticket = q->q->submit(std::move(s), std::vector<c128>(f, f + 2 * ny * nx), iterations);
   });

@@ -94,7 +94,7 @@ size_t estimate_device_bytes(const Spec& s, int iterations) {
   // sparse interface values (two sets) and lists, bounded by two planes
   bytes += 2 * plane + 2 * npix * 8;
   // Born: potential and emitted planes per slab, per-host tables, Krylov basis and work planes
-  if (!s.fluct.empty())
+  if (!s.fluct.empty() || (s.synth.on && s.synth.sigma > 0))
     bytes += static_cast<size_t>(s.nz) * (npix * elem + 2 * plane) + m * npix * 16 * elem + (31 + 12) * plane;
   return bytes;
 }

@@ -13,15 +13,25 @@ namespace bwpm {
 
 using c128 = std::complex<double>;
 
+// Gaussian-correlated real index fluctuation generated on the device: seeded normal noise
+// filtered by exp(−|q|²·corr²/4) (periodic in x, y and z) and scaled to standard deviation sigma.
+struct FluctSynth {
+  bool on = false;
+  double sigma = 0, corr = 0;  // corr in mm
+  unsigned long long seed = 0;
+};
+
 struct Spec {
   long long ny = 0, nx = 0, nz = 0;
   double dy = 0, dx = 0, dz = 0, k0 = 0;
   std::vector<c128> indices;
-  std::vector<int32_t> labels;  // (nz, ny, nx)
+  std::vector<int32_t> labels;  // (nz, ny, nx), or empty with uniform_label >= 0
+  int uniform_label = -1;       // >= 0: every pixel of every slab has this label
   c128 index_before{1.0, 0.0};
   std::vector<double> mask;     // (ny, nx) or empty
   std::vector<c128> fluct;      // (nz, ny, nx) or empty
   std::vector<int32_t> labels_before;  // labels of plane 0 (ny, nx); empty: index_before everywhere
+  FluctSynth synth;             // δn generated on the device (fluct must be empty)
   int precision = 32;           // 64: FP64; 32: FP32 compensated; 31: FP32 plain
   int device = 0;
 };
@@ -61,6 +71,9 @@ class EngineBase {
   virtual long long nx() const = 0;
   virtual int device() const = 0;
   virtual bool has_born() const = 0;
+  virtual long long nz() const = 0;
+  // δn of every slab (nz, ny, nx) as held on the device (zeros without δn).
+  virtual std::vector<c128> fluctuation() const = 0;
   // Run the step programs of `members` (all of this engine's type, grid and device; this engine
   // among them or not) in lockstep on `stream`: one column and one row launch per step serve every
   // member. Fields must have been uploaded. Returns the device time in ms.

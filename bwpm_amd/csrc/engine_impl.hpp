@@ -434,6 +434,13 @@ class Engine final : public EngineBase {
   long long nx() const override { return sp_.nx; }
   int device() const override { return sp_.device; }
   bool has_born() const override { return born_ != nullptr; }
+  long long nz() const override { return sp_.nz; }
+  std::vector<c128> fluctuation() const override {
+    HIP_CHECK(hipSetDevice(sp_.device));
+    HIP_CHECK(hipStreamSynchronize(stream_));
+    if (!born_) return std::vector<c128>(static_cast<size_t>(sp_.nz * npix_), c128(0.0, 0.0));
+    return born_->fluctuation();
+  }
 
   void run(const c128* field, int iterations, const std::vector<long long>& keep, RunOutput& out) override {
     HIP_CHECK(hipSetDevice(sp_.device));
@@ -981,14 +988,21 @@ class Engine final : public EngineBase {
 
   int medium_of_label(int32_t l) const { return l < 0 ? before_id_ : label_medium_[l]; }
   const int32_t* plane_labels(long long p) const {
-    return p == 0 ? before_.data() : sp_.labels.data() + (p - 1) * npix_;
+    if (p == 0) return before_.data();
+    return sp_.uniform_label >= 0 ? uniform_plane_.data() : sp_.labels.data() + (p - 1) * npix_;
   }
 
   void validate() {
     const Spec& s = sp_;
     require(s.ny >= 1 && s.nx >= 1, "grid must have ny, nx >= 1");
     require(s.nz >= 1, "labels must have shape (nz, ny, nx) with nz >= 1");
-    require(static_cast<long long>(s.labels.size()) == s.nz * s.ny * s.nx, "labels must have shape (nz, ny, nx)");
+    if (s.uniform_label >= 0)
+      require(s.labels.empty() && static_cast<size_t>(s.uniform_label) < s.indices.size(),
+              "a uniform label must index into `indices` (and no label array be given)");
+    else
+      require(static_cast<long long>(s.labels.size()) == s.nz * s.ny * s.nx, "labels must have shape (nz, ny, nx)");
+    require(!s.synth.on || (s.fluct.empty() && s.synth.sigma >= 0 && s.synth.corr >= 0),
+            "a synthesised fluctuation needs sigma >= 0, corr >= 0 and no fluctuation array");
     require(s.mask.empty() || static_cast<long long>(s.mask.size()) == s.ny * s.nx, "mask must have shape (ny, nx)");
     int32_t lo = 0, hi = 0;
     if (!s.labels.empty()) {
@@ -1026,16 +1040,28 @@ class Engine final : public EngineBase {
       before_.assign(npix_, -1);
     else
       before_ = s.labels_before;
+    if (s.uniform_label >= 0) uniform_plane_.assign(npix_, s.uniform_label);
 
     // Born hosts per slab (validation as the reference constructor, solver.cpp:47-62)
     has_host_.assign(s.nz, 0);
     host_.assign(s.nz, c128(0.0, 0.0));
-    if (!s.fluct.empty()) {
+    if (s.synth.on && s.synth.sigma > 0) {
+      // a generated δn fills every slab: each slab must lie in one host index
+      for (long long j = 0; j < s.nz; ++j) {
+        const int32_t* lab = plane_labels(j + 1);
+        const c128 idx = s.indices[lab[0]];
+        for (long long q = 1; q < npix_; ++q)
+          if (s.indices[lab[q]] != idx)
+            throw Error(kEinval, "slab " + std::to_string(j) + ": fluctuation must lie in one host index");
+        has_host_[j] = 1;
+        host_[j] = idx;
+      }
+    } else if (!s.fluct.empty()) {
       for (long long j = 0; j < s.nz; ++j)
         for (long long q = 0; q < npix_; ++q) {
           const c128 d = s.fluct[j * npix_ + q];
           if (d == c128(0.0, 0.0)) continue;
-          const c128 idx = s.indices[s.labels[j * npix_ + q]];
+          const c128 idx = s.indices[plane_labels(j + 1)[q]];
           if (has_host_[j] && host_[j] != idx)
             throw Error(kEinval, "slab " + std::to_string(j) + ": fluctuation must lie in one host index");
           has_host_[j] = 1;
@@ -1326,6 +1352,7 @@ class Engine final : public EngineBase {
   std::vector<int> label_medium_;
   int before_id_ = 0;
   std::vector<int32_t> before_;
+  std::vector<int32_t> uniform_plane_;  // the label plane of a uniform volume
   std::vector<char> has_host_;
   std::vector<c128> host_;
   DevBuf kx_, ky_;

@@ -18,6 +18,8 @@ from typing import Any
 
 import numpy as np
 
+from bwpm_amd.specs import FluctuationSynthesis, Uniform
+
 
 @dataclass
 class Problem:
@@ -45,7 +47,7 @@ class Problem:
 
     @property
     def nz(self) -> int:
-        return int(self.labels.shape[0])
+        return int(self.labels.nz) if isinstance(self.labels, Uniform) else int(self.labels.shape[0])
 
     @property
     def updates(self) -> int:
@@ -128,3 +130,48 @@ def born_3d(n: int = 512, nz: int = 1024, iterations: int = 6, sigma: float = 0.
     f[..., 0] = np.exp(1j * m * dk * x)[None, :]
     return Problem(f"born_3d_{n}x{n}x{nz}", n, n, d, d, d, lam, [complex(host)], labels, complex(host), f,
                    iterations, fluctuation=delta, meta={"m": m, "sigma": sigma})
+
+
+def born_3d_synth(n: int = 512, nz: int = 1024, iterations: int = 6, sigma: float = 0.01, corr_um: float = 1.0,
+                  seed: int = 3, wavelength_um: float = 0.8, step_um: float = 0.1, angle_deg: float = 5.0,
+                  host: float = 1.5) -> Problem:
+    """Config 3 geometry with the δn generated on the device (no (nz, ny, nx) arrays on the host):
+    a uniform host volume and a FluctuationSynthesis of the same σ and correlation length."""
+    lam = wavelength_um
+    d = step_um * 1e-3
+    k0 = 2 * np.pi / (lam * 1e-3)
+    dk = 2 * np.pi / (n * d)
+    m = int(round(host * k0 * np.sin(np.deg2rad(angle_deg)) / dk))
+    x = centred(n, d)
+    f = np.zeros((n, n, 2), dtype=np.complex128)
+    f[..., 0] = np.exp(1j * m * dk * x)[None, :]
+    return Problem(f"born_3d_synth_{n}x{n}x{nz}", n, n, d, d, d, lam, [complex(host)], Uniform(nz, 0),
+                   complex(host), f, iterations, fluctuation=FluctuationSynthesis(sigma, corr_um * 1e-3, seed),
+                   meta={"m": m, "sigma": sigma})
+
+
+def uneven_batch(count: int = 64, seed0: int = 1000, sizes: tuple[int, ...] = (128, 256, 384, 512),
+                 nz_range: tuple[int, int] = (100, 1000), iterations: int = 1) -> list[Problem]:
+    """Config 4: `count` independent Born solves of uneven sizes, one seed per solve (1000 + i):
+    n from `sizes`, nz uniform in `nz_range`, λ uniform in 0.4 … 1.6 µm, incidence 0 … 30° on the
+    nearest grid frequency, σ uniform in 0.005 … 0.02 with a 1 µm correlation length in a host of
+    n = 1.5, dx = dy = λ/6, dz = λ/4; δn generated on the device."""
+    out = []
+    for i in range(count):
+        rng = np.random.default_rng(seed0 + i)
+        n = int(rng.choice(sizes))
+        nz = int(rng.integers(nz_range[0], nz_range[1] + 1))
+        lam = float(rng.uniform(0.4, 1.6))
+        theta = float(rng.uniform(0.0, 30.0))
+        sigma = float(rng.uniform(0.005, 0.02))
+        dx, dz = lam * 1e-3 / 6, lam * 1e-3 / 4
+        k0 = 2 * np.pi / (lam * 1e-3)
+        dk = 2 * np.pi / (n * dx)
+        m = int(round(1.5 * k0 * np.sin(np.deg2rad(theta)) / dk))
+        x = centred(n, dx)
+        f = np.zeros((n, n, 2), dtype=np.complex128)
+        f[..., 0] = np.exp(1j * m * dk * x)[None, :]
+        out.append(Problem(f"uneven_{i}_{n}x{n}x{nz}", n, n, dx, dx, dz, lam, [1.5 + 0j], Uniform(nz, 0), 1.5 + 0j,
+                           f, iterations, fluctuation=FluctuationSynthesis(sigma, 1e-3, seed0 + i),
+                           meta={"m": m, "sigma": sigma, "theta_deg": theta}))
+    return out

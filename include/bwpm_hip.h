@@ -92,6 +92,19 @@ int bwpm_solver_create_ex(int64_t ny, int64_t nx, double dy, double dx, double d
                           const double* fluct, const int32_t* labels_before, int device,
                           int precision, bwpm_solver** out);
 
+/* A solver whose index fluctuation is generated on the device instead of copied from the host
+ * (no counterpart in the reference, which takes a (nz, ny, nx) complex array, bindings.cpp:115):
+ * real δn, Gaussian-correlated as the reference's test volumes (tests/test_fluctuation_3d.py:34-41):
+ * seeded normal noise filtered by exp(−|q|²·corr²/4) (corr in mm), periodic in x, y and z,
+ * scaled to standard deviation sigma. labels may be NULL for a volume of one label
+ * (uniform_label) in every pixel; every slab must then lie in one host index. Other arguments as
+ * bwpm_solver_create_ex. */
+int bwpm_solver_create_synth(int64_t ny, int64_t nx, double dy, double dx, double dz, double k0,
+                             const double* indices, int64_t m, const int32_t* labels, int uniform_label, int64_t nz,
+                             double ib_re, double ib_im, const double* mask, const int32_t* labels_before,
+                             double sigma, double corr, uint64_t seed, int device, int precision, bwpm_solver** out);
+/* The δn the solver holds, (nz, ny, nx) complex128 (zeros without δn). */
+int bwpm_solver_fluctuation(const bwpm_solver* s, double* out);
 int bwpm_solver_destroy(bwpm_solver* s);
 
 /* Bidirectional solve (iterations = 0: forward only) of field (ny, nx, 2), keeping the planes in
@@ -165,6 +178,12 @@ int bwpm_queue_submit(bwpm_queue* q, int64_t ny, int64_t nx, double dy, double d
                       const double* indices, int64_t m, const int32_t* labels, int64_t nz, double ib_re,
                       double ib_im, const double* mask, const double* fluct, const int32_t* labels_before,
                       int precision, const double* field, int iterations, int64_t* ticket);
+/* As bwpm_queue_submit with δn generated on the device (arguments as bwpm_solver_create_synth). */
+int bwpm_queue_submit_synth(bwpm_queue* q, int64_t ny, int64_t nx, double dy, double dx, double dz, double k0,
+                            const double* indices, int64_t m, const int32_t* labels, int uniform_label, int64_t nz,
+                            double ib_re, double ib_im, const double* mask, const int32_t* labels_before,
+                            double sigma, double corr, uint64_t seed, int precision, const double* field,
+                            int iterations, int64_t* ticket);
 int bwpm_queue_state(bwpm_queue* q, int64_t ticket, int* state);
 /* Waits until the ticket is done or failed, or timeout_s passes (timeout_s < 0: no limit). */
 int bwpm_queue_wait(bwpm_queue* q, int64_t ticket, double timeout_s, int* state);
