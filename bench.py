@@ -104,11 +104,30 @@ def run_solve(p, precision: str) -> tuple[dict, float]:
         s.close()
 
 
-def configs(c3_nz: int) -> dict:
-    """The reference's other configurations, each as one solve on the device: config 1 (2D, 1024
-    points, 200 slabs, 3 iterations), config 2 (3D 256² × 512 tilted interface, 4 iterations, E_x
-    and E_y) and config 3 (3D 512² Born volume, 6 iterations, at `c3_nz` of its 1024 slabs). FP32
-    errors are relative L2 against the FP64 device result (itself within 1e-10 of the oracle)."""
+def uneven_batch(count: int) -> dict:
+    """Config 4: `count` uneven Born solves (128² … 512², 100 … 1000 slabs, per-solve λ, angle, σ and
+    seed; δn generated on the device) submitted at once to the request queue, complex64."""
+    from bwpm_amd import serve, workloads
+
+    probs = workloads.uneven_batch(count)
+    with serve.Server(slots=16, chunk=64, precision="fp32") as srv:
+        t0 = time.perf_counter()
+        tickets = [srv.submit(p) for p in probs]
+        for t in tickets:
+            srv.result(t)
+        span = time.perf_counter() - t0
+        st = srv.stats()
+    return {"solves": count, "seconds": round(span, 2), "updates_per_s": sum(p.updates for p in probs) / span,
+            "p50_ms": round(st["p50_ms"], 1), "p99_ms": round(st["p99_ms"], 1), "failed": int(st["failed"]),
+            "peak_hbm_gb": round(st["peak_bytes"] / 1e9, 2)}
+
+
+def configs(c3_nz: int, c4_count: int) -> dict:
+    """The reference's other configurations: config 1 (2D, 1024 points, 200 slabs, 3 iterations),
+    config 2 (3D 256² × 512 tilted interface, 4 iterations, E_x and E_y) and config 3 (3D 512² Born
+    volume, `c3_nz` slabs, 6 iterations) as single solves; config 4 (uneven Born batch) on the
+    queue. FP32 errors are relative L2 against the FP64 device result (itself within 1e-10 of the
+    oracle)."""
     from bwpm_amd import workloads
 
     out: dict = {}
@@ -124,7 +143,9 @@ def configs(c3_nz: int) -> dict:
                          "fp32_rel_l2_transmitted": rel_l2(a32["transmitted"], a64["transmitted"]),
                          "fp32_rel_l2_reflected": rel_l2(a32["reflected"], a64["reflected"])}
     out["c2_tilted_256x256x512_it4"] = c2
-    p3 = workloads.born_3d(n=512, nz=c3_nz, iterations=6)
+    # config 3 at full depth: the δn volume is generated on the device (a (1024, 512, 512) complex
+    # array would be 4 GiB on the host)
+    p3 = workloads.born_3d_synth(n=512, nz=c3_nz, iterations=6)
     b64, q64 = run_solve(p3, "fp64")
     b32, q32 = run_solve(p3, "fp32")
     out[f"c3_born_512x512x{c3_nz}_it6"] = {
@@ -132,7 +153,9 @@ def configs(c3_nz: int) -> dict:
         "fp64_gmres_iterations": b64["gmres_iterations"],
         "fp32_rel_l2_transmitted": rel_l2(b32["transmitted"], b64["transmitted"]),
         "fp32_rel_l2_reflected": rel_l2(b32["reflected"], b64["reflected"]),
-        "note": f"{c3_nz} of the configuration's 1024 slabs (the δn volume and run time scale with nz)"}
+        "input": "Gaussian-correlated δn (σ = 0.01, correlation 1 µm, seed 3) generated on the device"}
+    if c4_count > 0:
+        out[f"c4_uneven_born_{c4_count}"] = uneven_batch(c4_count)
     return out
 
 
@@ -188,7 +211,8 @@ def main() -> int:
     ap.add_argument("--no-checks", action="store_true", help="skip the plain-path and accuracy runs")
     ap.add_argument("--serve-requests", type=int, default=24,
                     help="requests of the served mix (0: no serving run)")
-    ap.add_argument("--c3-nz", type=int, default=128, help="slabs of the Born configuration (0: skip configs)")
+    ap.add_argument("--c3-nz", type=int, default=1024, help="slabs of the Born configuration (0: skip configs)")
+    ap.add_argument("--c4-count", type=int, default=64, help="solves of the uneven Born batch (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU figures are not produced",
@@ -279,7 +303,7 @@ def main() -> int:
     for s in solvers:
         s.close()
     solvers = []
-    other = configs(args.c3_nz) if args.c3_nz > 0 else None
+    other = configs(args.c3_nz, args.c4_count) if args.c3_nz > 0 else None
     serving = None
     if args.serve_requests > 0:
         serving = served(args.serve_requests, args.precision)

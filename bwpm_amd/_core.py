@@ -95,6 +95,14 @@ _q_state = _sig("bwpm_queue_state", _p, _i64, ctypes.POINTER(_int))
 _q_wait = _sig("bwpm_queue_wait", _p, _i64, _d, ctypes.POINTER(_int))
 _q_result = _sig("bwpm_queue_result", _p, _i64, ctypes.POINTER(_p))
 _q_stats = _sig("bwpm_queue_stats", _p, _p)
+_a_create = _sig("bwpm_analysis_create", _int, _i64, _i64, _d, _d, _d, ctypes.POINTER(_p))
+_a_destroy = _sig("bwpm_analysis_destroy", _p)
+_a_power = _sig("bwpm_plane_power", _p, _p, _int, _i64, _d, _p)
+_a_refl = _sig("bwpm_reference_reflectance", _p, _p, _int, _d, _d, ctypes.POINTER(_d))
+_a_axial = _sig("bwpm_axial_profile", _p, _p, _int, _d, _p, _i64, _d, _p)
+_a_strehl = _sig("bwpm_strehl_profile", _p, _p, _int, _d, _d, _p, _i64, _p)
+_a_strehl_modes = _sig("bwpm_strehl_modes", _p, _p, _p, _i64, _d, _p, _i64, _p)
+_solver_plane = _sig("bwpm_solver_plane", _p, _int, _i64, ctypes.POINTER(_p), ctypes.POINTER(_int))
 _batch = _sig("bwpm_batch_run", _p, _i64, _int, _int, _int, ctypes.POINTER(_d), ctypes.POINTER(_d),
                ctypes.POINTER(_d), ctypes.POINTER(_i64), ctypes.POINTER(_i64))
 
@@ -354,6 +362,102 @@ class GpuSolver:
         b = _i64(0)
         _check(_dev_bytes(self._h, ctypes.byref(b)))
         return int(b.value)
+
+    def device_plane(self, which: str, plane: int = 0) -> "DevicePlane":
+        """A plane of the last run left on the device: "transmitted", "reflected", or the
+        "forward" / "backward" field at a kept plane. Valid until the next run or close; the
+        analysis functions of `bwpm_amd.pilot_imc3d` read it without a copy to the host."""
+        ptr, prec = _p(None), _int(0)
+        _check(_solver_plane(self._h, PLANE_KINDS[which], int(plane), ctypes.byref(ptr), ctypes.byref(prec)))
+        return DevicePlane(int(ptr.value), prec.value == 64, self.ny, self.nx, self)
+
+
+PLANE_KINDS = {"transmitted": 0, "reflected": 1, "forward": 2, "backward": 3}
+PLANE_HOST_C128, PLANE_DEVICE_C128, PLANE_DEVICE_C64 = 0, 1, 2
+
+
+class DevicePlane:
+    """Address of a (ny, nx, 2) plane in device memory (complex128 if `fp64`, else complex64),
+    with a reference to its owner so that the memory outlives the object."""
+
+    def __init__(self, ptr: int, fp64: bool, ny: int, nx: int, owner: Any = None) -> None:
+        self.ptr, self.fp64, self.ny, self.nx, self._owner = ptr, fp64, ny, nx, owner
+
+    @property
+    def where(self) -> int:
+        return PLANE_DEVICE_C128 if self.fp64 else PLANE_DEVICE_C64
+
+
+class GpuAnalysis:
+    """Analysis reductions of one grid on one device (bwpm_analysis_*): flux, Fresnel reference
+    reflectance, axial intensity scan and Strehl profiles, FP64 with deterministic sums."""
+
+    def __init__(self, ny: int, nx: int, dy: float, dx: float, k0: float, device: int = 0) -> None:
+        self.ny, self.nx = int(ny), int(nx)
+        self._h = _p(None)
+        _check(_a_create(int(device), self.ny, self.nx, float(dy), float(dx), float(k0), ctypes.byref(self._h)))
+
+    def close(self) -> None:
+        if self._h and self._h.value:
+            _a_destroy(self._h)
+            self._h = _p(None)
+
+    def __del__(self) -> None:
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _plane(self, field: Any, count: int | None = None) -> tuple[Any, int, Any]:
+        if isinstance(field, DevicePlane):
+            if (field.ny, field.nx) != (self.ny, self.nx):
+                raise ValueError("the plane's grid differs from the analysis grid")
+            return _p(field.ptr), field.where, field
+        a = np.ascontiguousarray(np.asarray(field, dtype=np.complex128))
+        if a.shape[-3:] != (self.ny, self.nx, 2):
+            raise ValueError(f"field must have shape (..., {self.ny}, {self.nx}, 2)")
+        return _ptr(a), PLANE_HOST_C128, a
+
+    def plane_power(self, field: Any, index: float) -> np.ndarray:
+        """Flux of each plane; field (ny, nx, 2), (count, ny, nx, 2) or a DevicePlane."""
+        ptr, where, keep = self._plane(field)
+        count = 1 if where != PLANE_HOST_C128 or keep.ndim == 3 else int(np.prod(keep.shape[:-3]))
+        out = np.zeros(count, dtype=np.float64)
+        _check(_a_power(self._h, ptr, where, count, float(index), _ptr(out)))
+        return out
+
+    def reference_reflectance(self, field: Any, n1: float, n2: float) -> float:
+        ptr, where, keep = self._plane(field)
+        r = _d(0)
+        _check(_a_refl(self._h, ptr, where, float(n1), float(n2), ctypes.byref(r)))
+        return float(r.value)
+
+    def axial_profile(self, field: Any, index: float, offsets: Any, na: float | None = None) -> np.ndarray:
+        ptr, where, keep = self._plane(field)
+        off = np.ascontiguousarray(np.asarray(offsets, dtype=np.float64).ravel())
+        out = np.zeros(off.size, dtype=np.float64)
+        _check(_a_axial(self._h, ptr, where, float(index), _ptr(off), off.size, -1.0 if na is None else float(na),
+                        _ptr(out)))
+        return out
+
+    def strehl_profile(self, spectrum: Any, k: float, na: float, offsets: Any) -> np.ndarray:
+        a = np.ascontiguousarray(np.asarray(spectrum, dtype=np.complex128))
+        if a.shape != (self.ny, self.nx):
+            raise ValueError(f"spectrum must have shape ({self.ny}, {self.nx})")
+        off = np.ascontiguousarray(np.asarray(offsets, dtype=np.float64).ravel())
+        out = np.zeros(off.size, dtype=np.float64)
+        _check(_a_strehl(self._h, _ptr(a), PLANE_HOST_C128, float(k), float(na), _ptr(off), off.size, _ptr(out)))
+        return out
+
+    def strehl_modes(self, ky: Any, spectrum: Any, k: float, offsets: Any) -> np.ndarray:
+        kyv = np.ascontiguousarray(np.asarray(ky, dtype=np.float64).ravel())
+        a = np.ascontiguousarray(np.asarray(spectrum, dtype=np.complex128).ravel())
+        if a.size != kyv.size:
+            raise ValueError("ky and spectrum must have the same length")
+        off = np.ascontiguousarray(np.asarray(offsets, dtype=np.float64).ravel())
+        out = np.zeros(off.size, dtype=np.float64)
+        _check(_a_strehl_modes(self._h, _ptr(kyv), _ptr(a), a.size, float(k), _ptr(off), off.size, _ptr(out)))
+        return out
 
 
 CudaSolver = GpuSolver  # the reference's name for its GPU solver class

@@ -432,7 +432,7 @@ class Born {
     ptrs_.alloc(2 * (kRestart + 2) * sizeof(void*));
     HIP_CHECK(hipHostMalloc(&hres_, (2 * (kRestart + 2) + 1) * sizeof(double2), hipHostMallocDefault));
     if (s.synth.on) synthesise(s, host);
-    choose_fft_geometry();
+    fft_.init(ny_, nx_, px_, py_, stream_);
     HIP_CHECK(hipStreamSynchronize(stream_));
     bytes_ = V_.bytes() + tabs_.bytes() + 10 * pb + basis_.bytes() + planes_.bytes();
   }
@@ -508,131 +508,11 @@ class Born {
     HIP_CHECK(hipGetLastError());
   }
 
-  // ---- 2D FFTs of up to kMaxPlanes planes -----------------------------------------------------------
-  void choose_fft_geometry() {
-    auto ls_of = [](int lines) { return lines <= 2 ? lines : (lines | 1); };
-    const int elems = Budget<T>::elems;
-    rows_fast_ = fast_n(nx_) && COMP == (sizeof(T) == 4);
-    cols_fast_ = fast_n(ny_) && COMP == (sizeof(T) == 4);
-    int rw = static_cast<int>(std::min<long long>(ny_, 64));
-    while (rw > 1 && nx_ * ls_of(2 * rw) > elems) rw /= 2;
-    rows_ = rw;
-    ls_row_ = ls_of(2 * rw);
-    int w = 16;
-    while (w > 1 && ny_ * ls_of(2 * w) > elems) w /= 2;
-    while (w > 1 && w > nx_) w /= 2;
-    cols_ = w;
-    ls_col_ = ls_of(2 * w);
-    require(nx_ * ls_row_ <= elems && ny_ * ls_col_ <= elems, "grid too large for the Born transforms");
-    HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fft2_rows<T, COMP, false>),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-    HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fft2_rows<T, COMP, true>),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-    HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fft2_cols<T, COMP, false>),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-    HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fft2_cols<T, COMP, true>),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-  }
-  static bool fast_n(long long n) {
-    return n == 64 || n == 128 || n == 256 || n == 512 || n == 1024 || n == 2048;
-  }
-
-  template <bool INV>
-  void rows_pass(const PlaneSet<T>& ps, int np) {
-    if (rows_fast_) {
-      switch (nx_) {
-        case 64: rows_fast_n<64, INV>(ps, np); return;
-        case 128: rows_fast_n<128, INV>(ps, np); return;
-        case 256: rows_fast_n<256, INV>(ps, np); return;
-        case 512: rows_fast_n<512, INV>(ps, np); return;
-        case 1024: rows_fast_n<1024, INV>(ps, np); return;
-        default: rows_fast_n<2048, INV>(ps, np); return;
-      }
-    }
-    const size_t lds = static_cast<size_t>(nx_) * ls_row_ * sizeof(cx<T>);
-    hipLaunchKernelGGL((k_fft2_rows<T, COMP, INV>), dim3(static_cast<unsigned>((ny_ + rows_ - 1) / rows_), np),
-                       dim3(kThreads), lds, stream_, ps, static_cast<int>(ny_), static_cast<int>(nx_), rows_, ls_row_, px_);
-    HIP_CHECK(hipGetLastError());
-  }
-  template <int N, bool INV>
-  void rows_fast_n(const PlaneSet<T>& ps, int np) {
-    constexpr int RW = (sizeof(T) == 4 ? 512 : 256) / (N / 8) > 0 ? (sizeof(T) == 4 ? 512 : 256) / (N / 8) : 1;
-    const auto* th = static_cast<const cx<T>*>(px_.tw_hi);
-    const auto* tl = static_cast<const cx<T>*>(px_.tw_lo);
-    if (ny_ % RW == 0)
-      hipLaunchKernelGGL((k_fft2_rows_fast<N, RW, T, COMP, INV>), dim3(static_cast<unsigned>(ny_ / RW), np),
-                         dim3(N / 8 * RW), 0, stream_, ps, th, tl);
-    else
-      hipLaunchKernelGGL((k_fft2_rows_fast<N, 1, T, COMP, INV>), dim3(static_cast<unsigned>(ny_), np), dim3(N / 8), 0,
-                         stream_, ps, th, tl);
-    HIP_CHECK(hipGetLastError());
-  }
-  template <bool INV>
-  void cols_pass(const PlaneSet<T>& ps, int np) {
-    if (cols_fast_) {
-      switch (ny_) {
-        case 64: cols_fast_n<64, INV>(ps, np); return;
-        case 128: cols_fast_n<128, INV>(ps, np); return;
-        case 256: cols_fast_n<256, INV>(ps, np); return;
-        case 512: cols_fast_n<512, INV>(ps, np); return;
-        case 1024: cols_fast_n<1024, INV>(ps, np); return;
-        default: cols_fast_n<2048, INV>(ps, np); return;
-      }
-    }
-    const size_t lds = static_cast<size_t>(ny_) * ls_col_ * sizeof(cx<T>);
-    hipLaunchKernelGGL((k_fft2_cols<T, COMP, INV>), dim3(static_cast<unsigned>((nx_ + cols_ - 1) / cols_), np),
-                       dim3(kThreads), lds, stream_, ps, static_cast<int>(ny_), static_cast<int>(nx_), cols_, ls_col_, py_);
-    HIP_CHECK(hipGetLastError());
-  }
-  template <int N, bool INV>
-  void cols_fast_n(const PlaneSet<T>& ps, int np) {
-    constexpr int W0 = (sizeof(T) == 4 ? 512 : 256) / (N / 8);
-    constexpr int W = W0 > 16 ? 16 : (W0 < 1 ? 1 : W0);
-    const auto* th = static_cast<const cx<T>*>(py_.tw_hi);
-    const auto* tl = static_cast<const cx<T>*>(py_.tw_lo);
-    if (nx_ % W == 0)
-      hipLaunchKernelGGL((k_fft2_cols_fast<N, W, T, COMP, INV>), dim3(static_cast<unsigned>(nx_ / W), np),
-                         dim3(N / 8 * W), 0, stream_, ps, static_cast<int>(nx_), th, tl);
-    else
-      hipLaunchKernelGGL((k_fft2_cols_fast<N, 1, T, COMP, INV>), dim3(static_cast<unsigned>(nx_), np), dim3(N / 8), 0,
-                         stream_, ps, static_cast<int>(nx_), th, tl);
-    HIP_CHECK(hipGetLastError());
-  }
-
-  // In-place 2D transform of np planes; inverse scaled by 1/N and optionally multiplied by `mul`.
+  // ---- 2D FFTs of up to kMaxPlanes planes (Fft2D) ---------------------------------------------------
   void fft2_planes(px2<T>* const* planes, int np, bool inverse, const cx<T>* mul) {
-    PlaneSet<T> a{}, b{};
-    for (int i = 0; i < np; ++i) {
-      a.in[i] = b.in[i] = planes[i];
-      a.out[i] = b.out[i] = planes[i];
-      b.mul[i] = mul;
-    }
-    a.scale = 1;
-    b.scale = inverse ? static_cast<T>(1.0 / static_cast<double>(npix_)) : static_cast<T>(1);
-    if (inverse) {
-      rows_pass<true>(a, np);
-      cols_pass<true>(b, np);
-    } else {
-      rows_pass<false>(a, np);
-      cols_pass<false>(b, np);
-    }
+    fft_.planes(planes, np, inverse, mul);
   }
-  void fft2(const px2<T>* in, px2<T>* out, bool inverse) {
-    PlaneSet<T> a{}, b{};
-    a.in[0] = in;
-    a.out[0] = out;
-    b.in[0] = out;
-    b.out[0] = out;
-    a.scale = 1;
-    b.scale = inverse ? static_cast<T>(1.0 / static_cast<double>(npix_)) : static_cast<T>(1);
-    if (inverse) {
-      rows_pass<true>(a, 1);
-      cols_pass<true>(b, 1);
-    } else {
-      rows_pass<false>(a, 1);
-      cols_pass<false>(b, 1);
-    }
-  }
+  void fft2(const px2<T>* in, px2<T>* out, bool inverse) { fft_.one(in, out, inverse); }
 
   // ---- the operators --------------------------------------------------------------------------------
 
@@ -914,8 +794,7 @@ class Born {
   double2* hres_ = nullptr;
   const void* host_ptrs_[2 * (kRestart + 2)] = {};
   double2 host_coef_[kRestart + 2] = {};
-  bool rows_fast_ = false, cols_fast_ = false;
-  int rows_ = 1, ls_row_ = 2, cols_ = 1, ls_col_ = 2;
+  Fft2D<T, COMP> fft_;
   long long iterations_ = 0;
   size_t bytes_ = 0;
 };

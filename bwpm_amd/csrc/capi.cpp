@@ -3,6 +3,7 @@
 #include <string>
 
 #include "../../include/bwpm_hip.h"
+#include "analysis.hpp"
 #include "engine.hpp"
 #include "serve.hpp"
 
@@ -20,6 +21,10 @@ struct bwpm_result {
 
 struct bwpm_queue {
   std::unique_ptr<bwpm::Queue> q;
+};
+
+struct bwpm_analysis {
+  std::unique_ptr<bwpm::Analysis> a;
 };
 
 namespace {
@@ -429,6 +434,70 @@ int bwpm_queue_stats(bwpm_queue* q, double* out) {
                                          st.budget_bytes,
                                          st.peak_bytes};
     std::memcpy(out, v, sizeof(v));
+  });
+}
+
+int bwpm_analysis_create(int device, int64_t ny, int64_t nx, double dy, double dx, double k0, bwpm_analysis** out) {
+  return guarded([&] {
+    bwpm::require(out != nullptr, "null argument");
+    *out = nullptr;
+    auto h = std::make_unique<bwpm_analysis>();
+    h->a = std::make_unique<bwpm::Analysis>(device, ny, nx, dy, dx, k0);
+    *out = h.release();
+  });
+}
+
+int bwpm_analysis_destroy(bwpm_analysis* a) {
+  return guarded([&] { delete a; });
+}
+
+int bwpm_plane_power(bwpm_analysis* a, const void* planes, int where, int64_t count, double index, double* power) {
+  return guarded([&] {
+    bwpm::require(a && power, "null argument");
+    const std::vector<double> v = a->a->plane_power(planes, where, count, index);
+    std::memcpy(power, v.data(), v.size() * sizeof(double));
+  });
+}
+
+int bwpm_reference_reflectance(bwpm_analysis* a, const void* plane, int where, double n1, double n2,
+                               double* reflectance) {
+  return guarded([&] {
+    bwpm::require(a && reflectance, "null argument");
+    *reflectance = a->a->reference_reflectance(plane, where, n1, n2);
+  });
+}
+
+int bwpm_axial_profile(bwpm_analysis* a, const void* plane, int where, double index, const double* offsets,
+                       int64_t noff, double na, double* intensity) {
+  return guarded([&] {
+    bwpm::require(a != nullptr, "null argument");
+    a->a->axial_profile(plane, where, index, offsets, noff, na, intensity);
+  });
+}
+
+int bwpm_strehl_profile(bwpm_analysis* a, const void* spectrum, int where, double k, double na,
+                        const double* offsets, int64_t noff, double* values) {
+  return guarded([&] {
+    bwpm::require(a != nullptr, "null argument");
+    a->a->strehl_profile(spectrum, where, k, na, offsets, noff, values);
+  });
+}
+
+int bwpm_strehl_modes(bwpm_analysis* a, const double* ky, const double* spectrum, int64_t m, double k,
+                      const double* offsets, int64_t noff, double* values) {
+  return guarded([&] {
+    bwpm::require(a != nullptr, "null argument");
+    a->a->strehl_modes(ky, spectrum, m, k, offsets, noff, values);
+  });
+}
+
+int bwpm_solver_plane(const bwpm_solver* s, int which, int64_t plane, const void** ptr, int* precision) {
+  return guarded([&] {
+    bwpm::require(s && ptr && precision, "null argument");
+    HIP_CHECK(hipSetDevice(s->engine->device()));
+    HIP_CHECK(hipStreamSynchronize(s->engine->stream()));
+    *ptr = s->engine->device_plane(which, plane);
+    *precision = s->engine->precision() == BWPM_FP64 ? BWPM_FP64 : BWPM_FP32;
   });
 }
 

@@ -74,6 +74,9 @@ class EngineBase {
   virtual long long nz() const = 0;
   // δn of every slab (nz, ny, nx) as held on the device (zeros without δn).
   virtual std::vector<c128> fluctuation() const = 0;
+  // Device plane (ny, nx) of px2<T> from the last run: 0 transmitted, 1 reflected (final
+  // iteration), 2 forward / 3 backward field at kept plane p. Valid until the next run.
+  virtual const void* device_plane(int which, long long p) const = 0;
   // Run the step programs of `members` (all of this engine's type, grid and device; this engine
   // among them or not) in lockstep on `stream`: one column and one row launch per step serve every
   // member. Fields must have been uploaded. Returns the device time in ms.

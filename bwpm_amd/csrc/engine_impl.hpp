@@ -13,6 +13,7 @@
 #include <unordered_map>
 
 #include "born.hpp"
+#include "device_mem.hpp"
 #include "engine.hpp"
 #include "kernels.hpp"
 #include "kernels_fast.hpp"
@@ -20,123 +21,6 @@
 namespace bwpm {
 
 namespace detail {
-
-// ---- device memory -------------------------------------------------------------------------
-
-class DevBuf {
- public:
-  DevBuf() = default;
-  explicit DevBuf(size_t bytes) { alloc(bytes); }
-  DevBuf(const DevBuf&) = delete;
-  DevBuf& operator=(const DevBuf&) = delete;
-  DevBuf(DevBuf&& o) noexcept : p_(o.p_), n_(o.n_) { o.p_ = nullptr; o.n_ = 0; }
-  DevBuf& operator=(DevBuf&& o) noexcept {
-    release();
-    p_ = o.p_; n_ = o.n_; o.p_ = nullptr; o.n_ = 0;
-    return *this;
-  }
-  ~DevBuf() { release(); }
-  void alloc(size_t bytes) {
-    release();
-    if (bytes == 0) return;
-    const hipError_t e = hipMalloc(&p_, bytes);
-    if (e != hipSuccess) {
-      p_ = nullptr;
-      throw Error(kEnomem, "device allocation of " + std::to_string(bytes) + " bytes failed: " +
-                               hipGetErrorString(e));
-    }
-    n_ = bytes;
-  }
-  void release() {
-    if (p_) (void)hipFree(p_);
-    p_ = nullptr;
-    n_ = 0;
-  }
-  template <typename U>
-  U* as() const { return static_cast<U*>(p_); }
-  size_t bytes() const { return n_; }
-
- private:
-  void* p_ = nullptr;
-  size_t n_ = 0;
-};
-
-// Page-locked host memory (allocated pinned; existing host memory is never registered).
-class PinnedBuf {
- public:
-  explicit PinnedBuf(size_t bytes) {
-    if (bytes == 0) return;
-    const hipError_t e = hipHostMalloc(&p_, bytes, hipHostMallocDefault);
-    if (e != hipSuccess) {
-      p_ = nullptr;
-      throw Error(kEnomem, "pinned host allocation of " + std::to_string(bytes) + " bytes failed: " +
-                               hipGetErrorString(e));
-    }
-  }
-  PinnedBuf(const PinnedBuf&) = delete;
-  PinnedBuf& operator=(const PinnedBuf&) = delete;
-  PinnedBuf(PinnedBuf&& o) noexcept : p_(o.p_) { o.p_ = nullptr; }
-  ~PinnedBuf() {
-    if (p_) (void)hipHostFree(p_);
-  }
-  template <typename U>
-  U* as() const { return static_cast<U*>(p_); }
-
- private:
-  void* p_ = nullptr;
-};
-
-template <typename U>
-inline DevBuf upload_vec(const std::vector<U>& v, hipStream_t s) {
-  DevBuf b(v.size() * sizeof(U));
-  if (!v.empty()) HIP_CHECK(hipMemcpyAsync(b.as<void>(), v.data(), v.size() * sizeof(U),
-                                           hipMemcpyHostToDevice, s));
-  return b;
-}
-
-// ---- FFT plans ---------------------------------------------------------------------------------
-
-inline std::vector<int> factorize(long long n) {
-  std::vector<int> r;
-  long long m = n;
-  for (int R : {8, 4, 2, 3, 5, 7, 11, 13})
-    while (m % R == 0) {
-      r.push_back(R);
-      m /= R;
-    }
-  if (m != 1)
-    throw Error(kEinval, "grid size " + std::to_string(n) +
-                             " has a prime factor above 13; supported sizes are 2^a·3^b·5^c·7^d·11^e·13^f");
-  // a single radix-2 stage is cheaper after the radix-8 stages; merge 2·2 into 4 if produced
-  if (r.size() > kMaxStages) throw Error(kEinval, "grid size " + std::to_string(n) + " needs too many FFT stages");
-  return r;
-}
-
-template <typename T>
-struct Twiddles {
-  DevBuf hi, lo;
-  FftPlan plan{};
-  void build(long long n, hipStream_t s) {
-    plan.n = static_cast<int>(n);
-    const std::vector<int> r = n > 1 ? factorize(n) : std::vector<int>{};
-    plan.nst = static_cast<int>(r.size());
-    for (size_t i = 0; i < r.size(); ++i) plan.rad[i] = r[i];
-    std::vector<cx<T>> h(n), l(n);
-    for (long long t = 0; t < n; ++t) {
-      // exp(−2πi t/n) from the mathematics in extended precision, then rounded
-      const long double a = -2.0L * 3.141592653589793238462643383279502884L * static_cast<long double>(t) /
-                            static_cast<long double>(n);
-      const double c = static_cast<double>(cosl(a)), sn = static_cast<double>(sinl(a));
-      const T ch = static_cast<T>(c), sh = static_cast<T>(sn);
-      h[t] = cx<T>{ch, sh};
-      l[t] = cx<T>{static_cast<T>(c - static_cast<double>(ch)), static_cast<T>(sn - static_cast<double>(sh))};
-    }
-    hi = upload_vec(h, s);
-    lo = upload_vec(l, s);
-    plan.tw_hi = hi.as<void>();
-    plan.tw_lo = lo.as<void>();
-  }
-};
 
 // ---- per-boundary structure ------------------------------------------------------------------
 
@@ -442,6 +326,21 @@ class Engine final : public EngineBase {
     return born_->fluctuation();
   }
 
+  const void* device_plane(int which, long long p) const override {
+    require(ran_, "the solver has not run yet");
+    const int nit = std::max(last_iterations_, 1);
+    if (which == 0) return iter_t_.as<px2<T>>() + (nit - 1) * npix_;
+    if (which == 1) {
+      require(last_iterations_ > 0, "a run without iterations has no reflected field");
+      return iter_r_.as<px2<T>>() + (nit - 1) * npix_;
+    }
+    require(which == 2 || which == 3, "which must be 0 (transmitted), 1 (reflected), 2 (forward) or 3 (backward)");
+    const int slot = keep_slot(p);
+    require(slot >= 0 && p >= 0 && p <= sp_.nz, "plane " + std::to_string(p) + " was not kept in the last run");
+    require(which == 2 || last_iterations_ > 0, "a run without iterations has no backward field");
+    return (which == 2 ? keepf_ : keepb_).as<px2<T>>() + slot * npix_;
+  }
+
   void run(const c128* field, int iterations, const std::vector<long long>& keep, RunOutput& out) override {
     HIP_CHECK(hipSetDevice(sp_.device));
     upload(field);
@@ -510,6 +409,8 @@ class Engine final : public EngineBase {
     const int nit = std::max(iterations, 1);
     const bool bidir = iterations > 0;
     keep_ = keep;
+    last_iterations_ = iterations;
+    ran_ = true;
     const size_t pb = npix_ * sizeof(px2<T>);
     if (iter_t_.bytes() < nit * pb) iter_t_.alloc(nit * pb);
     if (bidir && iter_r_.bytes() < nit * pb) iter_r_.alloc(nit * pb);
@@ -1374,6 +1275,8 @@ class Engine final : public EngineBase {
   std::vector<long long> keep_;
   std::unique_ptr<Born<T, COMP>> born_;
   long long last_steps_ = 0;
+  int last_iterations_ = 0;
+  bool ran_ = false;
   size_t bytes_ = 0;
 };
 

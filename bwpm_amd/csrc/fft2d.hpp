@@ -9,6 +9,8 @@
 #include "fft_reg.hpp"
 #include "kernels.hpp"
 
+#include <algorithm>
+
 namespace bwpm {
 
 inline constexpr int kMaxPlanes = 4;
@@ -124,5 +126,145 @@ __global__ __launch_bounds__(N / 8 * W) void k_fft2_cols_fast(PlaneSet<T> ps, in
     out[q] = post(v[m], mul, q, ps.scale);
   }
 }
+
+// ---- host driver ---------------------------------------------------------------------------------
+
+// In-place or out-of-place 2D transforms of up to kMaxPlanes (ny, nx, 2) planes on one stream: a
+// row pass then a column pass. Register kernels for power-of-two lines up to 2048 where the
+// precision mode matches them (FP64 plain, FP32 compensated), LDS Stockham kernels otherwise.
+template <typename T, bool COMP>
+class Fft2D {
+ public:
+  void init(long long ny, long long nx, const FftPlan& px, const FftPlan& py, hipStream_t stream) {
+    ny_ = ny;
+    nx_ = nx;
+    npix_ = ny * nx;
+    px_ = px;
+    py_ = py;
+    stream_ = stream;
+    auto ls_of = [](int lines) { return lines <= 2 ? lines : (lines | 1); };
+    const int elems = Budget<T>::elems;
+    rows_fast_ = fast_n(nx_) && COMP == (sizeof(T) == 4);
+    cols_fast_ = fast_n(ny_) && COMP == (sizeof(T) == 4);
+    int rw = static_cast<int>(std::min<long long>(ny_, 64));
+    while (rw > 1 && nx_ * ls_of(2 * rw) > elems) rw /= 2;
+    rows_ = rw;
+    ls_row_ = ls_of(2 * rw);
+    int w = 16;
+    while (w > 1 && ny_ * ls_of(2 * w) > elems) w /= 2;
+    while (w > 1 && w > nx_) w /= 2;
+    cols_ = w;
+    ls_col_ = ls_of(2 * w);
+    require(nx_ * ls_row_ <= elems && ny_ * ls_col_ <= elems, "grid too large for the 2D transforms");
+    for (const void* k : {reinterpret_cast<const void*>(&k_fft2_rows<T, COMP, false>),
+                          reinterpret_cast<const void*>(&k_fft2_rows<T, COMP, true>),
+                          reinterpret_cast<const void*>(&k_fft2_cols<T, COMP, false>),
+                          reinterpret_cast<const void*>(&k_fft2_cols<T, COMP, true>)})
+      HIP_CHECK(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+  }
+
+  // In-place transform of np planes; the inverse is scaled by 1/N and optionally multiplied by `mul`.
+  void planes(px2<T>* const* pl, int np, bool inverse, const cx<T>* mul) {
+    PlaneSet<T> a{}, b{};
+    for (int i = 0; i < np; ++i) {
+      a.in[i] = b.in[i] = pl[i];
+      a.out[i] = b.out[i] = pl[i];
+      b.mul[i] = mul;
+    }
+    run(a, b, np, inverse);
+  }
+  // out = F(in) or F⁻¹(in) (1/N scaled); `in` is left unchanged unless it is `out`.
+  void one(const px2<T>* in, px2<T>* out, bool inverse) {
+    PlaneSet<T> a{}, b{};
+    a.in[0] = in;
+    a.out[0] = out;
+    b.in[0] = out;
+    b.out[0] = out;
+    run(a, b, 1, inverse);
+  }
+
+ private:
+  static bool fast_n(long long n) {
+    return n == 64 || n == 128 || n == 256 || n == 512 || n == 1024 || n == 2048;
+  }
+  void run(PlaneSet<T>& a, PlaneSet<T>& b, int np, bool inverse) {
+    a.scale = 1;
+    b.scale = inverse ? static_cast<T>(1.0 / static_cast<double>(npix_)) : static_cast<T>(1);
+    if (inverse) {
+      rows_pass<true>(a, np);
+      cols_pass<true>(b, np);
+    } else {
+      rows_pass<false>(a, np);
+      cols_pass<false>(b, np);
+    }
+  }
+  template <bool INV>
+  void rows_pass(const PlaneSet<T>& ps, int np) {
+    if (rows_fast_) {
+      switch (nx_) {
+        case 64: rows_fast_n<64, INV>(ps, np); return;
+        case 128: rows_fast_n<128, INV>(ps, np); return;
+        case 256: rows_fast_n<256, INV>(ps, np); return;
+        case 512: rows_fast_n<512, INV>(ps, np); return;
+        case 1024: rows_fast_n<1024, INV>(ps, np); return;
+        default: rows_fast_n<2048, INV>(ps, np); return;
+      }
+    }
+    const size_t lds = static_cast<size_t>(nx_) * ls_row_ * sizeof(cx<T>);
+    hipLaunchKernelGGL((k_fft2_rows<T, COMP, INV>), dim3(static_cast<unsigned>((ny_ + rows_ - 1) / rows_), np),
+                       dim3(kThreads), lds, stream_, ps, static_cast<int>(ny_), static_cast<int>(nx_), rows_, ls_row_, px_);
+    HIP_CHECK(hipGetLastError());
+  }
+  template <int N, bool INV>
+  void rows_fast_n(const PlaneSet<T>& ps, int np) {
+    constexpr int RW = (sizeof(T) == 4 ? 512 : 256) / (N / 8) > 0 ? (sizeof(T) == 4 ? 512 : 256) / (N / 8) : 1;
+    const auto* th = static_cast<const cx<T>*>(px_.tw_hi);
+    const auto* tl = static_cast<const cx<T>*>(px_.tw_lo);
+    if (ny_ % RW == 0)
+      hipLaunchKernelGGL((k_fft2_rows_fast<N, RW, T, COMP, INV>), dim3(static_cast<unsigned>(ny_ / RW), np),
+                         dim3(N / 8 * RW), 0, stream_, ps, th, tl);
+    else
+      hipLaunchKernelGGL((k_fft2_rows_fast<N, 1, T, COMP, INV>), dim3(static_cast<unsigned>(ny_), np), dim3(N / 8), 0,
+                         stream_, ps, th, tl);
+    HIP_CHECK(hipGetLastError());
+  }
+  template <bool INV>
+  void cols_pass(const PlaneSet<T>& ps, int np) {
+    if (cols_fast_) {
+      switch (ny_) {
+        case 64: cols_fast_n<64, INV>(ps, np); return;
+        case 128: cols_fast_n<128, INV>(ps, np); return;
+        case 256: cols_fast_n<256, INV>(ps, np); return;
+        case 512: cols_fast_n<512, INV>(ps, np); return;
+        case 1024: cols_fast_n<1024, INV>(ps, np); return;
+        default: cols_fast_n<2048, INV>(ps, np); return;
+      }
+    }
+    const size_t lds = static_cast<size_t>(ny_) * ls_col_ * sizeof(cx<T>);
+    hipLaunchKernelGGL((k_fft2_cols<T, COMP, INV>), dim3(static_cast<unsigned>((nx_ + cols_ - 1) / cols_), np),
+                       dim3(kThreads), lds, stream_, ps, static_cast<int>(ny_), static_cast<int>(nx_), cols_, ls_col_, py_);
+    HIP_CHECK(hipGetLastError());
+  }
+  template <int N, bool INV>
+  void cols_fast_n(const PlaneSet<T>& ps, int np) {
+    constexpr int W0 = (sizeof(T) == 4 ? 512 : 256) / (N / 8);
+    constexpr int W = W0 > 16 ? 16 : (W0 < 1 ? 1 : W0);
+    const auto* th = static_cast<const cx<T>*>(py_.tw_hi);
+    const auto* tl = static_cast<const cx<T>*>(py_.tw_lo);
+    if (nx_ % W == 0)
+      hipLaunchKernelGGL((k_fft2_cols_fast<N, W, T, COMP, INV>), dim3(static_cast<unsigned>(nx_ / W), np),
+                         dim3(N / 8 * W), 0, stream_, ps, static_cast<int>(nx_), th, tl);
+    else
+      hipLaunchKernelGGL((k_fft2_cols_fast<N, 1, T, COMP, INV>), dim3(static_cast<unsigned>(nx_), np), dim3(N / 8), 0,
+                         stream_, ps, static_cast<int>(nx_), th, tl);
+    HIP_CHECK(hipGetLastError());
+  }
+
+  long long ny_ = 0, nx_ = 0, npix_ = 0;
+  FftPlan px_{}, py_{};
+  hipStream_t stream_ = nullptr;
+  bool rows_fast_ = false, cols_fast_ = false;
+  int rows_ = 1, ls_row_ = 2, cols_ = 1, ls_col_ = 2;
+};
 
 }  // namespace bwpm

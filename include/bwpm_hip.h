@@ -196,6 +196,53 @@ int bwpm_queue_result(bwpm_queue* q, int64_t ticket, bwpm_result** out);
 #define BWPM_QUEUE_NSTATS 11
 int bwpm_queue_stats(bwpm_queue* q, double* out);
 
+/* ---- analysis reductions over planes (the step after the path) ---------------------------------
+ *
+ * Replace the NumPy analysis of kept planes in `bwpm.pilot_imc3d` (src/bwpm/pilot_imc3d.py:19-131),
+ * the Strehl ratio of experiments/b3_scaling.py:88-97 and `bwpm.pilot_imc.strehl_from_spectrum`
+ * (src/bwpm/pilot_imc.py:184-197). The transforms, per-mode weights and sums run in FP64 on the
+ * device with deterministic reductions; a plane can stay on the device (a kept plane or output of
+ * a solver, bwpm_solver_plane). A handle is bound to one grid and device and is not safe for
+ * concurrent calls. */
+typedef struct bwpm_analysis bwpm_analysis;
+
+#define BWPM_PLANE_HOST_C128 0   /* host memory, complex128 (ny, nx, 2) */
+#define BWPM_PLANE_DEVICE_C128 1 /* device memory, complex128 */
+#define BWPM_PLANE_DEVICE_C64 2  /* device memory, complex64 (planes of an FP32 solver) */
+
+/* Grid (ny, nx) with steps dy, dx (mm) at vacuum wavenumber k0 = 2π/λ (1/mm), as `Grid3D`
+ * (src/bwpm/volume3d.py:38-67). */
+int bwpm_analysis_create(int device, int64_t ny, int64_t nx, double dy, double dx, double k0, bwpm_analysis** out);
+int bwpm_analysis_destroy(bwpm_analysis* a);
+/* power[count]: flux n·Σ cos θ (|E_s|² + |E_p|²) of each plane in a medium of index n
+ * (`plane_power`, src/bwpm/pilot_imc3d.py:55-63). */
+int bwpm_plane_power(bwpm_analysis* a, const void* planes, int where, int64_t count, double index, double* power);
+/* Flux-weighted Fresnel reflectance of n1 → n2 for the field (`reference_reflectance`,
+ * src/bwpm/pilot_imc3d.py:82-97); BWPM_EINVAL if the field carries no propagating flux. */
+int bwpm_reference_reflectance(bwpm_analysis* a, const void* plane, int where, double n1, double n2,
+                               double* reflectance);
+/* intensity[noff]: on-axis |E_x|² + |E_y|² + |E_z|² after each distance offsets[i] (mm), from the
+ * propagating modes, with na >= 0 only those with |k⊥| <= na·k0 (na < 0: no aperture). The host
+ * picks the first maximum, as `axial_focus` does (src/bwpm/pilot_imc3d.py:100-131). */
+int bwpm_axial_profile(bwpm_analysis* a, const void* plane, int where, double index, const double* offsets,
+                       int64_t noff, double na, double* intensity);
+/* values[noff]: |Σ A e^{i k_z d}|² / (Σ|A|)² over the modes of the spectrum (ny, nx) complex128 with
+ * |k⊥| <= na·k0, k_z = sqrt(k² − k⊥²) (experiments/b3_scaling.py:88-97). where: host or device
+ * complex128. */
+int bwpm_strehl_profile(bwpm_analysis* a, const void* spectrum, int where, double k, double na,
+                        const double* offsets, int64_t noff, double* values);
+/* values[noff] for m modes with transverse wavenumbers ky[m] and host amplitudes spectrum[m]
+ * (complex128), k_z = sqrt(max(k² − k_y², 0)) (`strehl_from_spectrum`, src/bwpm/pilot_imc.py:184-197);
+ * the handle supplies the device only. */
+int bwpm_strehl_modes(bwpm_analysis* a, const double* ky, const double* spectrum, int64_t m, double k,
+                      const double* offsets, int64_t noff, double* values);
+
+/* Device address of a plane of the solver's last run, valid until its next run or destroy:
+ * which 0 = transmitted, 1 = reflected (final iteration), 2 = forward / 3 = backward field at a
+ * kept plane `plane`. precision receives BWPM_FP64 (complex128 planes) or BWPM_FP32 (complex64).
+ * Feeds the analysis calls without a copy to the host (no counterpart in the reference). */
+int bwpm_solver_plane(const bwpm_solver* s, int which, int64_t plane, const void** ptr, int* precision);
+
 #ifdef __cplusplus
 }
 #endif
