@@ -18,7 +18,7 @@ served mix on the request queue (capacity, then p50/p99 latency of a Poisson str
 reference's CPU core timed on this host over a bounded sample of the same geometry.
 
     python bench.py [--steps K] [--warmup W] [--batch B] [--n 512] [--nz 1024] [--iterations 4]
-                    [--precision fp64|fp32|fp32_plain]
+                    [--precision fp64|mixed|fp32|fp32_plain]
 """
 
 from __future__ import annotations
@@ -38,7 +38,7 @@ sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBPS = 8000.0  # MI355X HBM3E
 # algorithmic bytes per grid-point·slab update: read + write of the (E_x, E_y) plane per slab step
-BYTES_PER_UPDATE = {"fp64": 64, "fp32": 32, "fp32_plain": 32}
+BYTES_PER_UPDATE = {"fp64": 64, "mixed": 32, "fp32": 32, "fp32_plain": 32}
 
 
 def profiled_traffic(precision: str, n: int, batch: int) -> tuple[int | None, str | None]:
@@ -104,6 +104,33 @@ def run_solve(p, precision: str) -> tuple[dict, float]:
         s.close()
 
 
+def precision_mode(probs: list, precision: str, iterations: int, ref: dict) -> dict:
+    """The headline lane in another precision: updates/s over one timed batch after one warm-up,
+    the kernel durations, and the relative L2 error of solve 0 against the FP64 device result."""
+    from bwpm_amd import _core
+
+    solvers = []
+    try:
+        for p in probs:
+            s = _core.GpuSolver(*p.solver_args(), precision=precision)
+            s.upload(p.field)
+            solvers.append(s)
+        _core.run_batch(solvers, iterations, max_lane=len(solvers))
+        t0 = time.perf_counter()
+        _core.run_batch(solvers, iterations, max_lane=len(solvers))
+        rate = sum(p.updates for p in probs) / (time.perf_counter() - t0)
+        kt = _core.run_batch(solvers, iterations, max_lane=len(solvers), time_kernels=True)
+        got = solvers[0].fetch(iterations)
+    finally:
+        for s in solvers:
+            s.close()
+    return {"updates_per_s": rate,
+            "k_col_us": round(1e3 * kt["col_ms"] / max(kt["col_launches"], 1), 3),
+            "k_row_us": round(1e3 * kt["row_ms"] / max(kt["row_launches"], 1), 3),
+            "rel_l2_transmitted": rel_l2(got["transmitted"], ref["transmitted"]),
+            "rel_l2_reflected": rel_l2(got["reflected"], ref["reflected"])}
+
+
 def uneven_batch(count: int) -> dict:
     """Config 4: `count` uneven Born solves (128² … 512², 100 … 1000 slabs, per-solve λ, angle, σ and
     seed; δn generated on the device) submitted at once to the request queue, complex64."""
@@ -126,33 +153,35 @@ def configs(c3_nz: int, c4_count: int) -> dict:
     """The reference's other configurations: config 1 (2D, 1024 points, 200 slabs, 3 iterations),
     config 2 (3D 256² × 512 tilted interface, 4 iterations, E_x and E_y) and config 3 (3D 512² Born
     volume, `c3_nz` slabs, 6 iterations) as single solves; config 4 (uneven Born batch) on the
-    queue. FP32 errors are relative L2 against the FP64 device result (itself within 1e-10 of the
-    oracle)."""
+    queue. Errors of the mixed and FP32 modes are relative L2 against the FP64 device result
+    (itself within 1e-10 of the oracle)."""
     from bwpm_amd import workloads
 
     out: dict = {}
     p1 = workloads.plain_2d(ny=1024, nz=200, iterations=3)
     _, r1 = run_solve(p1, "fp64")
     out["c1_2d_1024x200_it3"] = {"fp64_updates_per_s": r1}
+    def against(p, ref: dict) -> dict:
+        # the same solve with complex64 streaming planes (FP64 arithmetic) and in FP32 arithmetic
+        out_m = {}
+        for m in ("mixed", "fp32"):
+            r, rate = run_solve(p, m)
+            out_m[m] = {"updates_per_s": rate, "rel_l2_transmitted": rel_l2(r["transmitted"], ref["transmitted"]),
+                        "rel_l2_reflected": rel_l2(r["reflected"], ref["reflected"])}
+        return out_m
+
     c2 = {}
     for pol in ("x", "y"):
         p2 = workloads.tilted_3d(n=256, nz=512, iterations=4, polarisation=pol)
         a64, r64 = run_solve(p2, "fp64")
-        a32, r32 = run_solve(p2, "fp32")
-        c2[f"E{pol}"] = {"fp64_updates_per_s": r64, "fp32_updates_per_s": r32,
-                         "fp32_rel_l2_transmitted": rel_l2(a32["transmitted"], a64["transmitted"]),
-                         "fp32_rel_l2_reflected": rel_l2(a32["reflected"], a64["reflected"])}
+        c2[f"E{pol}"] = {"fp64_updates_per_s": r64, **against(p2, a64)}
     out["c2_tilted_256x256x512_it4"] = c2
     # config 3 at full depth: the δn volume is generated on the device (a (1024, 512, 512) complex
     # array would be 4 GiB on the host)
     p3 = workloads.born_3d_synth(n=512, nz=c3_nz, iterations=6)
     b64, q64 = run_solve(p3, "fp64")
-    b32, q32 = run_solve(p3, "fp32")
     out[f"c3_born_512x512x{c3_nz}_it6"] = {
-        "fp64_updates_per_s": q64, "fp32_updates_per_s": q32,
-        "fp64_gmres_iterations": b64["gmres_iterations"],
-        "fp32_rel_l2_transmitted": rel_l2(b32["transmitted"], b64["transmitted"]),
-        "fp32_rel_l2_reflected": rel_l2(b32["reflected"], b64["reflected"]),
+        "fp64_updates_per_s": q64, "fp64_gmres_iterations": b64["gmres_iterations"], **against(p3, b64),
         "input": "Gaussian-correlated δn (σ = 0.01, correlation 1 µm, seed 3) generated on the device"}
     if c4_count > 0:
         out[f"c4_uneven_born_{c4_count}"] = uneven_batch(c4_count)
@@ -204,11 +233,12 @@ def main() -> int:
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--nz", type=int, default=1024)
     ap.add_argument("--iterations", type=int, default=4)
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32", "fp32_plain"])
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed", "fp32", "fp32_plain"])
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "0") or 0))
     ap.add_argument("--cpu-nz", type=int, default=48)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-checks", action="store_true", help="skip the plain-path and accuracy runs")
+    ap.add_argument("--no-modes", action="store_true", help="skip the mixed and FP32 lanes of the headline")
     ap.add_argument("--serve-requests", type=int, default=24,
                     help="requests of the served mix (0: no serving run)")
     ap.add_argument("--c3-nz", type=int, default=1024, help="slabs of the Born configuration (0: skip configs)")
@@ -303,6 +333,10 @@ def main() -> int:
     for s in solvers:
         s.close()
     solvers = []
+    modes = None
+    if checks is not None and args.precision == "fp64" and not args.no_modes:
+        # the same lane with complex64 streaming planes (FP64 arithmetic) and in FP32 arithmetic
+        modes = {m: precision_mode(probs, m, args.iterations, alone) for m in ("mixed", "fp32")}
     other = configs(args.c3_nz, args.c4_count) if args.c3_nz > 0 else None
     serving = None
     if args.serve_requests > 0:
@@ -323,7 +357,7 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "c64" if args.precision.startswith("fp32") else "c128",
+        "dtype": {"fp64": "c128", "mixed": "c64 planes, f64 arithmetic"}.get(args.precision, "c64"),
         "data": "synthetic",
         "config": {
             "workload": f"tilted-interface 3D vector BWPM {args.n}x{args.n}x{args.nz} slabs, "
@@ -333,6 +367,7 @@ def main() -> int:
         },
         "roofline": roofline,
         "checks": checks,
+        "precision_modes": modes,
         "configs": other,
         "serving": serving,
         "cpu_baseline": cpu,

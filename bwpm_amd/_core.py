@@ -29,7 +29,7 @@ if not _LIB_PATH.exists():
 _lib = ctypes.CDLL(str(_LIB_PATH))
 
 OK, EINVAL, ENOCONV, EDEVICE, ENOMEM, EINTERNAL = 0, 1, 2, 3, 4, 5
-PRECISIONS = {"fp64": 64, "fp32": 32, "fp32_plain": 31}
+PRECISIONS = {"fp64": 64, "mixed": 48, "fp32": 32, "fp32_plain": 31}
 
 _d = ctypes.c_double
 _i64 = ctypes.c_int64
@@ -102,6 +102,7 @@ _a_refl = _sig("bwpm_reference_reflectance", _p, _p, _int, _d, _d, ctypes.POINTE
 _a_axial = _sig("bwpm_axial_profile", _p, _p, _int, _d, _p, _i64, _d, _p)
 _a_strehl = _sig("bwpm_strehl_profile", _p, _p, _int, _d, _d, _p, _i64, _p)
 _a_strehl_modes = _sig("bwpm_strehl_modes", _p, _p, _p, _i64, _d, _p, _i64, _p)
+_graph_stats = _sig("bwpm_lane_graph_stats", ctypes.POINTER(_i64), ctypes.POINTER(_i64), ctypes.POINTER(_i64))
 _solver_plane = _sig("bwpm_solver_plane", _p, _int, _i64, ctypes.POINTER(_p), ctypes.POINTER(_int))
 _batch = _sig("bwpm_batch_run", _p, _i64, _int, _int, _int, ctypes.POINTER(_d), ctypes.POINTER(_d),
                ctypes.POINTER(_d), ctypes.POINTER(_i64), ctypes.POINTER(_i64))
@@ -116,6 +117,13 @@ def _check(status: int) -> None:
     if status == EINVAL:
         raise ValueError(msg)
     raise RuntimeError(msg or f"bwpm_hip error {status}")
+
+
+def lane_graph_stats() -> dict[str, int]:
+    """Process-wide counts of continuous-lane graphs: built, replayed, fallbacks to direct launches."""
+    b, r, f = _i64(0), _i64(0), _i64(0)
+    _check(_graph_stats(ctypes.byref(b), ctypes.byref(r), ctypes.byref(f)))
+    return {"built": int(b.value), "replays": int(r.value), "fallbacks": int(f.value)}
 
 
 def gpu_device_count() -> int:

@@ -64,31 +64,47 @@ __global__ void k_kz_eval(zd n, double k0, const double* kp2, long long cnt, zd*
 std::unique_ptr<EngineBase> make_engine(Spec spec) {
   switch (spec.precision) {
     case 64: return make_engine_f64(std::move(spec));
+    case 48:
+      // Born steps are bound by their GMRES, not by the two streaming planes, and the Born operator
+      // amplifies the planes' storage rounding: solves with index fluctuations keep them FP64
+      if ((spec.synth.on && spec.synth.sigma > 0) ||
+          std::any_of(spec.fluct.begin(), spec.fluct.end(), [](const c128& v) { return v != c128(0.0, 0.0); }))
+        return make_engine_f64(std::move(spec));
+      return make_engine_mixed(std::move(spec));
     case 32: return make_engine_f32(std::move(spec), true);
     case 31: return make_engine_f32(std::move(spec), false);
-    default: throw Error(kEinval, "precision must be 64, 32 or 31");
+    default: throw Error(kEinval, "precision must be 64, 48, 32 or 31");
   }
+}
+
+LaneGraphCounters& lane_graph_counters() {
+  static LaneGraphCounters c;
+  return c;
 }
 
 void run_continuous_lane(int precision, LaneFeed& feed, int slots, int chunk, hipStream_t stream) {
   if (precision == 64)
     run_continuous_lane_f64(feed, slots, chunk, stream);
+  else if (precision == 48)
+    run_continuous_lane_mixed(feed, slots, chunk, stream);
   else
     run_continuous_lane_f32(feed, slots, chunk, stream, precision == 32);
 }
 
 size_t estimate_device_bytes(const Spec& s, int iterations) {
-  const size_t elem = s.precision == 64 ? 16 : 8;  // complex
+  const size_t elem = s.precision == 64 || s.precision == 48 ? 16 : 8;  // complex
+  const size_t selem = s.precision == 48 ? 8 : elem;                     // streaming planes
   const size_t npix = static_cast<size_t>(s.ny) * static_cast<size_t>(s.nx);
   const size_t plane = 2 * npix * elem;            // (E_x, E_y) plane
+  const size_t splane = 2 * npix * selem;
   const size_t nit = static_cast<size_t>(std::max(iterations, 1));
   std::vector<c128> media;
   for (const c128& v : s.indices)
     if (std::find(media.begin(), media.end(), v) == media.end()) media.push_back(v);
   if (std::find(media.begin(), media.end(), s.index_before) == media.end()) media.push_back(s.index_before);
   const size_t m = media.size();
-  // staging (complex128), input, X, e, up to four product planes, per-iteration outputs
-  size_t bytes = 2 * npix * 16 + plane * (3 + 4 + 2 * nit);
+  // staging (complex128), input, e, per-iteration outputs; X and up to four product planes
+  size_t bytes = 2 * npix * 16 + plane * (2 + 2 * nit) + splane * 5;
   // phase tables per medium and 2×2 tables per ordered medium pair and kind (hi and lo parts)
   bytes += m * 2 * npix * elem + (m > 1 ? m * (m - 1) * 4 * 6 * npix * elem : 0);
   // sparse interface values (two sets) and lists, bounded by two planes

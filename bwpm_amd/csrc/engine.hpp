@@ -6,6 +6,7 @@
 #include <map>
 #include <memory>
 #include <vector>
+#include <atomic>
 
 #include "common.hpp"
 
@@ -32,7 +33,8 @@ struct Spec {
   std::vector<c128> fluct;      // (nz, ny, nx) or empty
   std::vector<int32_t> labels_before;  // labels of plane 0 (ny, nx); empty: index_before everywhere
   FluctSynth synth;             // δn generated on the device (fluct must be empty)
-  int precision = 32;           // 64: FP64; 32: FP32 compensated; 31: FP32 plain
+  int precision = 32;           // 64: FP64; 48: complex64 streaming planes, FP64 arithmetic;
+                                // 32: FP32 compensated; 31: FP32 plain
   int device = 0;
 };
 
@@ -89,6 +91,7 @@ class EngineBase {
 std::unique_ptr<EngineBase> make_engine(Spec spec);
 std::unique_ptr<EngineBase> make_engine_f32(Spec spec, bool compensated);
 std::unique_ptr<EngineBase> make_engine_f64(Spec spec);
+std::unique_ptr<EngineBase> make_engine_mixed(Spec spec);
 
 // Batched solve of many engines: members are grouped into lanes of equal (precision, grid,
 // device, no δn); each lane runs in lockstep on its own stream, lanes run concurrently.
@@ -124,8 +127,16 @@ class LaneFeed {
 // steps are all plain launches replay one captured graph per shape for the lane's lifetime, so a
 // long-lived lane of small grids is not bound by launch overhead.
 void run_continuous_lane(int precision, LaneFeed& feed, int slots, int chunk, hipStream_t stream);
+
+// Process-wide counts of the continuous lanes' graphs: built, replayed, and lanes that fell back
+// to direct launches because a graph could not be built.
+struct LaneGraphCounters {
+  std::atomic<long long> built{0}, replays{0}, fallbacks{0};
+};
+LaneGraphCounters& lane_graph_counters();
 void run_continuous_lane_f32(LaneFeed& feed, int slots, int chunk, hipStream_t stream, bool compensated);
 void run_continuous_lane_f64(LaneFeed& feed, int slots, int chunk, hipStream_t stream);
+void run_continuous_lane_mixed(LaneFeed& feed, int slots, int chunk, hipStream_t stream);
 
 // Upper bound of the device bytes of one solve of `spec` with `iterations` (admission by HBM
 // footprint before the engine exists).

@@ -179,9 +179,42 @@ inline void tune_report(const TuneKey& key, int trial, double us_per_solve_step)
 
 using namespace detail;
 
+// Where a launch goes: onto a stream, or appended as a kernel node to a graph (after the previous
+// node, so the nodes of a chunk run in order). Building graphs node by node instead of capturing
+// a stream keeps them immune to what other threads of the process do meanwhile.
+struct Emit {
+  hipStream_t stream = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphNode_t last = nullptr;
+
+  template <typename... P, typename... A>
+  void operator()(void (*kern)(P...), dim3 grid, dim3 block, size_t lds, A... args) {
+    launch_args<P...>(kern, grid, block, lds, static_cast<P>(args)...);
+  }
+
+ private:
+  template <typename... P>
+  void launch_args(void (*kern)(P...), dim3 grid, dim3 block, size_t lds, P... args) {
+    if (!graph) {
+      hipLaunchKernelGGL(kern, grid, block, lds, stream, args...);
+      return;
+    }
+    void* params[] = {static_cast<void*>(&args)...};
+    hipKernelNodeParams np{};
+    np.func = reinterpret_cast<void*>(kern);
+    np.gridDim = grid;
+    np.blockDim = block;
+    np.sharedMemBytes = static_cast<unsigned>(lds);
+    np.kernelParams = params;
+    hipGraphNode_t node = nullptr;
+    HIP_CHECK(hipGraphAddKernelNode(&node, graph, last ? &last : nullptr, last ? 1 : 0, &np));
+    last = node;
+  }
+};
+
 // Launches of one slab step over a lane of S solves of one grid: the kernel geometry and the
 // twiddle tables (shared, so a lane runner may outlive the solve it took them from).
-template <typename T, bool COMP>
+template <typename T, bool COMP, typename ST = T>
 struct Launcher {
   long long ny = 0, nx = 0, npix = 0;
   Geometry geo;
@@ -189,7 +222,18 @@ struct Launcher {
 
   // multi: some solve of the launch has several products at this step (register kernels keep
   // their spectrum in LDS); the result bits do not depend on it.
-  void col(const ColDesc<T>* d, int S, hipStream_t s, bool multi) const {
+  void col(const ColDesc<T, ST>* d, int S, hipStream_t s, bool multi) const {
+    Emit e;
+    e.stream = s;
+    col(d, S, e, multi);
+  }
+  void row(const RowDesc<T, ST>* d, int S, hipStream_t s, bool multi) const {
+    Emit e;
+    e.stream = s;
+    row(d, S, e, multi);
+  }
+
+  void col(const ColDesc<T, ST>* d, int S, Emit& s, bool multi) const {
     if (geo.col_fast && COMP == (sizeof(T) == 4)) {
       switch (ny) {
         case 64: col_fast_n<64>(d, S, s, multi); return;
@@ -205,10 +249,10 @@ struct Launcher {
     ColGeom g{static_cast<int>(ny), static_cast<int>(nx), geo.cols2, geo.ls_col2, npix, twy->plan};
     const size_t lds = static_cast<size_t>(ny) * g.ls * sizeof(cx<T>);
     const dim3 grid(static_cast<unsigned>((nx + g.cols - 1) / g.cols), S);
-    hipLaunchKernelGGL((k_col_step<T, COMP>), grid, dim3(kThreads), lds, s, d, g);
+    s(&k_col_step<T, COMP, ST>, grid, dim3(kThreads), lds, d, g);
   }
 
-  void row(const RowDesc<T>* d, int S, hipStream_t s, bool multi) const {
+  void row(const RowDesc<T, ST>* d, int S, Emit& s, bool multi) const {
     if (geo.row_fast && COMP == (sizeof(T) == 4)) {
       switch (nx) {
         case 64: row_fast_n<64>(d, S, s, multi); return;
@@ -223,7 +267,7 @@ struct Launcher {
     RowGeom g{static_cast<int>(ny), static_cast<int>(nx), geo.rows, geo.ls_row, npix, twx->plan};
     const size_t lds = 2 * static_cast<size_t>(nx) * g.ls * sizeof(cx<T>);
     const dim3 grid(static_cast<unsigned>((ny + g.rows - 1) / g.rows), S);
-    hipLaunchKernelGGL((k_row_step<T, COMP>), grid, dim3(kThreads), lds, s, d, g);
+    s(&k_row_step<T, COMP, ST>, grid, dim3(kThreads), lds, d, g);
   }
 
  private:
@@ -235,29 +279,27 @@ struct Launcher {
            163840;
   }
   template <int N, int W>
-  void col_launch(const ColDesc<T>* d, int S, hipStream_t s, bool multi, const cx<T>* th, const cx<T>* tl) const {
+  void col_launch(const ColDesc<T, ST>* d, int S, Emit& s, bool multi, const cx<T>* th, const cx<T>* tl) const {
     const dim3 grid(static_cast<unsigned>(nx / W), S);
     if constexpr (stash_fits<N, W>())
       if (multi) {
-        hipLaunchKernelGGL((k_col_fast<N, W, T, COMP, true>), grid, dim3(N / 4 * W), 0, s, d, static_cast<int>(nx),
-                           npix, th, tl);
+        s(&k_col_fast<N, W, T, COMP, true, ST>, grid, dim3(N / 4 * W), 0, d, static_cast<int>(nx), npix, th, tl);
         return;
       }
-    hipLaunchKernelGGL((k_col_fast<N, W, T, COMP, false>), grid, dim3(N / 4 * W), 0, s, d, static_cast<int>(nx), npix,
-                       th, tl);
+    s(&k_col_fast<N, W, T, COMP, false, ST>, grid, dim3(N / 4 * W), 0, d, static_cast<int>(nx), npix, th, tl);
   }
   template <int N, int RW>
-  void row_launch(const RowDesc<T>* d, int S, hipStream_t s, bool multi, const cx<T>* th, const cx<T>* tl) const {
+  void row_launch(const RowDesc<T, ST>* d, int S, Emit& s, bool multi, const cx<T>* th, const cx<T>* tl) const {
     const dim3 grid(static_cast<unsigned>(ny / RW), S);
     if constexpr (stash_fits<N, RW>())
       if (multi) {
-        hipLaunchKernelGGL((k_row_fast<N, RW, T, COMP, true>), grid, dim3(N / 4 * RW), 0, s, d, npix, th, tl);
+        s(&k_row_fast<N, RW, T, COMP, true, ST>, grid, dim3(N / 4 * RW), 0, d, npix, th, tl);
         return;
       }
-    hipLaunchKernelGGL((k_row_fast<N, RW, T, COMP, false>), grid, dim3(N / 4 * RW), 0, s, d, npix, th, tl);
+    s(&k_row_fast<N, RW, T, COMP, false, ST>, grid, dim3(N / 4 * RW), 0, d, npix, th, tl);
   }
   template <int N>
-  void col_fast_n(const ColDesc<T>* d, int S, hipStream_t s, bool multi) const {
+  void col_fast_n(const ColDesc<T, ST>* d, int S, Emit& s, bool multi) const {
     constexpr int W = fast_w<T, COMP>(N);
     const cx<T>* th = twy->hi.template as<cx<T>>();
     const cx<T>* tl = twy->lo.template as<cx<T>>();
@@ -267,7 +309,7 @@ struct Launcher {
       col_launch<N, 1>(d, S, s, multi, th, tl);
   }
   template <int N>
-  void row_fast_n(const RowDesc<T>* d, int S, hipStream_t s, bool multi) const {
+  void row_fast_n(const RowDesc<T, ST>* d, int S, Emit& s, bool multi) const {
     constexpr int RW = fast_rw<T, COMP>(N);
     const cx<T>* th = twx->hi.template as<cx<T>>();
     const cx<T>* tl = twx->lo.template as<cx<T>>();
@@ -282,16 +324,17 @@ struct Launcher {
 // sub-step of a single-solve lane.
 enum OpKind : int { kOpPair = 0, kOpBornFwd = 1, kOpBornBwd = 2 };
 
-template <typename T>
+template <typename T, typename ST = T>
 struct Op {
   int kind = kOpPair;
-  ColDesc<T> col{};
-  RowDesc<T> row{};
+  ColDesc<T, ST> col{};
+  RowDesc<T, ST> row{};
   long long slab = 0;
   bool bidir = false;
 };
 
-template <typename T, bool COMP>
+// ST: storage of the two streaming planes (X and the product planes); T: the arithmetic.
+template <typename T, bool COMP, typename ST = T>
 class Engine final : public EngineBase {
  public:
   explicit Engine(Spec s) : sp_(std::move(s)) {
@@ -404,7 +447,7 @@ class Engine final : public EngineBase {
   //            e += src_fwd[p+1] + born_fwd[p+1]
   //  backward: e = born_bwd[nz]; for p = nz…1: [R3 → born_fwd[p]; e += R2] [BwdR → src_fwd[p]] BwdT;
   //            e += src_bwd[p−1] + born_bwd[p−1]
-  std::vector<Op<T>> program(int iterations, const std::vector<long long>& keep) {
+  std::vector<Op<T, ST>> program(int iterations, const std::vector<long long>& keep) {
     const long long nz = sp_.nz;
     const int nit = std::max(iterations, 1);
     const bool bidir = iterations > 0;
@@ -418,9 +461,9 @@ class Engine final : public EngineBase {
       if (keepf_.bytes() < keep_.size() * pb) keepf_.alloc(keep_.size() * pb);
       if (bidir && keepb_.bytes() < keep_.size() * pb) keepb_.alloc(keep_.size() * pb);
     }
-    px2<T>* X = X_.as<px2<T>>();
+    px2<ST>* X = X_.as<px2<ST>>();
     px2<T>* E = e_.as<px2<T>>();
-    std::vector<Op<T>> ops;
+    std::vector<Op<T, ST>> ops;
     long long steps = 0;
     auto born_at = [&](long long slab) { return born_ && slab >= 0 && slab < nz && has_host_[slab]; };
     for (int it = 0; it < nit; ++it) {
@@ -432,7 +475,7 @@ class Engine final : public EngineBase {
       };
       px2<T>* trans = iter_t_.as<px2<T>>() + it * npix_;
       {  // forward start: e = field (+ src_fwd[0], never present)
-        Op<T> o;
+        Op<T, ST> o;
         o.row.active = 1;
         o.row.e_in = field_in_.as<px2<T>>();
         o.row.e_out2 = kept(keepf_, 0);
@@ -444,13 +487,13 @@ class Engine final : public EngineBase {
         const Boundary& B = bnd_[p];
         const bool with_r = bidir && !B.rpair.empty();
         if (born_at(p)) {
-          Op<T> b;
+          Op<T, ST> b;
           b.kind = kOpBornFwd;
           b.slab = p;
           b.bidir = bidir;
           ops.push_back(b);
         }
-        Op<T> o;
+        Op<T, ST> o;
         o.col = col_desc(B, true, with_r);
         o.row = row_desc(B, p, with_r);
         o.row.inc = born_at(p) ? born_->increment() : nullptr;
@@ -469,7 +512,7 @@ class Engine final : public EngineBase {
       if (!bidir) break;
       px2<T>* refl = iter_r_.as<px2<T>>() + it * npix_;
       {  // backward start: e = 0 (+ src_bwd[nz], never present) + born_bwd[nz]
-        Op<T> o;
+        Op<T, ST> o;
         o.row.active = 1;
         o.row.dense = born_ ? born_->bwd_plane(nz) : nullptr;
         o.row.e_out2 = kept(keepb_, nz);
@@ -480,12 +523,12 @@ class Engine final : public EngineBase {
       for (long long p = nz; p >= 1; --p) {
         const Boundary& B = bnd_[p - 1];
         if (born_at(p - 1)) {
-          Op<T> b;  // R3 from e at plane p into born_fwd[p], then e += R2 (no mask) and the row FFT
+          Op<T, ST> b;  // R3 from e at plane p into born_fwd[p], then e += R2 (no mask) and the row FFT
           b.kind = kOpBornBwd;
           b.slab = p - 1;
           b.bidir = true;
           ops.push_back(b);
-          Op<T> o;
+          Op<T, ST> o;
           o.row.active = 1;
           o.row.e_in = E;
           o.row.inc = born_->increment();
@@ -493,7 +536,7 @@ class Engine final : public EngineBase {
           ops.push_back(o);
         }
         const bool with_r = !B.rpair.empty();
-        Op<T> o;
+        Op<T, ST> o;
         o.col = col_desc(B, false, with_r);
         o.row = row_desc(B, p - 1, with_r);
         if (B.sp_count > 0) {
@@ -524,25 +567,25 @@ class Engine final : public EngineBase {
               "a batch lane needs solves of equal grid, precision and device");
       es.push_back(e);
     }
-    std::vector<std::vector<Op<T>>> progs(S);
+    std::vector<std::vector<Op<T, ST>>> progs(S);
     size_t G = 0;
     for (int i = 0; i < S; ++i) {
       progs[i] = es[i]->program(iterations, keeps[i]);
       G = std::max(G, progs[i].size());
       if (S > 1)
-        for (const Op<T>& o : progs[i])
+        for (const Op<T, ST>& o : progs[i])
           require(o.kind == kOpPair, "solves with index fluctuations run one per lane");
     }
     // every member's buffers (tables, kept planes) must be ready before the lane starts
     for (Engine* e : es)
       if (e->stream_ != stream) HIP_CHECK(hipStreamSynchronize(e->stream_));
-    std::vector<ColDesc<T>> hcol(G * S);
-    std::vector<RowDesc<T>> hrow(G * S);
+    std::vector<ColDesc<T, ST>> hcol(G * S);
+    std::vector<RowDesc<T, ST>> hrow(G * S);
     std::vector<char> has_col(G, 0), has_row(G, 0), multi_col(G, 0), multi_row(G, 0);
     for (size_t g = 0; g < G; ++g)
       for (int i = 0; i < S; ++i) {
         if (g >= progs[i].size() || progs[i][g].kind != kOpPair) continue;
-        const Op<T>& o = progs[i][g];
+        const Op<T, ST>& o = progs[i][g];
         hcol[g * S + i] = o.col;
         hrow[g * S + i] = o.row;
         has_col[g] |= o.col.in != nullptr;
@@ -550,11 +593,11 @@ class Engine final : public EngineBase {
         multi_col[g] |= o.col.in != nullptr && o.col.nprod > 1;
         multi_row[g] |= o.row.active != 0 && o.row.nt > 1;
       }
-    if (lane_col_.bytes() < hcol.size() * sizeof(ColDesc<T>)) lane_col_.alloc(hcol.size() * sizeof(ColDesc<T>));
-    if (lane_row_.bytes() < hrow.size() * sizeof(RowDesc<T>)) lane_row_.alloc(hrow.size() * sizeof(RowDesc<T>));
-    HIP_CHECK(hipMemcpyAsync(lane_col_.as<void>(), hcol.data(), hcol.size() * sizeof(ColDesc<T>),
+    if (lane_col_.bytes() < hcol.size() * sizeof(ColDesc<T, ST>)) lane_col_.alloc(hcol.size() * sizeof(ColDesc<T, ST>));
+    if (lane_row_.bytes() < hrow.size() * sizeof(RowDesc<T, ST>)) lane_row_.alloc(hrow.size() * sizeof(RowDesc<T, ST>));
+    HIP_CHECK(hipMemcpyAsync(lane_col_.as<void>(), hcol.data(), hcol.size() * sizeof(ColDesc<T, ST>),
                              hipMemcpyHostToDevice, stream));
-    HIP_CHECK(hipMemcpyAsync(lane_row_.as<void>(), hrow.data(), hrow.size() * sizeof(RowDesc<T>),
+    HIP_CHECK(hipMemcpyAsync(lane_row_.as<void>(), hrow.data(), hrow.size() * sizeof(RowDesc<T, ST>),
                              hipMemcpyHostToDevice, stream));
     if (total_sparse_ > 0 || S > 1)
       for (Engine* e : es)
@@ -590,7 +633,7 @@ class Engine final : public EngineBase {
     // candidate on a uniform stretch of kTuneSteps slab steps first (not while timing kernels)
     constexpr size_t kTuneSteps = 32;
     const TuneKey tk{sp_.device, precision_code(), S, sp_.ny, sp_.nx};
-    Launcher<T, COMP> Lc = launch_;
+    Launcher<T, COMP, ST> Lc = launch_;
     for (size_t g = 0; g < G;) {
       int trial = -1;
       const std::pair<int, int> choice = tune_pick(tk, launch_.geo, trial);
@@ -608,15 +651,15 @@ class Engine final : public EngineBase {
       }
       for (; g < end; ++g) {
         if (S == 1 && progs[0][g].kind != kOpPair) {
-          const Op<T>& o = progs[0][g];
+          const Op<T, ST>& o = progs[0][g];
           if (o.kind == kOpBornFwd)
             born_->forward(e_.template as<px2<T>>(), o.slab, o.bidir);
           else
             born_->backward(e_.template as<px2<T>>(), o.slab);
           continue;
         }
-        const ColDesc<T>* dc = lane_col_.as<ColDesc<T>>() + g * S;
-        const RowDesc<T>* dr = lane_row_.as<RowDesc<T>>() + g * S;
+        const ColDesc<T, ST>* dc = lane_col_.as<ColDesc<T, ST>>() + g * S;
+        const RowDesc<T, ST>* dr = lane_row_.as<RowDesc<T, ST>>() + g * S;
         if (has_col[g]) timed(true, [&] { Lc.col(dc, S, stream, multi_col[g] != 0); });
         if (has_row[g]) timed(false, [&] { Lc.row(dr, S, stream, multi_row[g] != 0); });
       }
@@ -649,16 +692,16 @@ class Engine final : public EngineBase {
     require(S >= 1 && chunk >= 1, "lane slots and chunk must be >= 1");
     std::vector<Engine*> slot(S, nullptr);
     std::vector<LaneMember> mem(S);
-    std::vector<std::vector<Op<T>>> prog(S);
+    std::vector<std::vector<Op<T, ST>>> prog(S);
     std::vector<size_t> pc(S, 0);
-    Launcher<T, COMP> L;
+    Launcher<T, COMP, ST> L;
     bool have_launcher = false;
     const size_t n = static_cast<size_t>(chunk) * S;
-    DevBuf dcol(n * sizeof(ColDesc<T>)), drow(n * sizeof(RowDesc<T>));
+    DevBuf dcol(n * sizeof(ColDesc<T, ST>)), drow(n * sizeof(RowDesc<T, ST>));
     // descriptor staging in pinned memory, two sets: a set is rewritten only after the chunk that
     // copied it has completed (at most two chunks are in flight)
-    PinnedBuf hcol[2] = {PinnedBuf(n * sizeof(ColDesc<T>)), PinnedBuf(n * sizeof(ColDesc<T>))};
-    PinnedBuf hrow[2] = {PinnedBuf(n * sizeof(RowDesc<T>)), PinnedBuf(n * sizeof(RowDesc<T>))};
+    PinnedBuf hcol[2] = {PinnedBuf(n * sizeof(ColDesc<T, ST>)), PinnedBuf(n * sizeof(ColDesc<T, ST>))};
+    PinnedBuf hrow[2] = {PinnedBuf(n * sizeof(RowDesc<T, ST>)), PinnedBuf(n * sizeof(RowDesc<T, ST>))};
     std::vector<char> has_col(chunk), has_row(chunk), multi_col(chunk), multi_row(chunk);
     std::map<std::tuple<size_t, int, int>, hipGraphExec_t> graphs;
     bool graphs_ok = true;
@@ -686,7 +729,7 @@ class Engine final : public EngineBase {
               if (have_launcher)
                 require(e->sp_.ny == L.ny && e->sp_.nx == L.nx, "a lane needs solves of equal grid");
               prog[i] = e->program(m.iterations, {});
-              for (const Op<T>& o : prog[i])
+              for (const Op<T, ST>& o : prog[i])
                 require(o.kind == kOpPair, "solves with index fluctuations run one per lane");
               HIP_CHECK(hipStreamSynchronize(e->stream_));
               if (e->total_sparse_ > 0)
@@ -711,10 +754,10 @@ class Engine final : public EngineBase {
         size_t G = 0;
         for (int i = 0; i < S; ++i)
           if (slot[i]) G = std::max(G, std::min<size_t>(chunk, prog[i].size() - pc[i]));
-        auto* hc = hcol[set].template as<ColDesc<T>>();
-        auto* hr = hrow[set].template as<RowDesc<T>>();
-        std::fill(hc, hc + G * S, ColDesc<T>{});
-        std::fill(hr, hr + G * S, RowDesc<T>{});
+        auto* hc = hcol[set].template as<ColDesc<T, ST>>();
+        auto* hr = hrow[set].template as<RowDesc<T, ST>>();
+        std::fill(hc, hc + G * S, ColDesc<T, ST>{});
+        std::fill(hr, hr + G * S, RowDesc<T, ST>{});
         std::fill(has_col.begin(), has_col.end(), 0);
         std::fill(has_row.begin(), has_row.end(), 0);
         std::fill(multi_col.begin(), multi_col.end(), 0);
@@ -722,7 +765,7 @@ class Engine final : public EngineBase {
         for (size_t g = 0; g < G; ++g)
           for (int i = 0; i < S; ++i) {
             if (!slot[i] || pc[i] + g >= prog[i].size()) continue;
-            const Op<T>& o = prog[i][pc[i] + g];
+            const Op<T, ST>& o = prog[i][pc[i] + g];
             hc[g * S + i] = o.col;
             hr[g * S + i] = o.row;
             has_col[g] |= o.col.in != nullptr;
@@ -730,8 +773,8 @@ class Engine final : public EngineBase {
             multi_col[g] |= o.col.in != nullptr && o.col.nprod > 1;
             multi_row[g] |= o.row.active != 0 && o.row.nt > 1;
           }
-        HIP_CHECK(hipMemcpyAsync(dcol.as<void>(), hc, G * S * sizeof(ColDesc<T>), hipMemcpyHostToDevice, stream));
-        HIP_CHECK(hipMemcpyAsync(drow.as<void>(), hr, G * S * sizeof(RowDesc<T>), hipMemcpyHostToDevice, stream));
+        HIP_CHECK(hipMemcpyAsync(dcol.as<void>(), hc, G * S * sizeof(ColDesc<T, ST>), hipMemcpyHostToDevice, stream));
+        HIP_CHECK(hipMemcpyAsync(drow.as<void>(), hr, G * S * sizeof(RowDesc<T, ST>), hipMemcpyHostToDevice, stream));
         // uniform chunk: every step is one plain column and one plain row launch
         bool uniform = true;
         for (size_t g = 0; g < G; ++g)
@@ -741,28 +784,31 @@ class Engine final : public EngineBase {
         const TuneKey tk{slot_device(slot), precision_code(), S, L.ny, L.nx};
         int trial = -1;
         const std::pair<int, int> choice = tune_pick(tk, L.geo, trial);
-        Launcher<T, COMP> Lc = L;
+        Launcher<T, COMP, ST> Lc = L;
         Lc.geo.col_w = choice.first;
         Lc.geo.rw = choice.second;
-        auto launch_chunk = [&](bool check) {
+        auto launch_chunk = [&](Emit& em) {
+          const bool eager = em.graph == nullptr;
           for (size_t g = 0; g < G; ++g) {
             if (has_col[g]) {
-              Lc.col(dcol.as<ColDesc<T>>() + g * S, S, stream, multi_col[g] != 0);
-              if (check) HIP_CHECK(hipGetLastError());
+              Lc.col(dcol.as<ColDesc<T, ST>>() + g * S, S, em, multi_col[g] != 0);
+              if (eager) HIP_CHECK(hipGetLastError());
             }
             if (has_row[g]) {
-              Lc.row(drow.as<RowDesc<T>>() + g * S, S, stream, multi_row[g] != 0);
-              if (check) HIP_CHECK(hipGetLastError());
+              Lc.row(drow.as<RowDesc<T, ST>>() + g * S, S, em, multi_row[g] != 0);
+              if (eager) HIP_CHECK(hipGetLastError());
             }
           }
         };
+        Emit eager;
+        eager.stream = stream;
         if (uniform && trial >= 0) {
           // a timed trial of a tuning candidate
           hipEvent_t a, b;
           HIP_CHECK(hipEventCreate(&a));
           HIP_CHECK(hipEventCreate(&b));
           HIP_CHECK(hipEventRecord(a, stream));
-          launch_chunk(true);
+          launch_chunk(eager);
           HIP_CHECK(hipEventRecord(b, stream));
           HIP_CHECK(hipEventSynchronize(b));
           float ms = 0;
@@ -775,30 +821,38 @@ class Engine final : public EngineBase {
           const auto gk = std::make_tuple(G, Lc.geo.col_w, Lc.geo.rw);
           auto it = graphs.find(gk);
           if (it == graphs.end()) {
-            // relaxed capture: other threads of the process keep allocating and copying meanwhile
-            hipGraph_t graph = nullptr;
+            // the graph is built node by node (no stream capture, so other threads of the
+            // process may allocate, copy and synchronise meanwhile)
+            Emit build;
             hipGraphExec_t exec = nullptr;
-            bool ok = hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed) == hipSuccess;
+            bool ok = hipGraphCreate(&build.graph, 0) == hipSuccess;
             if (ok) {
-              launch_chunk(false);
-              ok = hipStreamEndCapture(stream, &graph) == hipSuccess && graph != nullptr;
-              ok = ok && hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) == hipSuccess;
-              if (graph) (void)hipGraphDestroy(graph);
+              try {
+                launch_chunk(build);
+              } catch (const Error&) {
+                ok = false;
+              }
+              ok = ok && hipGraphInstantiate(&exec, build.graph, nullptr, nullptr, 0) == hipSuccess;
+              (void)hipGraphDestroy(build.graph);
             }
             if (!ok) {
-              // the capture did not hold: clear its error, run this chunk directly and stop
-              // capturing on this lane
+              // no graph for this lane: clear the error, run this chunk directly from now on
               (void)hipGetLastError();
               if (exec) (void)hipGraphExecDestroy(exec);
               graphs_ok = false;
-              launch_chunk(true);
+              ++lane_graph_counters().fallbacks;
+              launch_chunk(eager);
             } else {
               it = graphs.emplace(gk, exec).first;
+              ++lane_graph_counters().built;
             }
           }
-          if (it != graphs.end()) HIP_CHECK(hipGraphLaunch(it->second, stream));
+          if (it != graphs.end()) {
+            HIP_CHECK(hipGraphLaunch(it->second, stream));
+            ++lane_graph_counters().replays;
+          }
         } else {
-          launch_chunk(true);
+          launch_chunk(eager);
         }
         for (int i = 0; i < S; ++i) {
           if (!slot[i]) continue;
@@ -832,7 +886,7 @@ class Engine final : public EngineBase {
     drop_graphs();
   }
 
-  static int precision_code() { return COMP ? 32 : (sizeof(T) == 8 ? 64 : 31); }
+  static int precision_code() { return sizeof(ST) < sizeof(T) ? 48 : COMP ? 32 : (sizeof(T) == 8 ? 64 : 31); }
   static int slot_device(const std::vector<Engine*>& slot) {
     for (const Engine* e : slot)
       if (e) return e->sp_.device;
@@ -853,21 +907,21 @@ class Engine final : public EngineBase {
     HIP_CHECK(hipGetLastError());
   }
 
-  ColDesc<T> col_desc(const Boundary& B, bool forward, bool with_r) {
-    ColDesc<T> c{};
-    c.in = X_.as<px2<T>>();
-    c.out = prod_.as<px2<T>>();
+  ColDesc<T, ST> col_desc(const Boundary& B, bool forward, bool with_r) {
+    ColDesc<T, ST> c{};
+    c.in = X_.as<px2<ST>>();
+    c.out = prod_.as<px2<ST>>();
     c.prods = prods_.as<Prod<T>>() + (forward ? B.fwd_prod : B.bwd_prod);
     c.nprod = static_cast<int>(B.pairs.size()) + (with_r ? static_cast<int>(B.rpair.size()) : 0);
     return c;
   }
 
-  RowDesc<T> row_desc(const Boundary& B, long long q, bool with_r) {
-    RowDesc<T> r{};
+  RowDesc<T, ST> row_desc(const Boundary& B, long long q, bool with_r) {
+    RowDesc<T, ST> r{};
     r.active = 1;
     r.nt = static_cast<int>(B.pairs.size());
     r.nr = with_r ? static_cast<int>(B.rpair.size()) : 0;
-    r.prod = prod_.as<px2<T>>();
+    r.prod = prod_.as<px2<ST>>();
     r.tsel = B.tsel_off >= 0 ? tsel_.as<uint8_t>() + B.tsel_off : nullptr;
     r.mask = mask_.as<T>();
     if (B.sp_count > 0) {
@@ -994,8 +1048,8 @@ class Engine final : public EngineBase {
     const size_t pb = npix_ * sizeof(px2<T>);
     stage_.alloc(npix_ * 2 * sizeof(c128));
     field_in_.alloc(pb);
-    X_.alloc(pb);
-    prod_.alloc(pb * std::max(1, max_prod_));
+    X_.alloc(npix_ * sizeof(px2<ST>));
+    prod_.alloc(npix_ * sizeof(px2<ST>) * std::max(1, max_prod_));
     e_.alloc(pb);
     const size_t sp_bytes = std::max<long long>(1, total_sparse_) * sizeof(px2<T>);
     fwd_vals_.alloc(sp_bytes);
@@ -1042,7 +1096,7 @@ class Engine final : public EngineBase {
               "grid size ny = " + std::to_string(s.ny) + " exceeds the column kernel's LDS budget");
       g.cols2 = w;
       g.ls_col2 = ls_of(2 * w);
-      HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_col_step<T, COMP>),
+      HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_col_step<T, COMP, ST>),
                                     hipFuncAttributeMaxDynamicSharedMemorySize, kLdsMax));
     }
     if (!g.row_fast) {
@@ -1053,7 +1107,7 @@ class Engine final : public EngineBase {
               "grid size nx = " + std::to_string(s.nx) + " exceeds the row kernel's LDS budget");
       g.rows = rw;
       g.ls_row = ls_of(2 * rw);
-      HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_row_step<T, COMP>),
+      HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_row_step<T, COMP, ST>),
                                     hipFuncAttributeMaxDynamicSharedMemorySize, kLdsMax));
     }
   }
@@ -1258,7 +1312,7 @@ class Engine final : public EngineBase {
   std::vector<c128> host_;
   DevBuf kx_, ky_;
   std::shared_ptr<Twiddles<T>> twx_, twy_;
-  Launcher<T, COMP> launch_;
+  Launcher<T, COMP, ST> launch_;
   Geometry geo_;
   std::vector<Boundary> bnd_;
   std::unordered_map<long long, long long> list_off_;

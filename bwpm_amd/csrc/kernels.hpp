@@ -40,21 +40,42 @@ struct Prod {
   const cx<T>* lo;
 };
 
+// Storage of the two streaming planes (X and the product planes) may be narrower than the
+// arithmetic: ST = float with T = double keeps them as complex64 in HBM and computes in FP64.
+template <typename T, typename U>
+__device__ __forceinline__ cx<T> widen(cx<U> z) {
+  return cx<T>{static_cast<T>(z.x), static_cast<T>(z.y)};
+}
+template <typename T, typename U>
+__device__ __forceinline__ px2<T> widen(px2<U> v) {
+  return px2<T>{widen<T>(v.a), widen<T>(v.b)};
+}
+
+// Streaming-plane access: element i (a complex index) of a plane stored as ST, computed as T.
+template <typename T, typename ST>
+__device__ __forceinline__ cx<T> sload(const px2<ST>* p, unsigned i) {
+  return widen<T>(gload(reinterpret_cast<const cx<ST>*>(p), i));
+}
+template <typename T, typename ST>
+__device__ __forceinline__ void sstore(px2<ST>* p, unsigned i, cx<T> v) {
+  gstore(reinterpret_cast<cx<ST>*>(p), i, widen<ST>(v));
+}
+
 // Column-kernel work of one solve for one step.
-template <typename T>
+template <typename T, typename ST = T>
 struct ColDesc {
-  const px2<T>* in;      // X plane; nullptr: no column work for this solve at this step
-  px2<T>* out;           // product planes, `plane` pixels apart
+  const px2<ST>* in;     // X plane; nullptr: no column work for this solve at this step
+  px2<ST>* out;          // product planes, `plane` pixels apart
   const Prod<T>* prods;  // nprod descriptors (T kinds first)
   int nprod;
 };
 
 // Row-kernel work of one solve for one step.
-template <typename T>
+template <typename T, typename ST = T>
 struct RowDesc {
   int active;                 // 0: no row work at this step
   int nt, nr;                 // T products (0: start a pass from e_in), R products
-  const px2<T>* prod;         // product planes
+  const px2<ST>* prod;        // product planes
   const uint8_t* tsel;        // T pair of each pixel (nt > 1)
   const T* mask;              // transmission mask or nullptr
   const px2<T>* e_in;         // nt == 0: start plane (nullptr: zero)
@@ -67,7 +88,7 @@ struct RowDesc {
   const px2<T>* dense;        // dense Born plane added (nullptr: none)
   px2<T>* e_out;              // real-space result (nullptr: not written)
   px2<T>* e_out2;             // second copy (kept plane)
-  px2<T>* x_out;              // row FFT of the result for the next step (nullptr: none)
+  px2<ST>* x_out;             // row FFT of the result for the next step (nullptr: none)
 };
 
 template <typename T, bool COMP>
@@ -92,9 +113,9 @@ struct ColGeom {
   FftPlan plan;  // length ny
 };
 
-template <typename T, bool COMP>
-__global__ __launch_bounds__(kThreads) void k_col_step(const ColDesc<T>* __restrict__ descs, ColGeom g) {
-  const ColDesc<T> d = descs[blockIdx.y];
+template <typename T, bool COMP, typename ST = T>
+__global__ __launch_bounds__(kThreads) void k_col_step(const ColDesc<T, ST>* __restrict__ descs, ColGeom g) {
+  const ColDesc<T, ST> d = descs[blockIdx.y];
   if (!d.in) return;
   cx<T>* A = lds<T>();
   const int W = g.cols, ny = g.ny, nx = g.nx, ls = g.ls;
@@ -104,19 +125,19 @@ __global__ __launch_bounds__(kThreads) void k_col_step(const ColDesc<T>* __restr
   for (int i = threadIdx.x; i < count; i += kThreads) {
     const int y = i / W, c = i - y * W, x = x0 + c;
     px2<T> v{};
-    if (x < nx) v = d.in[static_cast<long long>(y) * nx + x];
+    if (x < nx) v = widen<T>(d.in[static_cast<long long>(y) * nx + x]);
     A[y * ls + c] = v.a;
     A[y * ls + W + c] = v.b;
   }
   __syncthreads();
   fft_lines<COMP, false>(A, ls, lines, g.plan);
   // several products: keep the spectrum in this block's own positions of X and reload it per
-  // product, so one LDS buffer suffices
-  px2<T>* xio = const_cast<px2<T>*>(d.in);
+  // product, so one LDS buffer suffices (in the storage precision of X)
+  px2<ST>* xio = const_cast<px2<ST>*>(d.in);
   if (d.nprod > 1) {
     for (int i = threadIdx.x; i < count; i += kThreads) {
       const int y = i / W, c = i - y * W, x = x0 + c;
-      if (x < nx) xio[static_cast<long long>(y) * nx + x] = px2<T>{A[y * ls + c], A[y * ls + W + c]};
+      if (x < nx) xio[static_cast<long long>(y) * nx + x] = widen<ST>(px2<T>{A[y * ls + c], A[y * ls + W + c]});
     }
     __syncthreads();
   }
@@ -126,17 +147,17 @@ __global__ __launch_bounds__(kThreads) void k_col_step(const ColDesc<T>* __restr
       const int y = i / W, c = i - y * W, x = x0 + c;
       if (x >= nx) continue;
       const long long k = static_cast<long long>(y) * nx + x;
-      const px2<T> e = o == 0 ? px2<T>{A[y * ls + c], A[y * ls + W + c]} : xio[k];
+      const px2<T> e = o == 0 ? px2<T>{A[y * ls + c], A[y * ls + W + c]} : widen<T>(xio[k]);
       const px2<T> r = apply_prod<T, COMP>(pd, k, e);
       A[y * ls + c] = r.a;
       A[y * ls + W + c] = r.b;
     }
     __syncthreads();
     fft_lines<COMP, true>(A, ls, lines, g.plan);
-    px2<T>* out = d.out + o * g.plane;
+    px2<ST>* out = d.out + o * g.plane;
     for (int i = threadIdx.x; i < count; i += kThreads) {
       const int y = i / W, c = i - y * W, x = x0 + c;
-      if (x < nx) out[static_cast<long long>(y) * nx + x] = px2<T>{A[y * ls + c], A[y * ls + W + c]};
+      if (x < nx) out[static_cast<long long>(y) * nx + x] = widen<ST>(px2<T>{A[y * ls + c], A[y * ls + W + c]});
     }
     __syncthreads();
   }
@@ -148,21 +169,21 @@ struct RowGeom {
   FftPlan plan;  // length nx
 };
 
-template <typename T>
-__device__ __forceinline__ void load_rows(cx<T>* buf, const px2<T>* src, int y0, int nrow, int nx, int rows,
+template <typename T, typename U>
+__device__ __forceinline__ void load_rows(cx<T>* buf, const px2<U>* src, int y0, int nrow, int nx, int rows,
                                           int ls) {
   const int count = nrow * nx;
   for (int i = threadIdx.x; i < count; i += kThreads) {
     const int r = i / nx, x = i - r * nx;
-    const px2<T> v = src[static_cast<long long>(y0 + r) * nx + x];
+    const px2<T> v = widen<T>(src[static_cast<long long>(y0 + r) * nx + x]);
     buf[x * ls + r] = v.a;
     buf[x * ls + rows + r] = v.b;
   }
 }
 
-template <typename T, bool COMP>
-__global__ __launch_bounds__(kThreads) void k_row_step(const RowDesc<T>* __restrict__ descs, RowGeom g) {
-  const RowDesc<T> a = descs[blockIdx.y];
+template <typename T, bool COMP, typename ST = T>
+__global__ __launch_bounds__(kThreads) void k_row_step(const RowDesc<T, ST>* __restrict__ descs, RowGeom g) {
+  const RowDesc<T, ST> a = descs[blockIdx.y];
   if (!a.active) return;
   const int nx = g.nx, rows = g.rows, ls = g.ls;
   const int y0 = blockIdx.x * rows;
@@ -281,7 +302,7 @@ __global__ __launch_bounds__(kThreads) void k_row_step(const RowDesc<T>* __restr
     fft_lines<COMP, false>(acc, ls, lines, g.plan);
     for (int i = threadIdx.x; i < count; i += kThreads) {
       const int r = i / nx, x = i - r * nx;
-      a.x_out[static_cast<long long>(y0 + r) * nx + x] = px2<T>{acc[x * ls + r], acc[x * ls + rows + r]};
+      a.x_out[static_cast<long long>(y0 + r) * nx + x] = widen<ST>(px2<T>{acc[x * ls + r], acc[x * ls + rows + r]});
     }
   }
 }

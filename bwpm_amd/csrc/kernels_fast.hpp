@@ -103,9 +103,10 @@ __device__ __forceinline__ cx<T> apply_prod1(const Prod<T>& pd, unsigned k, int 
 // index (t·W + c)·2 + comp, so a wavefront reads W·sizeof(px2) contiguous bytes per row).
 // MULTI (launched for steps with several products): each thread keeps its eight spectrum values in
 // an LDS stash across the products instead of writing them back to X and reloading them. The
-// arithmetic is the same, so both forms give the same bits.
-template <int N, int W, typename T, bool COMP, bool MULTI>
-__global__ __launch_bounds__(N / 4 * W) void k_col_fast(const ColDesc<T>* __restrict__ descs, int nx,
+// arithmetic is the same, so both forms give the same bits where X is stored in the arithmetic
+// precision (ST = T); with narrower storage only the LDS stash keeps the spectrum unrounded.
+template <int N, int W, typename T, bool COMP, bool MULTI, typename ST = T>
+__global__ __launch_bounds__(N / 4 * W) void k_col_fast(const ColDesc<T, ST>* __restrict__ descs, int nx,
                                                         long long plane, const cx<T>* __restrict__ g_twh,
                                                         const cx<T>* __restrict__ g_twl) {
   constexpr int TT = N / 8;
@@ -115,7 +116,7 @@ __global__ __launch_bounds__(N / 4 * W) void k_col_fast(const ColDesc<T>* __rest
   __shared__ cx<T> tw_hi_s[TWL ? reg_tw_len<N>() : 1];
   __shared__ cx<T> tw_lo_s[TWL && COMP ? reg_tw_len<N>() : 1];
   __shared__ cx<T> stash[MULTI ? 2 * W * (N + 1) : 1];
-  const ColDesc<T> d = descs[blockIdx.y];
+  const ColDesc<T, ST> d = descs[blockIdx.y];
   if (!d.in) return;
   if constexpr (TWL) stage_twiddles<N, COMP>(tw_hi_s, tw_lo_s, g_twh, g_twl);
   const cx<T>* tw_hi = TWL ? tw_hi_s : g_twh;
@@ -125,14 +126,14 @@ __global__ __launch_bounds__(N / 4 * W) void k_col_fast(const ColDesc<T>* __rest
   const int c = q % W, t = q / W;
   const unsigned x = xcd_group(blockIdx.x, gridDim.x) * W + c;
   cx<T>* line = sh + (2 * c + comp) * LS;
-  cx<T>* xio = reinterpret_cast<cx<T>*>(const_cast<px2<T>*>(d.in));
+  px2<ST>* xio = const_cast<px2<ST>*>(d.in);
   // pixel index of row t + T·m and complex index of this lane's component (planes < 2^31 values)
   auto pix = [&](int m) { return static_cast<unsigned>(t + TT * m) * static_cast<unsigned>(nx) + x; };
   auto at = [&](int m) { return 2u * pix(m) + static_cast<unsigned>(comp); };
   const int nprod = uniform(d.nprod);
   cx<T> s[8];
 #pragma unroll
-  for (int m = 0; m < 8; ++m) s[m] = gload(xio, at(m));
+  for (int m = 0; m < 8; ++m) s[m] = sload<T>(xio, at(m));
   reg_fft<N, COMP, false>(s, line, t, tw_hi, tw_lo);
   // several products: keep the spectrum (in the LDS stash, or in the thread's own positions of X);
   // stash entry m of this lane is own[m·T], its component line is N + 1 entries long
@@ -144,7 +145,7 @@ __global__ __launch_bounds__(N / 4 * W) void k_col_fast(const ColDesc<T>* __rest
       if constexpr (MULTI)
         own[m * TT] = s[m];
       else
-        gstore(xio, at(m), s[m]);
+        sstore<T>(xio, at(m), s[m]);
     }
   }
   for (int o = 0; o < nprod; ++o) {
@@ -154,7 +155,7 @@ __global__ __launch_bounds__(N / 4 * W) void k_col_fast(const ColDesc<T>* __rest
         if constexpr (MULTI)
           s[m] = own[m * TT];
         else
-          s[m] = gload(xio, at(m));
+          s[m] = sload<T>(xio, at(m));
       }
     }
     const Prod<T>* pp = d.prods + o;
@@ -162,17 +163,17 @@ __global__ __launch_bounds__(N / 4 * W) void k_col_fast(const ColDesc<T>* __rest
 #pragma unroll
     for (int m = 0; m < 8; ++m) s[m] = apply_prod1<T, COMP>(pd, pix(m), comp, s[m]);
     reg_fft<N, COMP, true>(s, line, t, tw_hi, tw_lo);
-    cx<T>* out = reinterpret_cast<cx<T>*>(d.out + o * plane);
+    px2<ST>* out = d.out + o * plane;
 #pragma unroll
-    for (int m = 0; m < 8; ++m) gstore(out, at(m), s[m]);
+    for (int m = 0; m < 8; ++m) sstore<T>(out, at(m), s[m]);
   }
 }
 
 // Row kernel: RW rows per workgroup, N/8 threads per row and component (thread index
 // (r·N/8 + t)·2 + comp). MULTI (launched for steps with several transmitted products): the
 // selected pixels of each product are merged in an LDS stash instead of product plane 0.
-template <int N, int RW, typename T, bool COMP, bool MULTI>
-__global__ __launch_bounds__(N / 4 * RW) void k_row_fast(const RowDesc<T>* __restrict__ descs, long long plane,
+template <int N, int RW, typename T, bool COMP, bool MULTI, typename ST = T>
+__global__ __launch_bounds__(N / 4 * RW) void k_row_fast(const RowDesc<T, ST>* __restrict__ descs, long long plane,
                                                          const cx<T>* __restrict__ g_twh,
                                                          const cx<T>* __restrict__ g_twl) {
   constexpr int TT = N / 8;
@@ -182,7 +183,7 @@ __global__ __launch_bounds__(N / 4 * RW) void k_row_fast(const RowDesc<T>* __res
   __shared__ cx<T> tw_hi_s[TWL ? reg_tw_len<N>() : 1];
   __shared__ cx<T> tw_lo_s[TWL && COMP ? reg_tw_len<N>() : 1];
   __shared__ cx<T> stash[MULTI ? 2 * RW * (N + 1) : 1];
-  const RowDesc<T> a = descs[blockIdx.y];
+  const RowDesc<T, ST> a = descs[blockIdx.y];
   if (!a.active) return;
   if constexpr (TWL) stage_twiddles<N, COMP>(tw_hi_s, tw_lo_s, g_twh, g_twl);
   const cx<T>* tw_hi = TWL ? tw_hi_s : g_twh;
@@ -203,9 +204,9 @@ __global__ __launch_bounds__(N / 4 * RW) void k_row_fast(const RowDesc<T>* __res
     const int e0 = gload1(a.rowptr, y), e1 = gload1(a.rowptr, y + 1);
     cx<T>* store = reinterpret_cast<cx<T>*>(a.sp_store);
     for (int o = 0; o < a.nr; ++o) {
-      const cx<T>* src = cplane(a.prod + (a.nt + o) * plane);
+      const px2<ST>* src = a.prod + (a.nt + o) * plane;
 #pragma unroll
-      for (int m = 0; m < 8; ++m) acc[m] = gload(src, at(m));
+      for (int m = 0; m < 8; ++m) acc[m] = sload<T>(src, at(m));
       reg_fft<N, COMP, true>(acc, line, t, tw_hi, tw_lo);
 #pragma unroll
       for (int m = 0; m < 8; ++m) line[lpad(t + TT * m)] = acc[m];
@@ -222,19 +223,18 @@ __global__ __launch_bounds__(N / 4 * RW) void k_row_fast(const RowDesc<T>* __res
 #pragma unroll
     for (int m = 0; m < 8; ++m) acc[m] = ein ? gload(ein, at(m)) : cx<T>{};
   } else if (a.nt == 1) {
-    const cx<T>* src = cplane(a.prod);
 #pragma unroll
-    for (int m = 0; m < 8; ++m) acc[m] = gload(src, at(m));
+    for (int m = 0; m < 8; ++m) acc[m] = sload<T>(a.prod, at(m));
     reg_fft<N, COMP, true>(acc, line, t, tw_hi, tw_lo);
   } else {
     // the selected pixels are collected in the LDS stash or in product plane 0
-    cx<T>* merged = reinterpret_cast<cx<T>*>(const_cast<px2<T>*>(a.prod));
+    px2<ST>* merged = const_cast<px2<ST>*>(a.prod);
     cx<T>* own = stash;  // stash entry m of this lane is own[m·T]
     if constexpr (MULTI) own = stash + (2 * r + comp) * (N + 1) + t;
     for (int o = 0; o < a.nt; ++o) {
-      const cx<T>* src = cplane(a.prod + o * plane);
+      const px2<ST>* src = a.prod + o * plane;
 #pragma unroll
-      for (int m = 0; m < 8; ++m) acc[m] = gload(src, at(m));
+      for (int m = 0; m < 8; ++m) acc[m] = sload<T>(src, at(m));
       reg_fft<N, COMP, true>(acc, line, t, tw_hi, tw_lo);
 #pragma unroll
       for (int m = 0; m < 8; ++m)
@@ -242,7 +242,7 @@ __global__ __launch_bounds__(N / 4 * RW) void k_row_fast(const RowDesc<T>* __res
           if constexpr (MULTI)
             own[m * TT] = acc[m];
           else
-            gstore(merged, at(m), acc[m]);
+            sstore<T>(merged, at(m), acc[m]);
         }
     }
 #pragma unroll
@@ -250,7 +250,7 @@ __global__ __launch_bounds__(N / 4 * RW) void k_row_fast(const RowDesc<T>* __res
       if constexpr (MULTI)
         acc[m] = own[m * TT];
       else
-        acc[m] = gload(merged, at(m));
+        acc[m] = sload<T>(merged, at(m));
     }
   }
 
@@ -303,9 +303,8 @@ __global__ __launch_bounds__(N / 4 * RW) void k_row_fast(const RowDesc<T>* __res
   // 5. row FFT for the next slab step
   if (a.x_out) {
     reg_fft<N, COMP, false>(acc, line, t, tw_hi, tw_lo);
-    cx<T>* o = reinterpret_cast<cx<T>*>(a.x_out);
 #pragma unroll
-    for (int m = 0; m < 8; ++m) gstore(o, at(m), acc[m]);
+    for (int m = 0; m < 8; ++m) sstore<T>(a.x_out, at(m), acc[m]);
   }
 }
 

@@ -31,6 +31,9 @@ extern "C" {
 
 /* precision of the bulk fields */
 #define BWPM_FP64 64       /* complex128 throughout (parity mode against the FP64 reference) */
+#define BWPM_MIXED 48      /* complex64 storage of the two streaming planes of every slab step,
+                              FP64 arithmetic and every other buffer FP64; solves with index
+                              fluctuations keep the planes FP64 too */
 #define BWPM_FP32 32       /* complex64 bulk, FP64 tables, compensated twiddles/operators */
 #define BWPM_FP32_PLAIN 31 /* complex64 bulk without compensation (diagnostic) */
 
@@ -195,6 +198,10 @@ int bwpm_queue_result(bwpm_queue* q, int64_t ticket, bwpm_result** out);
  * admitted requests, device bytes in use, the budget and the peak in use. */
 #define BWPM_QUEUE_NSTATS 11
 int bwpm_queue_stats(bwpm_queue* q, double* out);
+/* Process-wide counts of the graphs of continuous lanes: graphs built (node by node, no stream
+ * capture), graph replays, and lanes that ran their chunks as direct launches because a graph
+ * could not be built (no counterpart in the reference). */
+int bwpm_lane_graph_stats(int64_t* built, int64_t* replays, int64_t* fallbacks);
 
 /* ---- analysis reductions over planes (the step after the path) ---------------------------------
  *

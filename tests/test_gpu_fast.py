@@ -45,7 +45,7 @@ def _solver(a: dict, precision: str):
 
 @pytest.mark.parametrize("shape", [(64, 64), (128, 128), (256, 128), (512, 512), (512, 12), (12, 256), (4096, 1),
                                    (2048, 2), (3072, 1), (384, 96)])
-@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("precision", ["fp64", "mixed", "fp32"])
 def test_register_kernels_match_oracle(shape: tuple[int, int], precision: str) -> None:
     ny, nx = shape
     nz = 9
@@ -67,10 +67,10 @@ def test_register_kernels_match_oracle(shape: tuple[int, int], precision: str) -
             # input, below FP32's range): the FP32 result must be as small
             assert np.max(np.abs(got)) < 1e-6 * scale
         else:
-            assert rel_l2(got, want) < 1e-5
+            assert rel_l2(got, want) < (1e-6 if precision == "mixed" else 1e-5)
 
 
-@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+@pytest.mark.parametrize("precision", ["fp32", "mixed", "fp64"])
 def test_batch_equals_alone_bitwise(precision: str) -> None:
     """Solves of two grid sizes, uneven depths and different geometries: the batched lanes give
     each solve exactly its result alone."""

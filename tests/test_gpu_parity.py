@@ -17,6 +17,9 @@ from oracle import bwpm_oracle as O
 pytestmark = pytest.mark.gpu
 
 L2_FP32 = 1e-5
+# complex64 streaming planes with FP64 arithmetic: only the storage rounding of two planes per step
+# (weak reflected planes of the random silver volumes see ~1.3e-6)
+L2 = {"mixed": 5e-6, "fp32": L2_FP32}
 
 
 def _core():
@@ -94,7 +97,7 @@ def test_longitudinal_wavenumber_matches_oracle() -> None:
     assert isinstance(c.longitudinal_wavenumber(1.5, k0, 0.0), complex)
 
 
-@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("precision", ["fp64", "mixed", "fp32"])
 def test_fpwpm2d_interface(precision: str) -> None:
     d = load_golden("fpwpm2d_interface")
     a = fixture_2d_args(d)
@@ -104,11 +107,11 @@ def test_fpwpm2d_interface(precision: str) -> None:
     if precision == "fp64":
         _close64(planes, d["expected"]["planes"])
     else:
-        assert rel_l2(planes, d["expected"]["planes"]) < L2_FP32
+        assert rel_l2(planes, d["expected"]["planes"]) < L2[precision]
     assert out["backward"] == {} and out["stored_values"] == 0
 
 
-@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("precision", ["fp64", "mixed", "fp32"])
 def test_bwpm2d_tilted_silver(precision: str) -> None:
     d = load_golden("bwpm2d_tilted_silver")
     a = fixture_2d_args(d)
@@ -125,12 +128,12 @@ def test_bwpm2d_tilted_silver(precision: str) -> None:
         _close64(bwd, d["expected"]["backward"])
         assert np.allclose(res, d["expected"]["energy_residual"], rtol=1e-10, atol=1e-12)
     else:
-        assert rel_l2(fwd, d["expected"]["forward"]) < L2_FP32
-        assert rel_l2(bwd, d["expected"]["backward"]) < L2_FP32
+        assert rel_l2(fwd, d["expected"]["forward"]) < L2[precision]
+        assert rel_l2(bwd, d["expected"]["backward"]) < L2[precision]
         assert np.allclose(res, d["expected"]["energy_residual"], atol=1e-4)
 
 
-@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("precision", ["fp64", "mixed", "fp32"])
 def test_bwpm2d_slab_exact(precision: str) -> None:
     d = load_golden("bwpm2d_slab_exact")
     a = fixture_2d_args(d)
@@ -142,11 +145,11 @@ def test_bwpm2d_slab_exact(precision: str) -> None:
         rr = complex(np.mean(r[:, 0] / d["inputs"]["field"][:, 0]))
         assert abs(rr - d["reference"]["r_te_airy"]) < d["reference_tolerance"]["abs"]
     else:
-        assert rel_l2(t, d["expected"]["transmitted"]) < L2_FP32
-        assert rel_l2(r, d["expected"]["reflected"]) < L2_FP32
+        assert rel_l2(t, d["expected"]["transmitted"]) < L2[precision]
+        assert rel_l2(r, d["expected"]["reflected"]) < L2[precision]
 
 
-@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("precision", ["fp64", "mixed", "fp32"])
 def test_bwpm3d_tilted(precision: str) -> None:
     d = load_golden("bwpm3d_tilted")
     a = fixture_3d_args(d)
@@ -164,11 +167,11 @@ def test_bwpm3d_tilted(precision: str) -> None:
     else:
         for got, want in ((out["transmitted"], e["transmitted"]), (out["reflected"], e["reflected"]),
                           (fwd, e["forward_planes"]), (bwd, e["backward_planes"])):
-            assert rel_l2(got, want) < L2_FP32
+            assert rel_l2(got, want) < L2[precision]
 
 
 @pytest.mark.parametrize("seed", [1, 2, 3])
-@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("precision", ["fp64", "mixed", "fp32"])
 def test_random_volumes_equal_oracle(seed: int, precision: str) -> None:
     """Random volumes with four materials incl. silver, odd and even sizes, dx ≠ dy (grazing
     points), 2 iterations; the reference's core-vs-Python test (test_core_solver.py:99-127)."""
@@ -187,12 +190,12 @@ def test_random_volumes_equal_oracle(seed: int, precision: str) -> None:
         if precision == "fp64":
             assert np.max(np.abs(got - want)) < 1e-10 * scale
         else:
-            assert rel_l2(got, want) < L2_FP32
+            assert rel_l2(got, want) < L2[precision]
 
 
 # ---- Born scattering at index fluctuations -------------------------------------------------------
 
-@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("precision", ["fp64", "mixed", "fp32"])
 def test_fpwpm2d_fluctuation(precision: str) -> None:
     d = load_golden("fpwpm2d_fluctuation")
     a = fixture_2d_args(d)
@@ -202,11 +205,11 @@ def test_fpwpm2d_fluctuation(precision: str) -> None:
     if precision == "fp64":
         _close64(planes, d["expected"]["planes"])
     else:
-        assert rel_l2(planes, d["expected"]["planes"]) < L2_FP32
+        assert rel_l2(planes, d["expected"]["planes"]) < L2[precision]
     assert out["gmres_iterations"] > 0
 
 
-@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("precision", ["fp64", "mixed", "fp32"])
 def test_bwpm3d_fluctuation(precision: str) -> None:
     d = load_golden("bwpm3d_fluctuation")
     a = fixture_3d_args(d)
@@ -221,7 +224,7 @@ def test_bwpm3d_fluctuation(precision: str) -> None:
         if precision == "fp64":
             _close64(got, want)
         else:
-            assert rel_l2(got, want) < L2_FP32
+            assert rel_l2(got, want) < L2[precision]
 
 
 def _fluct_volume(seed: int, back: bool):
@@ -253,7 +256,7 @@ def _fluct_volume(seed: int, back: bool):
 
 
 @pytest.mark.parametrize(("seed", "back"), [(1, False), (2, False), (3, True), (4, True)])
-@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("precision", ["fp64", "mixed", "fp32"])
 def test_fluctuations_equal_oracle(seed: int, back: bool, precision: str) -> None:
     a, its = _fluct_volume(seed, back)
     nz = a["labels"].shape[0]
@@ -269,7 +272,7 @@ def test_fluctuations_equal_oracle(seed: int, back: bool, precision: str) -> Non
         if precision == "fp64":
             assert np.max(np.abs(got - want)) < 1e-10 * scale
         else:
-            assert rel_l2(got, want) < L2_FP32
+            assert rel_l2(got, want) < L2[precision]
 
 
 def test_born_step_conserves_flux_and_absorbs_exactly() -> None:

@@ -127,6 +127,7 @@ def test_replayed_lane_graphs_match_eager_solves() -> None:
         p.index_before = 1.33 + 0j
         reqs.append(p)
     other = workloads.tilted_3d(n=96, nz=40, iterations=1)
+    before = _core.lane_graph_stats()
     with serve.Server(slots=2, chunk=16) as srv:
         tickets = [srv.submit(p) for p in reqs]
         t_other = srv.submit(other)
@@ -134,3 +135,6 @@ def test_replayed_lane_graphs_match_eager_solves() -> None:
         _same(srv.result(t_other, timeout=120), _alone(other, "fp64"))
     for p, res in zip(reqs, results):
         _same(res, _alone(p, "fp64"))
+    after = _core.lane_graph_stats()
+    assert after["built"] > before["built"] and after["replays"] > before["replays"] + 6
+    assert after["fallbacks"] == before["fallbacks"]
