@@ -131,6 +131,34 @@ def precision_mode(probs: list, precision: str, iterations: int, ref: dict) -> d
             "rel_l2_reflected": rel_l2(got["reflected"], ref["reflected"])}
 
 
+def small_lane(count: int = 64, n: int = 128, nz: int = 256, iterations: int = 2) -> dict:
+    """Batched over plain on small grids: `count` tilted-interface solves of n² × nz in one lane
+    against the same solves one at a time (one solve per launch, synchronised per solve)."""
+    from bwpm_amd import _core, workloads
+
+    probs = [workloads.tilted_3d(n=n, nz=nz, iterations=iterations, polarisation="xy"[i % 2]) for i in range(count)]
+    solvers = []
+    try:
+        for p in probs:
+            s = _core.GpuSolver(*p.solver_args())
+            s.upload(p.field)
+            solvers.append(s)
+        _core.run_batch(solvers, iterations, max_lane=count)
+        t0 = time.perf_counter()
+        _core.run_batch(solvers, iterations, max_lane=count)
+        batched = sum(p.updates for p in probs) / (time.perf_counter() - t0)
+        solvers[0].run_resident(iterations)
+        t0 = time.perf_counter()
+        for s in solvers:
+            s.run_resident(iterations)
+        plain = sum(p.updates for p in probs) / (time.perf_counter() - t0)
+    finally:
+        for s in solvers:
+            s.close()
+    return {"workload": f"{count} tilted-interface solves {n}x{n}x{nz}, {iterations} iterations, FP64",
+            "batched_updates_per_s": batched, "plain_updates_per_s": plain, "batched_over_plain": batched / plain}
+
+
 def uneven_batch(count: int) -> dict:
     """Config 4: `count` uneven Born solves (128² … 512², 100 … 1000 slabs, per-solve λ, angle, σ and
     seed; δn generated on the device) submitted at once to the request queue, complex64."""
@@ -333,6 +361,8 @@ def main() -> int:
     for s in solvers:
         s.close()
     solvers = []
+    if checks is not None:
+        checks["small_grids"] = small_lane()
     modes = None
     if checks is not None and args.precision == "fp64" and not args.no_modes:
         # the same lane with complex64 streaming planes (FP64 arithmetic) and in FP32 arithmetic

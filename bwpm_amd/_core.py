@@ -19,7 +19,7 @@ from typing import Any
 
 import numpy as np
 
-from bwpm_amd.specs import FluctuationSynthesis, Uniform
+from bwpm_amd.specs import FluctuationSynthesis, Geometry, Uniform
 
 _LIB_PATH = Path(os.environ.get("BWPM_HIP_LIB", Path(__file__).resolve().parent / "lib" / "libbwpm_hip.so"))
 if not _LIB_PATH.exists():
@@ -102,6 +102,11 @@ _a_refl = _sig("bwpm_reference_reflectance", _p, _p, _int, _d, _d, ctypes.POINTE
 _a_axial = _sig("bwpm_axial_profile", _p, _p, _int, _d, _p, _i64, _d, _p)
 _a_strehl = _sig("bwpm_strehl_profile", _p, _p, _int, _d, _d, _p, _i64, _p)
 _a_strehl_modes = _sig("bwpm_strehl_modes", _p, _p, _p, _i64, _d, _p, _i64, _p)
+_geom_labels = _sig("bwpm_geometry_labels", _i64, _i64, _d, _d, _d, _i64, _d, _int, _p, _i64, _i64, _p)
+_create_geom = _sig("bwpm_solver_create_geometry", _i64, _i64, _d, _d, _d, _d, _p, _i64, _i64, _d, _int, _p, _i64, _d,
+                    _d, _p, _p, _d, _d, ctypes.c_uint64, _int, _int, ctypes.POINTER(_p))
+_q_submit_geom = _sig("bwpm_queue_submit_geometry", _p, _i64, _i64, _d, _d, _d, _d, _p, _i64, _i64, _d, _int, _p, _i64,
+                      _d, _d, _p, _p, _d, _d, ctypes.c_uint64, _int, _p, _int, ctypes.POINTER(_i64))
 _graph_stats = _sig("bwpm_lane_graph_stats", ctypes.POINTER(_i64), ctypes.POINTER(_i64), ctypes.POINTER(_i64))
 _solver_plane = _sig("bwpm_solver_plane", _p, _int, _i64, ctypes.POINTER(_p), ctypes.POINTER(_int))
 _batch = _sig("bwpm_batch_run", _p, _i64, _int, _int, _int, ctypes.POINTER(_d), ctypes.POINTER(_d),
@@ -117,6 +122,15 @@ def _check(status: int) -> None:
     if status == EINVAL:
         raise ValueError(msg)
     raise RuntimeError(msg or f"bwpm_hip error {status}")
+
+
+def geometry_labels(g: Geometry, ny: int, nx: int, dy: float, dx: float, dz: float, nlabels: int) -> np.ndarray:
+    """The (nz, ny, nx) int32 label volume the library makes from a geometry (host code, no GPU)."""
+    out = np.empty((int(g.nz), int(ny), int(nx)), dtype=np.int32)
+    pk = g.packed()
+    _check(_geom_labels(int(ny), int(nx), float(dy), float(dx), float(dz), int(g.nz), float(g.z0), int(g.background),
+                        _ptr(pk) if pk.size else None, pk.shape[0], int(nlabels), _ptr(out)))
+    return out
 
 
 def lane_graph_stats() -> dict[str, int]:
@@ -197,9 +211,9 @@ def _solver_arrays(ny: int, nx: int, indices: Any, labels: Any, mask: Any, fluct
     """Checked, contiguous copies of a solve's arrays (shapes as cpp/python/bindings.cpp:100-115)."""
     if precision not in PRECISIONS:
         raise ValueError(f"precision must be one of {sorted(PRECISIONS)}")
-    if isinstance(labels, Uniform):
+    if isinstance(labels, (Uniform, Geometry)):
         if int(labels.nz) < 1:
-            raise ValueError("a uniform volume needs nz >= 1")
+            raise ValueError("a uniform or geometry volume needs nz >= 1")
         lab = labels
     else:
         lab = np.asarray(labels)
@@ -216,7 +230,7 @@ def _solver_arrays(ny: int, nx: int, indices: Any, labels: Any, mask: Any, fluct
             raise ValueError("sigma and corr_mm must be >= 0")
         f = fluctuation
     elif fluctuation is not None:
-        if isinstance(lab, Uniform):
+        if isinstance(lab, (Uniform, Geometry)):
             raise ValueError("a fluctuation array needs a label array")
         f = np.ascontiguousarray(np.asarray(fluctuation, dtype=np.complex128))
         if f.shape != lab.shape:
@@ -224,6 +238,8 @@ def _solver_arrays(ny: int, nx: int, indices: Any, labels: Any, mask: Any, fluct
     if isinstance(lab, Uniform):
         if not isinstance(f, FluctuationSynthesis):
             raise ValueError("a uniform label volume is given with a synthesised fluctuation")
+    elif isinstance(lab, Geometry):
+        pass
     elif lab.shape[1:] != (int(ny), int(nx)):
         raise ValueError("labels must have shape (nz, ny, nx)")
     lb = None
@@ -293,10 +309,19 @@ class GpuSolver:
         self._h = _p(None)
         idx, lab, m, f, lb = _solver_arrays(ny, nx, indices, labels, mask, fluctuation, labels_before, precision)
         ib = complex(index_before)
-        nz = int(lab.nz) if isinstance(lab, Uniform) else int(lab.shape[0])
+        nz = int(lab.nz) if isinstance(lab, (Uniform, Geometry)) else int(lab.shape[0])
         self.ny, self.nx, self.nz = int(ny), int(nx), nz
         self.precision = precision
         self.device = int(device)
+        if isinstance(lab, Geometry):
+            pk = lab.packed()
+            sig = f if isinstance(f, FluctuationSynthesis) else FluctuationSynthesis(0.0, 0.0, 0)
+            _check(_create_geom(int(ny), int(nx), float(dy), float(dx), float(dz), float(k0), _ptr(idx), idx.size, nz,
+                                float(lab.z0), int(lab.background), _ptr(pk) if pk.size else None, pk.shape[0],
+                                ib.real, ib.imag, None if m is None else _ptr(m), None if lb is None else _ptr(lb),
+                                float(sig.sigma), float(sig.corr_mm), int(sig.seed), int(device),
+                                PRECISIONS[precision], ctypes.byref(self._h)))
+            return
         if isinstance(f, FluctuationSynthesis):
             arr = None if isinstance(lab, Uniform) else _ptr(lab)
             ul = int(lab.label) if isinstance(lab, Uniform) else -1
@@ -530,7 +555,16 @@ class GpuQueue:
             raise ValueError("iterations must be >= 0")
         ib = complex(index_before)
         t = _i64(0)
-        if isinstance(f, FluctuationSynthesis):
+        if isinstance(lab, Geometry):
+            pk = lab.packed()
+            sig = f if isinstance(f, FluctuationSynthesis) else FluctuationSynthesis(0.0, 0.0, 0)
+            _check(_q_submit_geom(self._h, int(ny), int(nx), float(dy), float(dx), float(dz), float(k0), _ptr(idx),
+                                  idx.size, int(lab.nz), float(lab.z0), int(lab.background),
+                                  _ptr(pk) if pk.size else None, pk.shape[0], ib.real, ib.imag,
+                                  None if m is None else _ptr(m), None if lb is None else _ptr(lb), float(sig.sigma),
+                                  float(sig.corr_mm), int(sig.seed), PRECISIONS[precision], _ptr(e), int(iterations),
+                                  ctypes.byref(t)))
+        elif isinstance(f, FluctuationSynthesis):
             uni = isinstance(lab, Uniform)
             _check(_q_submit_synth(self._h, int(ny), int(nx), float(dy), float(dx), float(dz), float(k0), _ptr(idx),
                                    idx.size, None if uni else _ptr(lab), int(lab.label) if uni else -1,

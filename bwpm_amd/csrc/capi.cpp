@@ -1,4 +1,5 @@
 // extern "C" entry points of libbwpm_hip.so (declared in include/bwpm_hip.h).
+#include <cmath>
 #include <cstring>
 #include <string>
 
@@ -103,11 +104,88 @@ bwpm::Spec make_spec(int64_t ny, int64_t nx, double dy, double dx, double dz, do
   return s;
 }
 
+// Spec of a solve whose labels come from a geometry (kGeomStride doubles per primitive: kind,
+// label, a, b, d, zlo, zhi, cx, cy, r) and, with sigma > 0, whose δn is generated on the device.
+constexpr int kGeomStride = 10;
+bwpm::Spec geometry_spec(int64_t ny, int64_t nx, double dy, double dx, double dz, double k0, const double* indices,
+                         int64_t m, int64_t nz, double z0, int background, const double* prims, int64_t nprims,
+                         double ib_re, double ib_im, const double* mask, const int32_t* labels_before, double sigma,
+                         double corr, uint64_t seed, int device, int precision) {
+  bwpm::require(nprims >= 0 && (nprims == 0 || prims), "geometry primitives missing");
+  bwpm::require(std::isfinite(z0), "z0 must be finite");
+  bwpm::Spec s = make_spec(ny, nx, dy, dx, dz, k0, indices, m, nullptr, nz, ib_re, ib_im, mask, nullptr, labels_before,
+                           device, precision, 0);
+  s.uniform_label = -1;
+  s.geom.on = true;
+  s.geom.z0 = z0;
+  s.geom.background = background;
+  for (int64_t i = 0; i < nprims; ++i) {
+    const double* v = prims + kGeomStride * i;
+    // bounds may be infinite (an unbounded layer); nothing may be NaN
+    for (int k = 0; k < kGeomStride; ++k) bwpm::require(!std::isnan(v[k]), "geometry values must not be NaN");
+    for (int k : {0, 1, 2, 3, 4, 7, 8, 9})
+      bwpm::require(std::isfinite(v[k]), "geometry kinds, labels, planes and cylinders must be finite");
+    bwpm::GeomPrim p;
+    p.kind = static_cast<int>(v[0]);
+    p.label = static_cast<int>(v[1]);
+    bwpm::require(static_cast<double>(p.kind) == v[0] && static_cast<double>(p.label) == v[1],
+                  "geometry kind and label must be integers");
+    p.a = v[2];
+    p.b = v[3];
+    p.d = v[4];
+    p.zlo = v[5];
+    p.zhi = v[6];
+    p.cx = v[7];
+    p.cy = v[8];
+    p.r = v[9];
+    s.geom.prims.push_back(p);
+  }
+  if (sigma > 0) {
+    s.synth.on = true;
+    s.synth.sigma = sigma;
+    s.synth.corr = corr;
+    s.synth.seed = seed;
+  }
+  return s;
+}
+
 }  // namespace
 
 extern "C" {
 
 const char* bwpm_last_error(void) { return g_error.c_str(); }
+
+int bwpm_geometry_labels(int64_t ny, int64_t nx, double dy, double dx, double dz, int64_t nz, double z0,
+                         int background, const double* prims, int64_t nprims, int64_t m, int32_t* out) {
+  return guarded([&] {
+    bwpm::require(out != nullptr && ny >= 1 && nx >= 1 && nz >= 1 && m >= 1, "null output or empty grid");
+    std::vector<double> idx(2 * static_cast<size_t>(m), 1.0);  // labels are checked against m indices
+    bwpm::Spec s = geometry_spec(ny, nx, dy, dx, dz, 1.0, idx.data(), m, nz, z0, background, prims, nprims, 1.0, 0.0,
+                                 nullptr, nullptr, 0.0, 0.0, 0, 0, BWPM_FP64);
+    bwpm::geometry_labels(s, out);
+  });
+}
+
+int bwpm_solver_create_geometry(int64_t ny, int64_t nx, double dy, double dx, double dz, double k0,
+                                const double* indices, int64_t m, int64_t nz, double z0, int background,
+                                const double* prims, int64_t nprims, double ib_re, double ib_im, const double* mask,
+                                const int32_t* labels_before, double sigma, double corr, uint64_t seed, int device,
+                                int precision, bwpm_solver** out) {
+  return guarded([&] {
+    bwpm::require(out != nullptr, "null output handle");
+    *out = nullptr;
+    bwpm::Spec s = geometry_spec(ny, nx, dy, dx, dz, k0, indices, m, nz, z0, background, prims, nprims, ib_re, ib_im,
+                                 mask, labels_before, sigma, corr, seed, device, precision);
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0)
+      throw bwpm::Error(bwpm::kEdevice, "no GPU device available");
+    bwpm::require(device >= 0 && device < count, "device index out of range");
+    auto h = std::make_unique<bwpm_solver>();
+    h->engine = bwpm::make_engine(std::move(s));
+    h->device = device;
+    *out = h.release();
+  });
+}
 
 int bwpm_device_count(int* count) {
   return guarded([&] {
@@ -389,6 +467,20 @@ int bwpm_queue_submit_synth(bwpm_queue* q, int64_t ny, int64_t nx, double dy, do
     s.synth.sigma = sigma;
     s.synth.corr = corr;
     s.synth.seed = seed;
+    const c128* f = reinterpret_cast<const c128*>(field);
+    *ticket = q->q->submit(std::move(s), std::vector<c128>(f, f + 2 * ny * nx), iterations);
+  });
+}
+
+int bwpm_queue_submit_geometry(bwpm_queue* q, int64_t ny, int64_t nx, double dy, double dx, double dz, double k0,
+                               const double* indices, int64_t m, int64_t nz, double z0, int background,
+                               const double* prims, int64_t nprims, double ib_re, double ib_im, const double* mask,
+                               const int32_t* labels_before, double sigma, double corr, uint64_t seed, int precision,
+                               const double* field, int iterations, int64_t* ticket) {
+  return guarded([&] {
+    bwpm::require(q && field && ticket, "null argument");
+    bwpm::Spec s = geometry_spec(ny, nx, dy, dx, dz, k0, indices, m, nz, z0, background, prims, nprims, ib_re, ib_im,
+                                 mask, labels_before, sigma, corr, seed, 0, precision);
     const c128* f = reinterpret_cast<const c128*>(field);
     *ticket = q->q->submit(std::move(s), std::vector<c128>(f, f + 2 * ny * nx), iterations);
   });

@@ -77,6 +77,51 @@ std::unique_ptr<EngineBase> make_engine(Spec spec) {
   }
 }
 
+void geometry_labels(const Spec& s, int32_t* out) {
+  const LabelGeometry& g = s.geom;
+  require(g.background >= 0 && static_cast<size_t>(g.background) < s.indices.size(), "labels must index into `indices`");
+  for (const GeomPrim& p : g.prims) {
+    require(p.kind >= 0 && p.kind <= 2, "geometry primitive kind must be 0 (plane), 1 (layer) or 2 (cylinder)");
+    require(p.label >= 0 && static_cast<size_t>(p.label) < s.indices.size(), "labels must index into `indices`");
+  }
+  const long long ny = s.ny, nx = s.nx, nz = s.nz, npix = ny * nx;
+  std::vector<double> xs(nx), ys(ny);
+  for (long long m = 0; m < nx; ++m) xs[m] = static_cast<double>(m - nx / 2) * s.dx;
+  for (long long n = 0; n < ny; ++n) ys[n] = static_cast<double>(n - ny / 2) * s.dy;
+  auto slab = [&](long long j) {
+    const double z = g.z0 + (static_cast<double>(j) + 0.5) * s.dz;
+    int32_t* o = out + j * npix;
+    std::fill(o, o + npix, g.background);
+    for (const GeomPrim& p : g.prims) {
+      if (p.kind != 0 && !(p.zlo < z && z <= p.zhi)) continue;
+      if (p.kind == 1) {
+        std::fill(o, o + npix, p.label);
+        continue;
+      }
+      for (long long n = 0; n < ny; ++n)
+        for (long long m = 0; m < nx; ++m) {
+          const double x = xs[m], y = ys[n];
+          bool in;
+          if (p.kind == 0) {
+            in = z > p.d + p.a * x + p.b * y;
+          } else {
+            const double u = x - p.cx, v = y - p.cy;
+            in = std::sqrt(u * u + v * v) <= p.r;
+          }
+          if (in) o[n * nx + m] = p.label;
+        }
+    }
+  };
+  const unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  std::vector<std::thread> th;
+  std::atomic<long long> next{0};
+  for (unsigned t = 0; t < nt; ++t)
+    th.emplace_back([&] {
+      for (long long j = next++; j < nz; j = next++) slab(j);
+    });
+  for (auto& t : th) t.join();
+}
+
 LaneGraphCounters& lane_graph_counters() {
   static LaneGraphCounters c;
   return c;

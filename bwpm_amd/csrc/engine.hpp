@@ -22,6 +22,24 @@ struct FluctSynth {
   unsigned long long seed = 0;
 };
 
+// Label volume from a geometry by the pixel-centre rule (src/bwpm/geometry.py:24-35): slab j,
+// pixel (n, m) takes the label of the last primitive containing its centre
+// (x, y, z) = ((m − nx//2)·dx, (n − ny//2)·dy, z0 + (j + ½)·dz), else the background.
+struct GeomPrim {
+  int kind = 0;  // 0: above the plane, z > d + a·x + b·y; 1: layer, zlo < z ≤ zhi;
+                 // 2: cylinder, sqrt((x − cx)² + (y − cy)²) ≤ r and zlo < z ≤ zhi
+  int label = 0;
+  double a = 0, b = 0, d = 0, zlo = 0, zhi = 0, cx = 0, cy = 0, r = 0;
+};
+struct LabelGeometry {
+  bool on = false;
+  int background = 0;
+  double z0 = 0;
+  std::vector<GeomPrim> prims;
+};
+// Writes the (nz, ny, nx) labels of s.geom into out (several host threads).
+void geometry_labels(const struct Spec& s, int32_t* out);
+
 struct Spec {
   long long ny = 0, nx = 0, nz = 0;
   double dy = 0, dx = 0, dz = 0, k0 = 0;
@@ -33,6 +51,7 @@ struct Spec {
   std::vector<c128> fluct;      // (nz, ny, nx) or empty
   std::vector<int32_t> labels_before;  // labels of plane 0 (ny, nx); empty: index_before everywhere
   FluctSynth synth;             // δn generated on the device (fluct must be empty)
+  LabelGeometry geom;           // labels generated from a geometry (labels must be empty)
   int precision = 32;           // 64: FP64; 48: complex64 streaming planes, FP64 arithmetic;
                                 // 32: FP32 compensated; 31: FP32 plain
   int device = 0;

@@ -948,9 +948,15 @@ class Engine final : public EngineBase {
   }
 
   void validate() {
+    require(sp_.ny >= 1 && sp_.nx >= 1, "grid must have ny, nx >= 1");
+    require(sp_.nz >= 1, "labels must have shape (nz, ny, nx) with nz >= 1");
+    if (sp_.geom.on) {
+      // the label volume is made here from the geometry; the set-up frees it when done
+      require(sp_.labels.empty() && sp_.uniform_label < 0, "a geometry replaces the label array");
+      sp_.labels.resize(static_cast<size_t>(sp_.nz * sp_.ny * sp_.nx));
+      geometry_labels(sp_, sp_.labels.data());
+    }
     const Spec& s = sp_;
-    require(s.ny >= 1 && s.nx >= 1, "grid must have ny, nx >= 1");
-    require(s.nz >= 1, "labels must have shape (nz, ny, nx) with nz >= 1");
     if (s.uniform_label >= 0)
       require(s.labels.empty() && static_cast<size_t>(s.uniform_label) < s.indices.size(),
               "a uniform label must index into `indices` (and no label array be given)");

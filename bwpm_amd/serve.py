@@ -11,7 +11,8 @@ grid-point·slab updates/s.
 `served_request(seed)` draws one request of the served mix, and `run_stream` submits requests at
 Poisson arrival times and collects the results. The mix has 2D solves on 512 … 4096 points and 3D
 vector solves on 128² … 512² with 200 … 1024 slabs. Geometries are planar, tilted, a silver-like
-layer and δn volumes.
+layer and δn volumes; their label volumes are made inside the library from a `Geometry` and
+their δn on the device, so a request crosses the C ABI as a few hundred bytes plus its field.
 """
 
 from __future__ import annotations
@@ -24,6 +25,7 @@ from typing import Any, Callable, Iterator
 import numpy as np
 
 from bwpm_amd import _core, workloads
+from bwpm_amd.specs import Geometry, Layer
 
 SILVER = complex(0.13, 4.0)
 
@@ -72,14 +74,18 @@ SIZES_3D = (128, 192, 256, 384, 512)
 
 
 def planar_3d(n: int, nz: int, iterations: int, wavelength_um: float, n1: complex, n2: complex,
-              layer: tuple[int, int] | None = None, polarisation: str = "x") -> workloads.Problem:
-    """Planar interface n1 → n2 at mid-depth, or a layer of n2 in n1 between slabs layer[0] … layer[1]."""
+              layer: tuple[int, int] | None = None, polarisation: str = "x", geometry: bool = True) -> workloads.Problem:
+    """Planar interface n1 → n2 at mid-depth, or a layer of n2 in n1 in slabs layer[0] … layer[1] − 1.
+    With `geometry` the labels are made inside the library (no (nz, n, n) array on the host)."""
     p = workloads.tilted_3d(n=n, nz=nz, iterations=iterations, wavelength_um=wavelength_um, tilt_deg=0.0,
-                            n1=n1, n2=n2, polarisation=polarisation)
+                            n1=n1, n2=n2, polarisation=polarisation, geometry=geometry)
     if layer is not None:
-        lab = np.zeros((nz, n, n), dtype=np.int32)
-        lab[layer[0]:layer[1]] = 1
-        p.labels = lab
+        if geometry:
+            p.labels = Geometry(nz, (Layer(1, layer[0] * p.dz, layer[1] * p.dz),))
+        else:
+            lab = np.zeros((nz, n, n), dtype=np.int32)
+            lab[layer[0]:layer[1]] = 1
+            p.labels = lab
     return p
 
 
@@ -103,14 +109,14 @@ def served_request(seed: int, max_n: int = 512, max_nz: int = 1024, born_n: int 
         p = planar_3d(n, nz, its, lam, 1.0, float(rng.uniform(1.3, 1.7)), polarisation=pol)
     elif kind == "tilted":
         p = workloads.tilted_3d(n=n, nz=nz, iterations=its, wavelength_um=lam, tilt_deg=float(rng.uniform(2, 15)),
-                                n2=float(rng.uniform(1.3, 1.7)), polarisation=pol)
+                                n2=float(rng.uniform(1.3, 1.7)), polarisation=pol, geometry=True)
     elif kind == "silver":
         z0 = int(rng.integers(nz // 4, nz // 2))
         p = planar_3d(n, nz, its, lam, 1.0, SILVER, layer=(z0, z0 + int(rng.integers(2, 8))), polarisation=pol)
     else:
         nzb = int(rng.integers(50, 129))
-        p = workloads.born_3d(n=born_n, nz=nzb, iterations=1, sigma=float(rng.uniform(0.005, 0.02)),
-                              seed=int(rng.integers(1 << 30)))
+        p = workloads.born_3d_synth(n=born_n, nz=nzb, iterations=1, sigma=float(rng.uniform(0.005, 0.02)),
+                                    seed=int(rng.integers(1 << 30)))
     p.name = f"served_{kind}_{p.ny}x{p.nx}x{p.nz}"
     return p
 

@@ -18,7 +18,7 @@ from typing import Any
 
 import numpy as np
 
-from bwpm_amd.specs import FluctuationSynthesis, Uniform
+from bwpm_amd.specs import Above, FluctuationSynthesis, Geometry, Uniform
 
 
 @dataclass
@@ -47,7 +47,18 @@ class Problem:
 
     @property
     def nz(self) -> int:
-        return int(self.labels.nz) if isinstance(self.labels, Uniform) else int(self.labels.shape[0])
+        return int(self.labels.nz) if isinstance(self.labels, (Uniform, Geometry)) else int(self.labels.shape[0])
+
+    def label_volume(self) -> np.ndarray:
+        """The (nz, ny, nx) int32 labels, also of a uniform or geometry volume (made on the host;
+        a `Geometry` through the library's own label code)."""
+        if isinstance(self.labels, Uniform):
+            return np.full((self.labels.nz, self.ny, self.nx), self.labels.label, dtype=np.int32)
+        if isinstance(self.labels, Geometry):
+            from bwpm_amd import _core
+
+            return _core.geometry_labels(self.labels, self.ny, self.nx, self.dy, self.dx, self.dz, len(self.indices))
+        return np.asarray(self.labels, dtype=np.int32)
 
     @property
     def updates(self) -> int:
@@ -79,17 +90,22 @@ def plain_2d(ny: int = 1024, nz: int = 200, iterations: int = 3) -> Problem:
 
 def tilted_3d(n: int = 256, nz: int = 512, iterations: int = 4, wavelength_um: float = 1.55,
               tilt_deg: float = 10.0, n1: complex = 1.0, n2: complex = 1.46, polarisation: str = "x",
-              waist_mm: float = 8e-3) -> Problem:
+              waist_mm: float = 8e-3, geometry: bool = False) -> Problem:
     """Config 2 geometry: dx = dy = λ/6, dz = λ/4, interface z = z_mid + y·tan(tilt) between n1 and
-    n2 (pixel-centre rule), E_x- or E_y-polarised Gaussian of waist w₀ at plane 0."""
+    n2 (pixel-centre rule), E_x- or E_y-polarised Gaussian of waist w₀ at plane 0. With `geometry`
+    the labels are a `Geometry` made inside the library (the same volume, no host array)."""
     lam = wavelength_um
     dx, dz = lam * 1e-3 / 6, lam * 1e-3 / 4
     y = centred(n, dx)
     x = centred(n, dx)
-    zc = (np.arange(nz) + 0.5) * dz
-    surface = (nz // 2) * dz + y * np.tan(np.deg2rad(tilt_deg))
-    lab2 = (zc[:, None] > surface[None, :]).astype(np.int32)  # (nz, ny)
-    labels = np.ascontiguousarray(np.broadcast_to(lab2[:, :, None], (nz, n, n)))
+    tan = np.tan(np.deg2rad(tilt_deg))
+    if geometry:
+        labels = Geometry(nz, (Above(1, (nz // 2) * dz, 0.0, float(tan)),))
+    else:
+        zc = (np.arange(nz) + 0.5) * dz
+        surface = (nz // 2) * dz + y * tan
+        lab2 = (zc[:, None] > surface[None, :]).astype(np.int32)  # (nz, ny)
+        labels = np.ascontiguousarray(np.broadcast_to(lab2[:, :, None], (nz, n, n)))
     r2 = x[None, :] ** 2 + y[:, None] ** 2
     f = np.zeros((n, n, 2), dtype=np.complex128)
     f[..., 0 if polarisation == "x" else 1] = np.exp(-r2 / waist_mm ** 2)

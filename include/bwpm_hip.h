@@ -106,6 +106,24 @@ int bwpm_solver_create_synth(int64_t ny, int64_t nx, double dy, double dx, doubl
                              const double* indices, int64_t m, const int32_t* labels, int uniform_label, int64_t nz,
                              double ib_re, double ib_im, const double* mask, const int32_t* labels_before,
                              double sigma, double corr, uint64_t seed, int device, int precision, bwpm_solver** out);
+/* Label volumes from a geometry, by the pixel-centre rule of src/bwpm/geometry.py:24-35 (no
+ * counterpart in the reference's C++ core, which takes a (nz, ny, nx) int32 array, bindings.cpp:102).
+ * Slab j, pixel (n, m) has its centre at x = (m − nx//2)·dx, y = (n − ny//2)·dy,
+ * z = z0 + (j + ½)·dz and takes the label of the last primitive containing it, else `background`.
+ * prims: nprims × 10 doubles (kind, label, a, b, d, zlo, zhi, cx, cy, r):
+ *   kind 0, above a plane: z > d + a·x + b·y (a tilted or planar interface);
+ *   kind 1, a layer: zlo < z <= zhi;
+ *   kind 2, a cylinder along z: sqrt((x − cx)² + (y − cy)²) <= r and zlo < z <= zhi.
+ * bwpm_geometry_labels writes the (nz, ny, nx) labels (m: the number of indices they select). */
+int bwpm_geometry_labels(int64_t ny, int64_t nx, double dy, double dx, double dz, int64_t nz, double z0,
+                         int background, const double* prims, int64_t nprims, int64_t m, int32_t* out);
+/* A solver whose labels are made from a geometry inside the library (no label array crosses the
+ * ABI); with sigma > 0 its δn is generated on the device as in bwpm_solver_create_synth. */
+int bwpm_solver_create_geometry(int64_t ny, int64_t nx, double dy, double dx, double dz, double k0,
+                                const double* indices, int64_t m, int64_t nz, double z0, int background,
+                                const double* prims, int64_t nprims, double ib_re, double ib_im, const double* mask,
+                                const int32_t* labels_before, double sigma, double corr, uint64_t seed, int device,
+                                int precision, bwpm_solver** out);
 /* The δn the solver holds, (nz, ny, nx) complex128 (zeros without δn). */
 int bwpm_solver_fluctuation(const bwpm_solver* s, double* out);
 int bwpm_solver_destroy(bwpm_solver* s);
@@ -187,6 +205,12 @@ int bwpm_queue_submit_synth(bwpm_queue* q, int64_t ny, int64_t nx, double dy, do
                             double ib_re, double ib_im, const double* mask, const int32_t* labels_before,
                             double sigma, double corr, uint64_t seed, int precision, const double* field,
                             int iterations, int64_t* ticket);
+/* As bwpm_queue_submit with the labels made from a geometry (arguments as bwpm_solver_create_geometry). */
+int bwpm_queue_submit_geometry(bwpm_queue* q, int64_t ny, int64_t nx, double dy, double dx, double dz, double k0,
+                               const double* indices, int64_t m, int64_t nz, double z0, int background,
+                               const double* prims, int64_t nprims, double ib_re, double ib_im, const double* mask,
+                               const int32_t* labels_before, double sigma, double corr, uint64_t seed, int precision,
+                               const double* field, int iterations, int64_t* ticket);
 int bwpm_queue_state(bwpm_queue* q, int64_t ticket, int* state);
 /* Waits until the ticket is done or failed, or timeout_s passes (timeout_s < 0: no limit). */
 int bwpm_queue_wait(bwpm_queue* q, int64_t ticket, double timeout_s, int* state);

@@ -14,7 +14,7 @@ import numpy as np
 import pytest
 
 from bwpm_amd import _core, serve, workloads
-from bwpm_amd.specs import FluctuationSynthesis, Uniform
+from bwpm_amd.specs import FluctuationSynthesis, Geometry, Layer, Uniform
 from oracle import bwpm_oracle as O
 
 pytestmark = pytest.mark.gpu
@@ -101,6 +101,61 @@ def test_uniform_volume_equals_label_array() -> None:
 def test_generated_requests_on_the_queue_equal_the_solves_alone() -> None:
     reqs = workloads.uneven_batch(count=4, sizes=(32, 64), nz_range=(6, 14))
     with serve.Server(slots=4, chunk=8) as srv:
+        tickets = [srv.submit(p) for p in reqs]
+        results = [srv.result(t, timeout=120) for t in tickets]
+    for p, res in zip(reqs, results):
+        s = _solver(p)
+        try:
+            alone = s.run(p.field, p.iterations)
+        finally:
+            s.close()
+        assert np.array_equal(res["transmitted"], alone["transmitted"])
+        assert np.array_equal(res["reflected"], alone["reflected"])
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_geometry_volumes_equal_label_arrays(precision: str) -> None:
+    cases = [(workloads.tilted_3d(n=64, nz=30, iterations=2), workloads.tilted_3d(n=64, nz=30, iterations=2,
+                                                                                  geometry=True)),
+             (serve.planar_3d(96, 30, 2, 1.0, 1.0, serve.SILVER, layer=(10, 13), geometry=False),
+              serve.planar_3d(96, 30, 2, 1.0, 1.0, serve.SILVER, layer=(10, 13), geometry=True))]
+    for a, g in cases:
+        sa, sg = _solver(a, precision), _solver(g, precision)
+        try:
+            ra, rg = sa.run(a.field, a.iterations, [7]), sg.run(g.field, g.iterations, [7])
+        finally:
+            sa.close()
+            sg.close()
+        assert ra["stored_values"] == rg["stored_values"] > 0
+        for k in ("transmitted", "reflected"):
+            assert np.array_equal(ra[k], rg[k])
+        assert np.array_equal(ra["forward"][7], rg["forward"][7])
+
+
+def test_geometry_with_generated_fluctuation_equals_the_label_array() -> None:
+    geo = Geometry(12, (Layer(1, 4e-4, 8e-4),))
+    p = workloads.born_3d_synth(n=32, nz=12, iterations=1, sigma=0.02, corr_um=0.2, seed=4)
+    p.indices = [1.5 + 0j, 1.6 + 0j]
+    lab = _core.geometry_labels(geo, p.ny, p.nx, p.dy, p.dx, p.dz, 2)
+    assert 0 < lab.sum() < lab.size
+    outs = []
+    for labels in (geo, lab):
+        s = _core.GpuSolver(p.ny, p.nx, p.dy, p.dx, p.dz, p.k0, p.indices, labels, p.index_before, None,
+                            fluctuation=p.fluctuation)
+        try:
+            outs.append((s.run(p.field, 1), s.fluctuation()))
+        finally:
+            s.close()
+    assert np.array_equal(outs[0][0]["transmitted"], outs[1][0]["transmitted"])
+    assert np.array_equal(outs[0][0]["reflected"], outs[1][0]["reflected"])
+    assert np.array_equal(outs[0][1], outs[1][1])
+
+
+def test_geometry_requests_on_the_queue_equal_the_solves_alone() -> None:
+    reqs = [workloads.tilted_3d(n=64, nz=24 + 4 * i, iterations=1 + i % 2, geometry=True, tilt_deg=5.0 + i)
+            for i in range(4)]
+    reqs.append(serve.planar_3d(64, 20, 1, 1.0, 1.0, serve.SILVER, layer=(8, 11)))
+    with serve.Server(slots=3, chunk=8) as srv:
         tickets = [srv.submit(p) for p in reqs]
         results = [srv.result(t, timeout=120) for t in tickets]
     for p, res in zip(reqs, results):
