@@ -32,6 +32,8 @@ def _mix() -> list[tuple[workloads.Problem, str]]:
     reqs.append((workloads.plain_2d(ny=256, nz=50, iterations=2), "fp64"))
     reqs.append((workloads.born_3d(n=32, nz=12, iterations=1, sigma=0.01, seed=5), "fp64"))
     reqs.append((workloads.tilted_3d(n=64, nz=30, iterations=2), "fp32"))
+    reqs.append((workloads.tilted_3d(n=64, nz=26, iterations=2, geometry=True), "mixed"))
+    reqs.append((workloads.tilted_3d(n=64, nz=34, iterations=1, polarisation="y"), "mixed"))
     reqs.append((workloads.tilted_3d(n=64, nz=20, iterations=0), "fp64"))
     return reqs
 
