@@ -77,11 +77,16 @@ class PinnedBuf {
   void* p_ = nullptr;
 };
 
+// The copy has completed when this returns. Callers pass short-lived host vectors in pageable
+// memory, freed (and their blocks reused) as soon as the set-up step that built them returns; an
+// asynchronous copy still in flight would then move whatever the allocator put there.
 template <typename U>
 inline DevBuf upload_vec(const std::vector<U>& v, hipStream_t s) {
   DevBuf b(v.size() * sizeof(U));
-  if (!v.empty()) HIP_CHECK(hipMemcpyAsync(b.as<void>(), v.data(), v.size() * sizeof(U),
-                                           hipMemcpyHostToDevice, s));
+  if (!v.empty()) {
+    HIP_CHECK(hipMemcpyAsync(b.as<void>(), v.data(), v.size() * sizeof(U), hipMemcpyHostToDevice, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+  }
   return b;
 }
 
