@@ -115,7 +115,7 @@ __global__ __launch_bounds__(N / 8 * W) void k_fft2_cols_fast(PlaneSet<T> ps, in
   px2<T>* out = ps.out[blockIdx.y];
   const cx<T>* mul = ps.mul[blockIdx.y];
   const int c = threadIdx.x % W, t = threadIdx.x / W;
-  const int x = blockIdx.x * W + c;
+  const int x = xcd_group(blockIdx.x, gridDim.x) * W + c;
   px2<T> v[8];
 #pragma unroll
   for (int m = 0; m < 8; ++m) v[m] = in[static_cast<long long>(t + TT * m) * nx + x];
@@ -173,6 +173,11 @@ class Fft2D {
     }
     run(a, b, np, inverse);
   }
+  // Both passes run the register kernels (power-of-two lines in the precision they support).
+  bool fast() const { return rows_fast_ && cols_fast_; }
+  const FftPlan& row_plan() const { return px_; }
+  const FftPlan& col_plan() const { return py_; }
+
   // out = F(in) or F⁻¹(in) (1/N scaled); `in` is left unchanged unless it is `out`.
   void one(const px2<T>* in, px2<T>* out, bool inverse) {
     PlaneSet<T> a{}, b{};

@@ -341,6 +341,13 @@ __global__ void k_build_matrix(const double* kx, const double* ky, int ny, int n
   split(zmul(m.m11, scale), hi + 3 * k + 2, lo + 3 * k + 2);
 }
 
+// Workgroups are dispatched to the eight XCDs round-robin. Column groups that share cache lines
+// (W·sizeof(px2) < 128 B) are mapped so that neighbouring groups run on the same XCD and its L2:
+// XCD j takes the j-th contiguous eighth of the groups.
+__device__ __forceinline__ int xcd_group(int b, int n) {
+  return (n & 7) ? b : (b & 7) * (n >> 3) + (b >> 3);
+}
+
 // ---- small elementwise helpers ---------------------------------------------------------------
 
 template <typename T>

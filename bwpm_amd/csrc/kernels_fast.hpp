@@ -50,13 +50,6 @@ constexpr bool fast_tw_in_lds() {
   return fast_lds_bytes(N, LINES, SKEW, static_cast<int>(sizeof(cx<T>)), COMP) <= 163840;
 }
 
-// Workgroups are dispatched to the eight XCDs round-robin. Column groups that share cache lines
-// (W·sizeof(px2) < 128 B) are mapped so that neighbouring groups run on the same XCD and its L2:
-// XCD j takes the j-th contiguous eighth of the groups.
-__device__ __forceinline__ int xcd_group(int b, int n) {
-  return (n & 7) ? b : (b & 7) * (n >> 3) + (b >> 3);
-}
-
 // ---- lane-pair exchange (lanes 2i ↔ 2i + 1): DPP quad permutation [1, 0, 3, 2] ---------------
 
 __device__ __forceinline__ int pair_swap(int v) {
