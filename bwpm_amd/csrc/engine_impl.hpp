@@ -31,6 +31,7 @@ struct Boundary {
   std::vector<int> mleft, mright;                   // medium ids of each pair
   std::vector<int> rpair;                           // pair index of each R product
   long long tsel_off = -1;                          // into the tsel byte array (pairs > 1)
+  long long rowuse_off = -1;                        // ny per-row pair masks (pairs > 1)
   long long sp_off = 0, sp_count = 0;               // sparse interface entries
   long long rowptr_off = -1;                        // ny + 1 offsets (sp_count > 0)
   int fwd_prod = 0, bwd_prod = 0;                   // offsets into the product descriptor array
@@ -913,6 +914,9 @@ class Engine final : public EngineBase {
     c.out = prod_.as<px2<ST>>();
     c.prods = prods_.as<Prod<T>>() + (forward ? B.fwd_prod : B.bwd_prod);
     c.nprod = static_cast<int>(B.pairs.size()) + (with_r ? static_cast<int>(B.rpair.size()) : 0);
+    c.nt = static_cast<int>(B.pairs.size());
+    c.rowuse = B.rowuse_off >= 0 ? rowuse_.as<uint32_t>() + B.rowuse_off : nullptr;
+    c.rowptr = B.sp_count > 0 ? rowptr_.as<int>() + B.rowptr_off : nullptr;
     return c;
   }
 
@@ -923,6 +927,7 @@ class Engine final : public EngineBase {
     r.nr = with_r ? static_cast<int>(B.rpair.size()) : 0;
     r.prod = prod_.as<px2<ST>>();
     r.tsel = B.tsel_off >= 0 ? tsel_.as<uint8_t>() + B.tsel_off : nullptr;
+    r.rowuse = B.rowuse_off >= 0 ? rowuse_.as<uint32_t>() + B.rowuse_off : nullptr;
     r.mask = mask_.as<T>();
     if (B.sp_count > 0) {
       const long long list = list_of(q);
@@ -1070,8 +1075,8 @@ class Engine final : public EngineBase {
                                               twy_->plan, stream_);
     HIP_CHECK(hipStreamSynchronize(stream_));
     bytes_ = stage_.bytes() + field_in_.bytes() + X_.bytes() + prod_.bytes() + e_.bytes() + fwd_vals_.bytes() +
-             bwd_vals_.bytes() + mask_.bytes() + ops_bytes_ + tsel_.bytes() + sp_pix_.bytes() + sp_rid_.bytes() +
-             rowptr_.bytes() + (born_ ? born_->device_bytes() : 0);
+             bwd_vals_.bytes() + mask_.bytes() + ops_bytes_ + tsel_.bytes() + rowuse_.bytes() + sp_pix_.bytes() +
+             sp_rid_.bytes() + rowptr_.bytes() + (born_ ? born_->device_bytes() : 0);
     // the label volume and δn now live in the boundary structures and device tables
     sp_.labels = std::vector<int32_t>();
     sp_.fluct = std::vector<c128>();
@@ -1124,6 +1129,7 @@ class Engine final : public EngineBase {
     const long long nz = s.nz;
     bnd_.resize(nz);
     std::vector<uint8_t> tsel;
+    std::vector<uint32_t> rowuse;
     std::vector<int32_t> sp_pix, rowptr;
     std::vector<uint8_t> sp_rid;
     total_sparse_ = 0;
@@ -1185,6 +1191,14 @@ class Engine final : public EngineBase {
           }
           tsel[B.tsel_off + i] = id;
         }
+        // pairs each row selects: bit min(pair, 31)
+        B.rowuse_off = static_cast<long long>(rowuse.size());
+        rowuse.resize(rowuse.size() + s.ny, 0u);
+        for (long long y = 0; y < s.ny; ++y) {
+          uint32_t u = 0;
+          for (long long x = 0; x < s.nx; ++x) u |= 1u << std::min<int>(tsel[B.tsel_off + y * s.nx + x], 31);
+          rowuse[B.rowuse_off + y] = u;
+        }
       }
       if (!B.rpair.empty()) {
         B.sp_off = total_sparse_;
@@ -1210,6 +1224,7 @@ class Engine final : public EngineBase {
       }
     }
     tsel_ = upload_vec(tsel, stream_);
+    rowuse_ = upload_vec(rowuse, stream_);
     sp_pix_ = upload_vec(sp_pix, stream_);
     sp_rid_ = upload_vec(sp_rid, stream_);
     rowptr_ = upload_vec(rowptr, stream_);
@@ -1323,7 +1338,7 @@ class Engine final : public EngineBase {
   std::vector<Boundary> bnd_;
   std::unordered_map<long long, long long> list_off_;
   long long total_sparse_ = 0;
-  DevBuf tsel_, sp_pix_, sp_rid_, rowptr_;
+  DevBuf tsel_, rowuse_, sp_pix_, sp_rid_, rowptr_;
   std::map<int, std::shared_ptr<OpTable>> phase_ops_;
   std::map<long long, std::shared_ptr<OpTable>> matrix_ops_;
   size_t ops_bytes_ = 0;

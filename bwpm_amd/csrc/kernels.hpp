@@ -68,6 +68,9 @@ struct ColDesc {
   px2<ST>* out;          // product planes, `plane` pixels apart
   const Prod<T>* prods;  // nprod descriptors (T kinds first)
   int nprod;
+  int nt;                   // T products (the first nt); the rest are R products
+  const uint32_t* rowuse;   // per row: bit min(o, 31) set if a pixel selects T product o (nullptr: all)
+  const int* rowptr;        // ny + 1 offsets of the rows' interface pixels (nullptr: every row has some)
 };
 
 // Row-kernel work of one solve for one step.
@@ -77,6 +80,7 @@ struct RowDesc {
   int nt, nr;                 // T products (0: start a pass from e_in), R products
   const px2<ST>* prod;        // product planes
   const uint8_t* tsel;        // T pair of each pixel (nt > 1)
+  const uint32_t* rowuse;     // per row: bit min(o, 31) set if a pixel selects T pair o (nullptr: all)
   const T* mask;              // transmission mask or nullptr
   const px2<T>* e_in;         // nt == 0: start plane (nullptr: zero)
   const px2<T>* inc;          // Born increment, times mask if mask != nullptr

@@ -69,6 +69,15 @@ __device__ __forceinline__ cx<T> pair_swap(cx<T> z) {
   return cx<T>{pair_swap(z.x), pair_swap(z.y)};
 }
 
+// An empty statement that takes z's registers as read and written: every instruction producing z
+// stays before it. Settling all values before the first conditional store keeps the compiler from
+// sinking the last arithmetic into the stores' branches, where multiply and add would no longer be
+// fused as on the unconditional path (and the results would differ in the last bits).
+template <typename T>
+__device__ __forceinline__ void settle(cx<T>& z) {
+  asm volatile("" : "+v"(z.x), "+v"(z.y));
+}
+
 // Operator product on one component (comp 0: E_x, 1: E_y) of spectrum entry k. The phase type is
 // the same for both components; the symmetric matrix type takes the partner lane's component:
 // E_x' = m00·E_x + m01·E_y, E_y' = m01·E_x + m11·E_y (terms in this order). Every lane of the
@@ -157,8 +166,29 @@ __global__ __launch_bounds__(N / 4 * W) void k_col_fast(const ColDesc<T, ST>* __
     for (int m = 0; m < 8; ++m) s[m] = apply_prod1<T, COMP>(pd, pix(m), comp, s[m]);
     reg_fft<N, COMP, true>(s, line, t, tw_hi, tw_lo);
     px2<ST>* out = d.out + o * plane;
+    if constexpr (MULTI) {
+      // a product is stored only in the rows whose row transforms use it: rows with a pixel that
+      // selects T product o, rows with interface pixels for the R products (the row kernel
+      // transforms each row on its own, so values of the other rows never reach a used pixel).
+      // Only the interface variant does this; the plain variant keeps its registers.
+      const bool refl = o >= d.nt;
+      const uint32_t* rowuse = refl ? nullptr : d.rowuse;
+      const int* rowptr = refl ? d.rowptr : nullptr;
+      const unsigned bit = static_cast<unsigned>(o < 31 ? o : 31);
 #pragma unroll
-    for (int m = 0; m < 8; ++m) sstore<T>(out, at(m), s[m]);
+      for (int m = 0; m < 8; ++m) settle(s[m]);
+#pragma unroll
+      for (int m = 0; m < 8; ++m) {
+        const int y = t + TT * m;
+        bool keep = true;
+        if (rowuse) keep = (gload1(rowuse, y) >> bit) & 1u;
+        if (rowptr) keep = gload1(rowptr, y) != gload1(rowptr, y + 1);
+        if (keep) sstore<T>(out, at(m), s[m]);
+      }
+    } else {
+#pragma unroll
+      for (int m = 0; m < 8; ++m) sstore<T>(out, at(m), s[m]);
+    }
   }
 }
 
@@ -191,9 +221,23 @@ __global__ __launch_bounds__(N / 4 * RW) void k_row_fast(const RowDesc<T, ST>* _
   auto at = [&](int m) { return 2u * (row + t + TT * m) + static_cast<unsigned>(comp); };
   auto cplane = [](const px2<T>* p) { return reinterpret_cast<const cx<T>*>(p); };
   cx<T> acc[8];
+  // Work a product does only for rows that use it is skipped for the whole workgroup when none of
+  // its rows does (the conditions are read from memory the kernel does not write, so every thread
+  // evaluates them equally and the barriers inside the transforms stay uniform): transmitted
+  // products no pixel of these rows selects (bit o, or bit 31 for o ≥ 31, of rowuse), and the
+  // reflection products when these rows hold no interface pixel. Those products' values would
+  // only be stored at pixels of other rows, so the results are the same bits.
+  const int y0 = blockIdx.x * RW;
+  int use = -1;
+  if (a.rowuse) {
+    unsigned u = 0;
+#pragma unroll
+    for (int i = 0; i < RW; ++i) u |= gload1(a.rowuse, y0 + i);
+    use = uniform(static_cast<int>(u));
+  }
 
   // 1. reflection products: row IFFT and gather of the interface pixels
-  if (a.nr > 0 && a.rowptr && a.sp_store) {
+  if (a.nr > 0 && a.rowptr && a.sp_store && uniform(gload1(a.rowptr, y0)) != uniform(gload1(a.rowptr, y0 + RW))) {
     const int e0 = gload1(a.rowptr, y), e1 = gload1(a.rowptr, y + 1);
     cx<T>* store = reinterpret_cast<cx<T>*>(a.sp_store);
     for (int o = 0; o < a.nr; ++o) {
@@ -225,6 +269,7 @@ __global__ __launch_bounds__(N / 4 * RW) void k_row_fast(const RowDesc<T, ST>* _
     cx<T>* own = stash;  // stash entry m of this lane is own[m·T]
     if constexpr (MULTI) own = stash + (2 * r + comp) * (N + 1) + t;
     for (int o = 0; o < a.nt; ++o) {
+      if (!((static_cast<unsigned>(use) >> (o < 31 ? o : 31)) & 1u)) continue;
       const px2<ST>* src = a.prod + o * plane;
 #pragma unroll
       for (int m = 0; m < 8; ++m) acc[m] = sload<T>(src, at(m));
