@@ -5,6 +5,8 @@
 #pragma once
 
 #include <algorithm>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -31,7 +33,7 @@ struct Boundary {
   std::vector<int> mleft, mright;                   // medium ids of each pair
   std::vector<int> rpair;                           // pair index of each R product
   long long tsel_off = -1;                          // into the tsel byte array (pairs > 1)
-  long long rowuse_off = -1;                        // ny per-row pair masks (pairs > 1)
+  long long rowuse_off = -1;                        // ny per-row product-use masks (pairs > 1)
   long long sp_off = 0, sp_count = 0;               // sparse interface entries
   long long rowptr_off = -1;                        // ny + 1 offsets (sp_count > 0)
   int fwd_prod = 0, bwd_prod = 0;                   // offsets into the product descriptor array
@@ -156,6 +158,15 @@ inline std::pair<int, int> tune_pick(const TuneKey& key, const Geometry& g, int&
       }
     st.us.assign(st.cand.size(), -1.0);
     if (st.cand.size() == 1) st.best = 0;
+    // BWPM_LINES="<columns>,<rows>" fixes the choice when it is a candidate (reproducible
+    // profiles: hardware counters slow kernels unevenly, which can sway the timed choice)
+    if (const char* env = std::getenv("BWPM_LINES")) {
+      int cw = 0, rw = 0;
+      if (std::sscanf(env, "%d,%d", &cw, &rw) == 2) {
+        const auto it = std::find(st.cand.begin(), st.cand.end(), std::pair<int, int>{cw, rw});
+        if (it != st.cand.end()) st.best = static_cast<int>(it - st.cand.begin());
+      }
+    }
   }
   trial = -1;
   if (st.best >= 0) return st.cand[st.best];
@@ -916,7 +927,6 @@ class Engine final : public EngineBase {
     c.nprod = static_cast<int>(B.pairs.size()) + (with_r ? static_cast<int>(B.rpair.size()) : 0);
     c.nt = static_cast<int>(B.pairs.size());
     c.rowuse = B.rowuse_off >= 0 ? rowuse_.as<uint32_t>() + B.rowuse_off : nullptr;
-    c.rowptr = B.sp_count > 0 ? rowptr_.as<int>() + B.rowptr_off : nullptr;
     return c;
   }
 
@@ -1191,12 +1201,16 @@ class Engine final : public EngineBase {
           }
           tsel[B.tsel_off + i] = id;
         }
-        // pairs each row selects: bit min(pair, 31)
+        // pairs each row selects, and whether it holds interface pixels (row_use_bit, kRowUseR)
         B.rowuse_off = static_cast<long long>(rowuse.size());
         rowuse.resize(rowuse.size() + s.ny, 0u);
         for (long long y = 0; y < s.ny; ++y) {
           uint32_t u = 0;
-          for (long long x = 0; x < s.nx; ++x) u |= 1u << std::min<int>(tsel[B.tsel_off + y * s.nx + x], 31);
+          for (long long x = 0; x < s.nx; ++x) {
+            const int pi = tsel[B.tsel_off + y * s.nx + x];
+            u |= 1u << row_use_bit(pi);
+            if (rid_of_pair[pi] >= 0) u |= 1u << kRowUseR;
+          }
           rowuse[B.rowuse_off + y] = u;
         }
       }

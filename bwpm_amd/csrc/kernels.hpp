@@ -69,9 +69,13 @@ struct ColDesc {
   const Prod<T>* prods;  // nprod descriptors (T kinds first)
   int nprod;
   int nt;                   // T products (the first nt); the rest are R products
-  const uint32_t* rowuse;   // per row: bit min(o, 31) set if a pixel selects T product o (nullptr: all)
-  const int* rowptr;        // ny + 1 offsets of the rows' interface pixels (nullptr: every row has some)
+  const uint32_t* rowuse;   // per-row product use (kRowUse bits; nullptr: every row uses every product)
 };
+
+// Per-row product use at a boundary with several label pairs: bit min(o, 30) is set if a pixel of
+// the row selects T pair o, bit 31 if the row holds interface pixels (it uses the R products).
+inline constexpr int kRowUseR = 31;
+__host__ __device__ inline constexpr unsigned row_use_bit(int o) { return static_cast<unsigned>(o < 30 ? o : 30); }
 
 // Row-kernel work of one solve for one step.
 template <typename T, typename ST = T>
@@ -80,7 +84,7 @@ struct RowDesc {
   int nt, nr;                 // T products (0: start a pass from e_in), R products
   const px2<ST>* prod;        // product planes
   const uint8_t* tsel;        // T pair of each pixel (nt > 1)
-  const uint32_t* rowuse;     // per row: bit min(o, 31) set if a pixel selects T pair o (nullptr: all)
+  const uint32_t* rowuse;     // per-row product use (kRowUse bits; nullptr: every row uses every product)
   const T* mask;              // transmission mask or nullptr
   const px2<T>* e_in;         // nt == 0: start plane (nullptr: zero)
   const px2<T>* inc;          // Born increment, times mask if mask != nullptr

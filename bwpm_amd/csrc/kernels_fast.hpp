@@ -133,6 +133,16 @@ __global__ __launch_bounds__(N / 4 * W) void k_col_fast(const ColDesc<T, ST>* __
   auto pix = [&](int m) { return static_cast<unsigned>(t + TT * m) * static_cast<unsigned>(nx) + x; };
   auto at = [&](int m) { return 2u * pix(m) + static_cast<unsigned>(comp); };
   const int nprod = uniform(d.nprod);
+  // the rows' product use, for the interface variant's stores (all ones: store every row)
+  unsigned use[8];
+#pragma unroll
+  for (int m = 0; m < 8; ++m) use[m] = ~0u;
+  if constexpr (MULTI) {
+    if (d.rowuse) {
+#pragma unroll
+      for (int m = 0; m < 8; ++m) use[m] = gload1(d.rowuse, t + TT * m);
+    }
+  }
   cx<T> s[8];
 #pragma unroll
   for (int m = 0; m < 8; ++m) s[m] = sload<T>(xio, at(m));
@@ -171,20 +181,12 @@ __global__ __launch_bounds__(N / 4 * W) void k_col_fast(const ColDesc<T, ST>* __
       // selects T product o, rows with interface pixels for the R products (the row kernel
       // transforms each row on its own, so values of the other rows never reach a used pixel).
       // Only the interface variant does this; the plain variant keeps its registers.
-      const bool refl = o >= d.nt;
-      const uint32_t* rowuse = refl ? nullptr : d.rowuse;
-      const int* rowptr = refl ? d.rowptr : nullptr;
-      const unsigned bit = static_cast<unsigned>(o < 31 ? o : 31);
+      const unsigned bit = o >= d.nt ? static_cast<unsigned>(kRowUseR) : row_use_bit(o);
 #pragma unroll
       for (int m = 0; m < 8; ++m) settle(s[m]);
 #pragma unroll
-      for (int m = 0; m < 8; ++m) {
-        const int y = t + TT * m;
-        bool keep = true;
-        if (rowuse) keep = (gload1(rowuse, y) >> bit) & 1u;
-        if (rowptr) keep = gload1(rowptr, y) != gload1(rowptr, y + 1);
-        if (keep) sstore<T>(out, at(m), s[m]);
-      }
+      for (int m = 0; m < 8; ++m)
+        if ((use[m] >> bit) & 1u) sstore<T>(out, at(m), s[m]);
     } else {
 #pragma unroll
       for (int m = 0; m < 8; ++m) sstore<T>(out, at(m), s[m]);
@@ -224,20 +226,19 @@ __global__ __launch_bounds__(N / 4 * RW) void k_row_fast(const RowDesc<T, ST>* _
   // Work a product does only for rows that use it is skipped for the whole workgroup when none of
   // its rows does (the conditions are read from memory the kernel does not write, so every thread
   // evaluates them equally and the barriers inside the transforms stay uniform): transmitted
-  // products no pixel of these rows selects (bit o, or bit 31 for o ≥ 31, of rowuse), and the
-  // reflection products when these rows hold no interface pixel. Those products' values would
-  // only be stored at pixels of other rows, so the results are the same bits.
-  const int y0 = blockIdx.x * RW;
+  // products no pixel of these rows selects, and the reflection products when these rows hold no
+  // interface pixel. Those products' values would only be stored at pixels of other rows, so the
+  // results are the same bits.
   int use = -1;
   if (a.rowuse) {
     unsigned u = 0;
 #pragma unroll
-    for (int i = 0; i < RW; ++i) u |= gload1(a.rowuse, y0 + i);
+    for (int i = 0; i < RW; ++i) u |= gload1(a.rowuse, blockIdx.x * RW + i);
     use = uniform(static_cast<int>(u));
   }
 
   // 1. reflection products: row IFFT and gather of the interface pixels
-  if (a.nr > 0 && a.rowptr && a.sp_store && uniform(gload1(a.rowptr, y0)) != uniform(gload1(a.rowptr, y0 + RW))) {
+  if (a.nr > 0 && a.rowptr && a.sp_store && ((static_cast<unsigned>(use) >> kRowUseR) & 1u)) {
     const int e0 = gload1(a.rowptr, y), e1 = gload1(a.rowptr, y + 1);
     cx<T>* store = reinterpret_cast<cx<T>*>(a.sp_store);
     for (int o = 0; o < a.nr; ++o) {
@@ -269,7 +270,7 @@ __global__ __launch_bounds__(N / 4 * RW) void k_row_fast(const RowDesc<T, ST>* _
     cx<T>* own = stash;  // stash entry m of this lane is own[m·T]
     if constexpr (MULTI) own = stash + (2 * r + comp) * (N + 1) + t;
     for (int o = 0; o < a.nt; ++o) {
-      if (!((static_cast<unsigned>(use) >> (o < 31 ? o : 31)) & 1u)) continue;
+      if (!((static_cast<unsigned>(use) >> row_use_bit(o)) & 1u)) continue;
       const px2<ST>* src = a.prod + o * plane;
 #pragma unroll
       for (int m = 0; m < 8; ++m) acc[m] = sload<T>(src, at(m));
