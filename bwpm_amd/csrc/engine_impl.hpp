@@ -34,6 +34,7 @@ struct Boundary {
   std::vector<int> rpair;                           // pair index of each R product
   long long tsel_off = -1;                          // into the tsel byte array (pairs > 1)
   long long rowuse_off = -1;                        // ny per-row product-use masks (pairs > 1)
+  bool mixed_rows = false;                          // over a quarter of the rows select several T pairs
   long long sp_off = 0, sp_count = 0;               // sparse interface entries
   long long rowptr_off = -1;                        // ny + 1 offsets (sp_count > 0)
   int fwd_prod = 0, bwd_prod = 0;                   // offsets into the product descriptor array
@@ -341,6 +342,10 @@ struct Op {
   int kind = kOpPair;
   ColDesc<T, ST> col{};
   RowDesc<T, ST> row{};
+  // the row kernel merges several T products of a row in LDS (its interface variant): many rows
+  // mix pairs, or storage is narrower than the arithmetic (a merge through a product plane would
+  // round); otherwise rows that use one product take it directly in the plain variant
+  bool row_stash = false;
   long long slab = 0;
   bool bidir = false;
 };
@@ -508,6 +513,7 @@ class Engine final : public EngineBase {
         Op<T, ST> o;
         o.col = col_desc(B, true, with_r);
         o.row = row_desc(B, p, with_r);
+        o.row_stash = B.mixed_rows || sizeof(ST) != sizeof(T);
         o.row.inc = born_at(p) ? born_->increment() : nullptr;
         if (B.sp_count > 0) {
           o.row.sp_add = bidir ? sp_vals(fwd_vals_, p) : nullptr;
@@ -551,6 +557,7 @@ class Engine final : public EngineBase {
         Op<T, ST> o;
         o.col = col_desc(B, false, with_r);
         o.row = row_desc(B, p - 1, with_r);
+        o.row_stash = B.mixed_rows || sizeof(ST) != sizeof(T);
         if (B.sp_count > 0) {
           o.row.sp_add = sp_vals(bwd_vals_, p - 1);
           o.row.sp_store = with_r ? sp_vals(fwd_vals_, p - 1) : nullptr;
@@ -603,7 +610,7 @@ class Engine final : public EngineBase {
         has_col[g] |= o.col.in != nullptr;
         has_row[g] |= o.row.active != 0;
         multi_col[g] |= o.col.in != nullptr && o.col.nprod > 1;
-        multi_row[g] |= o.row.active != 0 && o.row.nt > 1;
+        multi_row[g] |= o.row.active != 0 && o.row.nt > 1 && o.row_stash;
       }
     if (lane_col_.bytes() < hcol.size() * sizeof(ColDesc<T, ST>)) lane_col_.alloc(hcol.size() * sizeof(ColDesc<T, ST>));
     if (lane_row_.bytes() < hrow.size() * sizeof(RowDesc<T, ST>)) lane_row_.alloc(hrow.size() * sizeof(RowDesc<T, ST>));
@@ -783,7 +790,7 @@ class Engine final : public EngineBase {
             has_col[g] |= o.col.in != nullptr;
             has_row[g] |= o.row.active != 0;
             multi_col[g] |= o.col.in != nullptr && o.col.nprod > 1;
-            multi_row[g] |= o.row.active != 0 && o.row.nt > 1;
+            multi_row[g] |= o.row.active != 0 && o.row.nt > 1 && o.row_stash;
           }
         HIP_CHECK(hipMemcpyAsync(dcol.as<void>(), hc, G * S * sizeof(ColDesc<T, ST>), hipMemcpyHostToDevice, stream));
         HIP_CHECK(hipMemcpyAsync(drow.as<void>(), hr, G * S * sizeof(RowDesc<T, ST>), hipMemcpyHostToDevice, stream));
@@ -1204,6 +1211,7 @@ class Engine final : public EngineBase {
         // pairs each row selects, and whether it holds interface pixels (row_use_bit, kRowUseR)
         B.rowuse_off = static_cast<long long>(rowuse.size());
         rowuse.resize(rowuse.size() + s.ny, 0u);
+        long long mixed = 0;
         for (long long y = 0; y < s.ny; ++y) {
           uint32_t u = 0;
           for (long long x = 0; x < s.nx; ++x) {
@@ -1212,7 +1220,9 @@ class Engine final : public EngineBase {
             if (rid_of_pair[pi] >= 0) u |= 1u << kRowUseR;
           }
           rowuse[B.rowuse_off + y] = u;
+          if (!single_t(static_cast<int>(u))) ++mixed;
         }
+        B.mixed_rows = 4 * mixed > s.ny;
       }
       if (!B.rpair.empty()) {
         B.sp_off = total_sparse_;

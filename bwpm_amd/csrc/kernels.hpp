@@ -75,7 +75,14 @@ struct ColDesc {
 // Per-row product use at a boundary with several label pairs: bit min(o, 30) is set if a pixel of
 // the row selects T pair o, bit 31 if the row holds interface pixels (it uses the R products).
 inline constexpr int kRowUseR = 31;
+inline constexpr unsigned kRowUseT = (1u << 30) - 1u;  // the bits of single T pairs 0 … 29
 __host__ __device__ inline constexpr unsigned row_use_bit(int o) { return static_cast<unsigned>(o < 30 ? o : 30); }
+// The rows of a use mask (−1: unknown) select exactly one T pair below 30.
+__host__ __device__ inline constexpr bool single_t(int use) {
+  return use != -1 && (static_cast<unsigned>(use) & ((1u << 31) - 1u)) != 0 &&
+         (static_cast<unsigned>(use) & ((1u << 31) - 1u) & ~kRowUseT) == 0 &&
+         ((static_cast<unsigned>(use) & kRowUseT) & ((static_cast<unsigned>(use) & kRowUseT) - 1u)) == 0;
+}
 
 // Row-kernel work of one solve for one step.
 template <typename T, typename ST = T>

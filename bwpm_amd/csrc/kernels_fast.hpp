@@ -260,10 +260,14 @@ __global__ __launch_bounds__(N / 4 * RW) void k_row_fast(const RowDesc<T, ST>* _
     const cx<T>* ein = cplane(a.e_in);
 #pragma unroll
     for (int m = 0; m < 8; ++m) acc[m] = ein ? gload(ein, at(m)) : cx<T>{};
-  } else if (a.nt == 1) {
+  } else if (a.nt == 1 || single_t(use)) {
+    // one transmitted product serves every pixel of these rows: no merge
+    const int o = a.nt == 1 ? 0 : __builtin_ctz(static_cast<unsigned>(use) & kRowUseT);
 #pragma unroll
-    for (int m = 0; m < 8; ++m) acc[m] = sload<T>(a.prod, at(m));
+    for (int m = 0; m < 8; ++m) acc[m] = sload<T>(a.prod + o * plane, at(m));
     reg_fft<N, COMP, true>(acc, line, t, tw_hi, tw_lo);
+#pragma unroll
+    for (int m = 0; m < 8; ++m) settle(acc[m]);
   } else {
     // the selected pixels are collected in the LDS stash or in product plane 0
     px2<ST>* merged = const_cast<px2<ST>*>(a.prod);
@@ -275,6 +279,10 @@ __global__ __launch_bounds__(N / 4 * RW) void k_row_fast(const RowDesc<T, ST>* _
 #pragma unroll
       for (int m = 0; m < 8; ++m) acc[m] = sload<T>(src, at(m));
       reg_fft<N, COMP, true>(acc, line, t, tw_hi, tw_lo);
+      // the transform is complete before the selection branches, as on the one-product path, so
+      // every path (and both variants) gives the same bits
+#pragma unroll
+      for (int m = 0; m < 8; ++m) settle(acc[m]);
 #pragma unroll
       for (int m = 0; m < 8; ++m)
         if (gload1(a.tsel, row + t + TT * m) == o) {

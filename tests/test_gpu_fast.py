@@ -105,3 +105,44 @@ def test_batch_equals_alone_bitwise(precision: str) -> None:
     tol = 1e-10 if precision == "fp64" else 1e-5
     assert rel_l2(got["transmitted"], want["transmitted"]) < tol
     assert rel_l2(got["reflected"], want["reflected"]) < tol
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_interface_row_variants_give_the_same_bits(precision: str) -> None:
+    """Interface steps run the row kernel's LDS-merge variant when a launch holds a boundary whose
+    rows mostly select several label pairs, and the plain variant otherwise (merging through
+    product plane 0 where a workgroup's rows mix pairs; rows of one pair take their product
+    directly). Three 64² solves cross interfaces in the same launches: a thin cylinder (its disc
+    rows, ~20 %, mix pairs), a plane tilted along x (every row mixes pairs, so the lane takes the
+    merge variant) and one tilted along y (every row has one pair). Alone, the cylinder and the
+    y-tilt run the plain variant; in the lane they run the merge variant. Same bits required."""
+    import dataclasses
+
+    from bwpm_amd import _core, workloads
+    from bwpm_amd.specs import Above, Cylinder, Geometry
+
+    base = workloads.tilted_3d(n=64, nz=24, iterations=2, geometry=True)
+    dz, dx = base.dz, base.dx
+    tan = float(np.tan(np.deg2rad(10.0)))
+    probs = [dataclasses.replace(base, labels=Geometry(24, (Cylinder(1, 6 * dx, 10 * dz, 14 * dz),))),
+             dataclasses.replace(base, labels=Geometry(24, (Above(1, 12 * dz, tan, 0.0),))),
+             workloads.tilted_3d(n=64, nz=24, iterations=2, polarisation="y", geometry=True)]
+    alone = []
+    for p in probs:
+        s = _core.GpuSolver(*p.solver_args(), precision=precision)
+        try:
+            alone.append(s.run(p.field, p.iterations))
+        finally:
+            s.close()
+    solvers = [_core.GpuSolver(*p.solver_args(), precision=precision) for p in probs]
+    try:
+        for s, p in zip(solvers, probs):
+            s.upload(p.field)
+        _core.run_batch(solvers, 2)
+        for i, (s, ref) in enumerate(zip(solvers, alone)):
+            got = s.fetch(2)
+            for key in ("transmitted", "reflected"):
+                assert np.array_equal(got[key], ref[key]), (i, key)
+    finally:
+        for s in solvers:
+            s.close()
