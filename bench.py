@@ -177,18 +177,21 @@ def uneven_batch(count: int) -> dict:
             "peak_hbm_gb": round(st["peak_bytes"] / 1e9, 2)}
 
 
-def configs(c3_nz: int, c4_count: int) -> dict:
+def configs(c3_nz: int, c4_count: int, which: str = "c1,c2,c3,c4") -> dict:
     """The reference's other configurations: config 1 (2D, 1024 points, 200 slabs, 3 iterations),
     config 2 (3D 256² × 512 tilted interface, 4 iterations, E_x and E_y) and config 3 (3D 512² Born
     volume, `c3_nz` slabs, 6 iterations) as single solves; config 4 (uneven Born batch) on the
     queue. Errors of the mixed and FP32 modes are relative L2 against the FP64 device result
-    (itself within 1e-10 of the oracle)."""
+    (itself within 1e-10 of the oracle). `which` selects configurations (comma-separated)."""
     from bwpm_amd import workloads
 
+    sel = {w.strip() for w in which.split(",")}
     out: dict = {}
-    p1 = workloads.plain_2d(ny=1024, nz=200, iterations=3)
-    _, r1 = run_solve(p1, "fp64")
-    out["c1_2d_1024x200_it3"] = {"fp64_updates_per_s": r1}
+    if "c1" in sel:
+        p1 = workloads.plain_2d(ny=1024, nz=200, iterations=3)
+        _, r1 = run_solve(p1, "fp64")
+        out["c1_2d_1024x200_it3"] = {"fp64_updates_per_s": r1}
+
     def against(p, ref: dict) -> dict:
         # the same solve with complex64 streaming planes (FP64 arithmetic) and in FP32 arithmetic
         out_m = {}
@@ -198,20 +201,22 @@ def configs(c3_nz: int, c4_count: int) -> dict:
                         "rel_l2_reflected": rel_l2(r["reflected"], ref["reflected"])}
         return out_m
 
-    c2 = {}
-    for pol in ("x", "y"):
-        p2 = workloads.tilted_3d(n=256, nz=512, iterations=4, polarisation=pol)
-        a64, r64 = run_solve(p2, "fp64")
-        c2[f"E{pol}"] = {"fp64_updates_per_s": r64, **against(p2, a64)}
-    out["c2_tilted_256x256x512_it4"] = c2
-    # config 3 at full depth: the δn volume is generated on the device (a (1024, 512, 512) complex
-    # array would be 4 GiB on the host)
-    p3 = workloads.born_3d_synth(n=512, nz=c3_nz, iterations=6)
-    b64, q64 = run_solve(p3, "fp64")
-    out[f"c3_born_512x512x{c3_nz}_it6"] = {
-        "fp64_updates_per_s": q64, "fp64_gmres_iterations": b64["gmres_iterations"], **against(p3, b64),
-        "input": "Gaussian-correlated δn (σ = 0.01, correlation 1 µm, seed 3) generated on the device"}
-    if c4_count > 0:
+    if "c2" in sel:
+        c2 = {}
+        for pol in ("x", "y"):
+            p2 = workloads.tilted_3d(n=256, nz=512, iterations=4, polarisation=pol)
+            a64, r64 = run_solve(p2, "fp64")
+            c2[f"E{pol}"] = {"fp64_updates_per_s": r64, **against(p2, a64)}
+        out["c2_tilted_256x256x512_it4"] = c2
+    if "c3" in sel:
+        # config 3 at full depth: the δn volume is generated on the device (a (1024, 512, 512)
+        # complex array would be 4 GiB on the host)
+        p3 = workloads.born_3d_synth(n=512, nz=c3_nz, iterations=6)
+        b64, q64 = run_solve(p3, "fp64")
+        out[f"c3_born_512x512x{c3_nz}_it6"] = {
+            "fp64_updates_per_s": q64, "fp64_gmres_iterations": b64["gmres_iterations"], **against(p3, b64),
+            "input": "Gaussian-correlated δn (σ = 0.01, correlation 1 µm, seed 3) generated on the device"}
+    if c4_count > 0 and "c4" in sel:
         out[f"c4_uneven_born_{c4_count}"] = uneven_batch(c4_count)
     return out
 
@@ -271,6 +276,7 @@ def main() -> int:
                     help="requests of the served mix (0: no serving run)")
     ap.add_argument("--c3-nz", type=int, default=1024, help="slabs of the Born configuration (0: skip configs)")
     ap.add_argument("--c4-count", type=int, default=64, help="solves of the uneven Born batch (0: skip)")
+    ap.add_argument("--configs", default="c1,c2,c3,c4", help="configurations to run (comma-separated)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU figures are not produced",
@@ -367,7 +373,7 @@ def main() -> int:
     if checks is not None and args.precision == "fp64" and not args.no_modes:
         # the same lane with complex64 streaming planes (FP64 arithmetic) and in FP32 arithmetic
         modes = {m: precision_mode(probs, m, args.iterations, alone) for m in ("mixed", "fp32")}
-    other = configs(args.c3_nz, args.c4_count) if args.c3_nz > 0 else None
+    other = configs(args.c3_nz, args.c4_count, args.configs) if args.c3_nz > 0 else None
     serving = None
     if args.serve_requests > 0:
         serving = served(args.serve_requests, args.precision)

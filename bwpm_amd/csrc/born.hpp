@@ -660,7 +660,8 @@ class Born {
   const px2<T>* bwd_plane(long long p) const { return plane(p - 1, 0); }
   // born_fwd[p]: the forward scattering R3 of slab p − 1
   const px2<T>* fwd_plane(long long p) const { return plane(p - 1, 1); }
-  long long iterations() const { return iterations_; }
+  long long iterations() const { return iterations_; }  // GMRES iterations since the last reset
+  void reset_iterations() { iterations_ = 0; }
   size_t device_bytes() const { return bytes_; }
 
   // δn of every slab, (nz, ny, nx) complex, from the potential V = 2ñ₀k₀²δn held on the device.

@@ -465,6 +465,7 @@ class Engine final : public EngineBase {
   //  backward: e = born_bwd[nz]; for p = nz…1: [R3 → born_fwd[p]; e += R2] [BwdR → src_fwd[p]] BwdT;
   //            e += src_bwd[p−1] + born_bwd[p−1]
   std::vector<Op<T, ST>> program(int iterations, const std::vector<long long>& keep) {
+    if (born_) born_->reset_iterations();  // the result reports this run's GMRES iterations
     const long long nz = sp_.nz;
     const int nit = std::max(iterations, 1);
     const bool bidir = iterations > 0;
