@@ -406,6 +406,39 @@ __device__ __forceinline__ void block_sums(const double (&ar)[kMaxBasis], const 
   }
 }
 
+// The second stage of a two-stage reduction done by the workgroup that consumes the values (so no
+// separate launch): out[i] = Σ_b partial[i·kRedBlocks + b] for i < k ≤ kMaxBasis, into the LDS
+// array `out` for the whole workgroup; workgroup 0 also stores them to `res` (for the host). Lanes
+// work in groups of g = 64, 32, 16 or 8 (the most that give every value a group inside the
+// kRedThreads threads): a group sums one value's partials, each lane a contiguous slice, then a
+// butterfly inside the group. The order depends only on k, so the sums are deterministic. Every
+// thread of the workgroup (kRedThreads) must call it.
+__device__ __forceinline__ void final_sums(const double2* __restrict__ partial, int k, double2* out,
+                                           double2* __restrict__ res) {
+  static_assert(kRedThreads == 256 && kMaxBasis <= 32 && kRedBlocks % 64 == 0, "group layout");
+  const int lg = k <= 4 ? 6 : (k <= 8 ? 5 : (k <= 16 ? 4 : 3));  // log2 of the group size g
+  const int i = threadIdx.x >> lg, sl = threadIdx.x & ((1 << lg) - 1);
+  const int per = kRedBlocks >> lg;  // partials per lane
+  double r = 0, m = 0;
+  if (i < k) {
+    const double2* p = partial + i * kRedBlocks + sl * per;
+    for (int b = 0; b < per; ++b) {
+      const double2 v = p[b];
+      r += v.x;
+      m += v.y;
+    }
+  }
+  for (int o = 1 << (lg - 1); o > 0; o >>= 1) {
+    r += __shfl_xor(r, o, 64);
+    m += __shfl_xor(m, o, 64);
+  }
+  if (sl == 0 && i < k) {
+    out[i] = double2{r, m};
+    if (res && blockIdx.x == 0) res[i] = double2{r, m};
+  }
+  __syncthreads();
+}
+
 // w −= Σ_i c_i·B_i (the arithmetic of k_sub_basis), then the partial dots <B_i, w> of the updated
 // w with every B_i (i < k ≤ kMaxBasis): the first projection of Gram–Schmidt and the dots of the
 // second in one pass. Fixed grid kRedBlocks × kRedThreads, so the sums are deterministic.
@@ -473,13 +506,18 @@ __global__ __launch_bounds__(kRedThreads) void k_sub_norm_basis(px2<T>* w, const
   block_sums(ar, ai, 1, partial);
 }
 
-// out = w / sqrt(n2.x), with n2 (a squared norm) read from device memory; a zero norm leaves out.
+// out = w/‖w‖ with ‖w‖² the second stage of the partials `npart` (workgroup 0 stores it to res); a
+// zero norm leaves out. kRedThreads threads per workgroup.
 template <typename T>
-__global__ void k_normalize(const px2<T>* __restrict__ w, const double2* __restrict__ n2, px2<T>* out,
-                            long long npix) {
+__global__ __launch_bounds__(kRedThreads) void k_normalize_sum(const px2<T>* __restrict__ w,
+                                                               const double2* __restrict__ npart,
+                                                               double2* __restrict__ res, px2<T>* out,
+                                                               long long npix) {
+  __shared__ double2 n2[1];
+  final_sums(npart, 1, n2, res);
   const long long q = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;
   if (q >= npix) return;
-  const double nrm = sqrt(n2->x);
+  const double nrm = sqrt(n2[0].x);
   if (!(nrm > 0.0)) return;
   const T s = static_cast<T>(1.0 / nrm);
   const px2<T> u = w[q];
@@ -954,7 +992,8 @@ class Born {
     };
     static_assert(kRestart + 1 <= kMaxBasis, "the fused projections hold every basis vector");
     dot(basis, k, w, res);
-    // w −= B·h1 with the dots h2 = Bᴴw of the result, then w −= B·h2 with ‖w‖², two passes
+    // w −= B·h1 with the dots h2 = Bᴴw of the result, then w −= B·h2 with ‖w‖², then B_{j+1}
+    // (the normalisation finishes the norm's reduction itself)
     hipLaunchKernelGGL(k_sub_dot_basis<T>, dim3(kRedBlocks), dim3(kRedThreads), 0, stream_, w, basis,
                        static_cast<long long>(npix_), static_cast<const double2*>(res), k, npix_, part);
     HIP_CHECK(hipGetLastError());
@@ -963,9 +1002,8 @@ class Born {
     hipLaunchKernelGGL(k_sub_norm_basis<T>, dim3(kRedBlocks), dim3(kRedThreads), 0, stream_, w, basis,
                        static_cast<long long>(npix_), static_cast<const double2*>(res + kH2), k, npix_, part);
     HIP_CHECK(hipGetLastError());
-    hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(kRedThreads), 0, stream_, part, res + kN2);
-    HIP_CHECK(hipGetLastError());
-    launch(k_normalize<T>, static_cast<const px2<T>*>(w), static_cast<const double2*>(res + kN2), B(j + 1), npix_);
+    launch(k_normalize_sum<T>, static_cast<const px2<T>*>(w), static_cast<const double2*>(part), res + kN2, B(j + 1),
+           npix_);
     HIP_CHECK(hipMemcpyAsync(hres_, res, (kN2 + 1) * sizeof(double2), hipMemcpyDeviceToHost, stream_));
     HIP_CHECK(hipStreamSynchronize(stream_));
     for (int i = 0; i < k; ++i)
