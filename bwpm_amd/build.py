@@ -25,6 +25,10 @@ FLAGS = [
     "-Wno-unused-result",
     # packed FP32 instructions stay off: ordinary float code only
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # multiply-adds are fused within one source expression only (not across statements by the
+    # backend), so the rounding of a computation does not depend on the control flow around it
+    # and every kernel variant that computes a value the same way gives the same bits
+    "-ffp-contract=on",
 ]
 
 

@@ -69,15 +69,6 @@ __device__ __forceinline__ cx<T> pair_swap(cx<T> z) {
   return cx<T>{pair_swap(z.x), pair_swap(z.y)};
 }
 
-// An empty statement that takes z's registers as read and written: every instruction producing z
-// stays before it. Settling all values before the first conditional store keeps the compiler from
-// sinking the last arithmetic into the stores' branches, where multiply and add would no longer be
-// fused as on the unconditional path (and the results would differ in the last bits).
-template <typename T>
-__device__ __forceinline__ void settle(cx<T>& z) {
-  asm volatile("" : "+v"(z.x), "+v"(z.y));
-}
-
 // Operator product on one component (comp 0: E_x, 1: E_y) of spectrum entry k. The phase type is
 // the same for both components; the symmetric matrix type takes the partner lane's component:
 // E_x' = m00·E_x + m01·E_y, E_y' = m01·E_x + m11·E_y (terms in this order). Every lane of the
@@ -183,8 +174,6 @@ __global__ __launch_bounds__(N / 4 * W) void k_col_fast(const ColDesc<T, ST>* __
       // Only the interface variant does this; the plain variant keeps its registers.
       const unsigned bit = o >= d.nt ? static_cast<unsigned>(kRowUseR) : row_use_bit(o);
 #pragma unroll
-      for (int m = 0; m < 8; ++m) settle(s[m]);
-#pragma unroll
       for (int m = 0; m < 8; ++m)
         if ((use[m] >> bit) & 1u) sstore<T>(out, at(m), s[m]);
     } else {
@@ -266,8 +255,6 @@ __global__ __launch_bounds__(N / 4 * RW) void k_row_fast(const RowDesc<T, ST>* _
 #pragma unroll
     for (int m = 0; m < 8; ++m) acc[m] = sload<T>(a.prod + o * plane, at(m));
     reg_fft<N, COMP, true>(acc, line, t, tw_hi, tw_lo);
-#pragma unroll
-    for (int m = 0; m < 8; ++m) settle(acc[m]);
   } else {
     // the selected pixels are collected in the LDS stash or in product plane 0
     px2<ST>* merged = const_cast<px2<ST>*>(a.prod);
@@ -279,10 +266,6 @@ __global__ __launch_bounds__(N / 4 * RW) void k_row_fast(const RowDesc<T, ST>* _
 #pragma unroll
       for (int m = 0; m < 8; ++m) acc[m] = sload<T>(src, at(m));
       reg_fft<N, COMP, true>(acc, line, t, tw_hi, tw_lo);
-      // the transform is complete before the selection branches, as on the one-product path, so
-      // every path (and both variants) gives the same bits
-#pragma unroll
-      for (int m = 0; m < 8; ++m) settle(acc[m]);
 #pragma unroll
       for (int m = 0; m < 8; ++m)
         if (gload1(a.tsel, row + t + TT * m) == o) {
