@@ -316,3 +316,76 @@ def test_two_hosts_in_a_slab_is_a_value_error() -> None:
     fl[0] = 0.01
     with pytest.raises(ValueError, match="host"):
         _core().GpuSolver(4, 4, 1e-4, 1e-4, 1e-4, 1e4, [1.0, 1.5], labels, 1.0, fluctuation=fl)
+
+
+# ---- the device path against the reference's own C++ core (oracle/_ref, built from its sources) --
+
+def _reference_core():
+    import importlib.util
+
+    from conftest import ROOT
+
+    ref_dir = ROOT / "oracle" / "_ref"
+    so = sorted(ref_dir.glob("_core*.so")) if ref_dir.exists() else []
+    if not so:
+        pytest.skip("reference core not built (oracle/build_ref.sh)")
+    spec = importlib.util.spec_from_file_location("_core", so[0])
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
+@pytest.mark.parametrize("pol", ["x", "y"])
+def test_device_equals_reference_core_tilted_256(pol: str) -> None:
+    """The FP64 register-kernel path (interface steps with several label pairs, reflections, two
+    iterations) against the reference's C++ core on the 256² tilted geometry of config 2, 64 slabs:
+    agreement to 1e-12 of the field's maximum, and the same number of stored interface values."""
+    from bwpm_amd import _core, workloads
+
+    R = _reference_core()
+    p = workloads.tilted_3d(n=256, nz=64, iterations=2, polarisation=pol)
+    s = _core.GpuSolver(*p.solver_args(), precision="fp64")
+    try:
+        got = s.run(p.field, p.iterations, [0, 32, 64])
+    finally:
+        s.close()
+    ref = R.Solver(p.ny, p.nx, p.dy, p.dx, p.dz, p.k0, np.asarray(p.indices, complex), p.labels.astype(np.int32),
+                   complex(p.index_before), None, 16).run(p.field, p.iterations, [0, 32, 64])
+    scale = float(np.max(np.abs(p.field)))
+    for key in ("transmitted", "reflected"):
+        assert np.max(np.abs(got[key] - ref[key])) < 1e-12 * scale, key
+    for q in (0, 32, 64):
+        assert np.max(np.abs(got["forward"][q] - ref["forward"][q])) < 1e-12 * scale
+        assert np.max(np.abs(got["backward"][q] - ref["backward"][q])) < 1e-12 * scale
+    assert got["stored_values"] == ref["stored_values"]
+
+
+def test_device_equals_reference_core_born() -> None:
+    """Born slabs (δn, device GMRES) with the back-scattering paths over three iterations against
+    the reference's C++ core, 64² × 12 slabs: agreement to 1e-10 of the field's maximum (GMRES stops
+    at a relative residual of 1e-12 in both)."""
+    from bwpm_amd import _core
+
+    R = _reference_core()
+    rng = np.random.default_rng(7)
+    ny = nx = 64
+    nz = 12
+    dx, dz, lam = 0.1e-3, 0.15e-3, 0.8
+    k0 = 2 * np.pi / (lam * 1e-3)
+    labels = np.zeros((nz, ny, nx), dtype=np.int32)
+    labels[nz - 3:] = 1
+    fl = np.zeros((nz, ny, nx), complex)
+    fl[1:nz - 4] = 0.01 * rng.normal(size=(nz - 5, ny, nx)) + 1e-3j
+    x = (np.arange(nx) - nx / 2) * dx
+    field = np.zeros((ny, nx, 2), complex)
+    field[..., 0] = np.exp(-(x[None, :] ** 2 + x[:, None] ** 2) / (1.5e-3) ** 2)
+    indices = np.array([1.38, 1.5 + 0.002j])
+    s = _core.GpuSolver(ny, nx, dx, dx, dz, k0, indices, labels, 1.38 + 0j, None, fluctuation=fl, precision="fp64")
+    try:
+        got = s.run(field, 3)
+    finally:
+        s.close()
+    ref = R.Solver(ny, nx, dx, dx, dz, k0, indices, labels, 1.38 + 0j, None, 16, fluctuation=fl).run(field, 3)
+    scale = float(np.max(np.abs(field)))
+    for key in ("transmitted", "reflected"):
+        assert np.max(np.abs(got[key] - ref[key])) < 1e-10 * scale, key
