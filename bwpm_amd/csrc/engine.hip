@@ -122,6 +122,27 @@ void geometry_labels(const Spec& s, int32_t* out) {
   for (auto& t : th) t.join();
 }
 
+bool geometry_slab_labels(const Spec& s, std::vector<int32_t>& out) {
+  const LabelGeometry& g = s.geom;
+  for (const GeomPrim& p : g.prims)
+    if (p.kind == 2 || (p.kind == 0 && (p.a != 0.0 || p.b != 0.0))) return false;
+  require(g.background >= 0 && static_cast<size_t>(g.background) < s.indices.size(), "labels must index into `indices`");
+  for (const GeomPrim& p : g.prims) {
+    require(p.kind >= 0 && p.kind <= 2, "geometry primitive kind must be 0 (plane), 1 (layer) or 2 (cylinder)");
+    require(p.label >= 0 && static_cast<size_t>(p.label) < s.indices.size(), "labels must index into `indices`");
+  }
+  out.assign(static_cast<size_t>(s.nz), g.background);
+  for (long long j = 0; j < s.nz; ++j) {
+    const double z = g.z0 + (static_cast<double>(j) + 0.5) * s.dz;
+    for (const GeomPrim& p : g.prims) {
+      // a plane without tilt: z > d + 0·x + 0·y is z > d at every pixel
+      const bool in = p.kind == 1 ? (p.zlo < z && z <= p.zhi) : z > p.d;
+      if (in) out[j] = p.label;
+    }
+  }
+  return true;
+}
+
 LaneGraphCounters& lane_graph_counters() {
   static LaneGraphCounters c;
   return c;

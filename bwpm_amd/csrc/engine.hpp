@@ -39,6 +39,10 @@ struct LabelGeometry {
 };
 // Writes the (nz, ny, nx) labels of s.geom into out (several host threads).
 void geometry_labels(const struct Spec& s, int32_t* out);
+// When every primitive of s.geom depends on z only (layers, planes without tilt), each slab has one
+// label: writes the nz labels (the same pixel-centre rule as geometry_labels) and returns true;
+// otherwise returns false.
+bool geometry_slab_labels(const struct Spec& s, std::vector<int32_t>& out);
 
 struct Spec {
   long long ny = 0, nx = 0, nz = 0;

@@ -967,6 +967,7 @@ class Engine final : public EngineBase {
   int medium_of_label(int32_t l) const { return l < 0 ? before_id_ : label_medium_[l]; }
   const int32_t* plane_labels(long long p) const {
     if (p == 0) return before_.data();
+    if (!slab_plane_.empty()) return slab_plane_[p - 1];
     return sp_.uniform_label >= 0 ? uniform_plane_.data() : sp_.labels.data() + (p - 1) * npix_;
   }
 
@@ -974,16 +975,24 @@ class Engine final : public EngineBase {
     require(sp_.ny >= 1 && sp_.nx >= 1, "grid must have ny, nx >= 1");
     require(sp_.nz >= 1, "labels must have shape (nz, ny, nx) with nz >= 1");
     if (sp_.geom.on) {
-      // the label volume is made here from the geometry; the set-up frees it when done
       require(sp_.labels.empty() && sp_.uniform_label < 0, "a geometry replaces the label array");
-      sp_.labels.resize(static_cast<size_t>(sp_.nz * sp_.ny * sp_.nx));
-      geometry_labels(sp_, sp_.labels.data());
+      if (geometry_slab_labels(sp_, slab_label_)) {
+        // one label per slab: one uniform plane per distinct label instead of the volume
+        const long long npix = sp_.ny * sp_.nx;
+        for (int32_t l : slab_label_)
+          if (!label_planes_.count(l)) label_planes_[l].assign(static_cast<size_t>(npix), l);
+        for (int32_t l : slab_label_) slab_plane_.push_back(label_planes_.at(l).data());
+      } else {
+        // the label volume is made here from the geometry; the set-up frees it when done
+        sp_.labels.resize(static_cast<size_t>(sp_.nz * sp_.ny * sp_.nx));
+        geometry_labels(sp_, sp_.labels.data());
+      }
     }
     const Spec& s = sp_;
     if (s.uniform_label >= 0)
       require(s.labels.empty() && static_cast<size_t>(s.uniform_label) < s.indices.size(),
               "a uniform label must index into `indices` (and no label array be given)");
-    else
+    else if (slab_plane_.empty())
       require(static_cast<long long>(s.labels.size()) == s.nz * s.ny * s.nx, "labels must have shape (nz, ny, nx)");
     require(!s.synth.on || (s.fluct.empty() && s.synth.sigma >= 0 && s.synth.corr >= 0),
             "a synthesised fluctuation needs sigma >= 0, corr >= 0 and no fluctuation array");
@@ -1030,13 +1039,17 @@ class Engine final : public EngineBase {
     has_host_.assign(s.nz, 0);
     host_.assign(s.nz, c128(0.0, 0.0));
     if (s.synth.on && s.synth.sigma > 0) {
-      // a generated δn fills every slab: each slab must lie in one host index
+      // a generated δn fills every slab: each slab must lie in one host index (a label plane
+      // shared with the previous slab was checked already)
+      const int32_t* checked = nullptr;
       for (long long j = 0; j < s.nz; ++j) {
         const int32_t* lab = plane_labels(j + 1);
         const c128 idx = s.indices[lab[0]];
-        for (long long q = 1; q < npix_; ++q)
-          if (s.indices[lab[q]] != idx)
-            throw Error(kEinval, "slab " + std::to_string(j) + ": fluctuation must lie in one host index");
+        if (lab != checked)
+          for (long long q = 1; q < npix_; ++q)
+            if (s.indices[lab[q]] != idx)
+              throw Error(kEinval, "slab " + std::to_string(j) + ": fluctuation must lie in one host index");
+        checked = lab;
         has_host_[j] = 1;
         host_[j] = idx;
       }
@@ -1151,10 +1164,12 @@ class Engine final : public EngineBase {
     std::vector<int32_t> sp_pix, rowptr;
     std::vector<uint8_t> sp_rid;
     total_sparse_ = 0;
-    // same[p]: label plane p equals plane p − 1 (one comparison per plane)
+    // same[p]: label plane p equals plane p − 1 (one comparison per plane; the same plane object
+    // needs none)
     std::vector<char> same(nz + 1, 0);
     for (long long p = 1; p <= nz; ++p)
-      same[p] = std::memcmp(plane_labels(p), plane_labels(p - 1), npix_ * sizeof(int32_t)) == 0;
+      same[p] = plane_labels(p) == plane_labels(p - 1) ||
+                std::memcmp(plane_labels(p), plane_labels(p - 1), npix_ * sizeof(int32_t)) == 0;
     for (long long q = 0; q < nz; ++q) {
       const int32_t* a = plane_labels(q);
       const int32_t* b = plane_labels(q + 1);
@@ -1354,6 +1369,11 @@ class Engine final : public EngineBase {
   int before_id_ = 0;
   std::vector<int32_t> before_;
   std::vector<int32_t> uniform_plane_;  // the label plane of a uniform volume
+  // a geometry whose primitives depend on z only: the label of each slab, one uniform plane per
+  // distinct label, and each slab's plane (set-up only; no (nz, ny, nx) volume is made)
+  std::vector<int32_t> slab_label_;
+  std::map<int32_t, std::vector<int32_t>> label_planes_;
+  std::vector<const int32_t*> slab_plane_;
   std::vector<char> has_host_;
   std::vector<c128> host_;
   DevBuf kx_, ky_;

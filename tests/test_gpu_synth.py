@@ -118,7 +118,10 @@ def test_geometry_volumes_equal_label_arrays(precision: str) -> None:
     cases = [(workloads.tilted_3d(n=64, nz=30, iterations=2), workloads.tilted_3d(n=64, nz=30, iterations=2,
                                                                                   geometry=True)),
              (serve.planar_3d(96, 30, 2, 1.0, 1.0, serve.SILVER, layer=(10, 13), geometry=False),
-              serve.planar_3d(96, 30, 2, 1.0, 1.0, serve.SILVER, layer=(10, 13), geometry=True))]
+              serve.planar_3d(96, 30, 2, 1.0, 1.0, serve.SILVER, layer=(10, 13), geometry=True)),
+             # an untilted plane: one label per slab, kept without a label volume
+             (serve.planar_3d(128, 24, 2, 0.9, 1.0, 1.5, geometry=False),
+              serve.planar_3d(128, 24, 2, 0.9, 1.0, 1.5, geometry=True))]
     for a, g in cases:
         sa, sg = _solver(a, precision), _solver(g, precision)
         try:
