@@ -1,6 +1,8 @@
 // Non-templated parts of the engine: precision dispatch, batched lanes, the exported FP64
 // interface functions and the HBM copy probe.
 #include <algorithm>
+#include <atomic>
+#include <cmath>
 #include <chrono>
 #include <map>
 #include <thread>
@@ -77,19 +79,56 @@ std::unique_ptr<EngineBase> make_engine(Spec spec) {
   }
 }
 
-void geometry_labels(const Spec& s, int32_t* out) {
+namespace {
+
+void check_geometry(const Spec& s) {
   const LabelGeometry& g = s.geom;
   require(g.background >= 0 && static_cast<size_t>(g.background) < s.indices.size(), "labels must index into `indices`");
   for (const GeomPrim& p : g.prims) {
     require(p.kind >= 0 && p.kind <= 2, "geometry primitive kind must be 0 (plane), 1 (layer) or 2 (cylinder)");
     require(p.label >= 0 && static_cast<size_t>(p.label) < s.indices.size(), "labels must index into `indices`");
   }
-  const long long ny = s.ny, nx = s.nx, nz = s.nz, npix = ny * nx;
-  std::vector<double> xs(nx), ys(ny);
-  for (long long m = 0; m < nx; ++m) xs[m] = static_cast<double>(m - nx / 2) * s.dx;
-  for (long long n = 0; n < ny; ++n) ys[n] = static_cast<double>(n - ny / 2) * s.dy;
-  auto slab = [&](long long j) {
-    const double z = g.z0 + (static_cast<double>(j) + 0.5) * s.dz;
+}
+
+// d + a·x + b·y of a plane primitive and the distance test of a cylinder: the pixel-centre rule's
+// expressions, shared by both ways of making labels so that they agree bit for bit.
+inline double plane_rhs(const GeomPrim& p, double x, double y) { return p.d + p.a * x + p.b * y; }
+inline bool in_disc(const GeomPrim& p, double x, double y) {
+  const double u = x - p.cx, v = y - p.cy;
+  return std::sqrt(u * u + v * v) <= p.r;
+}
+inline double slab_z(const Spec& s, long long j) { return s.geom.z0 + (static_cast<double>(j) + 0.5) * s.dz; }
+
+void pixel_coords(const Spec& s, std::vector<double>& xs, std::vector<double>& ys) {
+  xs.resize(s.nx);
+  ys.resize(s.ny);
+  for (long long m = 0; m < s.nx; ++m) xs[m] = static_cast<double>(m - s.nx / 2) * s.dx;
+  for (long long n = 0; n < s.ny; ++n) ys[n] = static_cast<double>(n - s.ny / 2) * s.dy;
+}
+
+// Runs f(j) for j < n on up to 16 host threads.
+template <typename F>
+void parallel_for(long long n, F f) {
+  const unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  std::vector<std::thread> th;
+  std::atomic<long long> next{0};
+  for (unsigned t = 0; t < nt; ++t)
+    th.emplace_back([&] {
+      for (long long j = next++; j < n; j = next++) f(j);
+    });
+  for (auto& t : th) t.join();
+}
+
+}  // namespace
+
+void geometry_labels(const Spec& s, int32_t* out) {
+  check_geometry(s);
+  const LabelGeometry& g = s.geom;
+  const long long ny = s.ny, nx = s.nx, npix = ny * nx;
+  std::vector<double> xs, ys;
+  pixel_coords(s, xs, ys);
+  parallel_for(s.nz, [&](long long j) {
+    const double z = slab_z(s, j);
     int32_t* o = out + j * npix;
     std::fill(o, o + npix, g.background);
     for (const GeomPrim& p : g.prims) {
@@ -100,47 +139,115 @@ void geometry_labels(const Spec& s, int32_t* out) {
       }
       for (long long n = 0; n < ny; ++n)
         for (long long m = 0; m < nx; ++m) {
-          const double x = xs[m], y = ys[n];
-          bool in;
-          if (p.kind == 0) {
-            in = z > p.d + p.a * x + p.b * y;
-          } else {
-            const double u = x - p.cx, v = y - p.cy;
-            in = std::sqrt(u * u + v * v) <= p.r;
-          }
+          const bool in = p.kind == 0 ? z > plane_rhs(p, xs[m], ys[n]) : in_disc(p, xs[m], ys[n]);
           if (in) o[n * nx + m] = p.label;
         }
     }
-  };
-  const unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-  std::vector<std::thread> th;
-  std::atomic<long long> next{0};
-  for (unsigned t = 0; t < nt; ++t)
-    th.emplace_back([&] {
-      for (long long j = next++; j < nz; j = next++) slab(j);
-    });
-  for (auto& t : th) t.join();
+  });
 }
 
-bool geometry_slab_labels(const Spec& s, std::vector<int32_t>& out) {
+void geometry_planes(const Spec& s, std::vector<std::vector<int32_t>>& planes, std::vector<int>& plane_of) {
+  check_geometry(s);
   const LabelGeometry& g = s.geom;
-  for (const GeomPrim& p : g.prims)
-    if (p.kind == 2 || (p.kind == 0 && (p.a != 0.0 || p.b != 0.0))) return false;
-  require(g.background >= 0 && static_cast<size_t>(g.background) < s.indices.size(), "labels must index into `indices`");
-  for (const GeomPrim& p : g.prims) {
-    require(p.kind >= 0 && p.kind <= 2, "geometry primitive kind must be 0 (plane), 1 (layer) or 2 (cylinder)");
-    require(p.label >= 0 && static_cast<size_t>(p.label) < s.indices.size(), "labels must index into `indices`");
-  }
-  out.assign(static_cast<size_t>(s.nz), g.background);
-  for (long long j = 0; j < s.nz; ++j) {
-    const double z = g.z0 + (static_cast<double>(j) + 0.5) * s.dz;
-    for (const GeomPrim& p : g.prims) {
-      // a plane without tilt: z > d + 0·x + 0·y is z > d at every pixel
-      const bool in = p.kind == 1 ? (p.zlo < z && z <= p.zhi) : z > p.d;
-      if (in) out[j] = p.label;
+  const long long ny = s.ny, nx = s.nx, npix = ny * nx, nz = s.nz;
+  std::vector<double> xs, ys;
+  pixel_coords(s, xs, ys);
+  // per primitive, once: the planes' right-hand sides and their range, the cylinders' discs
+  struct Pre {
+    std::vector<double> rhs;
+    std::vector<char> disc;
+    double lo = 0, hi = 0;
+    bool any = false, all = false;
+  };
+  std::vector<Pre> pre(g.prims.size());
+  for (size_t k = 0; k < g.prims.size(); ++k) {
+    const GeomPrim& p = g.prims[k];
+    Pre& q = pre[k];
+    if (p.kind == 0) {
+      q.rhs.resize(npix);
+      for (long long n = 0; n < ny; ++n)
+        for (long long m = 0; m < nx; ++m) q.rhs[n * nx + m] = plane_rhs(p, xs[m], ys[n]);
+      const auto mm = std::minmax_element(q.rhs.begin(), q.rhs.end());
+      q.lo = *mm.first;
+      q.hi = *mm.second;
+    } else if (p.kind == 2) {
+      q.disc.resize(npix);
+      for (long long n = 0; n < ny; ++n)
+        for (long long m = 0; m < nx; ++m) q.disc[n * nx + m] = in_disc(p, xs[m], ys[n]) ? 1 : 0;
+      q.any = std::any_of(q.disc.begin(), q.disc.end(), [](char c) { return c != 0; });
+      q.all = std::all_of(q.disc.begin(), q.disc.end(), [](char c) { return c != 0; });
     }
   }
-  return true;
+  // pass 1: the slabs whose primitives are all uniform over the grid, and their label
+  std::vector<int32_t> uni_label(nz, -1);
+  for (long long j = 0; j < nz; ++j) {
+    const double z = slab_z(s, j);
+    bool uni = true;
+    int32_t lab = g.background;
+    for (size_t k = 0; k < g.prims.size(); ++k) {
+      const GeomPrim& p = g.prims[k];
+      const Pre& q = pre[k];
+      int state;  // the pixels a primitive covers in this slab: 0 none, 1 all, 2 some
+      if (p.kind == 0)
+        state = z > q.hi ? 1 : (z > q.lo ? 2 : 0);  // z > rhs at all, some or no pixels
+      else if (!(p.zlo < z && z <= p.zhi))
+        state = 0;
+      else
+        state = p.kind == 1 ? 1 : (q.all ? 1 : (q.any ? 2 : 0));
+      if (state == 1) {  // overwrites every pixel: uniform again
+        uni = true;
+        lab = p.label;
+      } else if (state == 2) {
+        uni = false;
+      }
+    }
+    if (uni) uni_label[j] = lab;
+  }
+  // pass 2: the other slabs' planes, in parallel
+  std::vector<long long> mixed;
+  for (long long j = 0; j < nz; ++j)
+    if (uni_label[j] < 0) mixed.push_back(j);
+  std::vector<std::vector<int32_t>> own(mixed.size());
+  parallel_for(static_cast<long long>(mixed.size()), [&](long long i) {
+    const long long j = mixed[i];
+    const double z = slab_z(s, j);
+    std::vector<int32_t>& o = own[i];
+    o.assign(npix, g.background);
+    for (size_t k = 0; k < g.prims.size(); ++k) {
+      const GeomPrim& p = g.prims[k];
+      if (p.kind != 0 && !(p.zlo < z && z <= p.zhi)) continue;
+      if (p.kind == 1) {
+        std::fill(o.begin(), o.end(), p.label);
+        continue;
+      }
+      const Pre& q = pre[k];
+      for (long long t = 0; t < npix; ++t)
+        if (p.kind == 0 ? z > q.rhs[t] : q.disc[t] != 0) o[t] = p.label;
+    }
+  });
+  // assemble: one plane per uniform label; a mixed plane unless it equals the previous slab's
+  planes.clear();
+  plane_of.assign(nz, -1);
+  std::map<int32_t, int> uni_plane;
+  size_t next_mixed = 0;
+  for (long long j = 0; j < nz; ++j) {
+    if (uni_label[j] >= 0) {
+      auto it = uni_plane.find(uni_label[j]);
+      if (it == uni_plane.end()) {
+        planes.emplace_back(npix, uni_label[j]);
+        it = uni_plane.emplace(uni_label[j], static_cast<int>(planes.size()) - 1).first;
+      }
+      plane_of[j] = it->second;
+      continue;
+    }
+    std::vector<int32_t>& o = own[next_mixed++];
+    if (j > 0 && uni_label[j - 1] < 0 && planes[plane_of[j - 1]] == o) {
+      plane_of[j] = plane_of[j - 1];
+    } else {
+      planes.push_back(std::move(o));
+      plane_of[j] = static_cast<int>(planes.size()) - 1;
+    }
+  }
 }
 
 LaneGraphCounters& lane_graph_counters() {

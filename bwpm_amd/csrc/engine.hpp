@@ -39,10 +39,12 @@ struct LabelGeometry {
 };
 // Writes the (nz, ny, nx) labels of s.geom into out (several host threads).
 void geometry_labels(const struct Spec& s, int32_t* out);
-// When every primitive of s.geom depends on z only (layers, planes without tilt), each slab has one
-// label: writes the nz labels (the same pixel-centre rule as geometry_labels) and returns true;
-// otherwise returns false.
-bool geometry_slab_labels(const struct Spec& s, std::vector<int32_t>& out);
+// The label planes of s.geom without the (nz, ny, nx) volume (the same pixel-centre rule as
+// geometry_labels): slab j's labels are planes[plane_of[j]]. A slab where every primitive is
+// uniform over the grid (layers, planes that do not cross the slab, cylinders outside their z
+// range) shares one uniform plane per label; the other slabs get planes of their own, and
+// neighbouring equal planes are stored once.
+void geometry_planes(const struct Spec& s, std::vector<std::vector<int32_t>>& planes, std::vector<int>& plane_of);
 
 struct Spec {
   long long ny = 0, nx = 0, nz = 0;

@@ -976,17 +976,10 @@ class Engine final : public EngineBase {
     require(sp_.nz >= 1, "labels must have shape (nz, ny, nx) with nz >= 1");
     if (sp_.geom.on) {
       require(sp_.labels.empty() && sp_.uniform_label < 0, "a geometry replaces the label array");
-      if (geometry_slab_labels(sp_, slab_label_)) {
-        // one label per slab: one uniform plane per distinct label instead of the volume
-        const long long npix = sp_.ny * sp_.nx;
-        for (int32_t l : slab_label_)
-          if (!label_planes_.count(l)) label_planes_[l].assign(static_cast<size_t>(npix), l);
-        for (int32_t l : slab_label_) slab_plane_.push_back(label_planes_.at(l).data());
-      } else {
-        // the label volume is made here from the geometry; the set-up frees it when done
-        sp_.labels.resize(static_cast<size_t>(sp_.nz * sp_.ny * sp_.nx));
-        geometry_labels(sp_, sp_.labels.data());
-      }
+      // the label planes of the slabs, without a (nz, ny, nx) volume (set-up only)
+      std::vector<int> plane_of;
+      geometry_planes(sp_, geo_planes_, plane_of);
+      for (int i : plane_of) slab_plane_.push_back(geo_planes_[i].data());
     }
     const Spec& s = sp_;
     if (s.uniform_label >= 0)
@@ -1369,10 +1362,8 @@ class Engine final : public EngineBase {
   int before_id_ = 0;
   std::vector<int32_t> before_;
   std::vector<int32_t> uniform_plane_;  // the label plane of a uniform volume
-  // a geometry whose primitives depend on z only: the label of each slab, one uniform plane per
-  // distinct label, and each slab's plane (set-up only; no (nz, ny, nx) volume is made)
-  std::vector<int32_t> slab_label_;
-  std::map<int32_t, std::vector<int32_t>> label_planes_;
+  // a geometry's distinct label planes and each slab's plane (set-up only; no volume is made)
+  std::vector<std::vector<int32_t>> geo_planes_;
   std::vector<const int32_t*> slab_plane_;
   std::vector<char> has_host_;
   std::vector<c128> host_;

@@ -135,6 +135,35 @@ def test_geometry_volumes_equal_label_arrays(precision: str) -> None:
         assert np.array_equal(ra["forward"][7], rg["forward"][7])
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_composite_geometry_equals_its_label_array(precision: str) -> None:
+    """A cylinder, a plane tilted in x and y over it and a layer: slabs where every primitive is
+    uniform, slabs the plane or the cylinder cuts, and runs of equal slabs. The solver's labels made
+    slab by slab must equal the materialised volume (bwpm_geometry_labels) bit for bit."""
+    from bwpm_amd.specs import Above, Cylinder
+
+    base = workloads.tilted_3d(n=64, nz=40, iterations=2, geometry=True)
+    dz, dx = base.dz, base.dx
+    base.indices = [1.0 + 0j, 1.46 + 0j, 1.3 + 0.01j]
+    geo = Geometry(40, (Cylinder(2, 12 * dx, 8 * dz, 20 * dz, 3 * dx, -2 * dx),
+                        Above(1, 24 * dz, 0.05, -0.08), Layer(2, 33 * dz, 35 * dz)))
+    lab = _core.geometry_labels(geo, base.ny, base.nx, base.dy, base.dx, base.dz, 3)
+    assert len(np.unique(lab)) == 3
+    outs = []
+    for labels in (geo, lab):
+        s = _core.GpuSolver(base.ny, base.nx, base.dy, base.dx, base.dz, base.k0, base.indices, labels,
+                            base.index_before, None, precision=precision)
+        try:
+            outs.append(s.run(base.field, 2, [10, 30]))
+        finally:
+            s.close()
+    for k in ("transmitted", "reflected"):
+        assert np.array_equal(outs[0][k], outs[1][k])
+    for q in (10, 30):
+        assert np.array_equal(outs[0]["forward"][q], outs[1]["forward"][q])
+    assert outs[0]["stored_values"] == outs[1]["stored_values"] > 0
+
+
 def test_geometry_with_generated_fluctuation_equals_the_label_array() -> None:
     geo = Geometry(12, (Layer(1, 4e-4, 8e-4),))
     p = workloads.born_3d_synth(n=32, nz=12, iterations=1, sigma=0.02, corr_um=0.2, seed=4)
