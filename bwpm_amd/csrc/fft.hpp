@@ -18,7 +18,7 @@
 
 namespace bwpm {
 
-inline constexpr int kThreads = 256;  // threads per workgroup of every FFT kernel
+inline constexpr int kThreads = 512;  // threads per workgroup of every FFT kernel
 inline constexpr int kMaxStages = 12;
 
 struct FftPlan {
