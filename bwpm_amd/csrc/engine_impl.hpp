@@ -114,6 +114,7 @@ struct Geometry {
   int ls_col1 = 1, ls_col2 = 1;
   int rw = 1;                     // fast: rows per workgroup (fast_rw or 1)
   int rows = 1, ls_row = 1;       // generic
+  bool lean_col = false, lean_row = false;  // generic: the plan's radices are 2, 3, 4 and 8 only
 };
 
 // ---- lines-per-workgroup autotuning ----------------------------------------------------------
@@ -262,7 +263,10 @@ struct Launcher {
     ColGeom g{static_cast<int>(ny), static_cast<int>(nx), geo.cols2, geo.ls_col2, npix, twy->plan};
     const size_t lds = static_cast<size_t>(ny) * g.ls * sizeof(cx<T>);
     const dim3 grid(static_cast<unsigned>((nx + g.cols - 1) / g.cols), S);
-    s(&k_col_step<T, COMP, ST>, grid, dim3(kThreads), lds, d, g);
+    if (geo.lean_col)
+      s(&k_col_step<T, COMP, ST, true>, grid, dim3(kThreads), lds, d, g);
+    else
+      s(&k_col_step<T, COMP, ST>, grid, dim3(kThreads), lds, d, g);
   }
 
   void row(const RowDesc<T, ST>* d, int S, Emit& s, bool multi) const {
@@ -280,7 +284,10 @@ struct Launcher {
     RowGeom g{static_cast<int>(ny), static_cast<int>(nx), geo.rows, geo.ls_row, npix, twx->plan};
     const size_t lds = 2 * static_cast<size_t>(nx) * g.ls * sizeof(cx<T>);
     const dim3 grid(static_cast<unsigned>((ny + g.rows - 1) / g.rows), S);
-    s(&k_row_step<T, COMP, ST>, grid, dim3(kThreads), lds, d, g);
+    if (geo.lean_row)
+      s(&k_row_step<T, COMP, ST, true>, grid, dim3(kThreads), lds, d, g);
+    else
+      s(&k_row_step<T, COMP, ST>, grid, dim3(kThreads), lds, d, g);
   }
 
  private:
@@ -1131,7 +1138,10 @@ class Engine final : public EngineBase {
               "grid size ny = " + std::to_string(s.ny) + " exceeds the column kernel's LDS budget");
       g.cols2 = w;
       g.ls_col2 = ls_of(2 * w);
+      g.lean_col = lean_plan(twy_->plan);
       HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_col_step<T, COMP, ST>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, kLdsMax));
+      HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_col_step<T, COMP, ST, true>),
                                     hipFuncAttributeMaxDynamicSharedMemorySize, kLdsMax));
     }
     if (!g.row_fast) {
@@ -1142,7 +1152,10 @@ class Engine final : public EngineBase {
               "grid size nx = " + std::to_string(s.nx) + " exceeds the row kernel's LDS budget");
       g.rows = rw;
       g.ls_row = ls_of(2 * rw);
+      g.lean_row = lean_plan(twx_->plan);
       HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_row_step<T, COMP, ST>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, kLdsMax));
+      HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_row_step<T, COMP, ST, true>),
                                     hipFuncAttributeMaxDynamicSharedMemorySize, kLdsMax));
     }
   }

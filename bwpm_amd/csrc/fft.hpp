@@ -216,15 +216,33 @@ __device__ __forceinline__ void fft_stage(cx<T>* buf, int n, int ls, int lines, 
   __syncthreads();
 }
 
+// A plan whose radices are all 2, 3, 4 or 8 (sizes 2^a·3^b): kernels for it compile only those
+// stages, which keeps them out of register spills.
+inline bool lean_plan(const FftPlan& p) {
+  for (int s = 0; s < p.nst; ++s)
+    if (p.rad[s] != 2 && p.rad[s] != 3 && p.rad[s] != 4 && p.rad[s] != 8) return false;
+  return true;
+}
+
 // Full transform of `lines` interleaved lines in LDS (element j of line l at buf[j·ls + l]).
-// Must be called by all threads of the workgroup.
-template <bool COMP, bool INV, typename T>
+// Must be called by all threads of the workgroup. LEAN: the plan is lean_plan's.
+template <bool COMP, bool INV, bool LEAN = false, typename T>
 __device__ void fft_lines(cx<T>* buf, int ls, int lines, const FftPlan& p) {
   const cx<T>* twh = static_cast<const cx<T>*>(p.tw_hi);
   const cx<T>* twl = static_cast<const cx<T>*>(p.tw_lo);
   int ns = 1;
   for (int s = 0; s < p.nst; ++s) {
     const int R = p.rad[s];
+    if constexpr (LEAN) {
+      switch (R) {
+        case 2: fft_stage<2, COMP, INV>(buf, p.n, ls, lines, ns, twh, twl); break;
+        case 3: fft_stage<3, COMP, INV>(buf, p.n, ls, lines, ns, twh, twl); break;
+        case 4: fft_stage<4, COMP, INV>(buf, p.n, ls, lines, ns, twh, twl); break;
+        default: fft_stage<8, COMP, INV>(buf, p.n, ls, lines, ns, twh, twl); break;
+      }
+      ns *= R;
+      continue;
+    }
     switch (R) {
       case 2: fft_stage<2, COMP, INV>(buf, p.n, ls, lines, ns, twh, twl); break;
       case 3: fft_stage<3, COMP, INV>(buf, p.n, ls, lines, ns, twh, twl); break;

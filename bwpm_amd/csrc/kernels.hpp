@@ -128,7 +128,7 @@ struct ColGeom {
   FftPlan plan;  // length ny
 };
 
-template <typename T, bool COMP, typename ST = T>
+template <typename T, bool COMP, typename ST = T, bool LEAN = false>
 __global__ __launch_bounds__(kThreads) void k_col_step(const ColDesc<T, ST>* __restrict__ descs, ColGeom g) {
   const ColDesc<T, ST> d = descs[blockIdx.y];
   if (!d.in) return;
@@ -145,7 +145,7 @@ __global__ __launch_bounds__(kThreads) void k_col_step(const ColDesc<T, ST>* __r
     A[y * ls + W + c] = v.b;
   }
   __syncthreads();
-  fft_lines<COMP, false>(A, ls, lines, g.plan);
+  fft_lines<COMP, false, LEAN>(A, ls, lines, g.plan);
   // several products: keep the spectrum in this block's own positions of X and reload it per
   // product, so one LDS buffer suffices (in the storage precision of X)
   px2<ST>* xio = const_cast<px2<ST>*>(d.in);
@@ -168,7 +168,7 @@ __global__ __launch_bounds__(kThreads) void k_col_step(const ColDesc<T, ST>* __r
       A[y * ls + W + c] = r.b;
     }
     __syncthreads();
-    fft_lines<COMP, true>(A, ls, lines, g.plan);
+    fft_lines<COMP, true, LEAN>(A, ls, lines, g.plan);
     px2<ST>* out = d.out + o * g.plane;
     for (int i = threadIdx.x; i < count; i += kThreads) {
       const int y = i / W, c = i - y * W, x = x0 + c;
@@ -196,7 +196,7 @@ __device__ __forceinline__ void load_rows(cx<T>* buf, const px2<U>* src, int y0,
   }
 }
 
-template <typename T, bool COMP, typename ST = T>
+template <typename T, bool COMP, typename ST = T, bool LEAN = false>
 __global__ __launch_bounds__(kThreads) void k_row_step(const RowDesc<T, ST>* __restrict__ descs, RowGeom g) {
   const RowDesc<T, ST> a = descs[blockIdx.y];
   if (!a.active) return;
@@ -223,12 +223,12 @@ __global__ __launch_bounds__(kThreads) void k_row_step(const RowDesc<T, ST>* __r
   } else if (a.nt == 1) {
     load_rows(acc, a.prod, y0, nrow, nx, rows, ls);
     __syncthreads();
-    fft_lines<COMP, true>(acc, ls, lines, g.plan);
+    fft_lines<COMP, true, LEAN>(acc, ls, lines, g.plan);
   } else {
     for (int o = 0; o < a.nt; ++o) {
       load_rows(work, a.prod + o * g.plane, y0, nrow, nx, rows, ls);
       __syncthreads();
-      fft_lines<COMP, true>(work, ls, lines, g.plan);
+      fft_lines<COMP, true, LEAN>(work, ls, lines, g.plan);
       for (int i = threadIdx.x; i < count; i += kThreads) {
         const int r = i / nx, x = i - r * nx;
         if (a.tsel[static_cast<long long>(y0 + r) * nx + x] == o) {
@@ -290,7 +290,7 @@ __global__ __launch_bounds__(kThreads) void k_row_step(const RowDesc<T, ST>* __r
     for (int o = 0; o < a.nr; ++o) {
       load_rows(work, a.prod + (a.nt + o) * g.plane, y0, nrow, nx, rows, ls);
       __syncthreads();
-      fft_lines<COMP, true>(work, ls, lines, g.plan);
+      fft_lines<COMP, true, LEAN>(work, ls, lines, g.plan);
       for (int e = e0 + threadIdx.x; e < e1; e += kThreads) {
         if (a.sp_rid[e] != o) continue;
         const int pix = a.sp_pix[e];
@@ -314,7 +314,7 @@ __global__ __launch_bounds__(kThreads) void k_row_step(const RowDesc<T, ST>* __r
 
   // 5. row FFT for the next slab step
   if (a.x_out) {
-    fft_lines<COMP, false>(acc, ls, lines, g.plan);
+    fft_lines<COMP, false, LEAN>(acc, ls, lines, g.plan);
     for (int i = threadIdx.x; i < count; i += kThreads) {
       const int r = i / nx, x = i - r * nx;
       a.x_out[static_cast<long long>(y0 + r) * nx + x] = widen<ST>(px2<T>{acc[x * ls + r], acc[x * ls + rows + r]});
