@@ -39,7 +39,7 @@ __device__ __forceinline__ px2<T> post(px2<T> v, const cx<T>* mul, long long q, 
 
 // ---- generic lengths -----------------------------------------------------------------------------
 
-template <typename T, bool COMP, bool INV>
+template <typename T, bool COMP, bool INV, bool LEAN = false>
 __global__ __launch_bounds__(kThreads) void k_fft2_rows(PlaneSet<T> ps, int ny, int nx, int rows, int ls, FftPlan plan) {
   const px2<T>* in = ps.in[blockIdx.y];
   px2<T>* out = ps.out[blockIdx.y];
@@ -50,7 +50,7 @@ __global__ __launch_bounds__(kThreads) void k_fft2_rows(PlaneSet<T> ps, int ny, 
   const int count = nrow * nx;
   load_rows(buf, in, y0, nrow, nx, rows, ls);
   __syncthreads();
-  fft_lines<COMP, INV>(buf, ls, 2 * rows, plan);
+  fft_lines<COMP, INV, LEAN>(buf, ls, 2 * rows, plan);
   for (int i = threadIdx.x; i < count; i += kThreads) {
     const int r = i / nx, x = i - r * nx;
     const long long q = static_cast<long long>(y0 + r) * nx + x;
@@ -58,7 +58,7 @@ __global__ __launch_bounds__(kThreads) void k_fft2_rows(PlaneSet<T> ps, int ny, 
   }
 }
 
-template <typename T, bool COMP, bool INV>
+template <typename T, bool COMP, bool INV, bool LEAN = false>
 __global__ __launch_bounds__(kThreads) void k_fft2_cols(PlaneSet<T> ps, int ny, int nx, int cols, int ls, FftPlan plan) {
   const px2<T>* in = ps.in[blockIdx.y];
   px2<T>* out = ps.out[blockIdx.y];
@@ -75,7 +75,7 @@ __global__ __launch_bounds__(kThreads) void k_fft2_cols(PlaneSet<T> ps, int ny, 
     buf[y * ls + W + c] = v.b;
   }
   __syncthreads();
-  fft_lines<COMP, INV>(buf, ls, 2 * W, plan);
+  fft_lines<COMP, INV, LEAN>(buf, ls, 2 * W, plan);
   for (int i = threadIdx.x; i < count; i += kThreads) {
     const int y = i / W, c = i - y * W, x = x0 + c;
     if (x >= nx) continue;
@@ -156,10 +156,16 @@ class Fft2D {
     cols_ = w;
     ls_col_ = ls_of(2 * w);
     require(nx_ * ls_row_ <= elems && ny_ * ls_col_ <= elems, "grid too large for the 2D transforms");
+    lean_rows_ = lean_plan(px_);
+    lean_cols_ = lean_plan(py_);
     for (const void* k : {reinterpret_cast<const void*>(&k_fft2_rows<T, COMP, false>),
                           reinterpret_cast<const void*>(&k_fft2_rows<T, COMP, true>),
                           reinterpret_cast<const void*>(&k_fft2_cols<T, COMP, false>),
-                          reinterpret_cast<const void*>(&k_fft2_cols<T, COMP, true>)})
+                          reinterpret_cast<const void*>(&k_fft2_cols<T, COMP, true>),
+                          reinterpret_cast<const void*>(&k_fft2_rows<T, COMP, false, true>),
+                          reinterpret_cast<const void*>(&k_fft2_rows<T, COMP, true, true>),
+                          reinterpret_cast<const void*>(&k_fft2_cols<T, COMP, false, true>),
+                          reinterpret_cast<const void*>(&k_fft2_cols<T, COMP, true, true>)})
       HIP_CHECK(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
   }
 
@@ -216,8 +222,13 @@ class Fft2D {
       }
     }
     const size_t lds = static_cast<size_t>(nx_) * ls_row_ * sizeof(cx<T>);
-    hipLaunchKernelGGL((k_fft2_rows<T, COMP, INV>), dim3(static_cast<unsigned>((ny_ + rows_ - 1) / rows_), np),
-                       dim3(kThreads), lds, stream_, ps, static_cast<int>(ny_), static_cast<int>(nx_), rows_, ls_row_, px_);
+    const dim3 grid(static_cast<unsigned>((ny_ + rows_ - 1) / rows_), np);
+    if (lean_rows_)
+      hipLaunchKernelGGL((k_fft2_rows<T, COMP, INV, true>), grid, dim3(kThreads), lds, stream_, ps, static_cast<int>(ny_),
+                         static_cast<int>(nx_), rows_, ls_row_, px_);
+    else
+      hipLaunchKernelGGL((k_fft2_rows<T, COMP, INV>), grid, dim3(kThreads), lds, stream_, ps, static_cast<int>(ny_),
+                         static_cast<int>(nx_), rows_, ls_row_, px_);
     HIP_CHECK(hipGetLastError());
   }
   template <int N, bool INV>
@@ -246,8 +257,13 @@ class Fft2D {
       }
     }
     const size_t lds = static_cast<size_t>(ny_) * ls_col_ * sizeof(cx<T>);
-    hipLaunchKernelGGL((k_fft2_cols<T, COMP, INV>), dim3(static_cast<unsigned>((nx_ + cols_ - 1) / cols_), np),
-                       dim3(kThreads), lds, stream_, ps, static_cast<int>(ny_), static_cast<int>(nx_), cols_, ls_col_, py_);
+    const dim3 grid(static_cast<unsigned>((nx_ + cols_ - 1) / cols_), np);
+    if (lean_cols_)
+      hipLaunchKernelGGL((k_fft2_cols<T, COMP, INV, true>), grid, dim3(kThreads), lds, stream_, ps, static_cast<int>(ny_),
+                         static_cast<int>(nx_), cols_, ls_col_, py_);
+    else
+      hipLaunchKernelGGL((k_fft2_cols<T, COMP, INV>), grid, dim3(kThreads), lds, stream_, ps, static_cast<int>(ny_),
+                         static_cast<int>(nx_), cols_, ls_col_, py_);
     HIP_CHECK(hipGetLastError());
   }
   template <int N, bool INV>
@@ -269,6 +285,7 @@ class Fft2D {
   FftPlan px_{}, py_{};
   hipStream_t stream_ = nullptr;
   bool rows_fast_ = false, cols_fast_ = false;
+  bool lean_rows_ = false, lean_cols_ = false;  // generic plans of radices 2, 3, 4 and 8 only
   int rows_ = 1, ls_row_ = 2, cols_ = 1, ls_col_ = 2;
 };
 
