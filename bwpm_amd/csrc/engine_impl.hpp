@@ -115,7 +115,17 @@ struct Geometry {
   int rw = 1;                     // fast: rows per workgroup (fast_rw or 1)
   int rows = 1, ls_row = 1;       // generic
   bool lean_col = false, lean_row = false;  // generic: the plan's radices are 2, 3, 4 and 8 only
+  // generic, for launches of few workgroups: lines per workgroup within kNarrowLds of LDS, so that
+  // several workgroups share a CU
+  int cols2n = 1, ls_col2n = 1, rowsn = 1, ls_rown = 1;
 };
+
+// LDS per workgroup of the generic kernels' narrow geometry, and the workgroup count (of the wide
+// geometry) below which a launch takes it. Wide workgroups are the more efficient ones as long as
+// a launch has enough of them: a lane of 16 192² solves (192 workgroups) runs 1.5× faster wide,
+// while a single 384² solve (48) runs 1.4× and a single 192² solve (12) 1.5× faster narrow.
+inline constexpr long long kNarrowLds = 65536;
+inline constexpr long long kNarrowBelow = 128;
 
 // ---- lines-per-workgroup autotuning ----------------------------------------------------------
 
@@ -260,7 +270,9 @@ struct Launcher {
       }
     }
     // generic: one LDS buffer of ny × ls per block (products reload the spectrum from X)
-    ColGeom g{static_cast<int>(ny), static_cast<int>(nx), geo.cols2, geo.ls_col2, npix, twy->plan};
+    const bool narrow = S * ((nx + geo.cols2 - 1) / geo.cols2) < kNarrowBelow;
+    ColGeom g{static_cast<int>(ny), static_cast<int>(nx), narrow ? geo.cols2n : geo.cols2,
+              narrow ? geo.ls_col2n : geo.ls_col2, npix, twy->plan};
     const size_t lds = static_cast<size_t>(ny) * g.ls * sizeof(cx<T>);
     const dim3 grid(static_cast<unsigned>((nx + g.cols - 1) / g.cols), S);
     if (geo.lean_col)
@@ -281,7 +293,9 @@ struct Launcher {
         default: row_fast_n<4096>(d, S, s, multi); return;
       }
     }
-    RowGeom g{static_cast<int>(ny), static_cast<int>(nx), geo.rows, geo.ls_row, npix, twx->plan};
+    const bool narrow = S * ((ny + geo.rows - 1) / geo.rows) < kNarrowBelow;
+    RowGeom g{static_cast<int>(ny), static_cast<int>(nx), narrow ? geo.rowsn : geo.rows, narrow ? geo.ls_rown : geo.ls_row,
+              npix, twx->plan};
     const size_t lds = 2 * static_cast<size_t>(nx) * g.ls * sizeof(cx<T>);
     const dim3 grid(static_cast<unsigned>((ny + g.rows - 1) / g.rows), S);
     if (geo.lean_row)
@@ -1138,6 +1152,9 @@ class Engine final : public EngineBase {
               "grid size ny = " + std::to_string(s.ny) + " exceeds the column kernel's LDS budget");
       g.cols2 = w;
       g.ls_col2 = ls_of(2 * w);
+      while (w > 1 && s.ny * ls_of(2 * w) * (long long)sizeof(cx<T>) > kNarrowLds) w /= 2;
+      g.cols2n = w;
+      g.ls_col2n = ls_of(2 * w);
       g.lean_col = lean_plan(twy_->plan);
       HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_col_step<T, COMP, ST>),
                                     hipFuncAttributeMaxDynamicSharedMemorySize, kLdsMax));
@@ -1152,6 +1169,9 @@ class Engine final : public EngineBase {
               "grid size nx = " + std::to_string(s.nx) + " exceeds the row kernel's LDS budget");
       g.rows = rw;
       g.ls_row = ls_of(2 * rw);
+      while (rw > 1 && 2 * s.nx * ls_of(2 * rw) * (long long)sizeof(cx<T>) > kNarrowLds) rw /= 2;
+      g.rowsn = rw;
+      g.ls_rown = ls_of(2 * rw);
       g.lean_row = lean_plan(twx_->plan);
       HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_row_step<T, COMP, ST>),
                                     hipFuncAttributeMaxDynamicSharedMemorySize, kLdsMax));
