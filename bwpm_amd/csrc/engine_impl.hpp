@@ -120,12 +120,6 @@ struct Geometry {
   int cols2n = 1, ls_col2n = 1, rowsn = 1, ls_rown = 1;
 };
 
-// LDS per workgroup of the generic kernels' narrow geometry, and the workgroup count (of the wide
-// geometry) below which a launch takes it. Wide workgroups are the more efficient ones as long as
-// a launch has enough of them: a lane of 16 192² solves (192 workgroups) runs 1.5× faster wide,
-// while a single 384² solve (48) runs 1.4× and a single 192² solve (12) 1.5× faster narrow.
-inline constexpr long long kNarrowLds = 65536;
-inline constexpr long long kNarrowBelow = 128;
 
 // ---- lines-per-workgroup autotuning ----------------------------------------------------------
 

@@ -156,6 +156,13 @@ class Fft2D {
     cols_ = w;
     ls_col_ = ls_of(2 * w);
     require(nx_ * ls_row_ <= elems && ny_ * ls_col_ <= elems, "grid too large for the 2D transforms");
+    // narrow geometry for launches of few workgroups (kNarrowLds of LDS per workgroup)
+    while (rw > 1 && nx_ * ls_of(2 * rw) * static_cast<long long>(sizeof(cx<T>)) > kNarrowLds) rw /= 2;
+    rows_n_ = rw;
+    ls_row_n_ = ls_of(2 * rw);
+    while (w > 1 && ny_ * ls_of(2 * w) * static_cast<long long>(sizeof(cx<T>)) > kNarrowLds) w /= 2;
+    cols_n_ = w;
+    ls_col_n_ = ls_of(2 * w);
     lean_rows_ = lean_plan(px_);
     lean_cols_ = lean_plan(py_);
     for (const void* k : {reinterpret_cast<const void*>(&k_fft2_rows<T, COMP, false>),
@@ -221,14 +228,16 @@ class Fft2D {
         default: rows_fast_n<2048, INV>(ps, np); return;
       }
     }
-    const size_t lds = static_cast<size_t>(nx_) * ls_row_ * sizeof(cx<T>);
-    const dim3 grid(static_cast<unsigned>((ny_ + rows_ - 1) / rows_), np);
+    const bool narrow = (ny_ + rows_ - 1) / rows_ * np < kNarrowBelow;
+    const int rows = narrow ? rows_n_ : rows_, ls = narrow ? ls_row_n_ : ls_row_;
+    const size_t lds = static_cast<size_t>(nx_) * ls * sizeof(cx<T>);
+    const dim3 grid(static_cast<unsigned>((ny_ + rows - 1) / rows), np);
     if (lean_rows_)
       hipLaunchKernelGGL((k_fft2_rows<T, COMP, INV, true>), grid, dim3(kThreads), lds, stream_, ps, static_cast<int>(ny_),
-                         static_cast<int>(nx_), rows_, ls_row_, px_);
+                         static_cast<int>(nx_), rows, ls, px_);
     else
       hipLaunchKernelGGL((k_fft2_rows<T, COMP, INV>), grid, dim3(kThreads), lds, stream_, ps, static_cast<int>(ny_),
-                         static_cast<int>(nx_), rows_, ls_row_, px_);
+                         static_cast<int>(nx_), rows, ls, px_);
     HIP_CHECK(hipGetLastError());
   }
   template <int N, bool INV>
@@ -256,14 +265,16 @@ class Fft2D {
         default: cols_fast_n<2048, INV>(ps, np); return;
       }
     }
-    const size_t lds = static_cast<size_t>(ny_) * ls_col_ * sizeof(cx<T>);
-    const dim3 grid(static_cast<unsigned>((nx_ + cols_ - 1) / cols_), np);
+    const bool narrow = (nx_ + cols_ - 1) / cols_ * np < kNarrowBelow;
+    const int cols = narrow ? cols_n_ : cols_, ls = narrow ? ls_col_n_ : ls_col_;
+    const size_t lds = static_cast<size_t>(ny_) * ls * sizeof(cx<T>);
+    const dim3 grid(static_cast<unsigned>((nx_ + cols - 1) / cols), np);
     if (lean_cols_)
       hipLaunchKernelGGL((k_fft2_cols<T, COMP, INV, true>), grid, dim3(kThreads), lds, stream_, ps, static_cast<int>(ny_),
-                         static_cast<int>(nx_), cols_, ls_col_, py_);
+                         static_cast<int>(nx_), cols, ls, py_);
     else
       hipLaunchKernelGGL((k_fft2_cols<T, COMP, INV>), grid, dim3(kThreads), lds, stream_, ps, static_cast<int>(ny_),
-                         static_cast<int>(nx_), cols_, ls_col_, py_);
+                         static_cast<int>(nx_), cols, ls, py_);
     HIP_CHECK(hipGetLastError());
   }
   template <int N, bool INV>
@@ -286,6 +297,7 @@ class Fft2D {
   hipStream_t stream_ = nullptr;
   bool rows_fast_ = false, cols_fast_ = false;
   bool lean_rows_ = false, lean_cols_ = false;  // generic plans of radices 2, 3, 4 and 8 only
+  int rows_n_ = 1, ls_row_n_ = 1, cols_n_ = 1, ls_col_n_ = 1;  // narrow geometry
   int rows_ = 1, ls_row_ = 2, cols_ = 1, ls_col_ = 2;
 };
 
