@@ -44,7 +44,7 @@ def _solver(a: dict, precision: str):
 
 
 @pytest.mark.parametrize("shape", [(64, 64), (128, 128), (256, 128), (512, 512), (512, 12), (12, 256), (4096, 1),
-                                   (2048, 2), (3072, 1), (384, 96)])
+                                   (2048, 2), (3072, 1), (384, 96), (80, 120), (1000, 1)])
 @pytest.mark.parametrize("precision", ["fp64", "mixed", "fp32"])
 def test_register_kernels_match_oracle(shape: tuple[int, int], precision: str) -> None:
     ny, nx = shape
@@ -146,3 +146,33 @@ def test_interface_row_variants_give_the_same_bits(precision: str) -> None:
     finally:
         for s in solvers:
             s.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_generic_kernels_narrow_and_wide_give_the_same_bits(precision: str) -> None:
+    """Generic sizes (96² = 2^5·3 lean, 80×120 with a radix-5 stage): a solve alone launches few
+    workgroups and takes the narrow geometry (less LDS, fewer rows per workgroup); in a lane of 28
+    the launch is large enough for the wide one. Lines are transformed alike either way."""
+    from bwpm_amd import _core
+
+    for ny, nx in ((96, 96), (80, 120)):
+        cases = [layered_volume(ny, nx, 8, seed=40 + i) for i in range(28)]
+        alone = []
+        for a in cases[:3]:
+            s = _solver(a, precision)
+            try:
+                alone.append(s.run(a["field"], 2))
+            finally:
+                s.close()
+        solvers = [_solver(a, precision) for a in cases]
+        try:
+            for s, a in zip(solvers, cases):
+                s.upload(a["field"])
+            _core.run_batch(solvers, 2, max_lane=28)
+            for s, ref in zip(solvers[:3], alone):
+                got = s.fetch(2)
+                for key in ("transmitted", "reflected"):
+                    assert np.array_equal(got[key], ref[key]), (ny, nx, key)
+        finally:
+            for s in solvers:
+                s.close()
