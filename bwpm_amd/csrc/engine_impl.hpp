@@ -237,6 +237,16 @@ struct Launcher {
   long long ny = 0, nx = 0, npix = 0;
   Geometry geo;
   std::shared_ptr<Twiddles<T>> twx, twy;
+  // solves of a launch that do work (0: all S); the generic kernels size their workgroups by it,
+  // since the workgroups of idle slots return at once
+  int busy = 0;
+
+  // bit 0: the generic column kernel takes the narrow geometry; bit 1: the generic row kernel
+  int narrow_mask(int S) const {
+    const long long n = busy > 0 ? busy : S;
+    return (n * ((nx + geo.cols2 - 1) / geo.cols2) < kNarrowBelow ? 1 : 0) |
+           (n * ((ny + geo.rows - 1) / geo.rows) < kNarrowBelow ? 2 : 0);
+  }
 
   // multi: some solve of the launch has several products at this step (register kernels keep
   // their spectrum in LDS); the result bits do not depend on it.
@@ -264,7 +274,7 @@ struct Launcher {
       }
     }
     // generic: one LDS buffer of ny × ls per block (products reload the spectrum from X)
-    const bool narrow = S * ((nx + geo.cols2 - 1) / geo.cols2) < kNarrowBelow;
+    const bool narrow = (narrow_mask(S) & 1) != 0;
     ColGeom g{static_cast<int>(ny), static_cast<int>(nx), narrow ? geo.cols2n : geo.cols2,
               narrow ? geo.ls_col2n : geo.ls_col2, npix, twy->plan};
     const size_t lds = static_cast<size_t>(ny) * g.ls * sizeof(cx<T>);
@@ -287,7 +297,7 @@ struct Launcher {
         default: row_fast_n<4096>(d, S, s, multi); return;
       }
     }
-    const bool narrow = S * ((ny + geo.rows - 1) / geo.rows) < kNarrowBelow;
+    const bool narrow = (narrow_mask(S) & 2) != 0;
     RowGeom g{static_cast<int>(ny), static_cast<int>(nx), narrow ? geo.rowsn : geo.rows, narrow ? geo.ls_rown : geo.ls_row,
               npix, twx->plan};
     const size_t lds = 2 * static_cast<size_t>(nx) * g.ls * sizeof(cx<T>);
@@ -738,7 +748,7 @@ class Engine final : public EngineBase {
     PinnedBuf hcol[2] = {PinnedBuf(n * sizeof(ColDesc<T, ST>)), PinnedBuf(n * sizeof(ColDesc<T, ST>))};
     PinnedBuf hrow[2] = {PinnedBuf(n * sizeof(RowDesc<T, ST>)), PinnedBuf(n * sizeof(RowDesc<T, ST>))};
     std::vector<char> has_col(chunk), has_row(chunk), multi_col(chunk), multi_row(chunk);
-    std::map<std::tuple<size_t, int, int>, hipGraphExec_t> graphs;
+    std::map<std::tuple<size_t, int, int, int>, hipGraphExec_t> graphs;
     bool graphs_ok = true;
     auto drop_graphs = [&] {
       for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
@@ -822,6 +832,7 @@ class Engine final : public EngineBase {
         Launcher<T, COMP, ST> Lc = L;
         Lc.geo.col_w = choice.first;
         Lc.geo.rw = choice.second;
+        Lc.busy = active;
         auto launch_chunk = [&](Emit& em) {
           const bool eager = em.graph == nullptr;
           for (size_t g = 0; g < G; ++g) {
@@ -853,7 +864,7 @@ class Engine final : public EngineBase {
           tune_report(tk, trial, 1e3 * ms / (static_cast<double>(G) * active));
         } else if (uniform && graphs_ok) {
           // uniform chunks replay one captured graph per shape (descriptors live in dcol/drow)
-          const auto gk = std::make_tuple(G, Lc.geo.col_w, Lc.geo.rw);
+          const auto gk = std::make_tuple(G, Lc.geo.col_w, Lc.geo.rw, Lc.narrow_mask(S));
           auto it = graphs.find(gk);
           if (it == graphs.end()) {
             // the graph is built node by node (no stream capture, so other threads of the

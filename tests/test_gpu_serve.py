@@ -140,3 +140,18 @@ def test_replayed_lane_graphs_match_eager_solves() -> None:
     after = _core.lane_graph_stats()
     assert after["built"] > before["built"] and after["replays"] > before["replays"] + 6
     assert after["fallbacks"] == before["fallbacks"]
+
+
+def test_few_generic_solves_in_a_wide_lane() -> None:
+    # a 96² grid runs the generic kernels: with 2 of 16 slots busy the lane launches the narrow
+    # geometry, with 16 busy the wide one; every result equals its solve alone
+    few = [workloads.tilted_3d(n=96, nz=24 + 4 * i, iterations=1 + i, polarisation="xy"[i]) for i in range(2)]
+    hom = serve.planar_3d(96, 30, 1, 1.0, 1.0, 1.0)
+    many = [workloads.tilted_3d(n=96, nz=12 + i, iterations=1) for i in range(16)]
+    with serve.Server(slots=16, chunk=8) as srv:
+        t_few = [srv.submit(p) for p in few + [hom]]
+        r_few = [srv.result(t, timeout=120) for t in t_few]
+        t_many = [srv.submit(p) for p in many]
+        r_many = [srv.result(t, timeout=120) for t in t_many]
+    for p, r in zip(few + [hom] + many, r_few + r_many):
+        _same(r, _alone(p, "fp64"))
