@@ -11,7 +11,8 @@
 //
 // In the px2 form the two field components are transformed one after the other inside each
 // butterfly, which halves the live temporaries; the cx form transforms a single component. Twiddles
-// are read from a table the kernel stages in LDS once (or from global memory).
+// are read from the table W_N^e in global memory, or (cx form) from a copy the kernel stages in LDS
+// once, laid out per pass so that the lanes of a wavefront read consecutive entries.
 // LDS index i is padded to i + i/8 to spread the stride-8 writes of the first pass over banks.
 #pragma once
 
@@ -39,19 +40,46 @@ constexpr int reg_lds_len() {
 }
 __device__ __forceinline__ int lpad(int i) { return i + (i >> 3); }
 
-// Twiddle exponents e = r·k·N/(Ns·R) of every pass stay below (R − 1)·N/R ≤ 7N/8.
+// Staged twiddle layout: pass p ≥ 1 (stride Ns, radix R) keeps W_N^{r·k·N/(Ns·R)} at
+// reg_tw_off(p) + (r − 1)·Ns + k for r in [1, R), k in [0, Ns). In the table W_N^e the lanes of a
+// pass read entries r·N/(Ns·R) apart (all in one LDS bank group in the second pass); here
+// consecutive k are consecutive entries. Pass 0 (Ns = 1) takes no twiddles.
+template <int N>
+constexpr int reg_tw_off(int pass) {
+  int off = 0, ns = RegPlan<N>::rad[0];
+  for (int p = 1; p < pass; ++p) {
+    off += (RegPlan<N>::rad[p] - 1) * ns;
+    ns *= RegPlan<N>::rad[p];
+  }
+  return off;
+}
 template <int N>
 constexpr int reg_tw_len() {
-  return N < 8 ? N : 7 * N / 8;
+  return reg_tw_off<N>(RegPlan<N>::P);
 }
 
+template <int N, int PASS, int NS, bool COMP, typename T>
+__device__ __forceinline__ void stage_tw_pass(cx<T>* sh_hi, cx<T>* sh_lo, const cx<T>* __restrict__ g_hi,
+                                              const cx<T>* __restrict__ g_lo) {
+  if constexpr (PASS < RegPlan<N>::P) {
+    constexpr int R = RegPlan<N>::rad[PASS];
+    constexpr int off = reg_tw_off<N>(PASS);
+    for (int i = threadIdx.x; i < (R - 1) * NS; i += blockDim.x) {
+      const int r = i / NS + 1, k = i % NS;
+      const int e = r * k * (N / (NS * R));
+      sh_hi[off + i] = g_hi[e];
+      if constexpr (COMP) sh_lo[off + i] = g_lo[e];
+    }
+    stage_tw_pass<N, PASS + 1, NS * R, COMP>(sh_hi, sh_lo, g_hi, g_lo);
+  }
+}
+
+// Copies the twiddles of every pass from the table W_N^e (g_hi, and g_lo for COMP) into the staged
+// layout in LDS (reg_tw_len<N>() entries each); the cx-form reg_fft reads them with STAGED.
 template <int N, bool COMP, typename T>
 __device__ __forceinline__ void stage_twiddles(cx<T>* sh_hi, cx<T>* sh_lo, const cx<T>* __restrict__ g_hi,
                                                const cx<T>* __restrict__ g_lo) {
-  for (int i = threadIdx.x; i < reg_tw_len<N>(); i += blockDim.x) {
-    sh_hi[i] = g_hi[i];
-    if constexpr (COMP) sh_lo[i] = g_lo[i];
-  }
+  stage_tw_pass<N, 1, RegPlan<N>::rad[0], COMP>(sh_hi, sh_lo, g_hi, g_lo);
   __syncthreads();
 }
 
@@ -122,7 +150,7 @@ __device__ __forceinline__ void reg_pass(px2<T> (&v)[8], px2<T>* lds, int t, con
 
 // One field component per thread: the same passes on v[m] = a[t + T·m] of a single complex line
 // (the slab-step kernels give the two components of a pixel to two adjacent lanes).
-template <int N, int PASS, int NS, bool COMP, bool INV, typename T>
+template <int N, int PASS, int NS, bool COMP, bool INV, bool STAGED, typename T>
 __device__ __forceinline__ void reg_pass1(cx<T> (&v)[8], cx<T>* lds, int t, const cx<T>* twh, const cx<T>* twl) {
   using PL = RegPlan<N>;
   if constexpr (PASS < PL::P) {
@@ -141,7 +169,7 @@ __device__ __forceinline__ void reg_pass1(cx<T> (&v)[8], cx<T>* lds, int t, cons
       if constexpr (NS > 1) {
 #pragma unroll
         for (int r = 1; r < R; ++r) {
-          const int e = r * k * (N / (NS * R));
+          const int e = STAGED ? reg_tw_off<N>(PASS) + (r - 1) * NS + k : r * k * (N / (NS * R));
           const cx<T> wh = INV ? conj(twh[e]) : twh[e];
           cx<T> wl{};
           if constexpr (COMP) wl = INV ? conj(twl[e]) : twl[e];
@@ -164,21 +192,22 @@ __device__ __forceinline__ void reg_pass1(cx<T> (&v)[8], cx<T>* lds, int t, cons
       for (int m = 0; m < 8; ++m) v[m] = lds[lpad(t + TT * m)];
       __syncthreads();
     }
-    reg_pass1<N, PASS + 1, NS * R, COMP, INV>(v, lds, t, twh, twl);
+    reg_pass1<N, PASS + 1, NS * R, COMP, INV, STAGED>(v, lds, t, twh, twl);
   }
 }
 
 // Transform one line held as v[m] = a[t + T·m]; on return v[m] = A[t + T·m]. All threads of the
 // workgroup must call it together (it synchronises the workgroup between passes). twh/twl point
-// to the twiddle table W_N^e (LDS or global). The px2 form transforms both components of a pixel
-// vector in one thread, the cx form one component.
+// to the twiddle table W_N^e (LDS or global), or with STAGED (cx form) to the copy that
+// stage_twiddles made. The px2 form transforms both components of a pixel vector in one thread,
+// the cx form one component.
 template <int N, bool COMP, bool INV, typename T>
 __device__ __forceinline__ void reg_fft(px2<T> (&v)[8], px2<T>* lds, int t, const cx<T>* twh, const cx<T>* twl) {
   reg_pass<N, 0, 1, COMP, INV>(v, lds, t, twh, twl);
 }
-template <int N, bool COMP, bool INV, typename T>
+template <int N, bool COMP, bool INV, bool STAGED = false, typename T>
 __device__ __forceinline__ void reg_fft(cx<T> (&v)[8], cx<T>* lds, int t, const cx<T>* twh, const cx<T>* twl) {
-  reg_pass1<N, 0, 1, COMP, INV>(v, lds, t, twh, twl);
+  reg_pass1<N, 0, 1, COMP, INV, STAGED>(v, lds, t, twh, twl);
 }
 
 }  // namespace bwpm

@@ -25,13 +25,14 @@ template <int N, int SKEW>
 constexpr int fast_line_stride() {
   return reg_lds_len<N>() + SKEW;
 }
-constexpr int kColSkew = 1;
+constexpr int kColSkew = 2;
 constexpr int kRowSkew = 4;
 
-// Static LDS of a kernel with `lines` component lines of n points (complex size csize).
+// Static LDS of a kernel with `lines` component lines of n points (complex size csize), with the
+// staged twiddles (n − 8 entries for plans that start with radix 8: reg_tw_len).
 constexpr long long fast_lds_bytes(int n, int lines, int skew, int csize, bool comp) {
   return static_cast<long long>(lines) * (n + n / 8 + skew) * csize +
-         static_cast<long long>(7 * n / 8) * csize * (comp ? 2 : 1);
+         static_cast<long long>(n - 8) * csize * (comp ? 2 : 1);
 }
 
 // Lines (columns or rows) per workgroup: 256 threads (n/4 per line) where the line allows, and at
@@ -137,7 +138,7 @@ __global__ __launch_bounds__(N / 4 * W) void k_col_fast(const ColDesc<T, ST>* __
   cx<T> s[8];
 #pragma unroll
   for (int m = 0; m < 8; ++m) s[m] = sload<T>(xio, at(m));
-  reg_fft<N, COMP, false>(s, line, t, tw_hi, tw_lo);
+  reg_fft<N, COMP, false, TWL>(s, line, t, tw_hi, tw_lo);
   // several products: keep the spectrum (in the LDS stash, or in the thread's own positions of X);
   // stash entry m of this lane is own[m·T], its component line is N + 1 entries long
   cx<T>* own = stash;
@@ -165,7 +166,7 @@ __global__ __launch_bounds__(N / 4 * W) void k_col_fast(const ColDesc<T, ST>* __
     const Prod<T> pd{uniform(gload1(&pp->type, 0)), uniform(gload1(&pp->hi, 0)), uniform(gload1(&pp->lo, 0))};
 #pragma unroll
     for (int m = 0; m < 8; ++m) s[m] = apply_prod1<T, COMP>(pd, pix(m), comp, s[m]);
-    reg_fft<N, COMP, true>(s, line, t, tw_hi, tw_lo);
+    reg_fft<N, COMP, true, TWL>(s, line, t, tw_hi, tw_lo);
     px2<ST>* out = d.out + o * plane;
     if constexpr (MULTI) {
       // a product is stored only in the rows whose row transforms use it: rows with a pixel that
@@ -234,7 +235,7 @@ __global__ __launch_bounds__(N / 4 * RW) void k_row_fast(const RowDesc<T, ST>* _
       const px2<ST>* src = a.prod + (a.nt + o) * plane;
 #pragma unroll
       for (int m = 0; m < 8; ++m) acc[m] = sload<T>(src, at(m));
-      reg_fft<N, COMP, true>(acc, line, t, tw_hi, tw_lo);
+      reg_fft<N, COMP, true, TWL>(acc, line, t, tw_hi, tw_lo);
 #pragma unroll
       for (int m = 0; m < 8; ++m) line[lpad(t + TT * m)] = acc[m];
       __syncthreads();
@@ -254,7 +255,7 @@ __global__ __launch_bounds__(N / 4 * RW) void k_row_fast(const RowDesc<T, ST>* _
     const int o = a.nt == 1 ? 0 : __builtin_ctz(static_cast<unsigned>(use) & kRowUseT);
 #pragma unroll
     for (int m = 0; m < 8; ++m) acc[m] = sload<T>(a.prod + o * plane, at(m));
-    reg_fft<N, COMP, true>(acc, line, t, tw_hi, tw_lo);
+    reg_fft<N, COMP, true, TWL>(acc, line, t, tw_hi, tw_lo);
   } else {
     // the selected pixels are collected in the LDS stash or in product plane 0
     px2<ST>* merged = const_cast<px2<ST>*>(a.prod);
@@ -265,7 +266,7 @@ __global__ __launch_bounds__(N / 4 * RW) void k_row_fast(const RowDesc<T, ST>* _
       const px2<ST>* src = a.prod + o * plane;
 #pragma unroll
       for (int m = 0; m < 8; ++m) acc[m] = sload<T>(src, at(m));
-      reg_fft<N, COMP, true>(acc, line, t, tw_hi, tw_lo);
+      reg_fft<N, COMP, true, TWL>(acc, line, t, tw_hi, tw_lo);
 #pragma unroll
       for (int m = 0; m < 8; ++m)
         if (gload1(a.tsel, row + t + TT * m) == o) {
@@ -332,7 +333,7 @@ __global__ __launch_bounds__(N / 4 * RW) void k_row_fast(const RowDesc<T, ST>* _
 
   // 5. row FFT for the next slab step
   if (a.x_out) {
-    reg_fft<N, COMP, false>(acc, line, t, tw_hi, tw_lo);
+    reg_fft<N, COMP, false, TWL>(acc, line, t, tw_hi, tw_lo);
 #pragma unroll
     for (int m = 0; m < 8; ++m) sstore<T>(a.x_out, at(m), acc[m]);
   }
