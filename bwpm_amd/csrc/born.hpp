@@ -123,6 +123,15 @@ __global__ void k_born_finish_op(const px2<T>* __restrict__ xy, const px2<T>* __
 
 // The parts of the per-k table the operator reads, as three planes per host medium (so each pass
 // reads only the values it uses): 1/k_z or 0, k_z, i dz/(2k_z) or 0.
+// The compensated FP32 passes (two twiddle tables, hi and lo) stage the twiddles in LDS where they
+// fit beside the line buffers (`lines_bytes`) within one workgroup's 160 KB. The FP64 passes read
+// the one table from global memory: at 512² staging took them 0.7 % longer (one workgroup less per
+// CU), against 2.4 % less time in FP32.
+template <int N, typename T, bool COMP>
+constexpr bool op_tw_in_lds(long long lines_bytes) {
+  return COMP && lines_bytes + static_cast<long long>(reg_tw_len<N>()) * sizeof(cx<T>) * 2 <= 163840;
+}
+
 template <typename T>
 struct BornOp {
   const cx<T>* cz;
@@ -145,11 +154,17 @@ template <int N, int RW, typename T, bool COMP>
 __global__ __launch_bounds__(N / 8 * RW) void k_born_op_rows_inv(const px2<T>* __restrict__ v, BornOp<T> op,
                                                                 const T* __restrict__ kxT, const T* __restrict__ kyT,
                                                                 T d, px2<T>* __restrict__ xy, cx<T>* __restrict__ zl,
-                                                                const cx<T>* __restrict__ twh,
-                                                                const cx<T>* __restrict__ twl) {
+                                                                const cx<T>* __restrict__ g_twh,
+                                                                const cx<T>* __restrict__ g_twl) {
   constexpr int TT = N / 8;
   constexpr int LL = reg_lds_len<N>();
   __shared__ px2<T> sh[RW * LL];
+  constexpr bool TWL = op_tw_in_lds<N, T, COMP>(sizeof(sh));
+  __shared__ cx<T> tw_hi_s[TWL ? reg_tw_len<N>() : 1];
+  __shared__ cx<T> tw_lo_s[TWL && COMP ? reg_tw_len<N>() : 1];
+  if constexpr (TWL) stage_twiddles<N, COMP>(tw_hi_s, tw_lo_s, g_twh, g_twl);
+  const cx<T>* twh = TWL ? tw_hi_s : g_twh;
+  const cx<T>* twl = TWL ? tw_lo_s : g_twl;
   const int r = threadIdx.x / TT, t = threadIdx.x % TT;
   const int y = blockIdx.x * RW + r;
   const long long row = static_cast<long long>(y) * N;
@@ -163,7 +178,7 @@ __global__ __launch_bounds__(N / 8 * RW) void k_born_op_rows_inv(const px2<T>* _
       const px2<T> s = v[q];
       a[m] = prop ? s : px2<T>{};
     }
-    reg_fft<N, COMP, true>(a, sh + r * LL, t, twh, twl);
+    reg_fft<N, COMP, true, TWL>(a, sh + r * LL, t, twh, twl);
 #pragma unroll
     for (int m = 0; m < 8; ++m) xy[row + t + TT * m] = a[m];
   } else {
@@ -178,7 +193,7 @@ __global__ __launch_bounds__(N / 8 * RW) void k_born_op_rows_inv(const px2<T>* _
       const cx<T> ex = prop ? s.a : cx<T>{0, 0}, ey = prop ? s.b : cx<T>{0, 0};
       z[m] = (ex * kxT[t + TT * m] + ey * ky) * cz * (-d);
     }
-    reg_fft<N, COMP, true>(z, reinterpret_cast<cx<T>*>(sh + r * LL), t, twh, twl);
+    reg_fft<N, COMP, true, TWL>(z, reinterpret_cast<cx<T>*>(sh + r * LL), t, twh, twl);
 #pragma unroll
     for (int m = 0; m < 8; ++m) zl[row + t + TT * m] = z[m];
   }
@@ -189,21 +204,27 @@ __global__ __launch_bounds__(N / 8 * RW) void k_born_op_rows_inv(const px2<T>* _
 template <int N, int W, typename T, bool COMP>
 __global__ __launch_bounds__(N / 8 * W) void k_born_op_cols(px2<T>* xy, cx<T>* zl, int nx,
                                                            const cx<T>* __restrict__ pot, T scale,
-                                                           const cx<T>* __restrict__ twh,
-                                                           const cx<T>* __restrict__ twl) {
+                                                           const cx<T>* __restrict__ g_twh,
+                                                           const cx<T>* __restrict__ g_twl) {
   constexpr int TT = N / 8;
   constexpr int LL = reg_lds_len<N>();
   __shared__ px2<T> sh[W * LL];
+  constexpr bool TWL = op_tw_in_lds<N, T, COMP>(sizeof(sh));
+  __shared__ cx<T> tw_hi_s[TWL ? reg_tw_len<N>() : 1];
+  __shared__ cx<T> tw_lo_s[TWL && COMP ? reg_tw_len<N>() : 1];
+  if constexpr (TWL) stage_twiddles<N, COMP>(tw_hi_s, tw_lo_s, g_twh, g_twl);
+  const cx<T>* twh = TWL ? tw_hi_s : g_twh;
+  const cx<T>* twl = TWL ? tw_lo_s : g_twl;
   const int c = threadIdx.x % W, t = threadIdx.x / W;
   const int x = xcd_group(blockIdx.x, gridDim.x) * W + c;
   if (blockIdx.y == 0) {
     px2<T> a[8];
 #pragma unroll
     for (int m = 0; m < 8; ++m) a[m] = xy[static_cast<long long>(t + TT * m) * nx + x];
-    reg_fft<N, COMP, true>(a, sh + c * LL, t, twh, twl);
+    reg_fft<N, COMP, true, TWL>(a, sh + c * LL, t, twh, twl);
 #pragma unroll
     for (int m = 0; m < 8; ++m) a[m] = post(a[m], pot, static_cast<long long>(t + TT * m) * nx + x, scale);
-    reg_fft<N, COMP, false>(a, sh + c * LL, t, twh, twl);
+    reg_fft<N, COMP, false, TWL>(a, sh + c * LL, t, twh, twl);
 #pragma unroll
     for (int m = 0; m < 8; ++m) xy[static_cast<long long>(t + TT * m) * nx + x] = a[m];
   } else {
@@ -211,10 +232,10 @@ __global__ __launch_bounds__(N / 8 * W) void k_born_op_cols(px2<T>* xy, cx<T>* z
     cx<T> z[8];
 #pragma unroll
     for (int m = 0; m < 8; ++m) z[m] = zl[static_cast<long long>(t + TT * m) * nx + x];
-    reg_fft<N, COMP, true>(z, lds, t, twh, twl);
+    reg_fft<N, COMP, true, TWL>(z, lds, t, twh, twl);
 #pragma unroll
     for (int m = 0; m < 8; ++m) z[m] = z[m] * pot[static_cast<long long>(t + TT * m) * nx + x] * scale;
-    reg_fft<N, COMP, false>(z, lds, t, twh, twl);
+    reg_fft<N, COMP, false, TWL>(z, lds, t, twh, twl);
 #pragma unroll
     for (int m = 0; m < 8; ++m) zl[static_cast<long long>(t + TT * m) * nx + x] = z[m];
   }
@@ -228,11 +249,17 @@ __global__ __launch_bounds__(N / 8 * RW) void k_born_op_rows_fwd(const px2<T>* _
                                                                 const T* __restrict__ kxT, const T* __restrict__ kyT,
                                                                 T d, cx<T> inv_k2, const px2<T>* __restrict__ v,
                                                                 px2<T>* __restrict__ out, int mode,
-                                                                const cx<T>* __restrict__ twh,
-                                                                const cx<T>* __restrict__ twl) {
+                                                                const cx<T>* __restrict__ g_twh,
+                                                                const cx<T>* __restrict__ g_twl) {
   constexpr int TT = N / 8;
   constexpr int LL = reg_lds_len<N>();
   __shared__ px2<T> sh[RW * LL];
+  constexpr bool TWL = op_tw_in_lds<N, T, COMP>(sizeof(sh));
+  __shared__ cx<T> tw_hi_s[TWL ? reg_tw_len<N>() : 1];
+  __shared__ cx<T> tw_lo_s[TWL && COMP ? reg_tw_len<N>() : 1];
+  if constexpr (TWL) stage_twiddles<N, COMP>(tw_hi_s, tw_lo_s, g_twh, g_twl);
+  const cx<T>* twh = TWL ? tw_hi_s : g_twh;
+  const cx<T>* twl = TWL ? tw_lo_s : g_twl;
   const int r = threadIdx.x / TT, t = threadIdx.x % TT;
   const int y = blockIdx.x * RW + r;
   const long long row = static_cast<long long>(y) * N;
@@ -243,8 +270,8 @@ __global__ __launch_bounds__(N / 8 * RW) void k_born_op_rows_fwd(const px2<T>* _
     a[m] = xy[row + t + TT * m];
     z[m] = zl[row + t + TT * m];
   }
-  reg_fft<N, COMP, false>(a, sh + r * LL, t, twh, twl);
-  reg_fft<N, COMP, false>(z, reinterpret_cast<cx<T>*>(sh + r * LL), t, twh, twl);
+  reg_fft<N, COMP, false, TWL>(a, sh + r * LL, t, twh, twl);
+  reg_fft<N, COMP, false, TWL>(z, reinterpret_cast<cx<T>*>(sh + r * LL), t, twh, twl);
   const T ky = kyT[y];
 #pragma unroll
   for (int m = 0; m < 8; ++m) {

@@ -75,7 +75,7 @@ __device__ __forceinline__ void stage_tw_pass(cx<T>* sh_hi, cx<T>* sh_lo, const 
 }
 
 // Copies the twiddles of every pass from the table W_N^e (g_hi, and g_lo for COMP) into the staged
-// layout in LDS (reg_tw_len<N>() entries each); the cx-form reg_fft reads them with STAGED.
+// layout in LDS (reg_tw_len<N>() entries each); reg_fft reads them with STAGED.
 template <int N, bool COMP, typename T>
 __device__ __forceinline__ void stage_twiddles(cx<T>* sh_hi, cx<T>* sh_lo, const cx<T>* __restrict__ g_hi,
                                                const cx<T>* __restrict__ g_lo) {
@@ -90,7 +90,7 @@ __device__ __forceinline__ void dft_r(cx<T>* x) {
   else dft8<COMP, INV>(x);
 }
 
-template <int N, int PASS, int NS, bool COMP, bool INV, typename T>
+template <int N, int PASS, int NS, bool COMP, bool INV, bool STAGED, typename T>
 __device__ __forceinline__ void reg_pass(px2<T> (&v)[8], px2<T>* lds, int t, const cx<T>* twh, const cx<T>* twl) {
   using PL = RegPlan<N>;
   if constexpr (PASS < PL::P) {
@@ -107,7 +107,7 @@ __device__ __forceinline__ void reg_pass(px2<T> (&v)[8], px2<T>* lds, int t, con
       if constexpr (NS > 1) {
 #pragma unroll
         for (int r = 1; r < R; ++r) {
-          const int e = r * k * (N / (NS * R));
+          const int e = STAGED ? reg_tw_off<N>(PASS) + (r - 1) * NS + k : r * k * (N / (NS * R));
           wh[r] = INV ? conj(twh[e]) : twh[e];
           if constexpr (COMP) wl[r] = INV ? conj(twl[e]) : twl[e];
         }
@@ -144,7 +144,7 @@ __device__ __forceinline__ void reg_pass(px2<T> (&v)[8], px2<T>* lds, int t, con
       for (int m = 0; m < 8; ++m) v[m] = lds[lpad(t + TT * m)];
       __syncthreads();
     }
-    reg_pass<N, PASS + 1, NS * R, COMP, INV>(v, lds, t, twh, twl);
+    reg_pass<N, PASS + 1, NS * R, COMP, INV, STAGED>(v, lds, t, twh, twl);
   }
 }
 
@@ -198,12 +198,12 @@ __device__ __forceinline__ void reg_pass1(cx<T> (&v)[8], cx<T>* lds, int t, cons
 
 // Transform one line held as v[m] = a[t + T·m]; on return v[m] = A[t + T·m]. All threads of the
 // workgroup must call it together (it synchronises the workgroup between passes). twh/twl point
-// to the twiddle table W_N^e (LDS or global), or with STAGED (cx form) to the copy that
-// stage_twiddles made. The px2 form transforms both components of a pixel vector in one thread,
+// to the twiddle table W_N^e (LDS or global), or with STAGED to the copy that stage_twiddles
+// made. The px2 form transforms both components of a pixel vector in one thread,
 // the cx form one component.
-template <int N, bool COMP, bool INV, typename T>
+template <int N, bool COMP, bool INV, bool STAGED = false, typename T>
 __device__ __forceinline__ void reg_fft(px2<T> (&v)[8], px2<T>* lds, int t, const cx<T>* twh, const cx<T>* twl) {
-  reg_pass<N, 0, 1, COMP, INV>(v, lds, t, twh, twl);
+  reg_pass<N, 0, 1, COMP, INV, STAGED>(v, lds, t, twh, twl);
 }
 template <int N, bool COMP, bool INV, bool STAGED = false, typename T>
 __device__ __forceinline__ void reg_fft(cx<T> (&v)[8], cx<T>* lds, int t, const cx<T>* twh, const cx<T>* twl) {
