@@ -99,11 +99,11 @@ inline bool fast_size(long long n) {
 // columns / rows per workgroup of the register kernels (kernels_fast.hpp)
 template <typename T, bool COMP>
 constexpr int fast_w(int n) {
-  return fast_lines(n, kColSkew, static_cast<int>(sizeof(cx<T>)), COMP);
+  return fast_lines(n, true, static_cast<int>(sizeof(cx<T>)), COMP);
 }
 template <typename T, bool COMP>
 constexpr int fast_rw(int n) {
-  return fast_lines(n, kRowSkew, static_cast<int>(sizeof(cx<T>)), COMP);
+  return fast_lines(n, false, static_cast<int>(sizeof(cx<T>)), COMP);
 }
 
 // Launch geometry of one grid size (shared by every solve of a lane).
@@ -309,17 +309,17 @@ struct Launcher {
   }
 
  private:
-  // The LDS-stash form is instantiated where its extra LDS fits a workgroup.
-  template <int N, int W>
+  // The LDS-stash form is instantiated where its extra LDS fits a workgroup (GROUP: component
+  // lines per bank group, col_group_lines(W) or kRowGroupLines).
+  template <int N, int W, int GROUP>
   static constexpr bool stash_fits() {
-    return fast_lds_bytes(N, 2 * W, kColSkew, static_cast<int>(sizeof(cx<T>)), COMP) +
-               2LL * W * (N + 1) * static_cast<long long>(sizeof(cx<T>)) <=
-           163840;
+    constexpr int cs = static_cast<int>(sizeof(cx<T>));
+    return fast_lds_bytes(N, 2 * W, GROUP, cs, COMP) + 2LL * W * lds_stride(N, GROUP, cs) * cs <= 163840;
   }
   template <int N, int W>
   void col_launch(const ColDesc<T, ST>* d, int S, Emit& s, bool multi, const cx<T>* th, const cx<T>* tl) const {
     const dim3 grid(static_cast<unsigned>(nx / W), S);
-    if constexpr (stash_fits<N, W>())
+    if constexpr (stash_fits<N, W, col_group_lines(W)>())
       if (multi) {
         s(&k_col_fast<N, W, T, COMP, true, ST>, grid, dim3(N / 4 * W), 0, d, static_cast<int>(nx), npix, th, tl);
         return;
@@ -329,7 +329,7 @@ struct Launcher {
   template <int N, int RW>
   void row_launch(const RowDesc<T, ST>* d, int S, Emit& s, bool multi, const cx<T>* th, const cx<T>* tl) const {
     const dim3 grid(static_cast<unsigned>(ny / RW), S);
-    if constexpr (stash_fits<N, RW>())
+    if constexpr (stash_fits<N, RW, kRowGroupLines>())
       if (multi) {
         s(&k_row_fast<N, RW, T, COMP, true, ST>, grid, dim3(N / 4 * RW), 0, d, npix, th, tl);
         return;

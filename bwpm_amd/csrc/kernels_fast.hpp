@@ -19,36 +19,43 @@ namespace bwpm {
 
 // ---- geometry shared by the kernels and their launches --------------------------------------
 
-// LDS line stride (complex entries) of a component line: the padded line plus a skew that spreads
-// the lines of one wavefront's lanes over the banks.
-template <int N, int SKEW>
-constexpr int fast_line_stride() {
-  return reg_lds_len<N>() + SKEW;
+// LDS stride (complex entries of csize bytes) of lines of `len` entries. A 128-bit (64-bit) LDS
+// access serves a wavefront's lanes in groups of g = 128 / csize (8 for complex128, 16 for
+// complex64), and a group covers g / L consecutive positions of L lines side by side: L = 2W in
+// the column kernel (lanes (t·W + c)·2 + comp), 2 in the row kernel (the two components of a row).
+// A stride ≡ g / L (mod g) puts the g lanes of a group in distinct bank groups.
+constexpr int lds_stride(int len, int group_lines, int csize) {
+  const int g = 128 / csize;
+  const int target = g / (group_lines < g ? group_lines : g);
+  return len + ((target - len % g) % g + g) % g;
 }
-constexpr int kColSkew = 2;
-constexpr int kRowSkew = 4;
+// Lines of one bank group: column kernels with W columns, row kernels.
+constexpr int col_group_lines(int w) { return 2 * w; }
+constexpr int kRowGroupLines = 2;
 
-// Static LDS of a kernel with `lines` component lines of n points (complex size csize), with the
-// staged twiddles (n − 8 entries for plans that start with radix 8: reg_tw_len).
-constexpr long long fast_lds_bytes(int n, int lines, int skew, int csize, bool comp) {
-  return static_cast<long long>(lines) * (n + n / 8 + skew) * csize +
+// Static LDS of a kernel with `lines` component lines of n points (complex size csize), `group`
+// of them side by side in a bank group, with the staged twiddles (n − 8 entries for plans that
+// start with radix 8: reg_tw_len).
+constexpr long long fast_lds_bytes(int n, int lines, int group, int csize, bool comp) {
+  return static_cast<long long>(lines) * lds_stride(n + n / 8, group, csize) * csize +
          static_cast<long long>(n - 8) * csize * (comp ? 2 : 1);
 }
 
 // Lines (columns or rows) per workgroup: 256 threads (n/4 per line) where the line allows, and at
 // most 52 KB of LDS, so that three workgroups share a CU and hide each other's memory and barrier
 // waits.
-constexpr int fast_lines(int n, int skew, int csize, bool comp) {
+constexpr int fast_lines(int n, bool columns, int csize, bool comp) {
   int w = 1024 / n < 1 ? 1 : 1024 / n;
-  while (w > 1 && fast_lds_bytes(n, 2 * w, skew, csize, comp) > 53248) w /= 2;
+  while (w > 1 && fast_lds_bytes(n, 2 * w, columns ? col_group_lines(w) : kRowGroupLines, csize, comp) > 53248)
+    w /= 2;
   return w;
 }
 
 // Twiddles are staged in LDS when the line buffers leave room for them (160 KB per workgroup);
 // the 4096-point FP64 lines read them from global memory.
-template <int N, int LINES, int SKEW, typename T, bool COMP>
+template <int N, int LINES, int GROUP, typename T, bool COMP>
 constexpr bool fast_tw_in_lds() {
-  return fast_lds_bytes(N, LINES, SKEW, static_cast<int>(sizeof(cx<T>)), COMP) <= 163840;
+  return fast_lds_bytes(N, LINES, GROUP, static_cast<int>(sizeof(cx<T>)), COMP) <= 163840;
 }
 
 // ---- lane-pair exchange (lanes 2i ↔ 2i + 1): DPP quad permutation [1, 0, 3, 2] ---------------
@@ -104,12 +111,14 @@ __global__ __launch_bounds__(N / 4 * W) void k_col_fast(const ColDesc<T, ST>* __
                                                         long long plane, const cx<T>* __restrict__ g_twh,
                                                         const cx<T>* __restrict__ g_twl) {
   constexpr int TT = N / 8;
-  constexpr int LS = fast_line_stride<N, kColSkew>();
-  constexpr bool TWL = fast_tw_in_lds<N, 2 * W, kColSkew, T, COMP>();
+  constexpr int CS = static_cast<int>(sizeof(cx<T>));
+  constexpr int LS = lds_stride(reg_lds_len<N>(), col_group_lines(W), CS);
+  constexpr int SS = lds_stride(N, col_group_lines(W), CS);  // stash line stride
+  constexpr bool TWL = fast_tw_in_lds<N, 2 * W, col_group_lines(W), T, COMP>();
   __shared__ cx<T> sh[2 * W * LS];
   __shared__ cx<T> tw_hi_s[TWL ? reg_tw_len<N>() : 1];
   __shared__ cx<T> tw_lo_s[TWL && COMP ? reg_tw_len<N>() : 1];
-  __shared__ cx<T> stash[MULTI ? 2 * W * (N + 1) : 1];
+  __shared__ cx<T> stash[MULTI ? 2 * W * SS : 1];
   const ColDesc<T, ST> d = descs[blockIdx.y];
   if (!d.in) return;
   if constexpr (TWL) stage_twiddles<N, COMP>(tw_hi_s, tw_lo_s, g_twh, g_twl);
@@ -140,9 +149,9 @@ __global__ __launch_bounds__(N / 4 * W) void k_col_fast(const ColDesc<T, ST>* __
   for (int m = 0; m < 8; ++m) s[m] = sload<T>(xio, at(m));
   reg_fft<N, COMP, false, TWL>(s, line, t, tw_hi, tw_lo);
   // several products: keep the spectrum (in the LDS stash, or in the thread's own positions of X);
-  // stash entry m of this lane is own[m·T], its component line is N + 1 entries long
+  // stash entry m of this lane is own[m·T] on its component line
   cx<T>* own = stash;
-  if constexpr (MULTI) own = stash + (2 * c + comp) * (N + 1) + t;
+  if constexpr (MULTI) own = stash + (2 * c + comp) * SS + t;
   if (nprod > 1) {
 #pragma unroll
     for (int m = 0; m < 8; ++m) {
@@ -192,12 +201,14 @@ __global__ __launch_bounds__(N / 4 * RW) void k_row_fast(const RowDesc<T, ST>* _
                                                          const cx<T>* __restrict__ g_twh,
                                                          const cx<T>* __restrict__ g_twl) {
   constexpr int TT = N / 8;
-  constexpr int LS = fast_line_stride<N, kRowSkew>();
-  constexpr bool TWL = fast_tw_in_lds<N, 2 * RW, kRowSkew, T, COMP>();
+  constexpr int CS = static_cast<int>(sizeof(cx<T>));
+  constexpr int LS = lds_stride(reg_lds_len<N>(), kRowGroupLines, CS);
+  constexpr int SS = lds_stride(N, kRowGroupLines, CS);  // stash line stride
+  constexpr bool TWL = fast_tw_in_lds<N, 2 * RW, kRowGroupLines, T, COMP>();
   __shared__ cx<T> sh[2 * RW * LS];
   __shared__ cx<T> tw_hi_s[TWL ? reg_tw_len<N>() : 1];
   __shared__ cx<T> tw_lo_s[TWL && COMP ? reg_tw_len<N>() : 1];
-  __shared__ cx<T> stash[MULTI ? 2 * RW * (N + 1) : 1];
+  __shared__ cx<T> stash[MULTI ? 2 * RW * SS : 1];
   const RowDesc<T, ST> a = descs[blockIdx.y];
   if (!a.active) return;
   if constexpr (TWL) stage_twiddles<N, COMP>(tw_hi_s, tw_lo_s, g_twh, g_twl);
@@ -260,7 +271,7 @@ __global__ __launch_bounds__(N / 4 * RW) void k_row_fast(const RowDesc<T, ST>* _
     // the selected pixels are collected in the LDS stash or in product plane 0
     px2<ST>* merged = const_cast<px2<ST>*>(a.prod);
     cx<T>* own = stash;  // stash entry m of this lane is own[m·T]
-    if constexpr (MULTI) own = stash + (2 * r + comp) * (N + 1) + t;
+    if constexpr (MULTI) own = stash + (2 * r + comp) * SS + t;
     for (int o = 0; o < a.nt; ++o) {
       if (!((static_cast<unsigned>(use) >> row_use_bit(o)) & 1u)) continue;
       const px2<ST>* src = a.prod + o * plane;
