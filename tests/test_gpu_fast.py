@@ -44,7 +44,8 @@ def _solver(a: dict, precision: str):
 
 
 @pytest.mark.parametrize("shape", [(64, 64), (128, 128), (256, 128), (512, 512), (512, 12), (12, 256), (4096, 1),
-                                   (2048, 2), (3072, 1), (384, 96), (80, 120), (1000, 1)])
+                                   (2048, 2), (3072, 1), (384, 96), (80, 120), (1000, 1), (2, 1024), (1, 4096),
+                                   (4, 2048)])
 @pytest.mark.parametrize("precision", ["fp64", "mixed", "fp32"])
 def test_register_kernels_match_oracle(shape: tuple[int, int], precision: str) -> None:
     ny, nx = shape
