@@ -157,8 +157,8 @@ __global__ __launch_bounds__(N / 8 * RW) void k_born_op_rows_inv(const px2<T>* _
                                                                 const cx<T>* __restrict__ g_twh,
                                                                 const cx<T>* __restrict__ g_twl) {
   constexpr int TT = N / 8;
-  constexpr int LL = reg_lds_len<N>();
-  __shared__ px2<T> sh[RW * LL];
+  constexpr int LQ = lds_stride(2 * reg_lds_len<N>(), 1, sizeof(cx<T>));  // a line: E_x and E_y planes
+  __shared__ cx<T> sh[RW * LQ];
   constexpr bool TWL = op_tw_in_lds<N, T, COMP>(sizeof(sh));
   __shared__ cx<T> tw_hi_s[TWL ? reg_tw_len<N>() : 1];
   __shared__ cx<T> tw_lo_s[TWL && COMP ? reg_tw_len<N>() : 1];
@@ -178,7 +178,7 @@ __global__ __launch_bounds__(N / 8 * RW) void k_born_op_rows_inv(const px2<T>* _
       const px2<T> s = v[q];
       a[m] = prop ? s : px2<T>{};
     }
-    reg_fft<N, COMP, true, TWL>(a, sh + r * LL, t, twh, twl);
+    reg_fft<N, COMP, true, TWL>(a, sh + r * LQ, t, twh, twl);
 #pragma unroll
     for (int m = 0; m < 8; ++m) xy[row + t + TT * m] = a[m];
   } else {
@@ -193,7 +193,7 @@ __global__ __launch_bounds__(N / 8 * RW) void k_born_op_rows_inv(const px2<T>* _
       const cx<T> ex = prop ? s.a : cx<T>{0, 0}, ey = prop ? s.b : cx<T>{0, 0};
       z[m] = (ex * kxT[t + TT * m] + ey * ky) * cz * (-d);
     }
-    reg_fft<N, COMP, true, TWL>(z, reinterpret_cast<cx<T>*>(sh + r * LL), t, twh, twl);
+    reg_fft<N, COMP, true, TWL>(z, sh + r * LQ, t, twh, twl);
 #pragma unroll
     for (int m = 0; m < 8; ++m) zl[row + t + TT * m] = z[m];
   }
@@ -207,8 +207,8 @@ __global__ __launch_bounds__(N / 8 * W) void k_born_op_cols(px2<T>* xy, cx<T>* z
                                                            const cx<T>* __restrict__ g_twh,
                                                            const cx<T>* __restrict__ g_twl) {
   constexpr int TT = N / 8;
-  constexpr int LL = reg_lds_len<N>();
-  __shared__ px2<T> sh[W * LL];
+  constexpr int LQ = lds_stride(2 * reg_lds_len<N>(), W, sizeof(cx<T>));  // a line: E_x and E_y planes
+  __shared__ cx<T> sh[W * LQ];
   constexpr bool TWL = op_tw_in_lds<N, T, COMP>(sizeof(sh));
   __shared__ cx<T> tw_hi_s[TWL ? reg_tw_len<N>() : 1];
   __shared__ cx<T> tw_lo_s[TWL && COMP ? reg_tw_len<N>() : 1];
@@ -221,14 +221,14 @@ __global__ __launch_bounds__(N / 8 * W) void k_born_op_cols(px2<T>* xy, cx<T>* z
     px2<T> a[8];
 #pragma unroll
     for (int m = 0; m < 8; ++m) a[m] = xy[static_cast<long long>(t + TT * m) * nx + x];
-    reg_fft<N, COMP, true, TWL>(a, sh + c * LL, t, twh, twl);
+    reg_fft<N, COMP, true, TWL>(a, sh + c * LQ, t, twh, twl);
 #pragma unroll
     for (int m = 0; m < 8; ++m) a[m] = post(a[m], pot, static_cast<long long>(t + TT * m) * nx + x, scale);
-    reg_fft<N, COMP, false, TWL>(a, sh + c * LL, t, twh, twl);
+    reg_fft<N, COMP, false, TWL>(a, sh + c * LQ, t, twh, twl);
 #pragma unroll
     for (int m = 0; m < 8; ++m) xy[static_cast<long long>(t + TT * m) * nx + x] = a[m];
   } else {
-    cx<T>* lds = reinterpret_cast<cx<T>*>(sh + c * LL);
+    cx<T>* lds = sh + c * LQ;
     cx<T> z[8];
 #pragma unroll
     for (int m = 0; m < 8; ++m) z[m] = zl[static_cast<long long>(t + TT * m) * nx + x];
@@ -252,8 +252,8 @@ __global__ __launch_bounds__(N / 8 * RW) void k_born_op_rows_fwd(const px2<T>* _
                                                                 const cx<T>* __restrict__ g_twh,
                                                                 const cx<T>* __restrict__ g_twl) {
   constexpr int TT = N / 8;
-  constexpr int LL = reg_lds_len<N>();
-  __shared__ px2<T> sh[RW * LL];
+  constexpr int LQ = lds_stride(2 * reg_lds_len<N>(), 1, sizeof(cx<T>));  // a line: E_x and E_y planes
+  __shared__ cx<T> sh[RW * LQ];
   constexpr bool TWL = op_tw_in_lds<N, T, COMP>(sizeof(sh));
   __shared__ cx<T> tw_hi_s[TWL ? reg_tw_len<N>() : 1];
   __shared__ cx<T> tw_lo_s[TWL && COMP ? reg_tw_len<N>() : 1];
@@ -270,8 +270,8 @@ __global__ __launch_bounds__(N / 8 * RW) void k_born_op_rows_fwd(const px2<T>* _
     a[m] = xy[row + t + TT * m];
     z[m] = zl[row + t + TT * m];
   }
-  reg_fft<N, COMP, false, TWL>(a, sh + r * LL, t, twh, twl);
-  reg_fft<N, COMP, false, TWL>(z, reinterpret_cast<cx<T>*>(sh + r * LL), t, twh, twl);
+  reg_fft<N, COMP, false, TWL>(a, sh + r * LQ, t, twh, twl);
+  reg_fft<N, COMP, false, TWL>(z, sh + r * LQ, t, twh, twl);
   const T ky = kyT[y];
 #pragma unroll
   for (int m = 0; m < 8; ++m) {

@@ -90,8 +90,8 @@ template <int N, int RW, typename T, bool COMP, bool INV>
 __global__ __launch_bounds__(N / 8 * RW) void k_fft2_rows_fast(PlaneSet<T> ps, const cx<T>* __restrict__ twh,
                                                                const cx<T>* __restrict__ twl) {
   constexpr int TT = N / 8;
-  constexpr int LL = reg_lds_len<N>();
-  __shared__ px2<T> sh[RW * LL];
+  constexpr int LQ = lds_stride(2 * reg_lds_len<N>(), 1, sizeof(cx<T>));  // a line: E_x and E_y planes
+  __shared__ cx<T> sh[RW * LQ];
   const px2<T>* in = ps.in[blockIdx.y];
   px2<T>* out = ps.out[blockIdx.y];
   const cx<T>* mul = ps.mul[blockIdx.y];
@@ -100,7 +100,7 @@ __global__ __launch_bounds__(N / 8 * RW) void k_fft2_rows_fast(PlaneSet<T> ps, c
   px2<T> v[8];
 #pragma unroll
   for (int m = 0; m < 8; ++m) v[m] = in[row + t + TT * m];
-  reg_fft<N, COMP, INV>(v, sh + r * LL, t, twh, twl);
+  reg_fft<N, COMP, INV>(v, sh + r * LQ, t, twh, twl);
 #pragma unroll
   for (int m = 0; m < 8; ++m) out[row + t + TT * m] = post(v[m], mul, row + t + TT * m, ps.scale);
 }
@@ -109,8 +109,8 @@ template <int N, int W, typename T, bool COMP, bool INV>
 __global__ __launch_bounds__(N / 8 * W) void k_fft2_cols_fast(PlaneSet<T> ps, int nx, const cx<T>* __restrict__ twh,
                                                               const cx<T>* __restrict__ twl) {
   constexpr int TT = N / 8;
-  constexpr int LL = reg_lds_len<N>();
-  __shared__ px2<T> sh[W * LL];
+  constexpr int LQ = lds_stride(2 * reg_lds_len<N>(), W, sizeof(cx<T>));  // a line: E_x and E_y planes
+  __shared__ cx<T> sh[W * LQ];
   const px2<T>* in = ps.in[blockIdx.y];
   px2<T>* out = ps.out[blockIdx.y];
   const cx<T>* mul = ps.mul[blockIdx.y];
@@ -119,7 +119,7 @@ __global__ __launch_bounds__(N / 8 * W) void k_fft2_cols_fast(PlaneSet<T> ps, in
   px2<T> v[8];
 #pragma unroll
   for (int m = 0; m < 8; ++m) v[m] = in[static_cast<long long>(t + TT * m) * nx + x];
-  reg_fft<N, COMP, INV>(v, sh + c * LL, t, twh, twl);
+  reg_fft<N, COMP, INV>(v, sh + c * LQ, t, twh, twl);
 #pragma unroll
   for (int m = 0; m < 8; ++m) {
     const long long q = static_cast<long long>(t + TT * m) * nx + x;

@@ -20,6 +20,17 @@
 
 namespace bwpm {
 
+// LDS stride (complex entries of csize bytes) of lines of `len` entries. A 128-bit (64-bit) LDS
+// access serves a wavefront's lanes in groups of g = 128 / csize (8 for complex128, 16 for
+// complex64), and a group covers g / L consecutive positions of L lines side by side (L = 1 where
+// the lanes of a group share a line). A stride ≡ g / L (mod g) puts the g lanes of a group in
+// distinct bank groups.
+constexpr int lds_stride(int len, int group_lines, int csize) {
+  const int g = 128 / csize;
+  const int target = g / (group_lines < g ? group_lines : g);
+  return len + ((target - len % g) % g + g) % g;
+}
+
 template <int N>
 struct RegPlan;
 template <> struct RegPlan<16> { static constexpr int P = 2; static constexpr int rad[P] = {8, 2}; };
@@ -91,7 +102,7 @@ __device__ __forceinline__ void dft_r(cx<T>* x) {
 }
 
 template <int N, int PASS, int NS, bool COMP, bool INV, bool STAGED, typename T>
-__device__ __forceinline__ void reg_pass(px2<T> (&v)[8], px2<T>* lds, int t, const cx<T>* twh, const cx<T>* twl) {
+__device__ __forceinline__ void reg_pass(px2<T> (&v)[8], cx<T>* lds, int t, const cx<T>* twh, const cx<T>* twl) {
   using PL = RegPlan<N>;
   if constexpr (PASS < PL::P) {
     constexpr int R = PL::rad[PASS];
@@ -131,17 +142,17 @@ __device__ __forceinline__ void reg_pass(px2<T> (&v)[8], px2<T>* lds, int t, con
           }
         } else {
 #pragma unroll
-          for (int r = 0; r < R; ++r) {
-            cx<T>* dst = reinterpret_cast<cx<T>*>(lds + lpad(base + r * NS)) + comp;
-            *dst = x[r];
-          }
+          for (int r = 0; r < R; ++r) lds[comp * reg_lds_len<N>() + lpad(base + r * NS)] = x[r];
         }
       }
     }
     if constexpr (!in_regs) {
       __syncthreads();
 #pragma unroll
-      for (int m = 0; m < 8; ++m) v[m] = lds[lpad(t + TT * m)];
+      for (int m = 0; m < 8; ++m) {
+        v[m].a = lds[lpad(t + TT * m)];
+        v[m].b = lds[reg_lds_len<N>() + lpad(t + TT * m)];
+      }
       __syncthreads();
     }
     reg_pass<N, PASS + 1, NS * R, COMP, INV, STAGED>(v, lds, t, twh, twl);
@@ -197,12 +208,15 @@ __device__ __forceinline__ void reg_pass1(cx<T> (&v)[8], cx<T>* lds, int t, cons
 }
 
 // Transform one line held as v[m] = a[t + T·m]; on return v[m] = A[t + T·m]. All threads of the
-// workgroup must call it together (it synchronises the workgroup between passes). twh/twl point
+// workgroup must call it together (it synchronises the workgroup between passes). The px2 form
+// exchanges through 2·reg_lds_len<N>() LDS entries, the E_x plane and then the E_y plane, so that
+// each LDS instruction of a wavefront touches consecutive 16-byte entries of one plane; the cx form
+// through reg_lds_len<N>() entries. twh/twl point
 // to the twiddle table W_N^e (LDS or global), or with STAGED to the copy that stage_twiddles
 // made. The px2 form transforms both components of a pixel vector in one thread,
 // the cx form one component.
 template <int N, bool COMP, bool INV, bool STAGED = false, typename T>
-__device__ __forceinline__ void reg_fft(px2<T> (&v)[8], px2<T>* lds, int t, const cx<T>* twh, const cx<T>* twl) {
+__device__ __forceinline__ void reg_fft(px2<T> (&v)[8], cx<T>* lds, int t, const cx<T>* twh, const cx<T>* twl) {
   reg_pass<N, 0, 1, COMP, INV, STAGED>(v, lds, t, twh, twl);
 }
 template <int N, bool COMP, bool INV, bool STAGED = false, typename T>

@@ -19,17 +19,8 @@ namespace bwpm {
 
 // ---- geometry shared by the kernels and their launches --------------------------------------
 
-// LDS stride (complex entries of csize bytes) of lines of `len` entries. A 128-bit (64-bit) LDS
-// access serves a wavefront's lanes in groups of g = 128 / csize (8 for complex128, 16 for
-// complex64), and a group covers g / L consecutive positions of L lines side by side: L = 2W in
-// the column kernel (lanes (t·W + c)·2 + comp), 2 in the row kernel (the two components of a row).
-// A stride ≡ g / L (mod g) puts the g lanes of a group in distinct bank groups.
-constexpr int lds_stride(int len, int group_lines, int csize) {
-  const int g = 128 / csize;
-  const int target = g / (group_lines < g ? group_lines : g);
-  return len + ((target - len % g) % g + g) % g;
-}
-// Lines of one bank group: column kernels with W columns, row kernels.
+// Lines of one LDS bank group (lds_stride): column kernels with W columns (lanes
+// (t·W + c)·2 + comp), row kernels (the two components of a row).
 constexpr int col_group_lines(int w) { return 2 * w; }
 constexpr int kRowGroupLines = 2;
 
