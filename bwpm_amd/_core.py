@@ -496,6 +496,23 @@ class GpuAnalysis:
 CudaSolver = GpuSolver  # the reference's name for its GPU solver class
 
 
+class Solver(GpuSolver):
+    """The reference's `Solver` signature, `Solver(ny, nx, dy, dx, dz, k0, indices, labels,
+    index_before, mask, threads=1, fluctuation=None)` with `.run(field, iterations, keep=[])`
+    (cpp/python/bindings.cpp:139-153), so that callers choosing the reference's non-CUDA branch
+    (src/bwpm/fast.py:86-98) keep working. It computes on GPU 0 in FP64 through the same C ABI as
+    `GpuSolver`; `threads` (the reference's OpenMP thread count, any int, clamped to >= 1 there) is
+    accepted and has no effect on the result.
+    """
+
+    def __init__(self, ny: int, nx: int, dy: float, dx: float, dz: float, k0: float, indices: Any,
+                 labels: Any, index_before: complex, mask: Any = None, threads: int = 1,
+                 fluctuation: Any = None) -> None:
+        self.threads = max(int(threads), 1)
+        super().__init__(ny, nx, dy, dx, dz, k0, indices, labels, index_before, mask, device=0,
+                         fluctuation=fluctuation, precision="fp64")
+
+
 def run_batch(solvers: list[GpuSolver], iterations: int, max_lane: int = 64, time_kernels: bool = False) -> Any:
     """Run every solver (fields uploaded with `upload`) as one batch; returns the wall time in ms
     (with `time_kernels`, a dict that adds the summed per-launch HIP-event durations).

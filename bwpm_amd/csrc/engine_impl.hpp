@@ -1000,6 +1000,10 @@ class Engine final : public EngineBase {
   void validate() {
     require(sp_.ny >= 1 && sp_.nx >= 1, "grid must have ny, nx >= 1");
     require(sp_.nz >= 1, "labels must have shape (nz, ny, nx) with nz >= 1");
+    // the slab kernels form byte offsets into a plane as unsigned 32-bit values (two complex values
+    // per pixel)
+    require(sp_.ny * sp_.nx <= (1LL << 32) / (2 * static_cast<long long>(sizeof(cx<T>))),
+            "grid of ny·nx = " + std::to_string(sp_.ny * sp_.nx) + " pixels exceeds the plane size limit");
     if (sp_.geom.on) {
       require(sp_.labels.empty() && sp_.uniform_label < 0, "a geometry replaces the label array");
       // the label planes of the slabs, without a (nz, ny, nx) volume (set-up only)
@@ -1294,11 +1298,64 @@ class Engine final : public EngineBase {
         total_sparse_ += cnt;
       }
     }
+    check_boundary_tables(tsel, rowuse, sp_pix, sp_rid, rowptr);
     tsel_ = upload_vec(tsel, stream_);
     rowuse_ = upload_vec(rowuse, stream_);
     sp_pix_ = upload_vec(sp_pix, stream_);
     sp_rid_ = upload_vec(sp_rid, stream_);
     rowptr_ = upload_vec(rowptr, stream_);
+  }
+
+  // Every index the slab kernels read through the interface tables lies inside its table: the
+  // pair map (tsel, npix entries below the pair count), the per-row use masks (ny entries), the
+  // sparse row offsets (ny + 1, from 0 to the entry count, non-decreasing), the sparse pixels (each
+  // inside its own row) and their R products (below the R count), and the sparse value slots.
+  // Checked on the host for every boundary before the tables go to the device; a violation is an
+  // internal error of the set-up, reported instead of launching kernels on it.
+  void check_boundary_tables(const std::vector<uint8_t>& tsel, const std::vector<uint32_t>& rowuse,
+                             const std::vector<int32_t>& sp_pix, const std::vector<uint8_t>& sp_rid,
+                             const std::vector<int32_t>& rowptr) const {
+    const long long ny = sp_.ny, nx = sp_.nx;
+    auto fail = [](long long q, const std::string& what) {
+      throw Error(kEinternal, "interface tables of boundary " + std::to_string(q) + ": " + what);
+    };
+    if (sp_rid.size() != sp_pix.size()) fail(-1, "sparse pixel and product lists differ in length");
+    for (long long q = 0; q < sp_.nz; ++q) {
+      const Boundary& B = bnd_[q];
+      const long long np = static_cast<long long>(B.pairs.size());
+      const long long nr = static_cast<long long>(B.rpair.size());
+      if (np < 1 || np > 255) fail(q, "pair count " + std::to_string(np));
+      for (int r : B.rpair)
+        if (r < 0 || r >= np) fail(q, "R product of pair " + std::to_string(r));
+      if (np > 1) {
+        if (B.tsel_off < 0 || B.tsel_off + npix_ > static_cast<long long>(tsel.size())) fail(q, "pair map extent");
+        if (B.rowuse_off < 0 || B.rowuse_off + ny > static_cast<long long>(rowuse.size())) fail(q, "row masks extent");
+        if (B.source < 0)
+          for (long long i = 0; i < npix_; ++i)
+            if (tsel[B.tsel_off + i] >= np) fail(q, "pair map value at pixel " + std::to_string(i));
+      } else if (B.tsel_off >= 0 || B.rowuse_off >= 0) {
+        fail(q, "pair map on a boundary of one pair");
+      }
+      if (nr == 0) {
+        if (B.sp_count != 0 || B.rowptr_off >= 0) fail(q, "sparse entries without R products");
+        continue;
+      }
+      if (B.sp_off < 0 || B.sp_off + B.sp_count > total_sparse_) fail(q, "sparse value slots");
+      if (B.rowptr_off < 0 || B.rowptr_off + ny + 1 > static_cast<long long>(rowptr.size())) fail(q, "row offsets extent");
+      const int32_t* rp = rowptr.data() + B.rowptr_off;
+      if (rp[0] != 0 || rp[ny] != B.sp_count) fail(q, "row offsets do not span the entries");
+      const long long list = list_of(q);
+      if (list < 0 || list + B.sp_count > static_cast<long long>(sp_pix.size())) fail(q, "sparse list extent");
+      if (B.source >= 0) continue;  // the tables of a reused structure were checked at their source
+      for (long long y = 0; y < ny; ++y) {
+        if (rp[y + 1] < rp[y]) fail(q, "row offsets decrease at row " + std::to_string(y));
+        for (long long e = rp[y]; e < rp[y + 1]; ++e) {
+          const long long pix = sp_pix[list + e];
+          if (pix < y * nx || pix >= (y + 1) * nx) fail(q, "sparse pixel outside its row " + std::to_string(y));
+          if (sp_rid[list + e] >= nr) fail(q, "sparse entry of R product " + std::to_string(sp_rid[list + e]));
+        }
+      }
+    }
   }
 
   // Operator tables and per-step product descriptor lists.

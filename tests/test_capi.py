@@ -32,12 +32,18 @@ def test_library_exports_every_declared_symbol() -> None:
 
 def test_binding_covers_the_reference_module_surface() -> None:
     """bwpm._core names (cpp/python/bindings.cpp:122-180) are present in the binding."""
+    import inspect
+
     from bwpm_amd import _core
 
-    for name in ("transmission_matrix", "reflection_matrix", "longitudinal_wavenumber", "CudaSolver",
+    for name in ("transmission_matrix", "reflection_matrix", "longitudinal_wavenumber", "Solver", "CudaSolver",
                  "HAS_CUDA", "cuda_device_count"):
         assert hasattr(_core, name), name
     assert _core.CudaSolver is _core.GpuSolver
+    # the positional order of the reference's constructors (bindings.cpp:149-151, 170-172)
+    head = ["ny", "nx", "dy", "dx", "dz", "k0", "indices", "labels", "index_before", "mask"]
+    assert list(inspect.signature(_core.Solver).parameters)[:12] == head + ["threads", "fluctuation"]
+    assert list(inspect.signature(_core.CudaSolver).parameters)[:12] == head + ["device", "fluctuation"]
 
 
 def test_invalid_arguments_raise_value_error_before_any_device_work() -> None:

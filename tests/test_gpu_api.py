@@ -218,3 +218,23 @@ def test_backend_selection_and_estimate() -> None:
         fast.run_2d(_prop2d(d), d["inputs"]["field"], iterations=0, backend="cpu")
     with pytest.raises(ValueError):
         fast.run_2d(_prop2d(d), d["inputs"]["field"], iterations=0, backend="quantum")
+
+
+def test_reference_solver_name_equals_the_gpu_solver_bitwise() -> None:
+    """`_core.Solver(..., threads, fluctuation=...)` (the reference's non-CUDA branch,
+    src/bwpm/fast.py:98) gives the bits of `_core.CudaSolver(..., device, fluctuation=...)`."""
+    from bwpm_amd import _core, workloads
+
+    p = workloads.tilted_3d(n=64, nz=24, iterations=2)
+    a = _core.Solver(*p.solver_args(), 8, fluctuation=None)
+    b = _core.CudaSolver(*p.solver_args(), 0, fluctuation=None)
+    try:
+        ra, rb = a.run(p.field, 2, [0, 12]), b.run(p.field, 2, [0, 12])
+    finally:
+        a.close()
+        b.close()
+    for key in ("transmitted", "reflected"):
+        assert np.array_equal(ra[key], rb[key])
+    for q in (0, 12):
+        assert np.array_equal(ra["forward"][q], rb["forward"][q])
+    assert ra["stored_values"] == rb["stored_values"]

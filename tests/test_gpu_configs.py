@@ -1,0 +1,222 @@
+"""GPU parity at the sizes of the benchmark configurations, through the C ABI.
+
+- C1 (2D, 1024 points, 200 slabs, 3 iterations) at full size against the FP64 oracle;
+- C2 (3D 256² × 512 tilted interface, 4 iterations) at full depth against the reference's own C++
+  core (oracle/_ref, built from its sources);
+- C3 (3D 512² Born volume) over 8 slabs and 2 iterations against the reference's C++ core;
+- C4 (uneven Born batch) as six real-size draws of the batch submitted to the queue together,
+  each equal to its solve alone;
+- C5 (served stream) as a full-size subset of the served mix (512² × 1000 tilted, a silver layer,
+  a δn volume, a 4096-point 2D solve) in the queue together, each equal to its solve alone.
+
+Tolerances. FP64: the reference's own GPU bar (rel 1e-10, abs 1e-11·max,
+tests/test_core_cuda.py:34-36); Born solves 1e-10 of the field maximum (GMRES stops at a relative
+residual of 1e-12 on both sides). Narrow modes: relative field L2 ≤ 1e-5, and reflectance and
+transmittance (Poynting flux over the incident flux) within 1e-4 of the reference's.
+"""
+
+from __future__ import annotations
+
+import importlib.util
+from functools import lru_cache
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, rel_l2
+from oracle import bwpm_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+L2_NARROW = 1e-5
+RT_TOL = 1e-4
+
+
+def _core():
+    from bwpm_amd import _core
+
+    assert _core.gpu_device_count() > 0, "no GPU visible"
+    return _core
+
+
+def _reference_core():
+    so = sorted((ROOT / "oracle" / "_ref").glob("_core*.so")) if (ROOT / "oracle" / "_ref").exists() else []
+    if not so:
+        pytest.skip("reference core not built (oracle/build_ref.sh)")
+    spec = importlib.util.spec_from_file_location("_core", so[0])
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def _close64(a: np.ndarray, b: np.ndarray) -> None:
+    atol = 1e-11 * float(np.max(np.abs(b)) or 1.0)
+    assert np.allclose(a, b, rtol=1e-10, atol=atol), float(np.max(np.abs(a - b)))
+
+
+def _gpu(p, precision: str, keep=()) -> dict:
+    c = _core()
+    s = c.GpuSolver(*p.solver_args(), fluctuation=p.fluctuation, precision=precision)
+    try:
+        return s.run(p.field, p.iterations, list(keep))
+    finally:
+        s.close()
+
+
+def _last_index(p) -> complex:
+    lab = p.label_volume()
+    return complex(p.indices[int(lab[-1, p.ny // 2, p.nx // 2])])
+
+
+def _r_t(p, res: dict) -> tuple[float, float]:
+    """Reflectance and transmittance of a result: Poynting flux of the reflected field (backward,
+    in the medium before the volume) and of the transmitted field (forward, in the medium at the
+    centre of the last slab) over the incident flux (src/bwpm/fast.py:146-155, adapter.py:267-284)."""
+    args = (p.nx, p.ny, p.dx, p.dy, p.k0)
+    p_in = O.port_power(p.field, *args, p.index_before, 1)
+    p_r = O.port_power(res["reflected"], *args, p.index_before, -1)
+    p_t = O.port_power(res["transmitted"], *args, _last_index(p), 1)
+    return p_r / p_in, p_t / p_in
+
+
+def _narrow_ok(p, got: dict, ref: dict, what: str) -> None:
+    for key in ("transmitted", "reflected"):
+        err = rel_l2(got[key], ref[key])
+        print(f"{what} {key} rel L2 {err:.3e}", flush=True)
+        assert err <= L2_NARROW, (what, key, err)
+    (rg, tg), (rr, tr) = _r_t(p, got), _r_t(p, ref)
+    print(f"{what} |dR| {abs(rg - rr):.2e} |dT| {abs(tg - tr):.2e} (R {rr:.6f}, T {tr:.6f})", flush=True)
+    assert abs(rg - rr) <= RT_TOL and abs(tg - tr) <= RT_TOL, (what, rg, rr, tg, tr)
+
+
+# ---- C1 ---------------------------------------------------------------------------------------
+
+@lru_cache(maxsize=1)
+def _c1():
+    from bwpm_amd import workloads
+
+    p = workloads.plain_2d(ny=1024, nz=200, iterations=3)
+    ref = O.Solver(*p.solver_args()).run(p.field, p.iterations)
+    return p, ref
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed", "fp32"])
+def test_c1_full_size_against_the_oracle(precision: str) -> None:
+    p, ref = _c1()
+    print(f"C1 {p.ny}x{p.nz} it{p.iterations} {precision}", flush=True)
+    got = _gpu(p, precision)
+    if precision == "fp64":
+        for key in ("transmitted", "reflected"):
+            _close64(got[key], ref[key])
+        for a, b in zip(got["transmitted_iter"] + got["reflected_iter"], ref["transmitted_iter"] + ref["reflected_iter"]):
+            _close64(a, b)
+    else:
+        _narrow_ok(p, got, ref, f"C1 {precision}")
+    # lossless: the energy residual 1 − R − T of the reference's (ε, fast.py:148-155)
+    (rg, tg), (rr, tr) = _r_t(p, got), _r_t(p, ref)
+    assert abs((1 - rg - tg) - (1 - rr - tr)) <= (1e-10 if precision == "fp64" else RT_TOL)
+
+
+# ---- C2 ---------------------------------------------------------------------------------------
+
+@lru_cache(maxsize=2)
+def _c2(pol: str):
+    from bwpm_amd import workloads
+
+    R = _reference_core()
+    p = workloads.tilted_3d(n=256, nz=512, iterations=4, polarisation=pol)
+    print(f"C2 reference core {p.ny}x{p.nx}x{p.nz} it{p.iterations} E{pol}", flush=True)
+    ref = R.Solver(p.ny, p.nx, p.dy, p.dx, p.dz, p.k0, np.asarray(p.indices, complex), p.labels.astype(np.int32),
+                   complex(p.index_before), None, 16).run(p.field, p.iterations, [256, 512])
+    return p, ref
+
+
+@pytest.mark.parametrize("pol", ["x", "y"])
+@pytest.mark.parametrize("precision", ["fp64", "mixed", "fp32"])
+def test_c2_full_depth_against_the_reference_core(precision: str, pol: str) -> None:
+    p, ref = _c2(pol)
+    print(f"C2 {precision} E{pol}", flush=True)
+    got = _gpu(p, precision, keep=[256, 512])
+    if precision == "fp64":
+        scale = float(np.max(np.abs(p.field)))
+        for key in ("transmitted", "reflected"):
+            assert np.max(np.abs(got[key] - ref[key])) < 1e-11 * scale, key
+        for q in (256, 512):
+            assert np.max(np.abs(got["forward"][q] - ref["forward"][q])) < 1e-11 * scale
+            assert np.max(np.abs(got["backward"][q] - ref["backward"][q])) < 1e-11 * scale
+        assert got["stored_values"] == ref["stored_values"]
+    else:
+        _narrow_ok(p, got, ref, f"C2 {precision} E{pol}")
+
+
+# ---- C3 ---------------------------------------------------------------------------------------
+
+@lru_cache(maxsize=1)
+def _c3():
+    from bwpm_amd import workloads
+
+    R = _reference_core()
+    p = workloads.born_3d(n=512, nz=8, iterations=2)
+    print(f"C3 reference core {p.ny}x{p.nx}x{p.nz} it{p.iterations}", flush=True)
+    ref = R.Solver(p.ny, p.nx, p.dy, p.dx, p.dz, p.k0, np.asarray(p.indices, complex), p.labels.astype(np.int32),
+                   complex(p.index_before), None, 16, fluctuation=p.fluctuation).run(p.field, p.iterations)
+    return p, ref
+
+
+def test_c3_born_512_against_the_reference_core() -> None:
+    p, ref = _c3()
+    got = _gpu(p, "fp64")
+    scale = float(np.max(np.abs(p.field)))
+    for key in ("transmitted", "reflected"):
+        err = float(np.max(np.abs(got[key] - ref[key])))
+        print(f"C3 fp64 {key} max abs error {err:.3e} (field max {scale:.3e})", flush=True)
+        assert err < 1e-10 * scale, key
+    for a, b in zip(got["transmitted_iter"] + got["reflected_iter"], ref["transmitted_iter"] + ref["reflected_iter"]):
+        assert np.max(np.abs(a - b)) < 1e-10 * scale
+
+
+# ---- C4 and C5: the queue against solves alone ------------------------------------------------
+
+def _same(a: dict, b: dict) -> None:
+    assert np.array_equal(a["transmitted"], b["transmitted"])
+    assert np.array_equal(a["reflected"], b["reflected"])
+    for x, y in zip(a["transmitted_iter"] + a["reflected_iter"], b["transmitted_iter"] + b["reflected_iter"]):
+        assert np.array_equal(x, y)
+    assert a["stored_values"] == b["stored_values"]
+
+
+def test_c4_uneven_born_draws_in_the_queue_equal_their_solves_alone() -> None:
+    """Six draws of the 64-solve uneven batch (128², 256², 384² and 512², 100 … 651 slabs, own λ,
+    angle, σ and seed, δn generated on the device), all submitted at once."""
+    from bwpm_amd import serve, workloads
+
+    batch = workloads.uneven_batch(64)
+    picks = [batch[i] for i in (0, 1, 3, 4, 7, 9)]
+    assert {p.ny for p in picks} == {128, 256, 384, 512}
+    with serve.Server(slots=16, chunk=64, precision="fp64") as srv:
+        tickets = [srv.submit(p) for p in picks]
+        results = [srv.result(t, timeout=300) for t in tickets]
+    for p, res in zip(picks, results):
+        print("C4 compare", p.name, flush=True)
+        _same(res, _gpu(p, "fp64"))
+
+
+def test_c5_full_size_served_subset_in_the_mix_equals_the_solves_alone() -> None:
+    from bwpm_amd import serve, workloads
+
+    reqs = [
+        workloads.tilted_3d(n=512, nz=1000, iterations=1, geometry=True),
+        serve.planar_3d(384, 600, 2, 0.9, 1.0, serve.SILVER, layer=(200, 205), polarisation="y"),
+        workloads.born_3d_synth(n=256, nz=300, iterations=1, sigma=0.015, seed=77),
+        workloads.plain_2d(ny=4096, nz=800, iterations=2),
+        serve.planar_3d(192, 400, 3, 1.4, 1.0, 1.6),
+        workloads.tilted_3d(n=512, nz=700, iterations=2, polarisation="y", tilt_deg=4.0, geometry=True),
+    ]
+    with serve.Server(slots=4, chunk=64, precision="fp64") as srv:
+        tickets = [srv.submit(p) for p in reqs]
+        results = [srv.result(t, timeout=300) for t in tickets]
+        st = srv.stats()
+    assert st["failed"] == 0 and st["completed"] >= len(reqs)
+    for p, res in zip(reqs, results):
+        print("C5 compare", p.name, flush=True)
+        _same(res, _gpu(p, "fp64"))

@@ -418,3 +418,74 @@ def test_planted_mistakes_fail_the_fixture(mistake: str, monkeypatch: pytest.Mon
 
         monkeypatch.setattr(O.Solver, "__init__", init)
     assert not _fixture_ok()
+
+
+# ---- 3D port flux (oracle.port_power) ----------------------------------------------------------
+
+def _port_plane_wave(nx, ny, dx, dy, mx, my, e_perp):
+    x = np.arange(nx) * dx
+    y = np.arange(ny) * dy
+    kx = 2 * np.pi * mx / (nx * dx)
+    ky = 2 * np.pi * my / (ny * dy)
+    ph = np.exp(1j * (kx * x[None, :] + ky * y[:, None]))
+    return np.stack([e_perp[0] * ph, e_perp[1] * ph], axis=-1), kx, ky
+
+
+@pytest.mark.parametrize("direction", [1, -1])
+def test_port_power_reduces_to_the_2d_flux(direction: int) -> None:
+    # a field that does not vary along x (nx = 1): the 3D flux is dx times the 2D flux per unit x
+    rng = np.random.default_rng(11)
+    ny, dy, dx, k0, n = 64, 0.5e-3 / 8, 0.37e-3, 2 * np.pi / 0.5e-3, 1.3 + 0j
+    f = band_limited_field(rng, ny, 1, dy, dx, k0)
+    got = O.port_power(f, 1, ny, dx, dy, k0, n, direction)
+    want = dx * O.power_flux(f[:, 0, :], ny, dy, k0, n, direction)
+    assert abs(got - want) <= 1e-13 * abs(want)
+
+
+@pytest.mark.parametrize(("mx", "my", "n"), [(0, 0, 1.0), (3, 0, 1.0), (0, 5, 1.46), (4, -3, 1.46), (-6, 2, 2.1)])
+def test_port_power_of_plane_waves_is_n_k0_cos_theta(mx: int, my: int, n: float) -> None:
+    # a plane wave of unit |E| (E ⊥ k) carries n·k0·cosθ·|E|²·area for s and p polarisation alike;
+    # the p wave needs the E_z term with the right wave-vector components
+    nx, ny, dx, dy = 24, 20, 0.5e-3 / 6, 0.5e-3 / 5
+    k0 = 2 * np.pi / 0.5e-3
+    kxv = 2 * np.pi * mx / (nx * dx)
+    kyv = 2 * np.pi * my / (ny * dy)
+    kp = np.hypot(kxv, kyv)
+    kz = np.sqrt((n * k0) ** 2 - kp ** 2)
+    cos_t = kz / (n * k0)
+    area = nx * dx * ny * dy
+    if kp == 0:
+        s_dir, p_dir = np.array([0.0, 1.0]), np.array([1.0, 0.0])
+    else:
+        s_dir, p_dir = np.array([-kyv, kxv]) / kp, np.array([kxv, kyv]) / kp
+    a, b = 0.6 - 0.8j, 0.3 + 1.4j
+    # s wave: E = a·ŝ (transverse); p wave: |E| = |b|, transverse part b·cosθ along k⊥
+    for e_perp, e_total2 in ((s_dir * a, abs(a) ** 2), (p_dir * cos_t * b, abs(b) ** 2)):
+        field, _, _ = _port_plane_wave(nx, ny, dx, dy, mx, my, e_perp)
+        for direction in (1, -1):
+            got = O.port_power(field, nx, ny, dx, dy, k0, n + 0j, direction)
+            want = n * k0 * cos_t * e_total2 * area
+            assert abs(got - want) <= 1e-12 * want
+
+
+def test_port_power_of_an_evanescent_wave_is_zero() -> None:
+    nx, ny, dx, dy = 16, 16, 0.5e-3 / 4, 0.5e-3 / 4
+    k0 = 2 * np.pi / 0.5e-3
+    field, kx, ky = _port_plane_wave(nx, ny, dx, dy, 6, 5, (1.0, 0.5j))
+    assert np.hypot(kx, ky) > k0
+    # only FFT rounding in the propagating modes is left (relative to k0·|E|²·area)
+    scale = k0 * 1.25 * nx * dx * ny * dy
+    assert abs(O.port_power(field, nx, ny, dx, dy, k0, 1.0 + 0j)) <= 1e-26 * scale
+
+
+def test_planted_mistake_in_port_power_fails_the_closed_form(monkeypatch: pytest.MonkeyPatch) -> None:
+    # E_z from the wrong wave-vector component (k_y·E_x + k_x·E_y) must fail the p-wave check
+    orig = O.port_power
+
+    def swapped(field, nx, ny, dx, dy, k0, index, direction=1):
+        f = np.asarray(field)
+        return orig(f[..., ::-1], nx, ny, dx, dy, k0, index, direction)
+
+    monkeypatch.setattr(O, "port_power", swapped)
+    with pytest.raises(AssertionError):
+        test_port_power_of_plane_waves_is_n_k0_cos_theta(4, 0, 1.46)
