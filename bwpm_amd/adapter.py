@@ -1,17 +1,25 @@
-"""Plugin adapter of the GPU engine with the reference's `bwpm.adapter` API (src/bwpm/adapter.py:38-310).
+"""Plugin adapter over the MI355X engine, with the call surface of the reference's `bwpm.adapter`
+(src/bwpm/adapter.py:38-310): `describe`, `prepare`, `apply`, `estimate`, `port_power`,
+`front_port`, `back_port`, `plane_wave` and the types `Port`, `PortField`, `Structure`, `Handle`.
 
-- `describe()`: method, conventions, ports, backends and validity limits as data.
-- `prepare(structure, wavelengths)`: checks the structure, computes a content hash.
-- `apply(handle, port_field, ...)`: field incident on the front port (+z) → reflected field at the
-  front port (−z) and transmitted field at the back port (+z), solved on the GPU.
-- `estimate(handle, iterations)`: run time from the measured MI355X update rate and device memory
-  from the engine's memory model.
-- `port_power(port_field)`: Poynting flux through a port (device FP64 reduction).
+The names, the dataclass fields and the error messages are the reference's, because callers build
+these types and match these messages. Everything behind them is this package's:
+
+- `describe()` is assembled from the engine's own tables (precision modes, determinism, backends);
+- `prepare()` hashes a canonical JSON header of the scalar parameters, then each array with its
+  dtype and shape, under an engine tag;
+- solvers are kept per (wavelength, precision) on the handle;
+- `apply()` solves on the GPU through `fast.run_3d`;
+- `estimate()` prices a solve with the update rate measured on one MI355X and the engine's memory
+  model;
+- `port_power()` is the device FP64 flux reduction (`bwpm_power_flux`);
+- `plane_wave()` is the outer product of two axis phase vectors.
 """
 
 from __future__ import annotations
 
 import hashlib
+import json
 from dataclasses import dataclass, field
 from typing import Any
 
@@ -20,9 +28,10 @@ import numpy as np
 from bwpm_amd import __version__, _core, fast
 from bwpm_amd.volume3d import Bidirectional3D, Grid3D, absorbing_mask_2d, memory_estimate
 
-# seconds per grid point and plane pass on one MI355X (single solve, 512² plain slab steps, measured
-# by bench.py); the batched lanes of `bwpm_amd.serve` reach several times this rate
-SECONDS_PER_POINT_PASS = {"gpu": 1.6e-10}
+# seconds per grid point and plane pass of one solve alone on one MI355X (512² plain slab steps,
+# FP64, one solve per launch; bench.py `checks.plain_one_solve_per_launch` = 1.16e10 updates/s);
+# the batched lanes of `bwpm_amd.serve` reach several times this rate
+SECONDS_PER_POINT_PASS = {"gpu": 8.6e-11}
 
 
 @dataclass(frozen=True)
@@ -51,7 +60,7 @@ class PortField:
     field: Any
 
     def __post_init__(self) -> None:
-        if np.asarray(self.field).shape != (self.port.ny, self.port.nx, 2):
+        if np.shape(self.field) != (self.port.ny, self.port.nx, 2):
             raise ValueError("field must have shape (ny, nx, 2) of the port grid")
 
 
@@ -78,7 +87,42 @@ class Handle:
     structure: Structure
     wavelengths_um: tuple[float, ...]
     content_hash: str
-    _solvers: dict[float, Bidirectional3D] = field(default_factory=dict, repr=False)
+    _solvers: dict[tuple[float, str], Bidirectional3D] = field(default_factory=dict, repr=False)
+
+
+# ---- self-description ------------------------------------------------------------------------
+
+_METHOD = ("fast bidirectional vector wave propagation (Wende et al., Opt. Express 31, 28874, 2023): "
+           "angular-spectrum slab steps with 2x2 Fresnel interface operators, iterated forward and "
+           "backward; hand-written HIP kernels on one MI355X")
+
+_CONVENTIONS = {
+    "time": "harmonic factor exp(-i omega t); a plane wave is exp(i(k.r - omega t))",
+    "fresnel": "Convention A of Byrnes, arXiv:1603.02720v5, eq. 6",
+    "polarisation": "TE is s, TM is p; ports carry the transverse components (E_x, E_y)",
+    "units": {"length": "mm", "wavelength": "um", "k": "1/mm"},
+    "grid": "pixel m at x = (m - nx//2) dx (centred); spectra refer to the origin (0, 0)",
+    "kz": "branch Im k_z >= 0; |n^2 k0^2 - kperp^2| <= 1e-12 |n^2 k0^2| counts as grazing",
+}
+
+_PORTS = (
+    {"name": "front", "plane": "z0", "in": "+z", "out": "-z (reflected)"},
+    {"name": "back", "plane": "z0 + nz dz", "out": "+z (transmitted)"},
+)
+
+_VALIDITY = (
+    "reflection amplitudes are quantitative at interfaces normal to z; at tilted surfaces only "
+    "their direction and phase are",
+    "transmitted power through tilted surfaces holds to a few percent",
+    "fields trapped by total internal reflection do not settle over iterations: bound the count",
+    "absorbing edge zones need at least 20 wavelengths (super-Gaussian of order 8)",
+)
+
+_PRECISION = {
+    "fp64": "complex128 planes and arithmetic (the reference's)",
+    "mixed": "block-floating-point planes of complex64 size, FP64 arithmetic",
+    "fp32": "complex64 planes and arithmetic, FP64 operator tables (compensated)",
+}
 
 
 def _backends() -> list[str]:
@@ -86,37 +130,38 @@ def _backends() -> list[str]:
 
 
 def describe() -> dict[str, Any]:
-    """Self-description for a planner."""
+    """Self-description for a planner: method, conventions, ports, properties, validity."""
+    props = {"linear": True, "reciprocal": True, "operator": False}
     return {
         "name": "BWPM",
         "version": __version__,
-        "method": "fast bidirectional vector wave propagation (wende2023), FPWPM steps with Fresnel "
-                  "interface matrices, forward-backward iterations; MI355X engine",
-        "conventions": {
-            "time": "exp(-i omega t), plane waves exp(i(k.r - omega t))",
-            "fresnel": "Convention A (Byrnes arXiv:1603.02720v5 eq. 6)",
-            "polarisation": "s = TE, p = TM; port fields are transverse (E_x, E_y)",
-            "units": {"length": "mm", "wavelength": "um", "k": "1/mm"},
-            "grid": "centred, x_m = (m - nx//2) dx; spectra referred to (0, 0)",
-            "kz": "Im k_z >= 0; grazing rule |n^2 k0^2 - kperp^2| <= 1e-12 |n^2 k0^2|",
-        },
-        "ports": [
-            {"name": "front", "plane": "z0", "in": "+z", "out": "-z (reflected)"},
-            {"name": "back", "plane": "z0 + nz dz", "out": "+z (transmitted)"},
-        ],
+        "method": _METHOD,
+        "conventions": {k: (dict(v) if isinstance(v, dict) else v) for k, v in _CONVENTIONS.items()},
+        "ports": [dict(p) for p in _PORTS],
         "representation": "gridded transverse field",
-        "linear": True,
-        "reciprocal": True,
-        "operator": False,
-        "deterministic": {"gpu": "bitwise (fixed reduction order, batch-invariant)"},
-        "precision": {"fp64": "complex128 bulk", "fp32": "complex64 bulk, FP64 tables, compensated"},
+        **props,
+        "deterministic": {"gpu": "bitwise (fixed reduction order; batched and served solves equal solves alone)"},
+        "precision": dict(_PRECISION),
         "backends": _backends(),
-        "validity": [
-            "reflections quantitative only at z-normal interfaces; tilted surfaces: direction and phase only",
-            "waves trapped by total internal reflection do not converge; limit iterations",
-            "absorbing boundary zone >= 20 wavelengths, order 8",
-        ],
+        "validity": list(_VALIDITY),
     }
+
+
+# ---- preparation ------------------------------------------------------------------------------
+
+def _content_hash(s: Structure, labels: np.ndarray, wavelengths: np.ndarray) -> str:
+    def cplx(z: complex) -> list[float]:
+        z = complex(z)
+        return [z.real, z.imag]
+
+    header = {"engine": "bwpm_amd", "version": __version__,
+              "grid": [int(s.nx), int(s.ny), float(s.dx), float(s.dy), float(s.dz)],
+              "index_before": cplx(s.index_before), "z0": float(s.z0), "mask_width": float(s.mask_width)}
+    h = hashlib.sha256(json.dumps(header, sort_keys=True).encode())
+    for arr in (np.asarray(s.indices, dtype=np.complex128), labels, wavelengths):
+        h.update(f"{arr.dtype.str}{arr.shape}".encode())
+        h.update(np.ascontiguousarray(arr).tobytes())
+    return h.hexdigest()
 
 
 def prepare(structure: Structure, wavelengths_um: list[float] | tuple[float, ...]) -> Handle:
@@ -124,57 +169,64 @@ def prepare(structure: Structure, wavelengths_um: list[float] | tuple[float, ...
     labels = np.asarray(structure.labels)
     if labels.ndim != 3 or labels.shape[1:] != (structure.ny, structure.nx):
         raise ValueError("labels must have shape (nz, ny, nx)")
-    if labels.min() < 0 or labels.max() >= len(structure.indices):
+    lo, hi = (int(labels.min()), int(labels.max())) if labels.size else (0, -1)
+    if lo < 0 or hi >= len(structure.indices):
         raise ValueError("labels must index into `indices`")
-    if not wavelengths_um:
+    wl = np.asarray(list(wavelengths_um), dtype=np.float64)
+    if wl.size == 0:
         raise ValueError("at least one wavelength is required")
-    h = hashlib.sha256()
-    h.update(f"bwpm_amd {__version__}".encode())
-    for value in (structure.nx, structure.ny, structure.dx, structure.dy, structure.dz, structure.index_before,
-                  structure.z0, structure.mask_width):
-        h.update(repr(value).encode())
-    h.update(np.asarray(structure.indices, dtype=np.complex128).tobytes())
-    h.update(np.ascontiguousarray(labels, dtype=np.int32).tobytes())
-    h.update(np.asarray(wavelengths_um, dtype=np.float64).tobytes())
-    return Handle(structure, tuple(float(w) for w in wavelengths_um), h.hexdigest())
+    return Handle(structure, tuple(wl.tolist()), _content_hash(structure, labels.astype(np.int32), wl))
 
 
 def _solver(handle: Handle, wavelength_um: float, precision: str) -> Bidirectional3D:
     if wavelength_um not in handle.wavelengths_um:
         raise ValueError("wavelength was not prepared")
-    s = handle.structure
-    grid = Grid3D(s.nx, s.ny, s.dx, s.dy, s.dz, wavelength_um)
-    mask = absorbing_mask_2d(grid, s.mask_width) if s.mask_width > 0 else None
-    return Bidirectional3D(grid, list(s.indices), np.asarray(s.labels), s.index_before, mask, precision=precision)
+    key = (float(wavelength_um), precision)
+    if key not in handle._solvers:
+        s = handle.structure
+        grid = Grid3D(s.nx, s.ny, s.dx, s.dy, s.dz, wavelength_um)
+        mask = absorbing_mask_2d(grid, s.mask_width) if s.mask_width > 0 else None
+        handle._solvers[key] = Bidirectional3D(grid, list(s.indices), np.asarray(s.labels), s.index_before, mask,
+                                               precision=precision)
+    return handle._solvers[key]
+
+
+# ---- ports ------------------------------------------------------------------------------------
+
+def _port_at(s: Structure, z: float, direction: int, index: complex, wavelength_um: float) -> Port:
+    return Port(z, direction, index, wavelength_um, s.nx, s.ny, s.dx, s.dy)
 
 
 def front_port(handle: Handle, wavelength_um: float, direction: int = 1) -> Port:
+    """The plane z0 in the medium before the volume."""
     s = handle.structure
-    return Port(s.z0, direction, s.index_before, wavelength_um, s.nx, s.ny, s.dx, s.dy)
+    return _port_at(s, s.z0, direction, s.index_before, wavelength_um)
 
 
 def back_port(handle: Handle, wavelength_um: float) -> Port:
+    """The plane z0 + nz·dz, in the medium of the last slab's centre pixel, direction +1."""
     s = handle.structure
     labels = np.asarray(s.labels)
-    last = s.indices[int(labels[-1, s.ny // 2, s.nx // 2])]
-    return Port(s.z0 + labels.shape[0] * s.dz, 1, last, wavelength_um, s.nx, s.ny, s.dx, s.dy)
+    exit_label = int(labels[labels.shape[0] - 1, s.ny // 2, s.nx // 2])
+    return _port_at(s, s.z0 + labels.shape[0] * s.dz, 1, s.indices[exit_label], wavelength_um)
 
+
+# ---- solving ----------------------------------------------------------------------------------
 
 def apply(handle: Handle, incident: PortField, iterations: int = 3, backend: str = "auto",
           precision: str = "fp64") -> dict[str, PortField]:
     """Propagate a field incident on the front port; returns {"front": reflected, "back": transmitted}."""
-    p = incident.port
-    s = handle.structure
-    if p.direction != 1 or abs(p.z - s.z0) > 1e-15 or complex(p.index) != complex(s.index_before):
+    p, s = incident.port, handle.structure
+    on_front = p.direction == 1 and abs(p.z - s.z0) <= 1e-15 and complex(p.index) == complex(s.index_before)
+    if not on_front:
         raise ValueError("incident field must be on the front port, direction +1")
     if (p.nx, p.ny, p.dx, p.dy) != (s.nx, s.ny, s.dx, s.dy):
         raise ValueError("port grid differs from the structure grid")
-    solver = _solver(handle, p.wavelength_um, precision)
-    result = fast.run_3d(solver, np.asarray(incident.field), iterations, backend=backend)
-    return {
-        "front": PortField(front_port(handle, p.wavelength_um, -1), result.reflected),
-        "back": PortField(back_port(handle, p.wavelength_um), result.transmitted),
-    }
+    result = fast.run_3d(_solver(handle, p.wavelength_um, precision), np.asarray(incident.field), iterations,
+                         backend=backend)
+    wl = p.wavelength_um
+    return {"front": PortField(front_port(handle, wl, -1), result.reflected),
+            "back": PortField(back_port(handle, wl), result.transmitted)}
 
 
 def estimate(handle: Handle, iterations: int = 3, backend: str = "auto") -> dict[str, float]:
@@ -201,12 +253,13 @@ def port_power(port_field: PortField) -> float:
 
 def plane_wave(port: Port, mx: int, my: int, e_perp: tuple[complex, complex]) -> PortField:
     """Plane wave on the port grid with k⊥ = 2π(mx/(nx dx), my/(ny dy)), phase 1 at (0, 0)."""
-    x = (np.arange(port.nx) - port.nx // 2) * port.dx
-    y = (np.arange(port.ny) - port.ny // 2) * port.dy
-    kx = 2 * np.pi * mx / (port.nx * port.dx)
-    ky = 2 * np.pi * my / (port.ny * port.dy)
-    phase = np.exp(1j * (kx * x[None, :] + ky * y[:, None]))
-    return PortField(port, np.stack([e_perp[0] * phase, e_perp[1] * phase], axis=-1))
+
+    def axis(m: int, n: int) -> np.ndarray:
+        # e^{i k x_j} with k = 2π m/(n d) and x_j = (j − n//2) d
+        return np.exp(2j * np.pi * m * (np.arange(n) - n // 2) / n)
+
+    phase = np.outer(axis(my, port.ny), axis(mx, port.nx))
+    return PortField(port, phase[:, :, None] * np.asarray(e_perp, dtype=np.complex128))
 
 
 __all__ = ["Handle", "Port", "PortField", "Structure", "apply", "back_port", "describe", "estimate", "front_port",
