@@ -267,6 +267,8 @@ def main() -> int:
     ap.add_argument("--nz", type=int, default=1024)
     ap.add_argument("--iterations", type=int, default=4)
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed", "fp32", "fp32_plain"])
+    ap.add_argument("--absorber", type=float, default=0.0,
+                    help="absorbing edge zone of the headline geometry in wavelengths (0: none)")
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "0") or 0))
     ap.add_argument("--cpu-nz", type=int, default=48)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -286,7 +288,8 @@ def main() -> int:
     from bwpm_amd import _core, workloads
 
     probs = [workloads.tilted_3d(n=args.n, nz=args.nz, iterations=args.iterations,
-                                 polarisation="x" if i % 2 == 0 else "y") for i in range(args.batch)]
+                                 polarisation="x" if i % 2 == 0 else "y", absorber_wavelengths=args.absorber)
+             for i in range(args.batch)]
     solvers = []
     for p in probs:
         s = _core.GpuSolver(*p.solver_args(), precision=args.precision)

@@ -67,11 +67,6 @@ std::unique_ptr<EngineBase> make_engine(Spec spec) {
   switch (spec.precision) {
     case 64: return make_engine_f64(std::move(spec));
     case 48:
-      // Born steps are bound by their GMRES, not by the two streaming planes, and the Born operator
-      // amplifies the planes' storage rounding: solves with index fluctuations keep them FP64
-      if ((spec.synth.on && spec.synth.sigma > 0) ||
-          std::any_of(spec.fluct.begin(), spec.fluct.end(), [](const c128& v) { return v != c128(0.0, 0.0); }))
-        return make_engine_f64(std::move(spec));
       return make_engine_mixed(std::move(spec));
     case 32: return make_engine_f32(std::move(spec), true);
     case 31: return make_engine_f32(std::move(spec), false);

@@ -19,6 +19,7 @@
 #include "engine.hpp"
 #include "kernels.hpp"
 #include "kernels_fast.hpp"
+#include "kernels_plain.hpp"
 
 namespace bwpm {
 
@@ -118,6 +119,9 @@ struct Geometry {
   // generic, for launches of few workgroups: lines per workgroup within kNarrowLds of LDS, so that
   // several workgroups share a CU
   int cols2n = 1, ls_col2n = 1, rowsn = 1, ls_rown = 1;
+  // register grids: the plain steps take the persistent kernels of kernels_plain.hpp (else the
+  // general register kernels); chosen per lane shape by the tuner
+  bool plain_col = true, plain_row = true;
 };
 
 
@@ -135,9 +139,16 @@ struct TuneKey {
     return std::tie(device, precision, slots, ny, nx) < std::tie(o.device, o.precision, o.slots, o.ny, o.nx);
   }
 };
+// A tuning candidate: columns and rows per workgroup, and whether plain steps take the persistent
+// column / row kernels.
+struct TuneChoice {
+  int cw = 1, rw = 1;
+  bool pc = true, pr = true;
+  bool operator==(const TuneChoice& o) const { return cw == o.cw && rw == o.rw && pc == o.pc && pr == o.pr; }
+};
 struct TuneState {
-  std::vector<std::pair<int, int>> cand;  // (columns, rows) per workgroup
-  std::vector<double> us;                 // measured time per solve-step, < 0: not yet
+  std::vector<TuneChoice> cand;
+  std::vector<double> us;  // measured time per solve-step, < 0: not yet
   int best = -1;
 };
 inline std::mutex& tune_mutex() {
@@ -151,25 +162,29 @@ inline std::map<TuneKey, TuneState>& tune_cache() {
 
 // Candidate to run next for `key` (given the kernel geometry `g`): the best one when tuning is
 // done (trial = −1), else the next untimed candidate (trial = its index).
-inline std::pair<int, int> tune_pick(const TuneKey& key, const Geometry& g, int& trial) {
+inline TuneChoice tune_pick(const TuneKey& key, const Geometry& g, int& trial) {
   std::lock_guard<std::mutex> lock(tune_mutex());
   TuneState& st = tune_cache()[key];
   if (st.cand.empty()) {
     for (int cw : {g.col_w, 1})
-      for (int rw : {g.rw, 1}) {
-        if (!g.col_fast && cw != g.col_w) continue;
-        if (!g.row_fast && rw != g.rw) continue;
-        const std::pair<int, int> c{cw, rw};
-        if (std::find(st.cand.begin(), st.cand.end(), c) == st.cand.end()) st.cand.push_back(c);
-      }
+      for (int rw : {g.rw, 1})
+        for (bool pc : {true, false})
+          for (bool pr : {true, false}) {
+            if (!g.col_fast && (cw != g.col_w || !pc)) continue;
+            if (!g.row_fast && (rw != g.rw || !pr)) continue;
+            const TuneChoice c{cw, rw, pc, pr};
+            if (std::find(st.cand.begin(), st.cand.end(), c) == st.cand.end()) st.cand.push_back(c);
+          }
     st.us.assign(st.cand.size(), -1.0);
     if (st.cand.size() == 1) st.best = 0;
-    // BWPM_LINES="<columns>,<rows>" fixes the choice when it is a candidate (reproducible
-    // profiles: hardware counters slow kernels unevenly, which can sway the timed choice)
+    // BWPM_LINES="<columns>,<rows>[,<plain columns>,<plain rows>]" fixes the choice when it is a
+    // candidate (reproducible profiles: hardware counters slow kernels unevenly, which can sway the
+    // timed choice); the plain flags default to 1
     if (const char* env = std::getenv("BWPM_LINES")) {
-      int cw = 0, rw = 0;
-      if (std::sscanf(env, "%d,%d", &cw, &rw) == 2) {
-        const auto it = std::find(st.cand.begin(), st.cand.end(), std::pair<int, int>{cw, rw});
+      int cw = 0, rw = 0, pc = 1, pr = 1;
+      if (std::sscanf(env, "%d,%d,%d,%d", &cw, &rw, &pc, &pr) >= 2) {
+        const TuneChoice want{cw, rw, pc != 0 && g.col_fast, pr != 0 && g.row_fast};
+        const auto it = std::find(st.cand.begin(), st.cand.end(), want);
         if (it != st.cand.end()) st.best = static_cast<int>(it - st.cand.begin());
       }
     }
@@ -191,6 +206,39 @@ inline void tune_report(const TuneKey& key, int trial, double us_per_solve_step)
   st.us[trial] = us_per_solve_step;
   if (std::all_of(st.us.begin(), st.us.end(), [](double v) { return v >= 0; }))
     st.best = static_cast<int>(std::min_element(st.us.begin(), st.us.end()) - st.us.begin());
+}
+
+// Workgroups of `kern` (blocks of `threads`) the current device holds at once.
+inline int resident_groups(const void* kern, int threads) {
+  int dev = 0, cus = 0, per = 0;
+  HIP_CHECK(hipGetDevice(&dev));
+  HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kern, threads, 0));
+  return std::max(1, cus * std::max(per, 1));
+}
+
+// The plain kernels (kernels_plain.hpp) run only where they need no scratch memory: at 4096-point
+// lines (1024-thread workgroups, 128 registers per thread) the FP64 column kernel spilled 792 bytes
+// per thread and gave run-dependent, wrong columns in a 2D solve, while the general kernel of the
+// same shape was exact. Those shapes keep the general kernels.
+inline bool spill_free(const void* kern) {
+  hipFuncAttributes fa{};
+  HIP_CHECK(hipFuncGetAttributes(&fa, kern));
+  return fa.localSizeBytes == 0;
+}
+
+// Grid of a persistent launch over `tiles` tiles with room for `cap` resident workgroups: as few
+// workgroups as give every one the same number of tiles (± 1) in ceil(tiles / cap) rounds.
+inline int persistent_grid(long long tiles, int cap) {
+  const long long rounds = (tiles + cap - 1) / cap;
+  return static_cast<int>(std::max<long long>(1, (tiles + rounds - 1) / rounds));
+}
+
+// A row step every solve of the launch takes in the plain row kernel (kernels_plain.hpp): one
+// transmitted product, no reflections, sparse sources, Born increment or Born plane.
+template <typename T, typename ST>
+inline bool row_is_plain(const RowDesc<T, ST>& r) {
+  return r.active == 0 || (r.nt == 1 && r.nr == 0 && !(r.rowptr && r.sp_add) && r.dense == nullptr && r.inc == nullptr);
 }
 
 }  // namespace detail
@@ -336,6 +384,71 @@ struct Launcher {
       }
     s(&k_row_fast<N, RW, T, COMP, false, ST>, grid, dim3(N / 4 * RW), 0, d, npix, th, tl);
   }
+ public:
+  // Plain steps (every solve one column product; one transmitted row product and no reflections,
+  // sparse sources or Born terms) on the persistent kernels of kernels_plain.hpp; false if the grid
+  // has no register kernels for this direction.
+  bool col_plain(const ColDesc<T, ST>* d, int S, Emit& s) const {
+    if (!geo.col_fast || !geo.plain_col) return false;
+    switch (ny) {
+      case 64: return col_plain_n<64>(d, S, s);
+      case 128: return col_plain_n<128>(d, S, s);
+      case 256: return col_plain_n<256>(d, S, s);
+      case 512: return col_plain_n<512>(d, S, s);
+      case 1024: return col_plain_n<1024>(d, S, s);
+      case 2048: return col_plain_n<2048>(d, S, s);
+      case 4096: return col_plain_n<4096>(d, S, s);
+      default: return false;
+    }
+  }
+  bool row_plain(const RowDesc<T, ST>* d, int S, Emit& s) const {
+    if (!geo.row_fast || !geo.plain_row) return false;
+    switch (nx) {
+      case 64: return row_plain_n<64>(d, S, s);
+      case 128: return row_plain_n<128>(d, S, s);
+      case 256: return row_plain_n<256>(d, S, s);
+      case 512: return row_plain_n<512>(d, S, s);
+      case 1024: return row_plain_n<1024>(d, S, s);
+      case 2048: return row_plain_n<2048>(d, S, s);
+      case 4096: return row_plain_n<4096>(d, S, s);
+      default: return false;
+    }
+  }
+
+ private:
+  template <int N, int W>
+  bool col_plain_launch(const ColDesc<T, ST>* d, int S, Emit& s) const {
+    auto kern = &k_col_plain<N, W, T, COMP, ST>;
+    static const bool fits = spill_free(reinterpret_cast<const void*>(kern));
+    if (!fits) return false;
+    static const int cap = resident_groups(reinterpret_cast<const void*>(kern), N / 4 * W);
+    const int groups = static_cast<int>(nx / W);
+    s(kern, dim3(static_cast<unsigned>(persistent_grid(static_cast<long long>(S) * groups, cap))), dim3(N / 4 * W), 0,
+      d, S, groups, static_cast<int>(nx), twy->hi.template as<cx<T>>(), twy->lo.template as<cx<T>>());
+    return true;
+  }
+  template <int N, int RW>
+  bool row_plain_launch(const RowDesc<T, ST>* d, int S, Emit& s) const {
+    auto kern = &k_row_plain<N, RW, T, COMP, ST>;
+    static const bool fits = spill_free(reinterpret_cast<const void*>(kern));
+    if (!fits) return false;
+    static const int cap = resident_groups(reinterpret_cast<const void*>(kern), N / 4 * RW);
+    const int groups = static_cast<int>(ny / RW);
+    s(kern, dim3(static_cast<unsigned>(persistent_grid(static_cast<long long>(S) * groups, cap))), dim3(N / 4 * RW), 0,
+      d, S, groups, twx->hi.template as<cx<T>>(), twx->lo.template as<cx<T>>());
+    return true;
+  }
+  template <int N>
+  bool col_plain_n(const ColDesc<T, ST>* d, int S, Emit& s) const {
+    constexpr int W = fast_w<T, COMP>(N);
+    return geo.col_w == W ? col_plain_launch<N, W>(d, S, s) : col_plain_launch<N, 1>(d, S, s);
+  }
+  template <int N>
+  bool row_plain_n(const RowDesc<T, ST>* d, int S, Emit& s) const {
+    constexpr int RW = fast_rw<T, COMP>(N);
+    return geo.rw == RW ? row_plain_launch<N, RW>(d, S, s) : row_plain_launch<N, 1>(d, S, s);
+  }
+
   template <int N>
   void col_fast_n(const ColDesc<T, ST>* d, int S, Emit& s, bool multi) const {
     constexpr int W = fast_w<T, COMP>(N);
@@ -371,6 +484,7 @@ struct Op {
   // mix pairs, or storage is narrower than the arithmetic (a merge through a product plane would
   // round); otherwise rows that use one product take it directly in the plain variant
   bool row_stash = false;
+  bool col_phase = false;  // the column work is one phase product (the plain column kernel's case)
   long long slab = 0;
   bool bidir = false;
 };
@@ -538,6 +652,7 @@ class Engine final : public EngineBase {
         }
         Op<T, ST> o;
         o.col = col_desc(B, true, with_r);
+        o.col_phase = o.col.nprod == 1 && B.mleft[0] == B.mright[0];
         o.row = row_desc(B, p, with_r);
         o.row_stash = B.mixed_rows || sizeof(ST) != sizeof(T);
         o.row.inc = born_at(p) ? born_->increment() : nullptr;
@@ -582,6 +697,7 @@ class Engine final : public EngineBase {
         const bool with_r = !B.rpair.empty();
         Op<T, ST> o;
         o.col = col_desc(B, false, with_r);
+        o.col_phase = o.col.nprod == 1 && B.mleft[0] == B.mright[0];
         o.row = row_desc(B, p - 1, with_r);
         o.row_stash = B.mixed_rows || sizeof(ST) != sizeof(T);
         if (B.sp_count > 0) {
@@ -626,7 +742,7 @@ class Engine final : public EngineBase {
       if (e->stream_ != stream) HIP_CHECK(hipStreamSynchronize(e->stream_));
     std::vector<ColDesc<T, ST>> hcol(G * S);
     std::vector<RowDesc<T, ST>> hrow(G * S);
-    std::vector<char> has_col(G, 0), has_row(G, 0), multi_col(G, 0), multi_row(G, 0);
+    std::vector<char> has_col(G, 0), has_row(G, 0), multi_col(G, 0), multi_row(G, 0), plain_row(G, 1), plain_col(G, 1);
     for (size_t g = 0; g < G; ++g)
       for (int i = 0; i < S; ++i) {
         if (g >= progs[i].size() || progs[i][g].kind != kOpPair) continue;
@@ -637,6 +753,8 @@ class Engine final : public EngineBase {
         has_row[g] |= o.row.active != 0;
         multi_col[g] |= o.col.in != nullptr && o.col.nprod > 1;
         multi_row[g] |= o.row.active != 0 && o.row.nt > 1 && o.row_stash;
+        plain_row[g] &= row_is_plain(o.row);
+        plain_col[g] &= o.col.in == nullptr || o.col_phase;
       }
     if (lane_col_.bytes() < hcol.size() * sizeof(ColDesc<T, ST>)) lane_col_.alloc(hcol.size() * sizeof(ColDesc<T, ST>));
     if (lane_row_.bytes() < hrow.size() * sizeof(RowDesc<T, ST>)) lane_row_.alloc(hrow.size() * sizeof(RowDesc<T, ST>));
@@ -681,12 +799,15 @@ class Engine final : public EngineBase {
     Launcher<T, COMP, ST> Lc = launch_;
     for (size_t g = 0; g < G;) {
       int trial = -1;
-      const std::pair<int, int> choice = tune_pick(tk, launch_.geo, trial);
-      Lc.geo.col_w = choice.first;
-      Lc.geo.rw = choice.second;
+      const TuneChoice choice = tune_pick(tk, launch_.geo, trial);
+      Lc.geo.col_w = choice.cw;
+      Lc.geo.rw = choice.rw;
+      Lc.geo.plain_col = choice.pc;
+      Lc.geo.plain_row = choice.pr;
       bool uniform = !time_kernels && trial >= 0 && g + kTuneSteps <= G;
       for (size_t q = g; uniform && q < g + kTuneSteps; ++q)
-        uniform = (S > 1 || progs[0][q].kind == kOpPair) && has_col[q] && has_row[q] && !multi_col[q] && !multi_row[q];
+        uniform = (S > 1 || progs[0][q].kind == kOpPair) && has_col[q] && has_row[q] && !multi_col[q] &&
+                  !multi_row[q] && plain_col[q] && plain_row[q];
       const size_t end = uniform ? g + kTuneSteps : std::min(G, g + kTuneSteps);
       hipEvent_t a = nullptr, b = nullptr;
       if (uniform) {
@@ -705,8 +826,16 @@ class Engine final : public EngineBase {
         }
         const ColDesc<T, ST>* dc = lane_col_.as<ColDesc<T, ST>>() + g * S;
         const RowDesc<T, ST>* dr = lane_row_.as<RowDesc<T, ST>>() + g * S;
-        if (has_col[g]) timed(true, [&] { Lc.col(dc, S, stream, multi_col[g] != 0); });
-        if (has_row[g]) timed(false, [&] { Lc.row(dr, S, stream, multi_row[g] != 0); });
+        Emit em;
+        em.stream = stream;
+        if (has_col[g])
+          timed(true, [&] {
+            if (!plain_col[g] || !Lc.col_plain(dc, S, em)) Lc.col(dc, S, stream, multi_col[g] != 0);
+          });
+        if (has_row[g])
+          timed(false, [&] {
+            if (!plain_row[g] || !Lc.row_plain(dr, S, em)) Lc.row(dr, S, stream, multi_row[g] != 0);
+          });
       }
       if (uniform) {
         HIP_CHECK(hipEventRecord(b, stream));
@@ -747,8 +876,9 @@ class Engine final : public EngineBase {
     // copied it has completed (at most two chunks are in flight)
     PinnedBuf hcol[2] = {PinnedBuf(n * sizeof(ColDesc<T, ST>)), PinnedBuf(n * sizeof(ColDesc<T, ST>))};
     PinnedBuf hrow[2] = {PinnedBuf(n * sizeof(RowDesc<T, ST>)), PinnedBuf(n * sizeof(RowDesc<T, ST>))};
-    std::vector<char> has_col(chunk), has_row(chunk), multi_col(chunk), multi_row(chunk);
-    std::map<std::tuple<size_t, int, int, int>, hipGraphExec_t> graphs;
+    std::vector<char> has_col(chunk), has_row(chunk), multi_col(chunk), multi_row(chunk), plain_row(chunk),
+        plain_col(chunk);
+    std::map<std::tuple<size_t, int, int, int, int>, hipGraphExec_t> graphs;
     bool graphs_ok = true;
     auto drop_graphs = [&] {
       for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
@@ -807,6 +937,8 @@ class Engine final : public EngineBase {
         std::fill(has_row.begin(), has_row.end(), 0);
         std::fill(multi_col.begin(), multi_col.end(), 0);
         std::fill(multi_row.begin(), multi_row.end(), 0);
+        std::fill(plain_row.begin(), plain_row.end(), 1);
+        std::fill(plain_col.begin(), plain_col.end(), 1);
         for (size_t g = 0; g < G; ++g)
           for (int i = 0; i < S; ++i) {
             if (!slot[i] || pc[i] + g >= prog[i].size()) continue;
@@ -817,31 +949,37 @@ class Engine final : public EngineBase {
             has_row[g] |= o.row.active != 0;
             multi_col[g] |= o.col.in != nullptr && o.col.nprod > 1;
             multi_row[g] |= o.row.active != 0 && o.row.nt > 1 && o.row_stash;
+            plain_row[g] &= row_is_plain(o.row);
+            plain_col[g] &= o.col.in == nullptr || o.col_phase;
           }
         HIP_CHECK(hipMemcpyAsync(dcol.as<void>(), hc, G * S * sizeof(ColDesc<T, ST>), hipMemcpyHostToDevice, stream));
         HIP_CHECK(hipMemcpyAsync(drow.as<void>(), hr, G * S * sizeof(RowDesc<T, ST>), hipMemcpyHostToDevice, stream));
         // uniform chunk: every step is one plain column and one plain row launch
         bool uniform = true;
         for (size_t g = 0; g < G; ++g)
-          uniform = uniform && has_col[g] && has_row[g] && !multi_col[g] && !multi_row[g];
+          uniform = uniform && has_col[g] && has_row[g] && !multi_col[g] && !multi_row[g] && plain_row[g] && plain_col[g];
         int active = 0;
         for (int i = 0; i < S; ++i) active += slot[i] != nullptr;
         const TuneKey tk{slot_device(slot), precision_code(), S, L.ny, L.nx};
         int trial = -1;
-        const std::pair<int, int> choice = tune_pick(tk, L.geo, trial);
+        const TuneChoice choice = tune_pick(tk, L.geo, trial);
         Launcher<T, COMP, ST> Lc = L;
-        Lc.geo.col_w = choice.first;
-        Lc.geo.rw = choice.second;
+        Lc.geo.col_w = choice.cw;
+        Lc.geo.rw = choice.rw;
+        Lc.geo.plain_col = choice.pc;
+        Lc.geo.plain_row = choice.pr;
         Lc.busy = active;
         auto launch_chunk = [&](Emit& em) {
           const bool eager = em.graph == nullptr;
           for (size_t g = 0; g < G; ++g) {
             if (has_col[g]) {
-              Lc.col(dcol.as<ColDesc<T, ST>>() + g * S, S, em, multi_col[g] != 0);
+              const ColDesc<T, ST>* dc = dcol.as<ColDesc<T, ST>>() + g * S;
+              if (!plain_col[g] || !Lc.col_plain(dc, S, em)) Lc.col(dc, S, em, multi_col[g] != 0);
               if (eager) HIP_CHECK(hipGetLastError());
             }
             if (has_row[g]) {
-              Lc.row(drow.as<RowDesc<T, ST>>() + g * S, S, em, multi_row[g] != 0);
+              const RowDesc<T, ST>* dr = drow.as<RowDesc<T, ST>>() + g * S;
+              if (!plain_row[g] || !Lc.row_plain(dr, S, em)) Lc.row(dr, S, em, multi_row[g] != 0);
               if (eager) HIP_CHECK(hipGetLastError());
             }
           }
@@ -864,7 +1002,8 @@ class Engine final : public EngineBase {
           tune_report(tk, trial, 1e3 * ms / (static_cast<double>(G) * active));
         } else if (uniform && graphs_ok) {
           // uniform chunks replay one captured graph per shape (descriptors live in dcol/drow)
-          const auto gk = std::make_tuple(G, Lc.geo.col_w, Lc.geo.rw, Lc.narrow_mask(S));
+          const auto gk = std::make_tuple(G, Lc.geo.col_w, Lc.geo.rw, Lc.narrow_mask(S),
+                                          Lc.geo.plain_col * 2 + Lc.geo.plain_row);
           auto it = graphs.find(gk);
           if (it == graphs.end()) {
             // the graph is built node by node (no stream capture, so other threads of the
@@ -973,6 +1112,8 @@ class Engine final : public EngineBase {
     r.tsel = B.tsel_off >= 0 ? tsel_.as<uint8_t>() + B.tsel_off : nullptr;
     r.rowuse = B.rowuse_off >= 0 ? rowuse_.as<uint32_t>() + B.rowuse_off : nullptr;
     r.mask = mask_.as<T>();
+    r.mask_y = mask_y_.as<double>();
+    r.mask_x = mask_x_.as<double>();
     if (B.sp_count > 0) {
       const long long list = list_of(q);
       r.rowptr = rowptr_.as<int>() + B.rowptr_off;
@@ -1120,16 +1261,34 @@ class Engine final : public EngineBase {
     fwd_vals_.alloc(sp_bytes);
     bwd_vals_.alloc(sp_bytes);
     if (!s.mask.empty()) {
-      std::vector<T> m(npix_);
-      for (long long q = 0; q < npix_; ++q) m[q] = static_cast<T>(s.mask[q]);
-      mask_ = upload_vec(m, stream_);
+      // a mask that is a row profile times a column profile (the reference's absorbing_mask_2d) is
+      // applied from the two profiles, if their FP64 product gives every value of the plane bit for
+      // bit; the profiles are the plane's centre column and centre row
+      std::vector<double> my(s.ny), mx(s.nx);
+      for (long long y = 0; y < s.ny; ++y) my[y] = s.mask[y * s.nx + s.nx / 2];
+      for (long long x = 0; x < s.nx; ++x) mx[x] = s.mask[(s.ny / 2) * s.nx + x];
+      bool separable = true;
+      for (long long y = 0; separable && y < s.ny; ++y)
+        for (long long x = 0; x < s.nx; ++x)
+          if (!(my[y] * mx[x] == s.mask[y * s.nx + x])) {
+            separable = false;
+            break;
+          }
+      if (separable) {
+        mask_y_ = upload_vec(my, stream_);
+        mask_x_ = upload_vec(mx, stream_);
+      } else {
+        std::vector<T> m(npix_);
+        for (long long q = 0; q < npix_; ++q) m[q] = static_cast<T>(s.mask[q]);
+        mask_ = upload_vec(m, stream_);
+      }
     }
     if (std::any_of(has_host_.begin(), has_host_.end(), [](char c) { return c != 0; }))
       born_ = std::make_unique<Born<T, COMP>>(s, has_host_, host_, kx_.as<double>(), ky_.as<double>(), twx_->plan,
                                               twy_->plan, stream_);
     HIP_CHECK(hipStreamSynchronize(stream_));
     bytes_ = stage_.bytes() + field_in_.bytes() + X_.bytes() + prod_.bytes() + e_.bytes() + fwd_vals_.bytes() +
-             bwd_vals_.bytes() + mask_.bytes() + ops_bytes_ + tsel_.bytes() + rowuse_.bytes() + sp_pix_.bytes() +
+             bwd_vals_.bytes() + mask_.bytes() + mask_y_.bytes() + mask_x_.bytes() + ops_bytes_ + tsel_.bytes() + rowuse_.bytes() + sp_pix_.bytes() +
              sp_rid_.bytes() + rowptr_.bytes() + (born_ ? born_->device_bytes() : 0);
     // the label volume and δn now live in the boundary structures and device tables
     sp_.labels = std::vector<int32_t>();
@@ -1475,7 +1634,7 @@ class Engine final : public EngineBase {
   size_t ops_bytes_ = 0;
   DevBuf prods_;
   int max_prod_ = 1;
-  DevBuf stage_, field_in_, X_, prod_, e_, fwd_vals_, bwd_vals_, mask_;
+  DevBuf stage_, field_in_, X_, prod_, e_, fwd_vals_, bwd_vals_, mask_, mask_y_, mask_x_;
   DevBuf iter_t_, iter_r_, keepf_, keepb_;
   DevBuf lane_col_, lane_row_;
   std::vector<long long> keep_;

@@ -51,6 +51,57 @@ __device__ __forceinline__ px2<T> widen(px2<U> v) {
   return px2<T>{widen<T>(v.a), widen<T>(v.b)};
 }
 
+// Block-floating-point storage (ST = bfp, the mixed mode's streaming planes): one complex value in
+// 64 bits, its two parts as 28-bit two's-complement mantissas that share one exponent. With
+// m = max(|re|, |im|) = f·2^e (f in [0.5, 1)), each part is stored as q = rint(v·2^(27 − e)),
+// |q| ≤ 2^27 − 1, in bits 4 … 31 of its word; the biased exponent E = e + 126 sits in the two low
+// nibbles (real part's word: bits 0 … 3 of E, imaginary part's word: bits 4 … 7). The rounding error
+// is at most 2^−27·m, against 2^−24 per part for complex64, in the same 8 bytes, over complex64's
+// normal range: E = 0 is a zero (m below 2^−126 flushes to it), E = 255 a value with a part that is
+// not finite or m ≥ 2^128 (it reads back as NaN).
+struct bfp {
+  uint32_t w;
+};
+
+BWPM_HD cx<bfp> bfp_encode(cx<double> z) {
+  const double ax = fabs(z.x), ay = fabs(z.y);
+  const double m = ax > ay ? ax : ay;
+  int e = 0;
+  (void)frexp(m, &e);
+  // comparisons with NaN are false, so a NaN in either part is not finite
+  const bool finite = ax <= 1.7976931348623157e308 && ay <= 1.7976931348623157e308;
+  const bool over = !finite || e > 128;
+  const bool flush = !(m > 0) || e < -125;  // zero or below the range
+  const int s = 27 - e;
+  const double r0 = rint(ldexp(z.x, s)), r1 = rint(ldexp(z.y, s));
+  int q0 = static_cast<int>(fmin(r0, 134217727.0)), q1 = static_cast<int>(fmin(r1, 134217727.0));
+  unsigned E = static_cast<unsigned>(e + 126);
+  if (flush || over) {
+    q0 = 0;
+    q1 = 0;
+    E = over ? 255u : 0u;
+  }
+  return cx<bfp>{bfp{(static_cast<unsigned>(q0) << 4) | (E & 15u)}, bfp{(static_cast<unsigned>(q1) << 4) | (E >> 4)}};
+}
+
+BWPM_HD cx<double> bfp_decode(unsigned w0, unsigned w1) {
+  const int E = static_cast<int>((w0 & 15u) | ((w1 & 15u) << 4));
+  const int sh = E - 153;  // 2^(e − 27)
+  cx<double> z{ldexp(static_cast<double>(static_cast<int>(w0) >> 4), sh),
+               ldexp(static_cast<double>(static_cast<int>(w1) >> 4), sh)};
+  if (E == 255) z = cx<double>{__builtin_nan(""), __builtin_nan("")};
+  return z;
+}
+
+template <>
+__device__ __forceinline__ cx<bfp> widen<bfp, double>(cx<double> z) {
+  return bfp_encode(z);
+}
+template <>
+__device__ __forceinline__ cx<double> widen<double, bfp>(cx<bfp> z) {
+  return bfp_decode(z.x.w, z.y.w);
+}
+
 // Streaming-plane access: element i (a complex index) of a plane stored as ST, computed as T.
 template <typename T, typename ST>
 __device__ __forceinline__ cx<T> sload(const px2<ST>* p, unsigned i) {
@@ -59,6 +110,21 @@ __device__ __forceinline__ cx<T> sload(const px2<ST>* p, unsigned i) {
 template <typename T, typename ST>
 __device__ __forceinline__ void sstore(px2<ST>* p, unsigned i, cx<T> v) {
   gstore(reinterpret_cast<cx<ST>*>(p), i, widen<ST>(v));
+}
+template <>
+__device__ __forceinline__ cx<double> sload<double, bfp>(const px2<bfp>* p, unsigned i) {
+  typedef unsigned u2 __attribute__((ext_vector_type(2)));
+  const u2 v = *gaddr<px2<bfp>, u2>(p, i);
+  return bfp_decode(v.x, v.y);
+}
+template <>
+__device__ __forceinline__ void sstore<double, bfp>(px2<bfp>* p, unsigned i, cx<double> z) {
+  typedef unsigned u2 __attribute__((ext_vector_type(2)));
+  const cx<bfp> b = bfp_encode(z);
+  u2 v;
+  v.x = b.x.w;
+  v.y = b.y.w;
+  *gaddr<px2<bfp>, u2>(p, i) = v;
 }
 
 // Column-kernel work of one solve for one step.
@@ -92,7 +158,9 @@ struct RowDesc {
   const px2<ST>* prod;        // product planes
   const uint8_t* tsel;        // T pair of each pixel (nt > 1)
   const uint32_t* rowuse;     // per-row product use (kRowUse bits; nullptr: every row uses every product)
-  const T* mask;              // transmission mask or nullptr
+  const T* mask;              // transmission mask plane or nullptr
+  const double* mask_y;       // or the mask as the product of two profiles, mask_y[y]·mask_x[x] (FP64)
+  const double* mask_x;
   const px2<T>* e_in;         // nt == 0: start plane (nullptr: zero)
   const px2<T>* inc;          // Born increment, times mask if mask != nullptr
   const int* rowptr;          // ny + 1 offsets into the sparse list (nullptr: none)
@@ -105,6 +173,15 @@ struct RowDesc {
   px2<T>* e_out2;             // second copy (kept plane)
   px2<ST>* x_out;             // row FFT of the result for the next step (nullptr: none)
 };
+
+// The mask factor of pixel (y, x) (plane index q) of a row descriptor: the two profiles multiplied
+// in FP64 (the host checked that this product gives every value of the mask plane bit for bit), or
+// the mask plane itself.
+template <typename T, typename ST>
+__device__ __forceinline__ T mask_factor(const RowDesc<T, ST>& a, unsigned y, unsigned x, unsigned q) {
+  if (a.mask_y) return static_cast<T>(a.mask_y[y] * a.mask_x[x]);
+  return a.mask[q];
+}
 
 template <typename T, bool COMP>
 __device__ __forceinline__ px2<T> apply_prod(const Prod<T>& pd, long long k, px2<T> e) {
@@ -245,8 +322,8 @@ __global__ __launch_bounds__(kThreads) void k_row_step(const RowDesc<T, ST>* __r
     const int r = i / nx, x = i - r * nx;
     const long long q = static_cast<long long>(y0 + r) * nx + x;
     cx<T> ea = acc[x * ls + r], eb = acc[x * ls + rows + r];
-    if (a.mask) {
-      const T m = a.mask[q];
+    if (a.mask || a.mask_y) {
+      const T m = mask_factor(a, static_cast<unsigned>(y0 + r), static_cast<unsigned>(x), static_cast<unsigned>(q));
       ea = ea * m;
       eb = eb * m;
       if (a.inc) {

@@ -289,10 +289,10 @@ __global__ __launch_bounds__(N / 4 * RW) void k_row_fast(const RowDesc<T, ST>* _
 
   // 3. epilogue in the order of the reference: mask, Born increment, sparse sources, Born plane
   const cx<T>* inc = cplane(a.inc);
-  if (a.mask) {
+  if (a.mask || a.mask_y) {
 #pragma unroll
     for (int m = 0; m < 8; ++m) {
-      const T mk = gload1(a.mask, row + t + TT * m);
+      const T mk = mask_factor(a, static_cast<unsigned>(y), static_cast<unsigned>(t + TT * m), row + t + TT * m);
       acc[m] = acc[m] * mk;
       if (inc) acc[m] = acc[m] + gload(inc, at(m)) * mk;
     }

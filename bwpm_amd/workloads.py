@@ -90,10 +90,15 @@ def plain_2d(ny: int = 1024, nz: int = 200, iterations: int = 3) -> Problem:
 
 def tilted_3d(n: int = 256, nz: int = 512, iterations: int = 4, wavelength_um: float = 1.55,
               tilt_deg: float = 10.0, n1: complex = 1.0, n2: complex = 1.46, polarisation: str = "x",
-              waist_mm: float = 8e-3, geometry: bool = False) -> Problem:
+              waist_mm: float = 8e-3, geometry: bool = False, absorber_wavelengths: float = 0.0) -> Problem:
     """Config 2 geometry: dx = dy = λ/6, dz = λ/4, interface z = z_mid + y·tan(tilt) between n1 and
     n2 (pixel-centre rule), E_x- or E_y-polarised Gaussian of waist w₀ at plane 0. With `geometry`
-    the labels are a `Geometry` made inside the library (the same volume, no host array)."""
+    the labels are a `Geometry` made inside the library (the same volume, no host array). With
+    `absorber_wavelengths` > 0 the transverse edges carry the reference's absorbing zone of that
+    many wavelengths (super-Gaussian of order 8, `volume3d.absorbing_mask_2d`); its validity rule
+    asks for ≥ 20. Without it the periodic window lets waves refracted at the tilted interface
+    circulate: at 512² the reflected field grows about 400× per iteration (the reference's own core
+    gives the same fields), while 256² stays bounded."""
     lam = wavelength_um
     dx, dz = lam * 1e-3 / 6, lam * 1e-3 / 4
     y = centred(n, dx)
@@ -109,8 +114,14 @@ def tilted_3d(n: int = 256, nz: int = 512, iterations: int = 4, wavelength_um: f
     r2 = x[None, :] ** 2 + y[:, None] ** 2
     f = np.zeros((n, n, 2), dtype=np.complex128)
     f[..., 0 if polarisation == "x" else 1] = np.exp(-r2 / waist_mm ** 2)
+    mask = None
+    if absorber_wavelengths > 0:
+        from bwpm_amd.volume3d import Grid3D, absorbing_mask_2d
+
+        mask = absorbing_mask_2d(Grid3D(n, n, dx, dx, dz, lam), absorber_wavelengths * lam * 1e-3)
     return Problem(f"tilted_3d_{n}x{n}x{nz}", n, n, dx, dx, dz, lam, [complex(n1), complex(n2)], labels,
-                   complex(n1), f, iterations, meta={"tilt_deg": tilt_deg, "polarisation": polarisation})
+                   complex(n1), f, iterations, mask=mask,
+                   meta={"tilt_deg": tilt_deg, "polarisation": polarisation, "absorber_wavelengths": absorber_wavelengths})
 
 
 def gaussian_correlated(shape: tuple[int, int, int], spacing_mm: tuple[float, float, float], corr_mm: float,

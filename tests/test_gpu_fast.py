@@ -177,3 +177,21 @@ def test_generic_kernels_narrow_and_wide_give_the_same_bits(precision: str) -> N
         finally:
             for s in solvers:
                 s.close()
+
+
+@pytest.mark.parametrize("shape", [(4096, 1), (2048, 2), (1, 4096), (512, 512)])
+@pytest.mark.parametrize("precision", ["fp64", "mixed", "fp32"])
+def test_repeated_solves_are_bitwise_equal(shape: tuple[int, int], precision: str) -> None:
+    """The same solve twice gives the same bits (the kernels read nothing they have not written and
+    reduce in a fixed order), here at the line lengths of 4096 points whose FP64 kernels run with
+    128 registers per thread."""
+    ny, nx = shape
+    a = layered_volume(ny, nx, 7, seed=ny + 3 * nx)
+    s = _solver(a, precision)
+    try:
+        one = s.run(a["field"], 2)
+        two = s.run(a["field"], 2)
+    finally:
+        s.close()
+    for key in ("transmitted", "reflected"):
+        assert np.array_equal(one[key], two[key]), key
