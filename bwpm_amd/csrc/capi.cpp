@@ -610,6 +610,8 @@ int bwpm_solver_plane(const bwpm_solver* s, int which, int64_t plane, const void
 namespace bwpm {
 double power_flux_device(const c128* field, long long ny, long long nx, double dy, double dx, double k0, c128 index,
                          int direction, int device);
+double power_flux_noalloc(const c128* field, long long ny, long long nx, double dy, double dx, double k0, c128 index,
+                          int direction, int device);
 }
 
 extern "C" int bwpm_power_flux(const double* field, int64_t ny, int64_t nx, double dy, double dx, double k0,
@@ -619,5 +621,15 @@ extern "C" int bwpm_power_flux(const double* field, int64_t ny, int64_t nx, doub
     bwpm::require(direction == 1 || direction == -1, "direction must be +1 or -1");
     *out = bwpm::power_flux_device(reinterpret_cast<const c128*>(field), ny, nx, dy, dx, k0, c128(n_re, n_im),
                                    direction, device);
+  });
+}
+
+extern "C" int bwpm_power_flux_noalloc(const double* field, int64_t ny, int64_t nx, double dy, double dx, double k0,
+                                       double n_re, double n_im, int direction, int device, double* out) {
+  return guarded([&] {
+    bwpm::require(field && out, "null argument");
+    bwpm::require(direction == 1 || direction == -1, "direction must be +1 or -1");
+    *out = bwpm::power_flux_noalloc(reinterpret_cast<const c128*>(field), ny, nx, dy, dx, k0, c128(n_re, n_im),
+                                    direction, device);
   });
 }

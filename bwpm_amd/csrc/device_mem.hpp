@@ -92,20 +92,35 @@ inline DevBuf upload_vec(const std::vector<U>& v, hipStream_t s) {
 
 // ---- FFT plans ---------------------------------------------------------------------------------
 
-inline std::vector<int> factorize(long long n) {
-  std::vector<int> r;
+// Radices of the plan of an n-point line into plan.n, plan.nst, plan.rad (no allocation).
+inline void plan_radices(long long n, FftPlan& plan) {
+  plan.n = static_cast<int>(n);
+  plan.nst = 0;
   long long m = n;
   for (int R : {8, 4, 2, 3, 5, 7, 11, 13})
     while (m % R == 0) {
-      r.push_back(R);
+      if (plan.nst == kMaxStages) throw Error(kEinval, "grid size " + std::to_string(n) + " needs too many FFT stages");
+      plan.rad[plan.nst++] = R;
       m /= R;
     }
   if (m != 1)
     throw Error(kEinval, "grid size " + std::to_string(n) +
                              " has a prime factor above 13; supported sizes are 2^a·3^b·5^c·7^d·11^e·13^f");
-  // a single radix-2 stage is cheaper after the radix-8 stages; merge 2·2 into 4 if produced
-  if (r.size() > kMaxStages) throw Error(kEinval, "grid size " + std::to_string(n) + " needs too many FFT stages");
-  return r;
+}
+
+inline std::vector<int> factorize(long long n) {
+  FftPlan plan{};
+  plan_radices(n, plan);
+  return std::vector<int>(plan.rad, plan.rad + plan.nst);
+}
+
+// The twiddle W_n^t = exp(−2πi t/n), from the mathematics in extended precision, then rounded to
+// FP64 (c, s).
+inline void twiddle_value(long long t, long long n, double& c, double& s) {
+  const long double a = -2.0L * 3.141592653589793238462643383279502884L * static_cast<long double>(t) /
+                        static_cast<long double>(n);
+  c = static_cast<double>(cosl(a));
+  s = static_cast<double>(sinl(a));
 }
 
 template <typename T>
@@ -119,10 +134,8 @@ struct Twiddles {
     for (size_t i = 0; i < r.size(); ++i) plan.rad[i] = r[i];
     std::vector<cx<T>> h(n), l(n);
     for (long long t = 0; t < n; ++t) {
-      // exp(−2πi t/n) from the mathematics in extended precision, then rounded
-      const long double a = -2.0L * 3.141592653589793238462643383279502884L * static_cast<long double>(t) /
-                            static_cast<long double>(n);
-      const double c = static_cast<double>(cosl(a)), sn = static_cast<double>(sinl(a));
+      double c = 0, sn = 0;
+      twiddle_value(t, n, c, sn);
       const T ch = static_cast<T>(c), sh = static_cast<T>(sn);
       h[t] = cx<T>{ch, sh};
       l[t] = cx<T>{static_cast<T>(c - static_cast<double>(ch)), static_cast<T>(sn - static_cast<double>(sh))};

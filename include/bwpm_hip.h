@@ -77,6 +77,16 @@ int bwpm_fresnel_coefficients(const double* kx, const double* ky, int64_t n, dou
 int bwpm_power_flux(const double* field, int64_t ny, int64_t nx, double dy, double dx, double k0,
                     double n_re, double n_im, int direction, int device, double* out);
 
+/* bwpm_power_flux without device allocation, for callers that cannot allocate device memory (code
+ * under graph capture, tight serving loops, harnesses that forbid it); it has no counterpart in the
+ * reference. Same arguments and the same bits as bwpm_power_flux. Its device buffers (the plane and
+ * its spectrum, k and twiddle tables, reduction partials) are one static array of the library per
+ * device, about 8.3 MB, held until the process ends; the grid is limited to ny, nx <= 4096 and
+ * ny * nx <= 262144 pixels (larger grids: BWPM_EINVAL). Calls in one process take turns on a lock;
+ * the array keeps the last call's contents, and every call writes all it reads. */
+int bwpm_power_flux_noalloc(const double* field, int64_t ny, int64_t nx, double dy, double dx, double k0,
+                            double n_re, double n_im, int direction, int device, double* out);
+
 /* Solver over a labelled volume. labels: (nz, ny, nx) int32 into `indices` (m complex);
  * mask: (ny, nx) or NULL; fluct: δn (nz, ny, nx) complex or NULL; precision: BWPM_FP64/FP32.
  * Replaces `_core.CudaSolver(...)` / `_core.Solver(...)` (cpp/python/bindings.cpp:139-151,158-171,
