@@ -38,7 +38,7 @@ sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBPS = 8000.0  # MI355X HBM3E
 # algorithmic bytes per grid-point·slab update: read + write of the (E_x, E_y) plane per slab step
-BYTES_PER_UPDATE = {"fp64": 64, "mixed": 32, "fp32": 32, "fp32_plain": 32}
+BYTES_PER_UPDATE = {"fp64": 64, "mixed": 32, "bfp": 32, "fp32": 32, "fp32_plain": 32}
 
 
 def profiled_traffic(precision: str, n: int, batch: int) -> tuple[int | None, str | None]:
@@ -266,7 +266,7 @@ def main() -> int:
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--nz", type=int, default=1024)
     ap.add_argument("--iterations", type=int, default=4)
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed", "fp32", "fp32_plain"])
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed", "bfp", "fp32", "fp32_plain"])
     ap.add_argument("--absorber", type=float, default=0.0,
                     help="absorbing edge zone of the headline geometry in wavelengths (0: none)")
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "0") or 0))

@@ -29,7 +29,7 @@ if not _LIB_PATH.exists():
 _lib = ctypes.CDLL(str(_LIB_PATH))
 
 OK, EINVAL, ENOCONV, EDEVICE, ENOMEM, EINTERNAL = 0, 1, 2, 3, 4, 5
-PRECISIONS = {"fp64": 64, "mixed": 48, "fp32": 32, "fp32_plain": 31}
+PRECISIONS = {"fp64": 64, "mixed": 48, "bfp": 40, "fp32": 32, "fp32_plain": 31}
 
 _d = ctypes.c_double
 _i64 = ctypes.c_int64

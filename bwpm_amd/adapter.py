@@ -120,7 +120,8 @@ _VALIDITY = (
 
 _PRECISION = {
     "fp64": "complex128 planes and arithmetic (the reference's)",
-    "mixed": "block-floating-point planes of complex64 size, FP64 arithmetic",
+    "mixed": "complex64 streaming planes, FP64 arithmetic and operator tables",
+    "bfp": "block-floating-point streaming planes of complex64 size (28-bit mantissas), FP64 arithmetic",
     "fp32": "complex64 planes and arithmetic, FP64 operator tables (compensated)",
 }
 

@@ -13,7 +13,7 @@ HERE = Path(__file__).resolve().parent
 CSRC = HERE / "csrc"
 LIB_DIR = HERE / "lib"
 LIB = LIB_DIR / "libbwpm_hip.so"
-SOURCES = ["engine.hip", "engine_f32.hip", "engine_f64.hip", "engine_mixed.hip", "born.hip", "analysis.hip", "serve.cpp", "capi.cpp"]
+SOURCES = ["engine.hip", "engine_f32.hip", "engine_f64.hip", "engine_mixed.hip", "engine_bfp.hip", "born.hip", "analysis.hip", "serve.cpp", "capi.cpp"]
 ARCH = os.environ.get("BWPM_OFFLOAD_ARCH", "gfx950")
 
 FLAGS = [

@@ -76,9 +76,9 @@ bwpm::Spec make_spec(int64_t ny, int64_t nx, double dy, double dx, double dz, do
   bwpm::require(ny >= 1 && nx >= 1 && nz >= 1, "labels must have shape (nz, ny, nx)");
   bwpm::require(m >= 0 && (m == 0 || indices), "indices missing");
   bwpm::require(labels != nullptr || uniform_label >= 0, "labels missing");
-  bwpm::require(precision == BWPM_FP64 || precision == BWPM_MIXED || precision == BWPM_FP32 ||
-                    precision == BWPM_FP32_PLAIN,
-                "precision must be BWPM_FP64, BWPM_MIXED, BWPM_FP32 or BWPM_FP32_PLAIN");
+  bwpm::require(precision == BWPM_FP64 || precision == BWPM_MIXED || precision == BWPM_BFP ||
+                    precision == BWPM_FP32 || precision == BWPM_FP32_PLAIN,
+                "precision must be BWPM_FP64, BWPM_MIXED, BWPM_BFP, BWPM_FP32 or BWPM_FP32_PLAIN");
   bwpm::Spec s;
   s.ny = ny;
   s.nx = nx;
@@ -601,7 +601,7 @@ int bwpm_solver_plane(const bwpm_solver* s, int which, int64_t plane, const void
     HIP_CHECK(hipStreamSynchronize(s->engine->stream()));
     *ptr = s->engine->device_plane(which, plane);
     const int p = s->engine->precision();
-    *precision = p == BWPM_FP64 || p == BWPM_MIXED ? BWPM_FP64 : BWPM_FP32;
+    *precision = p == BWPM_FP64 || p == BWPM_MIXED || p == BWPM_BFP ? BWPM_FP64 : BWPM_FP32;
   });
 }
 

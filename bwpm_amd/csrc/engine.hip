@@ -68,9 +68,11 @@ std::unique_ptr<EngineBase> make_engine(Spec spec) {
     case 64: return make_engine_f64(std::move(spec));
     case 48:
       return make_engine_mixed(std::move(spec));
+    case 40:
+      return make_engine_bfp(std::move(spec));
     case 32: return make_engine_f32(std::move(spec), true);
     case 31: return make_engine_f32(std::move(spec), false);
-    default: throw Error(kEinval, "precision must be 64, 48, 32 or 31");
+    default: throw Error(kEinval, "precision must be 64, 48, 40, 32 or 31");
   }
 }
 
@@ -255,13 +257,16 @@ void run_continuous_lane(int precision, LaneFeed& feed, int slots, int chunk, hi
     run_continuous_lane_f64(feed, slots, chunk, stream);
   else if (precision == 48)
     run_continuous_lane_mixed(feed, slots, chunk, stream);
+  else if (precision == 40)
+    run_continuous_lane_bfp(feed, slots, chunk, stream);
   else
     run_continuous_lane_f32(feed, slots, chunk, stream, precision == 32);
 }
 
 size_t estimate_device_bytes(const Spec& s, int iterations) {
-  const size_t elem = s.precision == 64 || s.precision == 48 ? 16 : 8;  // complex
-  const size_t selem = s.precision == 48 ? 8 : elem;                     // streaming planes
+  const bool f64 = s.precision == 64 || s.precision == 48 || s.precision == 40;
+  const size_t elem = f64 ? 16 : 8;                                           // complex
+  const size_t selem = s.precision == 48 || s.precision == 40 ? 8 : elem;     // streaming planes
   const size_t npix = static_cast<size_t>(s.ny) * static_cast<size_t>(s.nx);
   const size_t plane = 2 * npix * elem;            // (E_x, E_y) plane
   const size_t splane = 2 * npix * selem;

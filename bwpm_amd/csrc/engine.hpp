@@ -59,6 +59,7 @@ struct Spec {
   FluctSynth synth;             // δn generated on the device (fluct must be empty)
   LabelGeometry geom;           // labels generated from a geometry (labels must be empty)
   int precision = 32;           // 64: FP64; 48: complex64 streaming planes, FP64 arithmetic;
+                                // 40: block-floating-point streaming planes, FP64 arithmetic;
                                 // 32: FP32 compensated; 31: FP32 plain
   int device = 0;
 };
@@ -117,6 +118,7 @@ std::unique_ptr<EngineBase> make_engine(Spec spec);
 std::unique_ptr<EngineBase> make_engine_f32(Spec spec, bool compensated);
 std::unique_ptr<EngineBase> make_engine_f64(Spec spec);
 std::unique_ptr<EngineBase> make_engine_mixed(Spec spec);
+std::unique_ptr<EngineBase> make_engine_bfp(Spec spec);
 
 // Batched solve of many engines: members are grouped into lanes of equal (precision, grid,
 // device, no δn); each lane runs in lockstep on its own stream, lanes run concurrently.
@@ -162,6 +164,7 @@ LaneGraphCounters& lane_graph_counters();
 void run_continuous_lane_f32(LaneFeed& feed, int slots, int chunk, hipStream_t stream, bool compensated);
 void run_continuous_lane_f64(LaneFeed& feed, int slots, int chunk, hipStream_t stream);
 void run_continuous_lane_mixed(LaneFeed& feed, int slots, int chunk, hipStream_t stream);
+void run_continuous_lane_bfp(LaneFeed& feed, int slots, int chunk, hipStream_t stream);
 
 // Upper bound of the device bytes of one solve of `spec` with `iterations` (admission by HBM
 // footprint before the engine exists).

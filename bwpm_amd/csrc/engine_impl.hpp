@@ -1071,7 +1071,10 @@ class Engine final : public EngineBase {
     drop_graphs();
   }
 
-  static int precision_code() { return sizeof(ST) < sizeof(T) ? 48 : COMP ? 32 : (sizeof(T) == 8 ? 64 : 31); }
+  static int precision_code() {
+    if (std::is_same<ST, bfp>::value) return 40;
+    return sizeof(ST) < sizeof(T) ? 48 : COMP ? 32 : (sizeof(T) == 8 ? 64 : 31);
+  }
   static int slot_device(const std::vector<Engine*>& slot) {
     for (const Engine* e : slot)
       if (e) return e->sp_.device;

@@ -32,8 +32,10 @@ extern "C" {
 /* precision of the bulk fields */
 #define BWPM_FP64 64       /* complex128 throughout (parity mode against the FP64 reference) */
 #define BWPM_MIXED 48      /* complex64 storage of the two streaming planes of every slab step,
-                              FP64 arithmetic and every other buffer FP64; solves with index
-                              fluctuations keep the planes FP64 too */
+                              FP64 arithmetic and every other buffer FP64 */
+#define BWPM_BFP 40        /* the two streaming planes in a block-floating-point complex format
+                              of complex64 size (two 28-bit mantissas sharing an 8-bit exponent),
+                              FP64 arithmetic and every other buffer FP64 */
 #define BWPM_FP32 32       /* complex64 bulk, FP64 tables, compensated twiddles/operators */
 #define BWPM_FP32_PLAIN 31 /* complex64 bulk without compensation (diagnostic) */
 

@@ -100,7 +100,7 @@ def _c1():
     return p, ref
 
 
-@pytest.mark.parametrize("precision", ["fp64", "mixed", "fp32"])
+@pytest.mark.parametrize("precision", ["fp64", "mixed", "bfp", "fp32"])
 def test_c1_full_size_against_the_oracle(precision: str) -> None:
     p, ref = _c1()
     print(f"C1 {p.ny}x{p.nz} it{p.iterations} {precision}", flush=True)
@@ -132,7 +132,7 @@ def _c2(pol: str):
 
 
 @pytest.mark.parametrize("pol", ["x", "y"])
-@pytest.mark.parametrize("precision", ["fp64", "mixed", "fp32"])
+@pytest.mark.parametrize("precision", ["fp64", "mixed", "bfp", "fp32"])
 def test_c2_full_depth_against_the_reference_core(precision: str, pol: str) -> None:
     p, ref = _c2(pol)
     print(f"C2 {precision} E{pol}", flush=True)

@@ -46,7 +46,7 @@ def _solver(a: dict, precision: str):
 @pytest.mark.parametrize("shape", [(64, 64), (128, 128), (256, 128), (512, 512), (512, 12), (12, 256), (4096, 1),
                                    (2048, 2), (3072, 1), (384, 96), (80, 120), (1000, 1), (2, 1024), (1, 4096),
                                    (4, 2048)])
-@pytest.mark.parametrize("precision", ["fp64", "mixed", "fp32"])
+@pytest.mark.parametrize("precision", ["fp64", "mixed", "bfp", "fp32"])
 def test_register_kernels_match_oracle(shape: tuple[int, int], precision: str) -> None:
     ny, nx = shape
     nz = 9
@@ -68,10 +68,10 @@ def test_register_kernels_match_oracle(shape: tuple[int, int], precision: str) -
             # input, below FP32's range): the FP32 result must be as small
             assert np.max(np.abs(got)) < 1e-6 * scale
         else:
-            assert rel_l2(got, want) < (1e-6 if precision == "mixed" else 1e-5)
+            assert rel_l2(got, want) < (1e-6 if precision in ("mixed", "bfp") else 1e-5)
 
 
-@pytest.mark.parametrize("precision", ["fp32", "mixed", "fp64"])
+@pytest.mark.parametrize("precision", ["fp32", "mixed", "bfp", "fp64"])
 def test_batch_equals_alone_bitwise(precision: str) -> None:
     """Solves of two grid sizes, uneven depths and different geometries: the batched lanes give
     each solve exactly its result alone."""
@@ -180,7 +180,7 @@ def test_generic_kernels_narrow_and_wide_give_the_same_bits(precision: str) -> N
 
 
 @pytest.mark.parametrize("shape", [(4096, 1), (2048, 2), (1, 4096), (512, 512)])
-@pytest.mark.parametrize("precision", ["fp64", "mixed", "fp32"])
+@pytest.mark.parametrize("precision", ["fp64", "mixed", "bfp", "fp32"])
 def test_repeated_solves_are_bitwise_equal(shape: tuple[int, int], precision: str) -> None:
     """The same solve twice gives the same bits (the kernels read nothing they have not written and
     reduce in a fixed order), here at the line lengths of 4096 points whose FP64 kernels run with

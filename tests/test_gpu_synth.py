@@ -113,7 +113,7 @@ def test_generated_requests_on_the_queue_equal_the_solves_alone() -> None:
         assert np.array_equal(res["reflected"], alone["reflected"])
 
 
-@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("precision", ["fp64", "mixed", "bfp"])
 def test_geometry_volumes_equal_label_arrays(precision: str) -> None:
     cases = [(workloads.tilted_3d(n=64, nz=30, iterations=2), workloads.tilted_3d(n=64, nz=30, iterations=2,
                                                                                   geometry=True)),
