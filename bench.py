@@ -3,22 +3,28 @@
 
 Workload (default): the tilted-interface 3D vector geometry of the reference's plain config
 (λ = 1.55 µm, dx = dy = λ/6, dz = λ/4, interface 1.0 → 1.46 tilted 10°, pixel-centre labels,
-Gaussian w₀ = 8 µm, 4 bidirectional iterations) at 512² × 1024 slabs, FP64 (complex128) bulk by
-default. One step is one batch of `--batch` independent solves (E_x and E_y polarised alternately)
-run on the GPU as one lane (every slab step is one column launch and one row launch for all solves
-of the batch) with fields and state resident in HBM; the timed region covers `--steps` such batches.
+Gaussian w₀ = 8 µm, 4 bidirectional iterations) at 512² × 1024 slabs with the reference's absorbing
+edge zone of 20 wavelengths (its validity rule; without it the periodic 512² window lets refracted
+waves circulate and the fields grow about 400× per iteration, in the reference's own core as here).
+Precision `mixed` by default: the two streaming planes of every slab step stored as complex64 in
+HBM, FP64 arithmetic and operator tables. One step is one batch of `--batch` independent solves
+(E_x and E_y polarised alternately) run on the GPU as one lane (every slab step is one column launch
+and one row launch for all solves of the batch) with fields and state resident in HBM; the timed
+region covers `--steps` such batches.
 
 Printed: one JSON line with grid-point·slab updates/s (whole job), the roofline of the slab step
 (HIP-event durations of the column and the row kernel on the lane's stream; algorithmic bytes per
-update = one read and one write of the (E_x, E_y) plane: 64 B in complex128, 32 B in complex64),
-the plain one-solve-per-launch rate of the same solve on the same GPU, the accuracy of the run
-(batch result against the solve alone; FP32 against the FP64 device path), the reference's other
-configurations as single solves (2D, 3D 256² tilted, 3D 512² Born volume at reduced depth), the
-served mix on the request queue (capacity, then p50/p99 latency of a Poisson stream at 70 % of it), and the
-reference's CPU core timed on this host over a bounded sample of the same geometry.
+update = one read and one write of the (E_x, E_y) plane: 64 B in complex128, 32 B in complex64 or
+the block-floating-point format), and for every result the relative field L2 error, reflectance R,
+transmittance T and A = 1 − R − T against the FP64 device result of the same solve (the FP64
+device path equals the FP64 oracle and the reference's C++ core to 1e-10 in the GPU tests), with
+|ΔR| and |ΔT|. Also: the plain one-solve-per-launch rates, the other precision modes, the
+reference's other configurations (2D; 3D 256² tilted; 3D 512² Born volume; the uneven Born batch),
+the served mix on the request queue, and the reference's compiled CPU core timed on this host (all
+cores and one core, one warm-up run and the median of five) on bounded samples.
 
     python bench.py [--steps K] [--warmup W] [--batch B] [--n 512] [--nz 1024] [--iterations 4]
-                    [--precision fp64|mixed|fp32|fp32_plain]
+                    [--precision mixed|bfp|fp32|fp32_plain|fp64] [--absorber 20]
 """
 
 from __future__ import annotations
@@ -27,6 +33,8 @@ import argparse
 import json
 import os
 import re
+import statistics
+import subprocess
 import sys
 import time
 from pathlib import Path
@@ -39,12 +47,17 @@ sys.path.insert(0, str(ROOT))
 HBM_PEAK_GBPS = 8000.0  # MI355X HBM3E
 # algorithmic bytes per grid-point·slab update: read + write of the (E_x, E_y) plane per slab step
 BYTES_PER_UPDATE = {"fp64": 64, "mixed": 32, "bfp": 32, "fp32": 32, "fp32_plain": 32}
+# arithmetic type of each mode, and the storage of its streaming planes
+ARITHMETIC = {"fp64": "f64", "mixed": "f64", "bfp": "f64", "fp32": "f32", "fp32_plain": "f32"}
+PLANES = {"fp64": "complex128", "mixed": "complex64", "bfp": "block-floating-point (8 B per complex value)",
+          "fp32": "complex64", "fp32_plain": "complex64"}
+NARROW = ("mixed", "bfp", "fp32")
 
 
-def profiled_traffic(precision: str, n: int, batch: int) -> tuple[int | None, str | None]:
+def profiled_traffic(precision: str, n: int, batch: int, absorber: float) -> tuple[int | None, str | None]:
     """HBM bytes per launch pair from the committed rocprofv3 summary of the same configuration
     (FETCH_SIZE and WRITE_SIZE per dispatch, scaled by the copy-kernel calibration)."""
-    tag = f"config: precision={precision} n={n} batch={batch}"
+    tag = f"config: precision={precision} n={n} batch={batch} absorber={absorber:g}"
     for f in sorted((ROOT / "profiles").glob("*.md")):
         text = f.read_text()
         m = re.search(r"^traffic_bytes_per_launch_pair: (\d+)$", text, re.M)
@@ -58,34 +71,47 @@ def rel_l2(a: np.ndarray, b: np.ndarray) -> float:
     return float(np.linalg.norm(a - b)) / nb if nb > 0 else float(np.linalg.norm(a))
 
 
-def cpu_baseline(n: int, nz_sample: int, threads: int) -> dict:
-    """The reference's compiled CPU core (oracle/_ref) on a bounded sample of the workload, or the
-    NumPy oracle when that build is absent."""
-    from bwpm_amd import workloads
+# ---- reflectance, transmittance and errors ---------------------------------------------------
 
-    prob = workloads.tilted_3d(n=n, nz=nz_sample, iterations=1)
-    args = (prob.ny, prob.nx, prob.dy, prob.dx, prob.dz, prob.k0, np.asarray(prob.indices, complex),
-            prob.labels.astype(np.int32), prob.index_before, None)
-    ref_dir = ROOT / "oracle" / "_ref"
-    so = sorted(ref_dir.glob("_core*.so")) if ref_dir.exists() else []
-    if so:
-        import importlib.util
+def exit_index(p) -> complex:
+    """Index of the medium after the volume: the label of the last slab at the centre pixel
+    (src/bwpm/adapter.py:267-284)."""
+    from bwpm_amd import _core
+    from bwpm_amd.specs import Geometry, Uniform
 
-        spec = importlib.util.spec_from_file_location("_core", so[0])
-        mod = importlib.util.module_from_spec(spec)
-        spec.loader.exec_module(mod)
-        solver = mod.Solver(*args, threads)
-        kind, cores = "reference", threads
+    lab = p.labels
+    if isinstance(lab, Uniform):
+        label = lab.label
+    elif isinstance(lab, Geometry):
+        last = Geometry(1, lab.prims, lab.background, lab.z0 + (lab.nz - 1) * p.dz)
+        label = int(_core.geometry_labels(last, p.ny, p.nx, p.dy, p.dx, p.dz, len(p.indices))[0, p.ny // 2, p.nx // 2])
     else:
-        from oracle import bwpm_oracle
+        label = int(np.asarray(lab)[-1, p.ny // 2, p.nx // 2])
+    return complex(p.indices[label])
 
-        solver = bwpm_oracle.Solver(*args)
-        kind, cores = "port", 1
-    t0 = time.perf_counter()
-    solver.run(prob.field, prob.iterations, [])
-    dt = time.perf_counter() - t0
-    return {"value": prob.updates / dt, "unit": "grid-point·slab updates/s", "cores": cores, "kind": kind,
-            "sample": f"tilted_3d {n}x{n}x{nz_sample} slabs, 1 iteration ({prob.updates} updates) in {dt:.2f} s"}
+
+def r_t_a(p, res: dict) -> dict:
+    """R and T as Poynting fluxes over the incident flux (device FP64 reductions), A = 1 − R − T
+    (the energy residual of src/bwpm/fast.py:146-155)."""
+    from bwpm_amd import _core
+
+    def flux(f, n, d):
+        return _core.power_flux(f, p.dy, p.dx, p.k0, n, d)
+
+    p_in = flux(p.field, p.index_before, 1)
+    r = flux(res["reflected"], p.index_before, -1) / p_in
+    t = flux(res["transmitted"], exit_index(p), 1) / p_in
+    return {"R": r, "T": t, "A": 1.0 - r - t}
+
+
+def accuracy(p, got: dict, ref: dict, ref_rta: dict | None = None) -> dict:
+    """Field L2 errors and R/T/A of `got` against the FP64 result `ref` of the same solve."""
+    g = r_t_a(p, got)
+    r = ref_rta or r_t_a(p, ref)
+    return {"rel_l2_transmitted": rel_l2(got["transmitted"], ref["transmitted"]),
+            "rel_l2_reflected": rel_l2(got["reflected"], ref["reflected"]),
+            "R": g["R"], "T": g["T"], "A": g["A"],
+            "abs_dR": abs(g["R"] - r["R"]), "abs_dT": abs(g["T"] - r["T"]), "abs_dA": abs(g["A"] - r["A"])}
 
 
 def run_solve(p, precision: str) -> tuple[dict, float]:
@@ -104,9 +130,27 @@ def run_solve(p, precision: str) -> tuple[dict, float]:
         s.close()
 
 
-def precision_mode(probs: list, precision: str, iterations: int, ref: dict) -> dict:
-    """The headline lane in another precision: updates/s over one timed batch after one warm-up,
-    the kernel durations, and the relative L2 error of solve 0 against the FP64 device result."""
+def with_modes(p, modes=NARROW) -> dict:
+    """The solve in FP64 (rate, R, T, A) and in each narrow mode (rate and accuracy against FP64)."""
+    ref, rate = run_solve(p, "fp64")
+    ref_rta = r_t_a(p, ref)
+    out = {"fp64": {"updates_per_s": rate, **ref_rta}}
+    if "gmres_iterations" in ref:
+        out["fp64"]["gmres_iterations"] = ref["gmres_iterations"]
+    for m in modes:
+        got, r = run_solve(p, m)
+        out[m] = {"updates_per_s": r, **accuracy(p, got, ref, ref_rta)}
+        if "gmres_iterations" in got:
+            out[m]["gmres_iterations"] = got["gmres_iterations"]
+    return out
+
+
+# ---- the headline lane and its variants --------------------------------------------------------
+
+def lane(probs: list, precision: str, iterations: int, warmup: int, steps: int) -> tuple[list, float, dict, list]:
+    """Solvers of `probs` in one lane: the seconds of `steps` timed batches after `warmup`, the
+    HIP-event kernel durations of one more batch, and the results of solves 0 and 1. The caller
+    closes the returned solvers."""
     from bwpm_amd import _core
 
     solvers = []
@@ -115,25 +159,28 @@ def precision_mode(probs: list, precision: str, iterations: int, ref: dict) -> d
             s = _core.GpuSolver(*p.solver_args(), precision=precision)
             s.upload(p.field)
             solvers.append(s)
-        _core.run_batch(solvers, iterations, max_lane=len(solvers))
+        for _ in range(warmup):
+            _core.run_batch(solvers, iterations, max_lane=len(solvers))
         t0 = time.perf_counter()
-        _core.run_batch(solvers, iterations, max_lane=len(solvers))
-        rate = sum(p.updates for p in probs) / (time.perf_counter() - t0)
+        for _ in range(steps):
+            _core.run_batch(solvers, iterations, max_lane=len(solvers))
+        elapsed = time.perf_counter() - t0
         kt = _core.run_batch(solvers, iterations, max_lane=len(solvers), time_kernels=True)
-        got = solvers[0].fetch(iterations)
-    finally:
+        results = [solvers[i].fetch(iterations) for i in range(min(2, len(solvers)))]
+    except BaseException:
         for s in solvers:
             s.close()
-    return {"updates_per_s": rate,
-            "k_col_us": round(1e3 * kt["col_ms"] / max(kt["col_launches"], 1), 3),
-            "k_row_us": round(1e3 * kt["row_ms"] / max(kt["row_launches"], 1), 3),
-            "rel_l2_transmitted": rel_l2(got["transmitted"], ref["transmitted"]),
-            "rel_l2_reflected": rel_l2(got["reflected"], ref["reflected"])}
+        raise
+    return solvers, elapsed, kt, results
+
+
+def kernel_us(kt: dict) -> tuple[float, float]:
+    return (1e3 * kt["col_ms"] / max(kt["col_launches"], 1), 1e3 * kt["row_ms"] / max(kt["row_launches"], 1))
 
 
 def small_lane(count: int = 64, n: int = 128, nz: int = 256, iterations: int = 2) -> dict:
     """Batched over plain on small grids: `count` tilted-interface solves of n² × nz in one lane
-    against the same solves one at a time (one solve per launch, synchronised per solve)."""
+    against the same solves one at a time (one solve per launch, synchronised per solve), FP64."""
     from bwpm_amd import _core, workloads
 
     probs = [workloads.tilted_3d(n=n, nz=nz, iterations=iterations, polarisation="xy"[i % 2]) for i in range(count)]
@@ -147,100 +194,109 @@ def small_lane(count: int = 64, n: int = 128, nz: int = 256, iterations: int = 2
         t0 = time.perf_counter()
         _core.run_batch(solvers, iterations, max_lane=count)
         batched = sum(p.updates for p in probs) / (time.perf_counter() - t0)
+        in_batch = solvers[0].fetch(iterations)
         solvers[0].run_resident(iterations)
         t0 = time.perf_counter()
         for s in solvers:
             s.run_resident(iterations)
         plain = sum(p.updates for p in probs) / (time.perf_counter() - t0)
+        alone = solvers[0].fetch(iterations)
     finally:
         for s in solvers:
             s.close()
+    same = bool(np.array_equal(in_batch["transmitted"], alone["transmitted"])
+                and np.array_equal(in_batch["reflected"], alone["reflected"]))
     return {"workload": f"{count} tilted-interface solves {n}x{n}x{nz}, {iterations} iterations, FP64",
-            "batched_updates_per_s": batched, "plain_updates_per_s": plain, "batched_over_plain": batched / plain}
+            "batched_updates_per_s": batched, "plain_updates_per_s": plain, "batched_over_plain": batched / plain,
+            "batch_vs_alone_bitwise": same}
 
 
-def uneven_batch(count: int) -> dict:
+# ---- the reference's other configurations ------------------------------------------------------
+
+def uneven_batch(count: int, precision: str, check: int = 4) -> dict:
     """Config 4: `count` uneven Born solves (128² … 512², 100 … 1000 slabs, per-solve λ, angle, σ and
-    seed; δn generated on the device) submitted at once to the request queue, complex64."""
+    seed; δn generated on the device) submitted at once to the request queue. Accuracy: the
+    `check` smallest solves again alone in FP64 against their results from the queue."""
     from bwpm_amd import serve, workloads
 
     probs = workloads.uneven_batch(count)
-    with serve.Server(slots=16, chunk=64, precision="fp32") as srv:
+    order = sorted(range(count), key=lambda i: probs[i].updates)[:check]
+    with serve.Server(slots=16, chunk=64, precision=precision) as srv:
         t0 = time.perf_counter()
         tickets = [srv.submit(p) for p in probs]
-        for t in tickets:
-            srv.result(t)
+        kept = {}
+        for i, t in enumerate(tickets):
+            r = srv.result(t)
+            if i in order:
+                kept[i] = r
         span = time.perf_counter() - t0
         st = srv.stats()
-    return {"solves": count, "seconds": round(span, 2), "updates_per_s": sum(p.updates for p in probs) / span,
+    acc = {}
+    for i in order:
+        ref, _ = run_solve(probs[i], "fp64")
+        acc[probs[i].name] = accuracy(probs[i], kept[i], ref)
+    return {"solves": count, "precision": precision, "seconds": round(span, 2),
+            "updates_per_s": sum(p.updates for p in probs) / span,
             "p50_ms": round(st["p50_ms"], 1), "p99_ms": round(st["p99_ms"], 1), "failed": int(st["failed"]),
-            "peak_hbm_gb": round(st["peak_bytes"] / 1e9, 2)}
+            "peak_hbm_gb": round(st["peak_bytes"] / 1e9, 2),
+            "accuracy_vs_fp64_alone": acc}
 
 
-def configs(c3_nz: int, c4_count: int, which: str = "c1,c2,c3,c4") -> dict:
-    """The reference's other configurations: config 1 (2D, 1024 points, 200 slabs, 3 iterations),
-    config 2 (3D 256² × 512 tilted interface, 4 iterations, E_x and E_y) and config 3 (3D 512² Born
-    volume, `c3_nz` slabs, 6 iterations) as single solves; config 4 (uneven Born batch) on the
-    queue. Errors of the mixed and FP32 modes are relative L2 against the FP64 device result
-    (itself within 1e-10 of the oracle). `which` selects configurations (comma-separated)."""
+def configs(c3_nz: int, c4_count: int, precision: str, which: str = "c1,c2,c3,c4") -> dict:
+    """Config 1 (2D, 1024 points, 200 slabs, 3 iterations), config 2 (3D 256² × 512 tilted
+    interface, 4 iterations, E_x and E_y) and config 3 (3D 512² Born volume, `c3_nz` slabs, 6
+    iterations) as single solves in FP64 and in each narrow mode; config 4 (uneven Born batch) on
+    the queue in `precision`. `which` selects configurations (comma-separated)."""
     from bwpm_amd import workloads
 
     sel = {w.strip() for w in which.split(",")}
     out: dict = {}
     if "c1" in sel:
-        p1 = workloads.plain_2d(ny=1024, nz=200, iterations=3)
-        _, r1 = run_solve(p1, "fp64")
-        out["c1_2d_1024x200_it3"] = {"fp64_updates_per_s": r1}
-
-    def against(p, ref: dict) -> dict:
-        # the same solve with complex64 streaming planes (FP64 arithmetic) and in FP32 arithmetic
-        out_m = {}
-        for m in ("mixed", "fp32"):
-            r, rate = run_solve(p, m)
-            out_m[m] = {"updates_per_s": rate, "rel_l2_transmitted": rel_l2(r["transmitted"], ref["transmitted"]),
-                        "rel_l2_reflected": rel_l2(r["reflected"], ref["reflected"])}
-        return out_m
-
+        out["c1_2d_1024x200_it3"] = with_modes(workloads.plain_2d(ny=1024, nz=200, iterations=3))
     if "c2" in sel:
-        c2 = {}
-        for pol in ("x", "y"):
-            p2 = workloads.tilted_3d(n=256, nz=512, iterations=4, polarisation=pol)
-            a64, r64 = run_solve(p2, "fp64")
-            c2[f"E{pol}"] = {"fp64_updates_per_s": r64, **against(p2, a64)}
-        out["c2_tilted_256x256x512_it4"] = c2
-    if "c3" in sel:
-        # config 3 at full depth: the δn volume is generated on the device (a (1024, 512, 512)
-        # complex array would be 4 GiB on the host)
-        p3 = workloads.born_3d_synth(n=512, nz=c3_nz, iterations=6)
-        b64, q64 = run_solve(p3, "fp64")
+        out["c2_tilted_256x256x512_it4"] = {
+            f"E{pol}": with_modes(workloads.tilted_3d(n=256, nz=512, iterations=4, polarisation=pol))
+            for pol in ("x", "y")}
+    if "c3" in sel and c3_nz > 0:
+        # the δn volume is generated on the device (a (1024, 512, 512) complex array would be
+        # 4 GiB on the host)
         out[f"c3_born_512x512x{c3_nz}_it6"] = {
-            "fp64_updates_per_s": q64, "fp64_gmres_iterations": b64["gmres_iterations"], **against(p3, b64),
+            **with_modes(workloads.born_3d_synth(n=512, nz=c3_nz, iterations=6)),
             "input": "Gaussian-correlated δn (σ = 0.01, correlation 1 µm, seed 3) generated on the device"}
     if c4_count > 0 and "c4" in sel:
-        out[f"c4_uneven_born_{c4_count}"] = uneven_batch(c4_count)
+        out[f"c4_uneven_born_{c4_count}"] = uneven_batch(c4_count, precision)
     return out
 
 
-def served(count: int, precision: str) -> dict:
+def served(count: int, precision: str, check: int = 3) -> dict:
     """The served mix on the request queue: a burst of `count` requests measures capacity, then a
-    Poisson stream of `count` other requests at 70 % of it gives latency under load."""
+    Poisson stream of `count` other requests at 70 % of it gives latency under load. Accuracy: the
+    first `check` requests of the burst again alone in FP64."""
     from bwpm_amd import serve
 
     burst = [serve.served_request(1000 + i) for i in range(count)]
     with serve.Server(slots=16, chunk=64, precision=precision) as srv:
         t0 = time.perf_counter()
         tickets = [srv.submit(p) for p in burst]
-        for t in tickets:
-            srv.result(t)
+        kept = {}
+        for i, t in enumerate(tickets):
+            r = srv.result(t)
+            if i < check:
+                kept[i] = r
         span = time.perf_counter() - t0
         cap = srv.stats()
     rate = 0.7 * count / span
     with serve.Server(slots=16, chunk=64, precision=precision) as srv:
         rep = serve.run_stream(srv, (serve.served_request(2000 + i) for i in range(count)), rate_hz=rate, seed=1)
     st = rep.stats
+    acc = {}
+    for i in range(min(check, count)):
+        ref, _ = run_solve(burst[i], "fp64")
+        acc[burst[i].name] = accuracy(burst[i], kept[i], ref)
     return {
         "workload": "served mix: 2D 512-4096 points and 3D vector 128^2-512^2, 200-1024 slabs, planar / tilted / "
                     "silver-layer / index-fluctuation geometries, 1-3 iterations (seeded)",
+        "precision": precision,
         "requests": count,
         "capacity_req_per_s": round(count / span, 3),
         "capacity_updates_per_s": float(sum(p.updates for p in burst) / span),
@@ -254,8 +310,80 @@ def served(count: int, precision: str) -> dict:
         "completed": int(st["completed"]),
         "failed": int(st["failed"]) + len(rep.failures),
         "peak_hbm_gb": round(st["peak_bytes"] / 1e9, 2),
+        "accuracy_vs_fp64_alone": acc,
     }
 
+
+# ---- CPU baseline ------------------------------------------------------------------------------
+
+def host_cores() -> dict:
+    """Physical cores from lscpu, its CPU model, and the CPUs this process may run on."""
+    info: dict = {"allowed_cpus": len(os.sched_getaffinity(0))}
+    try:
+        out = subprocess.run(["lscpu", "-p=CORE,SOCKET"], capture_output=True, text=True, timeout=10).stdout
+        pairs = {tuple(line.split(",")) for line in out.splitlines() if line and not line.startswith("#")}
+        info["physical_cores"] = len(pairs)
+        model = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        m = re.search(r"^Model name:\s*(.+)$", model, re.M)
+        if m:
+            info["model"] = m.group(1).strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return info
+
+
+def cpu_baseline(reps: int = 5) -> dict:
+    """The reference's compiled C++ CPU core (oracle/_ref, built from its own sources) on bounded
+    samples of the headline and of configs 1–3, on all CPUs this process may use and on one core:
+    one warm-up run (which also fills its operator cache), then the median of `reps` timed runs."""
+    import importlib.util
+
+    from bwpm_amd import workloads
+
+    so = sorted((ROOT / "oracle" / "_ref").glob("_core*.so")) if (ROOT / "oracle" / "_ref").exists() else []
+    if not so:
+        return {"error": "oracle/_ref not built (oracle/build_ref.sh)"}
+    spec = importlib.util.spec_from_file_location("_core", so[0])
+    ref = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(ref)
+    cores = host_cores()
+    env_threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    all_threads = min(cores["allowed_cpus"], env_threads) if env_threads > 0 else cores["allowed_cpus"]
+    samples = {
+        "headline": (workloads.tilted_3d(n=512, nz=2, iterations=1, absorber_wavelengths=20.0),
+                     "tilted 512x512, 2 slabs, 1 iteration, 20-wavelength absorber"),
+        "c1": (workloads.plain_2d(ny=1024, nz=200, iterations=3), "2D 1024 points, 200 slabs, 3 iterations (full)"),
+        "c2": (workloads.tilted_3d(n=256, nz=8, iterations=1), "tilted 256x256, 8 slabs, 1 iteration"),
+        "c3": (workloads.born_3d(n=512, nz=1, iterations=1), "Born 512x512, 1 slab, 1 iteration"),
+    }
+
+    def timed(p, threads: int) -> tuple[float, list[float]]:
+        s = ref.Solver(p.ny, p.nx, p.dy, p.dx, p.dz, p.k0, np.asarray(p.indices, complex),
+                       np.asarray(p.labels, dtype=np.int32), complex(p.index_before), p.mask, threads,
+                       fluctuation=p.fluctuation)
+        ts = []
+        for _ in range(reps + 1):
+            t0 = time.perf_counter()
+            s.run(p.field, p.iterations, [])
+            ts.append(time.perf_counter() - t0)
+        return p.updates / statistics.median(ts[1:]), ts
+
+    out: dict = {}
+    for key, (p, what) in samples.items():
+        entry = {"sample": what, "updates": p.updates}
+        for label, threads in (("all_cores", all_threads), ("one_core", 1)):
+            rate, ts = timed(p, threads)
+            entry[label] = {"threads": threads, "updates_per_s": rate, "seconds": [round(t, 4) for t in ts]}
+        out[key] = entry
+    head = out["headline"]["all_cores"]
+    return {"value": head["updates_per_s"], "unit": "grid-point·slab updates/s", "cores": all_threads,
+            "kind": "reference",
+            "sample": f"{samples['headline'][1]} ({samples['headline'][0].updates} updates); one warm-up run, "
+                      f"median of {reps}",
+            "host": cores, "configs": out}
+
+
+# ---- main --------------------------------------------------------------------------------------
 
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
@@ -266,19 +394,18 @@ def main() -> int:
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--nz", type=int, default=1024)
     ap.add_argument("--iterations", type=int, default=4)
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed", "bfp", "fp32", "fp32_plain"])
-    ap.add_argument("--absorber", type=float, default=0.0,
+    ap.add_argument("--precision", default="mixed", choices=list(BYTES_PER_UPDATE))
+    ap.add_argument("--absorber", type=float, default=20.0,
                     help="absorbing edge zone of the headline geometry in wavelengths (0: none)")
-    ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "0") or 0))
-    ap.add_argument("--cpu-nz", type=int, default=48)
+    ap.add_argument("--cpu-reps", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-checks", action="store_true", help="skip the plain-path and accuracy runs")
-    ap.add_argument("--no-modes", action="store_true", help="skip the mixed and FP32 lanes of the headline")
+    ap.add_argument("--no-checks", action="store_true", help="skip the accuracy and plain-path runs")
+    ap.add_argument("--no-modes", action="store_true", help="skip the lanes of the other precision modes")
     ap.add_argument("--serve-requests", type=int, default=24,
                     help="requests of the served mix (0: no serving run)")
-    ap.add_argument("--c3-nz", type=int, default=1024, help="slabs of the Born configuration (0: skip configs)")
+    ap.add_argument("--c3-nz", type=int, default=1024, help="slabs of the Born configuration (0: skip it)")
     ap.add_argument("--c4-count", type=int, default=64, help="solves of the uneven Born batch (0: skip)")
-    ap.add_argument("--configs", default="c1,c2,c3,c4", help="configurations to run (comma-separated)")
+    ap.add_argument("--configs", default="c1,c2,c3,c4", help="configurations to run (comma-separated; '' for none)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU figures are not produced",
@@ -287,43 +414,28 @@ def main() -> int:
 
     from bwpm_amd import _core, workloads
 
-    probs = [workloads.tilted_3d(n=args.n, nz=args.nz, iterations=args.iterations,
-                                 polarisation="x" if i % 2 == 0 else "y", absorber_wavelengths=args.absorber)
-             for i in range(args.batch)]
-    solvers = []
-    for p in probs:
-        s = _core.GpuSolver(*p.solver_args(), precision=args.precision)
-        s.upload(p.field)
-        solvers.append(s)
-
-    def one_step() -> None:
-        # one lane: every slab step is one column and one row launch for all solves of the batch
-        _core.run_batch(solvers, args.iterations, max_lane=args.batch)
-
-    for _ in range(args.warmup):
-        one_step()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        one_step()
-    elapsed = time.perf_counter() - t0
+    # E_x and E_y polarised solves alternate; solves of one polarisation share one problem (the
+    # solvers copy their inputs)
+    pol = {c: workloads.tilted_3d(n=args.n, nz=args.nz, iterations=args.iterations, polarisation=c,
+                                  absorber_wavelengths=args.absorber) for c in "xy"}
+    probs = [pol["x" if i % 2 == 0 else "y"] for i in range(args.batch)]
+    solvers, elapsed, kt, batch_res = lane(probs, args.precision, args.iterations, args.warmup, args.steps)
     updates = sum(p.updates for p in probs) * args.steps
     value = updates / elapsed
     ms_per_step = 1e3 * elapsed / max(args.steps, 1)
 
     # kernel durations inside the lane: HIP events around every launch on the lane's stream
-    kt = _core.run_batch(solvers, args.iterations, max_lane=args.batch, time_kernels=True)
     bpu = BYTES_PER_UPDATE[args.precision]
     npix = args.n * args.n * args.batch  # pixels one launch processes
-    col_us = 1e3 * kt["col_ms"] / max(kt["col_launches"], 1)
-    row_us = 1e3 * kt["row_ms"] / max(kt["row_launches"], 1)
-    step_us = col_us + row_us
-    achieved = bpu * npix / (step_us * 1e-6) / 1e9
-    traffic, traffic_src = profiled_traffic(args.precision, args.n, args.batch)
+    col_us, row_us = kernel_us(kt)
+    achieved = bpu * npix / ((col_us + row_us) * 1e-6) / 1e9
+    traffic, traffic_src = profiled_traffic(args.precision, args.n, args.batch, args.absorber)
     copy_bytes = 1 << 30
     copy_ms = _core.copy_probe(copy_bytes, 10)
     roofline = {
         "bound": "hbm",
-        "kernel": "k_col_fast + k_row_fast (one slab step of the lane)",
+        "kernel": "slab step: column + row kernel (k_col_plain/k_row_plain on plain steps, k_col_fast/k_row_fast "
+                  "at interfaces and pass ends), averaged over every launch of one batch",
         "achieved": round(achieved, 2),
         "peak": HBM_PEAK_GBPS,
         "unit": "GB/s",
@@ -338,53 +450,65 @@ def main() -> int:
     }
 
     checks = None
-    if not args.no_checks:
-        batch_res = solvers[0].fetch(args.iterations)
-        # the plain path: the same solve alone, one solve per launch, synchronised per solve
-        solvers[0].run_resident(args.iterations)
-        t0 = time.perf_counter()
-        solvers[0].run_resident(args.iterations)
-        plain = probs[0].updates / (time.perf_counter() - t0)
-        alone = solvers[0].fetch(args.iterations)
-        # the same solve through host buffers (field copied in, results copied out over PCIe)
-        t0 = time.perf_counter()
-        solvers[0].run(probs[0].field, args.iterations)
-        host_rate = probs[0].updates / (time.perf_counter() - t0)
-        solvers[0].upload(probs[0].field)
-        checks = {
-            "plain_one_solve_per_launch": plain,
-            "one_solve_with_host_buffers": host_rate,
-            "batched_over_plain": value / plain,
-            "batch_vs_alone_bitwise": bool(np.array_equal(batch_res["transmitted"], alone["transmitted"])
-                                           and np.array_equal(batch_res["reflected"], alone["reflected"])),
-        }
-        if args.precision != "fp64":
-            ref = _core.GpuSolver(*probs[0].solver_args(), precision="fp64")
-            ref.upload(probs[0].field)
-            ref.run_resident(args.iterations)
-            r64 = ref.fetch(args.iterations)
-            ref.close()
-            checks["rel_l2_vs_fp64_device"] = {"transmitted": rel_l2(alone["transmitted"], r64["transmitted"]),
-                                               "reflected": rel_l2(alone["reflected"], r64["reflected"])}
-
-    for s in solvers:
-        s.close()
-    solvers = []
-    if checks is not None:
-        checks["small_grids"] = small_lane()
+    headline_acc = None
     modes = None
-    if checks is not None and args.precision == "fp64" and not args.no_modes:
-        # the same lane with complex64 streaming planes (FP64 arithmetic) and in FP32 arithmetic
-        modes = {m: precision_mode(probs, m, args.iterations, alone) for m in ("mixed", "fp32")}
-    other = configs(args.c3_nz, args.c4_count, args.configs) if args.c3_nz > 0 else None
-    serving = None
-    if args.serve_requests > 0:
-        serving = served(args.serve_requests, args.precision)
-
-    cpu = None
-    if not args.no_cpu_baseline:
-        threads = args.cpu_threads if args.cpu_threads > 0 else min(16, os.cpu_count() or 1)
-        cpu = cpu_baseline(args.n, args.cpu_nz, threads)
+    try:
+        if not args.no_checks:
+            # the plain path: the same solve alone, one solve per launch, synchronised per solve
+            solvers[0].run_resident(args.iterations)
+            t0 = time.perf_counter()
+            solvers[0].run_resident(args.iterations)
+            plain = probs[0].updates / (time.perf_counter() - t0)
+            alone = solvers[0].fetch(args.iterations)
+            # the same solve through host buffers (field copied in, results copied out over PCIe)
+            t0 = time.perf_counter()
+            solvers[0].run(probs[0].field, args.iterations)
+            host_rate = probs[0].updates / (time.perf_counter() - t0)
+            solvers[0].upload(probs[0].field)
+            checks = {
+                "plain_one_solve_per_launch": plain,
+                "one_solve_with_host_buffers": host_rate,
+                "batched_over_plain": value / plain,
+                "batch_vs_alone_bitwise": bool(np.array_equal(batch_res[0]["transmitted"], alone["transmitted"])
+                                               and np.array_equal(batch_res[0]["reflected"], alone["reflected"])),
+            }
+    finally:
+        for s in solvers:
+            s.close()
+        solvers = []
+    if checks is not None:
+        # accuracy of the batch's E_x and E_y solves against the FP64 device result of each
+        refs = {c: run_solve(pol[c], "fp64") for c in "xy"}
+        ref_rta = {c: r_t_a(pol[c], refs[c][0]) for c in "xy"}
+        headline_acc = {f"E{c}": accuracy(pol[c], batch_res[i], refs[c][0], ref_rta[c]) for i, c in enumerate("xy")}
+        headline_acc["fp64_reference"] = {f"E{c}": ref_rta[c] for c in "xy"}
+        # the plain one-solve-per-launch FP32 path (unbatched complex64, synchronised per solve)
+        _, plain32 = run_solve(pol["x"], "fp32")
+        checks["plain_fp32_one_solve_per_launch"] = plain32
+        checks["batched_over_plain_fp32"] = value / plain32
+        checks["plain_fp64_one_solve_per_launch"] = refs["x"][1]
+        checks["small_grids"] = small_lane()
+        if not args.no_modes:
+            modes = {}
+            for m in ("fp64", "mixed", "bfp", "fp32"):
+                if m == args.precision:
+                    continue
+                ss, el, kt_m, res = lane(probs, m, args.iterations, 1, 1)
+                for s in ss:
+                    s.close()
+                cu, ru = kernel_us(kt_m)
+                entry = {"updates_per_s": sum(p.updates for p in probs) / el, "k_col_us": round(cu, 3),
+                         "k_row_us": round(ru, 3),
+                         "roofline_frac": round(BYTES_PER_UPDATE[m] * npix / ((cu + ru) * 1e-6) / 1e9 / HBM_PEAK_GBPS,
+                                                5)}
+                if m != "fp64":
+                    for i, c in enumerate("xy"):
+                        entry[f"E{c}"] = accuracy(pol[c], res[i], refs[c][0], ref_rta[c])
+                modes[m] = entry
+    del pol, probs
+    other = configs(args.c3_nz, args.c4_count, args.precision, args.configs) if args.configs else None
+    serving = served(args.serve_requests, args.precision) if args.serve_requests > 0 else None
+    cpu = None if args.no_cpu_baseline else cpu_baseline(args.cpu_reps)
     line = {
         "metric": "grid-point·slab updates/s (3D vector 512²)",
         "value": value,
@@ -396,15 +520,19 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"fp64": "c128", "mixed": "c64 planes, f64 arithmetic"}.get(args.precision, "c64"),
+        "dtype": ARITHMETIC[args.precision],
         "data": "synthetic",
         "config": {
             "workload": f"tilted-interface 3D vector BWPM {args.n}x{args.n}x{args.nz} slabs, "
-                        f"{args.iterations} iterations, batch of {args.batch} solves in one lane",
+                        f"{args.iterations} iterations, absorbing zone {args.absorber:g} wavelengths, "
+                        f"batch of {args.batch} solves in one lane",
             "n": args.n, "nz": args.nz, "iterations": args.iterations, "batch": args.batch,
-            "precision": args.precision, "updates_per_solve": probs[0].updates,
+            "absorber_wavelengths": args.absorber, "precision": args.precision,
+            "streaming_planes": PLANES[args.precision], "arithmetic": ARITHMETIC[args.precision],
+            "updates_per_solve": updates // max(args.steps, 1) // args.batch,
         },
         "roofline": roofline,
+        "accuracy": headline_acc,
         "checks": checks,
         "precision_modes": modes,
         "configs": other,
@@ -412,8 +540,6 @@ def main() -> int:
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
-    for s in solvers:
-        s.close()
     return 0
 
 

@@ -384,7 +384,9 @@ double copy_probe(long long bytes, int reps, int device) {
   hipEvent_t e0, e1;
   HIP_CHECK(hipEventCreate(&e0));
   HIP_CHECK(hipEventCreate(&e1));
-  const unsigned grid = static_cast<unsigned>(std::min<long long>((n + 255) / 256, 256 * 64));
+  // one float4 per thread (a grid-stride loop with fewer workgroups reads HBM more slowly on this
+  // device: 6.2 against about 4.5 TB/s at 1 GiB)
+  const unsigned grid = static_cast<unsigned>(std::min<long long>((n + 255) / 256, 0x7fffffffLL));
   hipLaunchKernelGGL(k_copy_probe, dim3(grid), dim3(256), 0, s, a.as<float4>(), b.as<float4>(), n);
   HIP_CHECK(hipEventRecord(e0, s));
   for (int r = 0; r < reps; ++r)

@@ -228,10 +228,20 @@ inline bool spill_free(const void* kern) {
 }
 
 // Grid of a persistent launch over `tiles` tiles with room for `cap` resident workgroups: as few
-// workgroups as give every one the same number of tiles (± 1) in ceil(tiles / cap) rounds.
-inline int persistent_grid(long long tiles, int cap) {
+// workgroups as give every one the same number of tiles (± 1) in ceil(tiles / cap) rounds. With
+// `align` (the XCD count, for kernels that map tiles with xcd_tile) and `tiles` a multiple of it,
+// the grid is rounded up to a multiple of `align`, so that the tiles b, b + P, b + 2P, … of
+// workgroup b keep b's residue: workgroup b runs on XCD b mod align, and every tile the mapping
+// gives to one XCD is then processed there (with a grid of 683, neighbouring column groups ran on
+// different XCDs and each XCD fetched the 128-byte lines they share: twice the column reads).
+inline int persistent_grid(long long tiles, int cap, int align = 1) {
   const long long rounds = (tiles + cap - 1) / cap;
-  return static_cast<int>(std::max<long long>(1, (tiles + rounds - 1) / rounds));
+  long long g = std::max<long long>(1, (tiles + rounds - 1) / rounds);
+  if (align > 1 && tiles % align == 0 && cap >= align) {
+    g = std::min<long long>((g + align - 1) / align * align, std::max<long long>(align, cap / align * align));
+    g = std::min<long long>(g, tiles);
+  }
+  return static_cast<int>(g);
 }
 
 // A row step every solve of the launch takes in the plain row kernel (kernels_plain.hpp): one
@@ -423,8 +433,9 @@ struct Launcher {
     if (!fits) return false;
     static const int cap = resident_groups(reinterpret_cast<const void*>(kern), N / 4 * W);
     const int groups = static_cast<int>(nx / W);
-    s(kern, dim3(static_cast<unsigned>(persistent_grid(static_cast<long long>(S) * groups, cap))), dim3(N / 4 * W), 0,
-      d, S, groups, static_cast<int>(nx), twy->hi.template as<cx<T>>(), twy->lo.template as<cx<T>>());
+    s(kern, dim3(static_cast<unsigned>(persistent_grid(static_cast<long long>(S) * groups, cap, kXcds))),
+      dim3(N / 4 * W), 0, d, S, groups, static_cast<int>(nx), twy->hi.template as<cx<T>>(),
+      twy->lo.template as<cx<T>>());
     return true;
   }
   template <int N, int RW>

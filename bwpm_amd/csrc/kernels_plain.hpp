@@ -12,11 +12,13 @@
 // workgroup instead of once per tile. Per tile the arithmetic is exactly that of the general
 // kernels (same transforms, same operator products in the same order), so both give the same bits.
 //
-// Column tiles are W adjacent columns of one solve. Workgroups are dispatched to the eight XCDs in
-// turn (b mod 8), so tile u is mapped to the linear tile (u mod 8)·(T/8) + u/8: the workgroups of one
-// XCD work on one contiguous range of column groups, and neighbouring groups, which share the
-// 128-byte lines of X and of the operator tables, meet in that XCD's L2. (The mapping only affects
-// speed; any dispatch order gives the same results.)
+// Column tiles are W adjacent columns of one solve, G column groups per solve. Workgroups are
+// dispatched to the eight XCDs in turn (b mod 8), and the host makes the grid a multiple of 8, so
+// every tile of a workgroup has the workgroup's residue. Tile u goes to XCD x = u mod 8, which takes
+// the column groups [x·G/8, (x+1)·G/8) of every solve in turn: neighbouring groups, which share the
+// 128-byte lines of X, meet in that XCD's L2, and so does the XCD's slice of the phase table, which
+// the solves of a lane with one structure share (one table in HBM), from one solve to the next.
+// (The mapping only affects speed; any dispatch order gives the same results.)
 #pragma once
 
 #include "kernels_fast.hpp"
@@ -36,9 +38,16 @@ __device__ __forceinline__ cx<bfp> sraw<bfp>(const px2<bfp>* p, unsigned i) {
   return cx<bfp>{bfp{v.x}, bfp{v.y}};
 }
 
-// Tile order of the column kernel (see the file comment): linear tile of dispatch index u.
-__device__ __forceinline__ int xcd_tile(int u, int ntiles) {
-  return (ntiles & 7) ? u : (u & 7) * (ntiles >> 3) + (u >> 3);
+// XCDs of the device; workgroup b of a launch runs on XCD b mod kXcds.
+constexpr int kXcds = 8;
+
+// Tile order of the column kernel (see the file comment): linear tile (solve·G + group) of
+// dispatch index u, for G column groups per solve. A bijection of [0, S·G) when G is a multiple of
+// 8 (the identity otherwise).
+__device__ __host__ __forceinline__ int xcd_tile(int u, int groups) {
+  if (groups & 7) return u;
+  const int g8 = groups >> 3, k = u >> 3;
+  return (k / g8) * groups + (u & 7) * g8 + k % g8;
 }
 
 template <int N, int W, typename T, bool COMP, typename ST>
@@ -57,7 +66,7 @@ __global__ __launch_bounds__(N / 4 * W) void k_col_plain(const ColDesc<T, ST>* _
   // uniform, so descriptors are read with scalar loads (which do not wait behind vector loads)
   auto advance = [&](int u) {
     for (u = uniform(u); u < ntiles; u = uniform(u + static_cast<int>(gridDim.x)))
-      if (descs[xcd_tile(u, ntiles) / groups].in != nullptr) break;
+      if (descs[xcd_tile(u, groups) / groups].in != nullptr) break;
     return u;
   };
   int u = advance(static_cast<int>(blockIdx.x));
@@ -73,7 +82,7 @@ __global__ __launch_bounds__(N / 4 * W) void k_col_plain(const ColDesc<T, ST>* _
   auto pix = [&](unsigned x, int m) { return static_cast<unsigned>(t + TT * m) * static_cast<unsigned>(nx) + x; };
   auto at = [&](unsigned x, int m) { return 2u * pix(x, m) + static_cast<unsigned>(comp); };
   auto load = [&](int uu, cx<ST> (&r)[8]) {
-    const int lin = xcd_tile(uu, ntiles);
+    const int lin = xcd_tile(uu, groups);
     const px2<ST>* in = descs[lin / groups].in;
     const unsigned x = static_cast<unsigned>((lin % groups) * W + c);
 #pragma unroll
@@ -82,7 +91,7 @@ __global__ __launch_bounds__(N / 4 * W) void k_col_plain(const ColDesc<T, ST>* _
   // one tile from `cur`, with the next tile's loads issued into `nxt`; the loop alternates the two
   // buffers (no register copy, which would wait for the loads, and for the stores behind them)
   auto step = [&](cx<ST> (&cur)[8], cx<ST> (&nxt)[8]) {
-    const int lin = xcd_tile(u, ntiles);
+    const int lin = xcd_tile(u, groups);
     const ColDesc<T, ST>& d = descs[lin / groups];
     // the product as scalars (table loads then address as base + offset)
     const Prod<T>* pp = d.prods;
