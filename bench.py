@@ -130,16 +130,28 @@ def run_solve(p, precision: str) -> tuple[dict, float]:
         s.close()
 
 
+def iteration(res: dict, i: int) -> dict:
+    return {"transmitted": res["transmitted_iter"][i], "reflected": res["reflected_iter"][i]}
+
+
 def with_modes(p, modes=NARROW) -> dict:
-    """The solve in FP64 (rate, R, T, A) and in each narrow mode (rate and accuracy against FP64)."""
+    """The solve in FP64 (rate, R, T, A, and R and T after each iteration) and in each narrow mode
+    (rate and accuracy against FP64, also after each iteration: an iteration that does not settle
+    amplifies the rounding of the narrow modes with its own growth)."""
     ref, rate = run_solve(p, "fp64")
     ref_rta = r_t_a(p, ref)
-    out = {"fp64": {"updates_per_s": rate, **ref_rta}}
+    its = len(ref.get("reflected_iter", []))
+    per_it = [r_t_a(p, iteration(ref, i)) for i in range(its)]
+    out = {"fp64": {"updates_per_s": rate, **ref_rta, "R_per_iteration": [e["R"] for e in per_it],
+                    "T_per_iteration": [e["T"] for e in per_it]}}
     if "gmres_iterations" in ref:
         out["fp64"]["gmres_iterations"] = ref["gmres_iterations"]
     for m in modes:
         got, r = run_solve(p, m)
         out[m] = {"updates_per_s": r, **accuracy(p, got, ref, ref_rta)}
+        acc_it = [accuracy(p, iteration(got, i), iteration(ref, i), per_it[i]) for i in range(its)]
+        for key in ("rel_l2_transmitted", "rel_l2_reflected", "abs_dR", "abs_dT"):
+            out[m][key + "_per_iteration"] = [e[key] for e in acc_it]
         if "gmres_iterations" in got:
             out[m]["gmres_iterations"] = got["gmres_iterations"]
     return out
