@@ -90,7 +90,8 @@ int bwpm_power_flux_noalloc(const double* field, int64_t ny, int64_t nx, double 
                             double n_re, double n_im, int direction, int device, double* out);
 
 /* Solver over a labelled volume. labels: (nz, ny, nx) int32 into `indices` (m complex);
- * mask: (ny, nx) or NULL; fluct: δn (nz, ny, nx) complex or NULL; precision: BWPM_FP64/FP32.
+ * mask: (ny, nx) or NULL; fluct: δn (nz, ny, nx) complex or NULL; precision: one of the BWPM_FP64 …
+ * BWPM_FP32_PLAIN codes above.
  * Replaces `_core.CudaSolver(...)` / `_core.Solver(...)` (cpp/python/bindings.cpp:139-151,158-171,
  * cpp/src/solver.cpp:27-63; cpp/src/solver_cuda.cu:643-719). */
 int bwpm_solver_create(int64_t ny, int64_t nx, double dy, double dx, double dz, double k0,
