@@ -93,9 +93,38 @@ inline int pow2_floor(int v) {
 
 // ---- kernel geometry ---------------------------------------------------------------------------
 
+// Line lengths with register kernels: powers of two and 3·2^k, 9·2^k (fft_reg.hpp's plans).
 template <typename T>
 inline bool fast_size(long long n) {
-  return n == 64 || n == 128 || n == 256 || n == 512 || n == 1024 || n == 2048 || n == 4096;
+  switch (n) {
+    case 64: case 128: case 192: case 256: case 384: case 512: case 576: case 768: case 1024: case 1152:
+    case 1536: case 2048: case 3072: case 4096:
+      return true;
+    default:
+      return false;
+  }
+}
+
+// Calls f.template operator()<N>() with the compile-time line length N = n (a fast_size).
+template <typename F>
+inline auto with_fast_size(long long n, F&& f) {
+  switch (n) {
+    case 64: return f.template operator()<64>();
+    case 128: return f.template operator()<128>();
+    case 192: return f.template operator()<192>();
+    case 256: return f.template operator()<256>();
+    case 384: return f.template operator()<384>();
+    case 512: return f.template operator()<512>();
+    case 576: return f.template operator()<576>();
+    case 768: return f.template operator()<768>();
+    case 1024: return f.template operator()<1024>();
+    case 1152: return f.template operator()<1152>();
+    case 1536: return f.template operator()<1536>();
+    case 2048: return f.template operator()<2048>();
+    case 3072: return f.template operator()<3072>();
+    case 4096: return f.template operator()<4096>();
+    default: throw Error(kEinternal, "no register kernels for lines of " + std::to_string(n) + " points");
+  }
 }
 // columns / rows per workgroup of the register kernels (kernels_fast.hpp)
 template <typename T, bool COMP>
@@ -321,15 +350,8 @@ struct Launcher {
 
   void col(const ColDesc<T, ST>* d, int S, Emit& s, bool multi) const {
     if (geo.col_fast && COMP == (sizeof(T) == 4)) {
-      switch (ny) {
-        case 64: col_fast_n<64>(d, S, s, multi); return;
-        case 128: col_fast_n<128>(d, S, s, multi); return;
-        case 256: col_fast_n<256>(d, S, s, multi); return;
-        case 512: col_fast_n<512>(d, S, s, multi); return;
-        case 1024: col_fast_n<1024>(d, S, s, multi); return;
-        case 2048: col_fast_n<2048>(d, S, s, multi); return;
-        default: col_fast_n<4096>(d, S, s, multi); return;
-      }
+      with_fast_size(ny, [&]<int N>() { col_fast_n<N>(d, S, s, multi); });
+      return;
     }
     // generic: one LDS buffer of ny × ls per block (products reload the spectrum from X)
     const bool narrow = (narrow_mask(S) & 1) != 0;
@@ -345,15 +367,8 @@ struct Launcher {
 
   void row(const RowDesc<T, ST>* d, int S, Emit& s, bool multi) const {
     if (geo.row_fast && COMP == (sizeof(T) == 4)) {
-      switch (nx) {
-        case 64: row_fast_n<64>(d, S, s, multi); return;
-        case 128: row_fast_n<128>(d, S, s, multi); return;
-        case 256: row_fast_n<256>(d, S, s, multi); return;
-        case 512: row_fast_n<512>(d, S, s, multi); return;
-        case 1024: row_fast_n<1024>(d, S, s, multi); return;
-        case 2048: row_fast_n<2048>(d, S, s, multi); return;
-        default: row_fast_n<4096>(d, S, s, multi); return;
-      }
+      with_fast_size(nx, [&]<int N>() { row_fast_n<N>(d, S, s, multi); });
+      return;
     }
     const bool narrow = (narrow_mask(S) & 2) != 0;
     RowGeom g{static_cast<int>(ny), static_cast<int>(nx), narrow ? geo.rowsn : geo.rows, narrow ? geo.ls_rown : geo.ls_row,
@@ -400,29 +415,11 @@ struct Launcher {
   // has no register kernels for this direction.
   bool col_plain(const ColDesc<T, ST>* d, int S, Emit& s) const {
     if (!geo.col_fast || !geo.plain_col) return false;
-    switch (ny) {
-      case 64: return col_plain_n<64>(d, S, s);
-      case 128: return col_plain_n<128>(d, S, s);
-      case 256: return col_plain_n<256>(d, S, s);
-      case 512: return col_plain_n<512>(d, S, s);
-      case 1024: return col_plain_n<1024>(d, S, s);
-      case 2048: return col_plain_n<2048>(d, S, s);
-      case 4096: return col_plain_n<4096>(d, S, s);
-      default: return false;
-    }
+    return with_fast_size(ny, [&]<int N>() { return col_plain_n<N>(d, S, s); });
   }
   bool row_plain(const RowDesc<T, ST>* d, int S, Emit& s) const {
     if (!geo.row_fast || !geo.plain_row) return false;
-    switch (nx) {
-      case 64: return row_plain_n<64>(d, S, s);
-      case 128: return row_plain_n<128>(d, S, s);
-      case 256: return row_plain_n<256>(d, S, s);
-      case 512: return row_plain_n<512>(d, S, s);
-      case 1024: return row_plain_n<1024>(d, S, s);
-      case 2048: return row_plain_n<2048>(d, S, s);
-      case 4096: return row_plain_n<4096>(d, S, s);
-      default: return false;
-    }
+    return with_fast_size(nx, [&]<int N>() { return row_plain_n<N>(d, S, s); });
   }
 
  private:

@@ -1,13 +1,19 @@
-// Register-resident radix-8 Stockham FFT of power-of-two lines (16 … 4096 points) for the hot
-// slab-step kernels.
+// Register-resident mixed-radix Stockham FFT for the hot slab-step kernels: power-of-two lines
+// (16 … 4096 points) and lines of 3·2^k and 9·2^k points (192 … 3072).
 //
 // A line of N points is owned by T = N/8 threads. Thread t holds the eight pixel vectors
 // a[t + T·m] (m = 0 … 7, both field components), which is exactly the read pattern of every
-// Stockham pass of this plan: a pass of radix R lets each thread run 8/R butterflies
+// Stockham pass whose radix R divides 8: such a pass lets each thread run 8/R butterflies
 // j = t + T·b on v[b + r·8/R], multiplies by W_N^{r·(j mod Ns)·N/(Ns·R)}, applies the size-R DFT
 // and writes y_r to LDS at (j − k)·R + k + r·Ns; the next pass reads a[t + T·m] back. The first
 // and the last pass are radix 8, so the input is taken in natural order from registers and the
 // last pass leaves A[t + T·r] in v[r] (natural order) without touching LDS: P − 1 LDS exchanges.
+//
+// A pass of radix 3, 6, 9 or 12 (an "irregular" pass: R does not divide the eight values of a
+// thread) has N/R butterflies, which the T threads take as j = t, t + T, … (the last round only
+// partly): it reads its inputs x_r = a[j + r·N/R] straight from the LDS the previous pass wrote,
+// synchronises, and writes its outputs back to LDS in the same Stockham order. A 384-point line is
+// 8·6·8, so it has two exchanges like a 512-point one.
 //
 // In the px2 form the two field components are transformed one after the other inside each
 // butterfly, which halves the live temporaries; the cx form transforms a single component. Twiddles
@@ -42,8 +48,24 @@ template <> struct RegPlan<512> { static constexpr int P = 3; static constexpr i
 template <> struct RegPlan<1024> { static constexpr int P = 4; static constexpr int rad[P] = {8, 2, 8, 8}; };
 template <> struct RegPlan<2048> { static constexpr int P = 4; static constexpr int rad[P] = {8, 4, 8, 8}; };
 template <> struct RegPlan<4096> { static constexpr int P = 4; static constexpr int rad[P] = {8, 8, 8, 8}; };
+template <> struct RegPlan<192> { static constexpr int P = 3; static constexpr int rad[P] = {8, 3, 8}; };
+template <> struct RegPlan<384> { static constexpr int P = 3; static constexpr int rad[P] = {8, 6, 8}; };
+template <> struct RegPlan<576> { static constexpr int P = 3; static constexpr int rad[P] = {8, 9, 8}; };
+template <> struct RegPlan<768> { static constexpr int P = 3; static constexpr int rad[P] = {8, 12, 8}; };
+template <> struct RegPlan<1152> { static constexpr int P = 4; static constexpr int rad[P] = {8, 9, 2, 8}; };
+template <> struct RegPlan<1536> { static constexpr int P = 4; static constexpr int rad[P] = {8, 8, 3, 8}; };
+template <> struct RegPlan<3072> { static constexpr int P = 4; static constexpr int rad[P] = {8, 8, 6, 8}; };
 
 // The size-16 and size-32 plans end with a radix below 8; their last pass writes through LDS too.
+
+// A pass whose radix divides the eight values a thread holds takes its inputs from registers.
+constexpr bool reg_regular(int radix) { return 8 % radix == 0; }
+template <int N>
+constexpr bool reg_all_regular() {
+  for (int p = 0; p < RegPlan<N>::P; ++p)
+    if (!reg_regular(RegPlan<N>::rad[p])) return false;
+  return true;
+}
 
 template <int N>
 constexpr int reg_lds_len() {
@@ -94,16 +116,103 @@ __device__ __forceinline__ void stage_twiddles(cx<T>* sh_hi, cx<T>* sh_lo, const
   __syncthreads();
 }
 
+// cos(2π·m/36) for any integer m, from cos(10°·i), i = 0 … 9 (correctly rounded literals).
+constexpr double cos_36th(int m) {
+  constexpr double c[10] = {1.0,
+                            0.98480775301220805936674302458952,
+                            0.93969262078590838405410927732473,
+                            0.86602540378443864676372317075294,
+                            0.76604444311897803520239265055542,
+                            0.64278760968653932632264340990726,
+                            0.5,
+                            0.34202014332566873304409961468226,
+                            0.17364817766693034885171662676931,
+                            0.0};
+  m %= 36;
+  if (m < 0) m += 36;
+  if (m <= 9) return c[m];
+  if (m <= 18) return -c[18 - m];
+  if (m <= 27) return -c[m - 18];
+  return c[36 - m];
+}
+
+template <int R, bool COMP, bool INV, typename T>
+__device__ __forceinline__ void dft_r(cx<T>* x);
+
+// x ← W_R^e·x for the size-R DFT's internal twiddles (W_R = e^{∓2πi/R}, R dividing 36): exact for
+// multiples of a quarter turn, else a (compensated) complex product with the constant.
+template <int R, int E, bool COMP, bool INV, typename T>
+__device__ __forceinline__ cx<T> dft_twiddle(cx<T> x) {
+  constexpr int m = (E * (36 / R)) % 36;
+  if constexpr (m == 0) {
+    return x;
+  } else if constexpr (m == 9) {
+    return INV ? mul_i(x) : mul_mi(x);
+  } else if constexpr (m == 18) {
+    return cx<T>{-x.x, -x.y};
+  } else if constexpr (m == 27) {
+    return INV ? mul_mi(x) : mul_i(x);
+  } else {
+    constexpr double c = cos_36th(m), sn = (INV ? 1.0 : -1.0) * cos_36th(9 - m);
+    constexpr cx<T> hi{static_cast<T>(c), static_cast<T>(sn)};
+    constexpr cx<T> lo{static_cast<T>(c - static_cast<double>(static_cast<T>(c))),
+                       static_cast<T>(sn - static_cast<double>(static_cast<T>(sn)))};
+    return cmul2<COMP>(x, hi, lo);
+  }
+}
+
+template <int R1, int R2, int N2, int K1, bool COMP, bool INV, typename T>
+__device__ __forceinline__ void dft_ct_tw(cx<T> (&y)[R2][R1], const cx<T>* a) {
+  if constexpr (K1 < R1) {
+    y[N2][K1] = dft_twiddle<R1 * R2, N2 * K1, COMP, INV>(a[K1]);
+    dft_ct_tw<R1, R2, N2, K1 + 1, COMP, INV>(y, a);
+  }
+}
+// Size R1·R2 DFT by one Cooley–Tukey split: input n = R2·n1 + n2, output k = k1 + R1·k2. R2 DFTs of
+// size R1 over n1, the twiddles W_R^{n2·k1}, then R1 DFTs of size R2 over n2.
+template <int R1, int R2, bool COMP, bool INV, typename T, int N2 = 0>
+__device__ __forceinline__ void dft_ct_rows(cx<T> (&y)[R2][R1], const cx<T>* x) {
+  if constexpr (N2 < R2) {
+    cx<T> a[R1];
+#pragma unroll
+    for (int n1 = 0; n1 < R1; ++n1) a[n1] = x[R2 * n1 + N2];
+    dft_r<R1, COMP, INV>(a);
+    y[N2][0] = a[0];
+    dft_ct_tw<R1, R2, N2, 1, COMP, INV>(y, a);
+    dft_ct_rows<R1, R2, COMP, INV, T, N2 + 1>(y, x);
+  }
+}
+template <int R1, int R2, bool COMP, bool INV, typename T>
+__device__ __forceinline__ void dft_ct(cx<T>* x) {
+  cx<T> y[R2][R1];
+  dft_ct_rows<R1, R2, COMP, INV>(y, x);
+#pragma unroll
+  for (int k1 = 0; k1 < R1; ++k1) {
+    cx<T> b[R2];
+#pragma unroll
+    for (int n2 = 0; n2 < R2; ++n2) b[n2] = y[n2][k1];
+    dft_r<R2, COMP, INV>(b);
+#pragma unroll
+    for (int k2 = 0; k2 < R2; ++k2) x[k1 + R1 * k2] = b[k2];
+  }
+}
+
 template <int R, bool COMP, bool INV, typename T>
 __device__ __forceinline__ void dft_r(cx<T>* x) {
   if constexpr (R == 2) dft2<COMP, INV>(x);
+  else if constexpr (R == 3) dft3<COMP, INV>(x);
   else if constexpr (R == 4) dft4<COMP, INV>(x);
-  else dft8<COMP, INV>(x);
+  else if constexpr (R == 6) dft_ct<3, 2, COMP, INV>(x);
+  else if constexpr (R == 8) dft8<COMP, INV>(x);
+  else if constexpr (R == 9) dft_ct<3, 3, COMP, INV>(x);
+  else if constexpr (R == 12) dft_ct<4, 3, COMP, INV>(x);
+  else static_assert(R == 2, "register FFT radices: 2, 3, 4, 6, 8, 9, 12");
 }
 
 template <int N, int PASS, int NS, bool COMP, bool INV, bool STAGED, typename T>
 __device__ __forceinline__ void reg_pass(px2<T> (&v)[8], cx<T>* lds, int t, const cx<T>* twh, const cx<T>* twl) {
   using PL = RegPlan<N>;
+  static_assert(reg_all_regular<N>(), "the two-component form runs plans of radices 2, 4 and 8");
   if constexpr (PASS < PL::P) {
     constexpr int R = PL::rad[PASS];
     constexpr int B = 8 / R;
@@ -113,7 +222,7 @@ __device__ __forceinline__ void reg_pass(px2<T> (&v)[8], cx<T>* lds, int t, cons
 #pragma unroll
     for (int b = 0; b < B; ++b) {
       const int j = t + TT * b;
-      const int k = j & (NS - 1);
+      const int k = j % NS;
       cx<T> wh[R], wl[R];
       if constexpr (NS > 1) {
 #pragma unroll
@@ -160,23 +269,25 @@ __device__ __forceinline__ void reg_pass(px2<T> (&v)[8], cx<T>* lds, int t, cons
 }
 
 // One field component per thread: the same passes on v[m] = a[t + T·m] of a single complex line
-// (the slab-step kernels give the two components of a pixel to two adjacent lanes).
+// (the slab-step kernels give the two components of a pixel to two adjacent lanes). Regular passes
+// run from registers; an irregular pass runs from the LDS the previous pass left (see the file
+// comment), so a regular pass followed by an irregular one does not read its output back.
 template <int N, int PASS, int NS, bool COMP, bool INV, bool STAGED, typename T>
 __device__ __forceinline__ void reg_pass1(cx<T> (&v)[8], cx<T>* lds, int t, const cx<T>* twh, const cx<T>* twl) {
   using PL = RegPlan<N>;
   if constexpr (PASS < PL::P) {
     constexpr int R = PL::rad[PASS];
-    constexpr int B = 8 / R;
     constexpr int TT = N / 8;
+    constexpr int M = N / R;  // butterflies of the pass
     constexpr bool last = PASS == PL::P - 1;
+    constexpr bool regular = reg_regular(R);
     constexpr bool in_regs = last && R == 8;
-#pragma unroll
-    for (int b = 0; b < B; ++b) {
-      const int j = t + TT * b;
-      const int k = j & (NS - 1);
-      cx<T> x[R];
-#pragma unroll
-      for (int r = 0; r < R; ++r) x[r] = v[b + r * B];
+    // the next pass reads its inputs from LDS itself
+    constexpr bool next_lds = !last && !reg_regular(PL::rad[PASS + 1 < PL::P ? PASS + 1 : 0]);
+    static_assert(PASS > 0 || (regular && NS == 1), "the first pass takes natural order from registers");
+    // twiddled DFT of butterfly j on x, written to LDS (or kept in registers by the last pass)
+    auto butterfly = [&](int j, cx<T>* x) {
+      const int k = j % NS;
       if constexpr (NS > 1) {
 #pragma unroll
         for (int r = 1; r < R; ++r) {
@@ -188,20 +299,51 @@ __device__ __forceinline__ void reg_pass1(cx<T> (&v)[8], cx<T>* lds, int t, cons
         }
       }
       dft_r<R, COMP, INV>(x);
-      if constexpr (in_regs) {
-#pragma unroll
-        for (int r = 0; r < R; ++r) v[r] = x[r];
-      } else {
+      if constexpr (!in_regs) {
         const int base = (j - k) * R + k;
 #pragma unroll
         for (int r = 0; r < R; ++r) lds[lpad(base + r * NS)] = x[r];
       }
+    };
+    if constexpr (regular) {
+      constexpr int B = 8 / R;
+#pragma unroll
+      for (int b = 0; b < B; ++b) {
+        cx<T> x[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) x[r] = v[b + r * B];
+        butterfly(t + TT * b, x);
+        if constexpr (in_regs) {
+#pragma unroll
+          for (int r = 0; r < R; ++r) v[r] = x[r];
+        }
+      }
+    } else {
+      // butterflies j = t + T·b; the last round covers j < M only
+      constexpr int B = (M + TT - 1) / TT;
+      cx<T> x[B][R];
+#pragma unroll
+      for (int b = 0; b < B; ++b) {
+        const int j = t + TT * b;
+        if (b * TT + TT <= M || j < M) {
+#pragma unroll
+          for (int r = 0; r < R; ++r) x[b][r] = lds[lpad(j + r * M)];
+        }
+      }
+      __syncthreads();  // every input is read before the outputs overwrite the line
+#pragma unroll
+      for (int b = 0; b < B; ++b) {
+        const int j = t + TT * b;
+        if (b * TT + TT <= M || j < M) butterfly(j, x[b]);
+      }
     }
     if constexpr (!in_regs) {
       __syncthreads();
+      if constexpr (!next_lds) {
 #pragma unroll
-      for (int m = 0; m < 8; ++m) v[m] = lds[lpad(t + TT * m)];
-      __syncthreads();
+        for (int m = 0; m < 8; ++m) v[m] = lds[lpad(t + TT * m)];
+        __syncthreads();
+      }
     }
     reg_pass1<N, PASS + 1, NS * R, COMP, INV, STAGED>(v, lds, t, twh, twl);
   }

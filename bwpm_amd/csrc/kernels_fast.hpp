@@ -32,11 +32,12 @@ constexpr long long fast_lds_bytes(int n, int lines, int group, int csize, bool 
          static_cast<long long>(n - 8) * csize * (comp ? 2 : 1);
 }
 
-// Lines (columns or rows) per workgroup: 256 threads (n/4 per line) where the line allows, and at
-// most 52 KB of LDS, so that three workgroups share a CU and hide each other's memory and barrier
-// waits.
+// Lines (columns or rows) per workgroup: a power of two, 256 threads (n/4 per line) where the line
+// allows (192 for lines of 3·2^k points), and at most 52 KB of LDS, so that three workgroups share
+// a CU and hide each other's memory and barrier waits.
 constexpr int fast_lines(int n, bool columns, int csize, bool comp) {
-  int w = 1024 / n < 1 ? 1 : 1024 / n;
+  int w = 1;
+  while (2 * w * n <= 1024) w *= 2;
   while (w > 1 && fast_lds_bytes(n, 2 * w, columns ? col_group_lines(w) : kRowGroupLines, csize, comp) > 53248)
     w /= 2;
   return w;

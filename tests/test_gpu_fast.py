@@ -45,7 +45,10 @@ def _solver(a: dict, precision: str):
 
 @pytest.mark.parametrize("shape", [(64, 64), (128, 128), (256, 128), (512, 512), (512, 12), (12, 256), (4096, 1),
                                    (2048, 2), (3072, 1), (384, 96), (80, 120), (1000, 1), (2, 1024), (1, 4096),
-                                   (4, 2048)])
+                                   (4, 2048),
+                                   # lines of 3·2^k and 9·2^k points (irregular register passes)
+                                   (192, 192), (384, 384), (768, 64), (192, 1536), (1, 576), (1, 1152), (2, 3072),
+                                   (1536, 4), (576, 24), (1152, 2)])
 @pytest.mark.parametrize("precision", ["fp64", "mixed", "bfp", "fp32"])
 def test_register_kernels_match_oracle(shape: tuple[int, int], precision: str) -> None:
     ny, nx = shape
