@@ -108,6 +108,7 @@ _create_geom = _sig("bwpm_solver_create_geometry", _i64, _i64, _d, _d, _d, _d, _
 _q_submit_geom = _sig("bwpm_queue_submit_geometry", _p, _i64, _i64, _d, _d, _d, _d, _p, _i64, _i64, _d, _int, _p, _i64,
                       _d, _d, _p, _p, _d, _d, ctypes.c_uint64, _int, _p, _int, ctypes.POINTER(_i64))
 _graph_stats = _sig("bwpm_lane_graph_stats", ctypes.POINTER(_i64), ctypes.POINTER(_i64), ctypes.POINTER(_i64))
+_fft_plans = _sig("bwpm_fft_plans", ctypes.c_char_p, _i64, ctypes.POINTER(_i64))
 _solver_plane = _sig("bwpm_solver_plane", _p, _int, _i64, ctypes.POINTER(_p), ctypes.POINTER(_int))
 _batch = _sig("bwpm_batch_run", _p, _i64, _int, _int, _int, ctypes.POINTER(_d), ctypes.POINTER(_d),
                ctypes.POINTER(_d), ctypes.POINTER(_i64), ctypes.POINTER(_i64))
@@ -138,6 +139,21 @@ def lane_graph_stats() -> dict[str, int]:
     b, r, f = _i64(0), _i64(0), _i64(0)
     _check(_graph_stats(ctypes.byref(b), ctypes.byref(r), ctypes.byref(f)))
     return {"built": int(b.value), "replays": int(r.value), "fallbacks": int(f.value)}
+
+
+def fft_plans() -> list[dict]:
+    """The radix plans the plan tuner chose in this process: per precision code, direction and line
+    length, the plan taken (0: default, 1: alternative) and the timed launch of each (µs)."""
+    n = _i64(0)
+    _check(_fft_plans(None, 0, ctypes.byref(n)))
+    buf = ctypes.create_string_buffer(int(n.value) + 1)
+    _check(_fft_plans(buf, int(n.value) + 1, ctypes.byref(n)))
+    out = []
+    for line in buf.value.decode().splitlines():
+        prec, direction, length, plan, us0, us1 = line.split()
+        out.append({"precision": int(prec), "direction": direction, "n": int(length), "plan": int(plan),
+                    "us": [float(us0), float(us1)]})
+    return out
 
 
 def gpu_device_count() -> int:

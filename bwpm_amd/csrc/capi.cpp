@@ -1,4 +1,5 @@
 // extern "C" entry points of libbwpm_hip.so (declared in include/bwpm_hip.h).
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -591,6 +592,19 @@ int bwpm_lane_graph_stats(int64_t* built, int64_t* replays, int64_t* fallbacks) 
     *built = c.built.load();
     *replays = c.replays.load();
     *fallbacks = c.fallbacks.load();
+  });
+}
+
+int bwpm_fft_plans(char* buf, int64_t cap, int64_t* len) {
+  return guarded([&] {
+    bwpm::require(len != nullptr && cap >= 0 && (buf != nullptr || cap == 0), "null argument");
+    const std::string r = bwpm::fft_plan_report();
+    *len = static_cast<int64_t>(r.size());
+    if (cap > 0) {
+      const size_t n = std::min(static_cast<size_t>(cap - 1), r.size());
+      std::memcpy(buf, r.data(), n);
+      buf[n] = '\0';
+    }
   });
 }
 

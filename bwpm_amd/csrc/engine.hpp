@@ -161,6 +161,9 @@ struct LaneGraphCounters {
   std::atomic<long long> built{0}, replays{0}, fallbacks{0};
 };
 LaneGraphCounters& lane_graph_counters();
+// The radix plans the plan tuner chose in this process, one line per (precision, direction, line
+// length): "<precision> <col|row> <n> <plan> <µs of plan 0> <µs of plan 1>".
+std::string fft_plan_report();
 void run_continuous_lane_f32(LaneFeed& feed, int slots, int chunk, hipStream_t stream, bool compensated);
 void run_continuous_lane_f64(LaneFeed& feed, int slots, int chunk, hipStream_t stream);
 void run_continuous_lane_mixed(LaneFeed& feed, int slots, int chunk, hipStream_t stream);

@@ -181,4 +181,6 @@ double power_flux_noalloc(const c128* field, long long ny, long long nx, double 
                        reinterpret_cast<double2*>(b + kOffOut), ny, nx, dy, dx, k0, index, direction, s);
 }
 
+std::string fft_plan_report() { return plan_report(); }
+
 }  // namespace bwpm

@@ -126,6 +126,17 @@ inline auto with_fast_size(long long n, F&& f) {
     default: throw Error(kEinternal, "no register kernels for lines of " + std::to_string(n) + " points");
   }
 }
+
+// Calls f.template operator()<PL>() with radix plan `variant` of line length n (0: the default
+// plan, 1: the alternative; lengths with a single plan take it for either).
+template <typename F>
+inline auto with_fast_plan(long long n, int variant, F&& f) {
+  return with_fast_size(n, [&]<int N>() {
+    if constexpr (reg_plan_count<N>() > 1)
+      if (variant == 1) return f.template operator()<RegPlanV<N, 1>>();
+    return f.template operator()<RegPlanV<N, 0>>();
+  });
+}
 // columns / rows per workgroup of the register kernels (kernels_fast.hpp)
 template <typename T, bool COMP>
 constexpr int fast_w(int n) {
@@ -151,6 +162,9 @@ struct Geometry {
   // register grids: the plain steps take the persistent kernels of kernels_plain.hpp (else the
   // general register kernels); chosen per lane shape by the tuner
   bool plain_col = true, plain_row = true;
+  // register grids: the radix plan of the column and of the row transforms (0: default, 1: the
+  // alternative), fixed per device, precision and line length by the plan tuner
+  int plan_col = 0, plan_row = 0;
 };
 
 
@@ -350,7 +364,7 @@ struct Launcher {
 
   void col(const ColDesc<T, ST>* d, int S, Emit& s, bool multi) const {
     if (geo.col_fast && COMP == (sizeof(T) == 4)) {
-      with_fast_size(ny, [&]<int N>() { col_fast_n<N>(d, S, s, multi); });
+      with_fast_plan(ny, geo.plan_col, [&]<typename PL>() { col_fast_n<PL>(d, S, s, multi); });
       return;
     }
     // generic: one LDS buffer of ny × ls per block (products reload the spectrum from X)
@@ -367,7 +381,7 @@ struct Launcher {
 
   void row(const RowDesc<T, ST>* d, int S, Emit& s, bool multi) const {
     if (geo.row_fast && COMP == (sizeof(T) == 4)) {
-      with_fast_size(nx, [&]<int N>() { row_fast_n<N>(d, S, s, multi); });
+      with_fast_plan(nx, geo.plan_row, [&]<typename PL>() { row_fast_n<PL>(d, S, s, multi); });
       return;
     }
     const bool narrow = (narrow_mask(S) & 2) != 0;
@@ -389,25 +403,25 @@ struct Launcher {
     constexpr int cs = static_cast<int>(sizeof(cx<T>));
     return fast_lds_bytes(N, 2 * W, GROUP, cs, COMP) + 2LL * W * lds_stride(N, GROUP, cs) * cs <= 163840;
   }
-  template <int N, int W>
+  template <typename PL, int W>
   void col_launch(const ColDesc<T, ST>* d, int S, Emit& s, bool multi, const cx<T>* th, const cx<T>* tl) const {
     const dim3 grid(static_cast<unsigned>(nx / W), S);
-    if constexpr (stash_fits<N, W, col_group_lines(W)>())
+    if constexpr (stash_fits<PL::N, W, col_group_lines(W)>())
       if (multi) {
-        s(&k_col_fast<N, W, T, COMP, true, ST>, grid, dim3(N / 4 * W), 0, d, static_cast<int>(nx), npix, th, tl);
+        s(&k_col_fast<PL, W, T, COMP, true, ST>, grid, dim3(PL::N / 4 * W), 0, d, static_cast<int>(nx), npix, th, tl);
         return;
       }
-    s(&k_col_fast<N, W, T, COMP, false, ST>, grid, dim3(N / 4 * W), 0, d, static_cast<int>(nx), npix, th, tl);
+    s(&k_col_fast<PL, W, T, COMP, false, ST>, grid, dim3(PL::N / 4 * W), 0, d, static_cast<int>(nx), npix, th, tl);
   }
-  template <int N, int RW>
+  template <typename PL, int RW>
   void row_launch(const RowDesc<T, ST>* d, int S, Emit& s, bool multi, const cx<T>* th, const cx<T>* tl) const {
     const dim3 grid(static_cast<unsigned>(ny / RW), S);
-    if constexpr (stash_fits<N, RW, kRowGroupLines>())
+    if constexpr (stash_fits<PL::N, RW, kRowGroupLines>())
       if (multi) {
-        s(&k_row_fast<N, RW, T, COMP, true, ST>, grid, dim3(N / 4 * RW), 0, d, npix, th, tl);
+        s(&k_row_fast<PL, RW, T, COMP, true, ST>, grid, dim3(PL::N / 4 * RW), 0, d, npix, th, tl);
         return;
       }
-    s(&k_row_fast<N, RW, T, COMP, false, ST>, grid, dim3(N / 4 * RW), 0, d, npix, th, tl);
+    s(&k_row_fast<PL, RW, T, COMP, false, ST>, grid, dim3(PL::N / 4 * RW), 0, d, npix, th, tl);
   }
  public:
   // Plain steps (every solve one column product; one transmitted row product and no reflections,
@@ -415,69 +429,209 @@ struct Launcher {
   // has no register kernels for this direction.
   bool col_plain(const ColDesc<T, ST>* d, int S, Emit& s) const {
     if (!geo.col_fast || !geo.plain_col) return false;
-    return with_fast_size(ny, [&]<int N>() { return col_plain_n<N>(d, S, s); });
+    return with_fast_plan(ny, geo.plan_col, [&]<typename PL>() { return col_plain_n<PL>(d, S, s); });
   }
   bool row_plain(const RowDesc<T, ST>* d, int S, Emit& s) const {
     if (!geo.row_fast || !geo.plain_row) return false;
-    return with_fast_size(nx, [&]<int N>() { return row_plain_n<N>(d, S, s); });
+    return with_fast_plan(nx, geo.plan_row, [&]<typename PL>() { return row_plain_n<PL>(d, S, s); });
   }
 
  private:
-  template <int N, int W>
+  template <typename PL, int W>
   bool col_plain_launch(const ColDesc<T, ST>* d, int S, Emit& s) const {
-    auto kern = &k_col_plain<N, W, T, COMP, ST>;
+    auto kern = &k_col_plain<PL, W, T, COMP, ST>;
     static const bool fits = spill_free(reinterpret_cast<const void*>(kern));
     if (!fits) return false;
-    static const int cap = resident_groups(reinterpret_cast<const void*>(kern), N / 4 * W);
+    static const int cap = resident_groups(reinterpret_cast<const void*>(kern), PL::N / 4 * W);
     const int groups = static_cast<int>(nx / W);
     s(kern, dim3(static_cast<unsigned>(persistent_grid(static_cast<long long>(S) * groups, cap, kXcds))),
-      dim3(N / 4 * W), 0, d, S, groups, static_cast<int>(nx), twy->hi.template as<cx<T>>(),
+      dim3(PL::N / 4 * W), 0, d, S, groups, static_cast<int>(nx), twy->hi.template as<cx<T>>(),
       twy->lo.template as<cx<T>>());
     return true;
   }
-  template <int N, int RW>
+  template <typename PL, int RW>
   bool row_plain_launch(const RowDesc<T, ST>* d, int S, Emit& s) const {
-    auto kern = &k_row_plain<N, RW, T, COMP, ST>;
+    auto kern = &k_row_plain<PL, RW, T, COMP, ST>;
     static const bool fits = spill_free(reinterpret_cast<const void*>(kern));
     if (!fits) return false;
-    static const int cap = resident_groups(reinterpret_cast<const void*>(kern), N / 4 * RW);
+    static const int cap = resident_groups(reinterpret_cast<const void*>(kern), PL::N / 4 * RW);
     const int groups = static_cast<int>(ny / RW);
-    s(kern, dim3(static_cast<unsigned>(persistent_grid(static_cast<long long>(S) * groups, cap))), dim3(N / 4 * RW), 0,
+    s(kern, dim3(static_cast<unsigned>(persistent_grid(static_cast<long long>(S) * groups, cap))), dim3(PL::N / 4 * RW), 0,
       d, S, groups, twx->hi.template as<cx<T>>(), twx->lo.template as<cx<T>>());
     return true;
   }
-  template <int N>
+  template <typename PL>
   bool col_plain_n(const ColDesc<T, ST>* d, int S, Emit& s) const {
-    constexpr int W = fast_w<T, COMP>(N);
-    return geo.col_w == W ? col_plain_launch<N, W>(d, S, s) : col_plain_launch<N, 1>(d, S, s);
+    constexpr int W = fast_w<T, COMP>(PL::N);
+    return geo.col_w == W ? col_plain_launch<PL, W>(d, S, s) : col_plain_launch<PL, 1>(d, S, s);
   }
-  template <int N>
+  template <typename PL>
   bool row_plain_n(const RowDesc<T, ST>* d, int S, Emit& s) const {
-    constexpr int RW = fast_rw<T, COMP>(N);
-    return geo.rw == RW ? row_plain_launch<N, RW>(d, S, s) : row_plain_launch<N, 1>(d, S, s);
+    constexpr int RW = fast_rw<T, COMP>(PL::N);
+    return geo.rw == RW ? row_plain_launch<PL, RW>(d, S, s) : row_plain_launch<PL, 1>(d, S, s);
   }
 
-  template <int N>
+  template <typename PL>
   void col_fast_n(const ColDesc<T, ST>* d, int S, Emit& s, bool multi) const {
-    constexpr int W = fast_w<T, COMP>(N);
+    constexpr int W = fast_w<T, COMP>(PL::N);
     const cx<T>* th = twy->hi.template as<cx<T>>();
     const cx<T>* tl = twy->lo.template as<cx<T>>();
     if (geo.col_w == W)
-      col_launch<N, W>(d, S, s, multi, th, tl);
+      col_launch<PL, W>(d, S, s, multi, th, tl);
     else
-      col_launch<N, 1>(d, S, s, multi, th, tl);
+      col_launch<PL, 1>(d, S, s, multi, th, tl);
   }
-  template <int N>
+  template <typename PL>
   void row_fast_n(const RowDesc<T, ST>* d, int S, Emit& s, bool multi) const {
-    constexpr int RW = fast_rw<T, COMP>(N);
+    constexpr int RW = fast_rw<T, COMP>(PL::N);
     const cx<T>* th = twx->hi.template as<cx<T>>();
     const cx<T>* tl = twx->lo.template as<cx<T>>();
     if (geo.rw == RW)
-      row_launch<N, RW>(d, S, s, multi, th, tl);
+      row_launch<PL, RW>(d, S, s, multi, th, tl);
     else
-      row_launch<N, 1>(d, S, s, multi, th, tl);
+      row_launch<PL, 1>(d, S, s, multi, th, tl);
   }
 };
+
+// ---- radix-plan tuning ---------------------------------------------------------------------------
+
+// The radix plan of the register transforms (fft_reg.hpp's RegPlans: a default and an alternative
+// per line length) is chosen once per device, precision, line length and direction, before any
+// solve of that key runs. Every solve of a process then uses one plan, so results stay
+// bit-identical between a solve alone, in a lane and in the queue. The tuner times the persistent
+// plain kernel of each plan (or the general kernel where the plain one needs scratch memory) on
+// zeroed scratch planes: a lane of S solves of min(n, 512) lines of n points (at most 4 M pixels),
+// one warm launch and the best of three timed launches per plan, alternating.
+// BWPM_PLAN=0 (or 1) fixes the default (or the alternative) for every length.
+struct PlanKey {
+  int device = 0, precision = 0;
+  long long n = 0;
+  bool col = true;
+  bool operator<(const PlanKey& o) const {
+    return std::tie(device, precision, n, col) < std::tie(o.device, o.precision, o.n, o.col);
+  }
+};
+struct PlanChoice {
+  int variant = 0;
+  double us[2] = {-1.0, -1.0};  // timed launch of each plan (µs), −1: not timed
+};
+inline std::mutex& plan_mutex() {
+  static std::mutex m;
+  return m;
+}
+inline std::map<PlanKey, PlanChoice>& plan_cache() {
+  static std::map<PlanKey, PlanChoice> c;
+  return c;
+}
+
+// Every tuned choice as text: one line "<precision> <col|row> <n> <variant> <µs plan 0> <µs plan 1>".
+inline std::string plan_report() {
+  std::lock_guard<std::mutex> lock(plan_mutex());
+  std::string out;
+  for (const auto& [k, c] : plan_cache()) {
+    char line[160];
+    std::snprintf(line, sizeof(line), "%d %s %lld %d %.3f %.3f\n", k.precision, k.col ? "col" : "row", k.n,
+                  c.variant, c.us[0], c.us[1]);
+    out += line;
+  }
+  return out;
+}
+
+template <typename T, bool COMP, typename ST>
+int plan_pick(int device, int precision, long long n, bool col, const Geometry& g0, hipStream_t stream) {
+  if (!with_fast_size(n, [&]<int N>() { return reg_plan_count<N>() > 1; })) return 0;
+  if (const char* env = std::getenv("BWPM_PLAN")) {
+    if (env[0] == '0' || env[0] == '1') return env[0] - '0';
+  }
+  std::lock_guard<std::mutex> lock(plan_mutex());
+  const PlanKey key{device, precision, n, col};
+  auto it = plan_cache().find(key);
+  if (it != plan_cache().end()) return it->second.variant;
+
+  const long long lines = std::min<long long>(n, 512);
+  const int S = static_cast<int>(std::max<long long>(1, std::min<long long>(16, (4LL << 20) / (lines * n))));
+  Launcher<T, COMP, ST> L;
+  L.ny = col ? n : lines;
+  L.nx = col ? lines : n;
+  L.npix = L.ny * L.nx;
+  L.geo = g0;
+  if (col) {
+    L.geo.col_fast = true;
+    L.geo.plain_col = true;
+    const int w = fast_w<T, COMP>(static_cast<int>(n));
+    L.geo.col_w = (L.nx % w == 0) ? w : 1;
+  } else {
+    L.geo.row_fast = true;
+    L.geo.plain_row = true;
+    const int r = fast_rw<T, COMP>(static_cast<int>(n));
+    L.geo.rw = (L.ny % r == 0) ? r : 1;
+  }
+  auto tw = std::make_shared<Twiddles<T>>();
+  tw->build(n, stream);
+  L.twx = tw;
+  L.twy = tw;
+  const size_t plane = static_cast<size_t>(L.npix) * sizeof(px2<ST>);
+  DevBuf X, P, tab, lo, dprod, dcol, drow;
+  X.alloc(plane * S);
+  P.alloc(plane * S);
+  tab.alloc(static_cast<size_t>(L.npix) * sizeof(cx<T>));
+  lo.alloc(static_cast<size_t>(L.npix) * sizeof(cx<T>));
+  HIP_CHECK(hipMemsetAsync(X.as<void>(), 0, plane * S, stream));
+  HIP_CHECK(hipMemsetAsync(P.as<void>(), 0, plane * S, stream));
+  HIP_CHECK(hipMemsetAsync(tab.as<void>(), 0, tab.bytes(), stream));
+  HIP_CHECK(hipMemsetAsync(lo.as<void>(), 0, lo.bytes(), stream));
+  const std::vector<Prod<T>> prod{Prod<T>{0, tab.as<const cx<T>>(), lo.as<const cx<T>>()}};
+  dprod = upload_vec(prod, stream);
+  std::vector<ColDesc<T, ST>> cd(S);
+  std::vector<RowDesc<T, ST>> rd(S);
+  for (int i = 0; i < S; ++i) {
+    px2<ST>* xi = X.as<px2<ST>>() + static_cast<long long>(i) * L.npix;
+    px2<ST>* pi = P.as<px2<ST>>() + static_cast<long long>(i) * L.npix;
+    cd[i] = ColDesc<T, ST>{xi, pi, dprod.as<const Prod<T>>(), 1, 1, nullptr};
+    rd[i] = RowDesc<T, ST>{};
+    rd[i].active = 1;
+    rd[i].nt = 1;
+    rd[i].prod = pi;
+    rd[i].x_out = xi;
+  }
+  dcol = upload_vec(cd, stream);
+  drow = upload_vec(rd, stream);
+  auto run = [&](int v) {
+    Emit e;
+    e.stream = stream;
+    if (col) {
+      L.geo.plan_col = v;
+      if (!L.col_plain(dcol.as<const ColDesc<T, ST>>(), S, e)) L.col(dcol.as<const ColDesc<T, ST>>(), S, stream, false);
+    } else {
+      L.geo.plan_row = v;
+      if (!L.row_plain(drow.as<const RowDesc<T, ST>>(), S, e)) L.row(drow.as<const RowDesc<T, ST>>(), S, stream, false);
+    }
+    HIP_CHECK(hipGetLastError());
+  };
+  hipEvent_t e0, e1;
+  HIP_CHECK(hipEventCreate(&e0));
+  HIP_CHECK(hipEventCreate(&e1));
+  PlanChoice c;
+  run(0);
+  run(1);
+  for (int rep = 0; rep < 3; ++rep)
+    for (int v = 0; v < 2; ++v) {
+      HIP_CHECK(hipEventRecord(e0, stream));
+      run(v);
+      HIP_CHECK(hipEventRecord(e1, stream));
+      HIP_CHECK(hipEventSynchronize(e1));
+      float ms = 0.f;
+      HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+      const double us = 1e3 * ms;
+      if (c.us[v] < 0 || us < c.us[v]) c.us[v] = us;
+    }
+  HIP_CHECK(hipStreamSynchronize(stream));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  c.variant = c.us[1] < c.us[0] ? 1 : 0;
+  plan_cache()[key] = c;
+  return c.variant;
+}
 
 // A lane step: one column launch and one row launch over every solve of the lane, or a Born
 // sub-step of a single-solve lane.
@@ -1320,6 +1474,8 @@ class Engine final : public EngineBase {
       const int r = fast_rw<T, COMP>(static_cast<int>(s.nx));
       g.rw = (s.ny % r == 0) ? r : 1;
     }
+    if (g.col_fast) g.plan_col = plan_pick<T, COMP, ST>(s.device, precision_code(), s.ny, true, g, stream_);
+    if (g.row_fast) g.plan_row = plan_pick<T, COMP, ST>(s.device, precision_code(), s.nx, false, g, stream_);
     const int elems = Budget<T>::elems;
     auto ls_of = [](int lines) { return lines <= 2 ? lines : (lines | 1); };
     if (!g.col_fast) {

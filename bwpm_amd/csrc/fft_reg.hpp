@@ -22,6 +22,8 @@
 // LDS index i is padded to i + i/8 to spread the stride-8 writes of the first pass over banks.
 #pragma once
 
+#include <type_traits>
+
 #include "fft.hpp"
 
 namespace bwpm {
@@ -37,33 +39,55 @@ constexpr int lds_stride(int len, int group_lines, int csize) {
   return len + ((target - len % g) % g + g) % g;
 }
 
+// A radix plan: the passes' radices in order; N is their product.
+template <int... R>
+struct Plan {
+  static constexpr int P = sizeof...(R);
+  static constexpr int rad[P] = {R...};
+  static constexpr int N = (R * ...);
+};
+
+// Plans per line length: the default, and an alternative that the plan tuner times against it
+// (alt = type where a length has only one plan worth running). Every plan starts with a radix that
+// divides 8, so the first pass reads natural order from registers; every 3·2^k and 9·2^k plan has
+// one irregular pass.
 template <int N>
-struct RegPlan;
-template <> struct RegPlan<16> { static constexpr int P = 2; static constexpr int rad[P] = {8, 2}; };
-template <> struct RegPlan<32> { static constexpr int P = 2; static constexpr int rad[P] = {8, 4}; };
-template <> struct RegPlan<64> { static constexpr int P = 2; static constexpr int rad[P] = {8, 8}; };
-template <> struct RegPlan<128> { static constexpr int P = 3; static constexpr int rad[P] = {8, 2, 8}; };
-template <> struct RegPlan<256> { static constexpr int P = 3; static constexpr int rad[P] = {8, 4, 8}; };
-template <> struct RegPlan<512> { static constexpr int P = 3; static constexpr int rad[P] = {8, 8, 8}; };
-template <> struct RegPlan<1024> { static constexpr int P = 4; static constexpr int rad[P] = {8, 2, 8, 8}; };
-template <> struct RegPlan<2048> { static constexpr int P = 4; static constexpr int rad[P] = {8, 4, 8, 8}; };
-template <> struct RegPlan<4096> { static constexpr int P = 4; static constexpr int rad[P] = {8, 8, 8, 8}; };
-template <> struct RegPlan<192> { static constexpr int P = 3; static constexpr int rad[P] = {8, 3, 8}; };
-template <> struct RegPlan<384> { static constexpr int P = 3; static constexpr int rad[P] = {8, 6, 8}; };
-template <> struct RegPlan<576> { static constexpr int P = 3; static constexpr int rad[P] = {8, 9, 8}; };
-template <> struct RegPlan<768> { static constexpr int P = 3; static constexpr int rad[P] = {8, 12, 8}; };
-template <> struct RegPlan<1152> { static constexpr int P = 4; static constexpr int rad[P] = {8, 9, 2, 8}; };
-template <> struct RegPlan<1536> { static constexpr int P = 4; static constexpr int rad[P] = {8, 8, 3, 8}; };
-template <> struct RegPlan<3072> { static constexpr int P = 4; static constexpr int rad[P] = {8, 8, 6, 8}; };
+struct RegPlans;
+template <> struct RegPlans<16> { using type = Plan<8, 2>; using alt = type; };
+template <> struct RegPlans<32> { using type = Plan<8, 4>; using alt = type; };
+template <> struct RegPlans<64> { using type = Plan<8, 8>; using alt = type; };
+template <> struct RegPlans<128> { using type = Plan<8, 2, 8>; using alt = Plan<4, 4, 8>; };
+template <> struct RegPlans<256> { using type = Plan<8, 4, 8>; using alt = Plan<4, 8, 8>; };
+template <> struct RegPlans<512> { using type = Plan<8, 8, 8>; using alt = Plan<8, 4, 2, 8>; };
+template <> struct RegPlans<1024> { using type = Plan<8, 2, 8, 8>; using alt = Plan<4, 4, 8, 8>; };
+template <> struct RegPlans<2048> { using type = Plan<8, 4, 8, 8>; using alt = Plan<4, 8, 8, 8>; };
+template <> struct RegPlans<4096> { using type = Plan<8, 8, 8, 8>; using alt = type; };
+template <> struct RegPlans<192> { using type = Plan<8, 3, 8>; using alt = Plan<4, 6, 8>; };
+template <> struct RegPlans<384> { using type = Plan<8, 6, 8>; using alt = Plan<4, 12, 8>; };
+template <> struct RegPlans<576> { using type = Plan<8, 9, 8>; using alt = Plan<8, 3, 3, 8>; };
+template <> struct RegPlans<768> { using type = Plan<8, 12, 8>; using alt = Plan<8, 6, 2, 8>; };
+template <> struct RegPlans<1152> { using type = Plan<8, 9, 2, 8>; using alt = Plan<4, 4, 9, 8>; };
+template <> struct RegPlans<1536> { using type = Plan<8, 8, 3, 8>; using alt = Plan<4, 8, 6, 8>; };
+template <> struct RegPlans<3072> { using type = Plan<8, 8, 6, 8>; using alt = Plan<4, 8, 12, 8>; };
+
+template <int N>
+using RegPlan = typename RegPlans<N>::type;
+// Plan v (0: the default, 1: the alternative) of line length N.
+template <int N, int V>
+using RegPlanV = std::conditional_t<V == 0, typename RegPlans<N>::type, typename RegPlans<N>::alt>;
+template <int N>
+constexpr int reg_plan_count() {
+  return std::is_same_v<typename RegPlans<N>::type, typename RegPlans<N>::alt> ? 1 : 2;
+}
 
 // The size-16 and size-32 plans end with a radix below 8; their last pass writes through LDS too.
 
 // A pass whose radix divides the eight values a thread holds takes its inputs from registers.
 constexpr bool reg_regular(int radix) { return 8 % radix == 0; }
-template <int N>
-constexpr bool reg_all_regular() {
-  for (int p = 0; p < RegPlan<N>::P; ++p)
-    if (!reg_regular(RegPlan<N>::rad[p])) return false;
+template <typename PL>
+constexpr bool plan_all_regular() {
+  for (int p = 0; p < PL::P; ++p)
+    if (!reg_regular(PL::rad[p])) return false;
   return true;
 }
 
@@ -74,46 +98,61 @@ constexpr int reg_lds_len() {
 __device__ __forceinline__ int lpad(int i) { return i + (i >> 3); }
 
 // Staged twiddle layout: pass p ≥ 1 (stride Ns, radix R) keeps W_N^{r·k·N/(Ns·R)} at
-// reg_tw_off(p) + (r − 1)·Ns + k for r in [1, R), k in [0, Ns). In the table W_N^e the lanes of a
+// plan_tw_off(p) + (r − 1)·Ns + k for r in [1, R), k in [0, Ns). In the table W_N^e the lanes of a
 // pass read entries r·N/(Ns·R) apart (all in one LDS bank group in the second pass); here
-// consecutive k are consecutive entries. Pass 0 (Ns = 1) takes no twiddles.
-template <int N>
-constexpr int reg_tw_off(int pass) {
-  int off = 0, ns = RegPlan<N>::rad[0];
+// consecutive k are consecutive entries. Pass 0 (Ns = 1) takes no twiddles; the layout holds
+// N − rad[0] entries.
+template <typename PL>
+constexpr int plan_tw_off(int pass) {
+  int off = 0, ns = PL::rad[0];
   for (int p = 1; p < pass; ++p) {
-    off += (RegPlan<N>::rad[p] - 1) * ns;
-    ns *= RegPlan<N>::rad[p];
+    off += (PL::rad[p] - 1) * ns;
+    ns *= PL::rad[p];
   }
   return off;
 }
+template <typename PL>
+constexpr int plan_tw_len() {
+  return plan_tw_off<PL>(PL::P);
+}
+template <int N>
+constexpr int reg_tw_off(int pass) {
+  return plan_tw_off<RegPlan<N>>(pass);
+}
 template <int N>
 constexpr int reg_tw_len() {
-  return reg_tw_off<N>(RegPlan<N>::P);
+  return plan_tw_len<RegPlan<N>>();
 }
 
-template <int N, int PASS, int NS, bool COMP, typename T>
+template <typename PL, int PASS, int NS, bool COMP, typename T>
 __device__ __forceinline__ void stage_tw_pass(cx<T>* sh_hi, cx<T>* sh_lo, const cx<T>* __restrict__ g_hi,
                                               const cx<T>* __restrict__ g_lo) {
-  if constexpr (PASS < RegPlan<N>::P) {
-    constexpr int R = RegPlan<N>::rad[PASS];
-    constexpr int off = reg_tw_off<N>(PASS);
+  if constexpr (PASS < PL::P) {
+    constexpr int N = PL::N;
+    constexpr int R = PL::rad[PASS];
+    constexpr int off = plan_tw_off<PL>(PASS);
     for (int i = threadIdx.x; i < (R - 1) * NS; i += blockDim.x) {
       const int r = i / NS + 1, k = i % NS;
       const int e = r * k * (N / (NS * R));
       sh_hi[off + i] = g_hi[e];
       if constexpr (COMP) sh_lo[off + i] = g_lo[e];
     }
-    stage_tw_pass<N, PASS + 1, NS * R, COMP>(sh_hi, sh_lo, g_hi, g_lo);
+    stage_tw_pass<PL, PASS + 1, NS * R, COMP>(sh_hi, sh_lo, g_hi, g_lo);
   }
 }
 
-// Copies the twiddles of every pass from the table W_N^e (g_hi, and g_lo for COMP) into the staged
-// layout in LDS (reg_tw_len<N>() entries each); reg_fft reads them with STAGED.
+// Copies the twiddles of every pass of plan PL from the table W_N^e (g_hi, and g_lo for COMP) into
+// the staged layout in LDS (plan_tw_len<PL>() entries each); reg_fft reads them with STAGED.
+template <typename PL, bool COMP, typename T>
+__device__ __forceinline__ void stage_twiddles(cx<T>* sh_hi, cx<T>* sh_lo, const cx<T>* __restrict__ g_hi,
+                                               const cx<T>* __restrict__ g_lo) {
+  stage_tw_pass<PL, 1, PL::rad[0], COMP>(sh_hi, sh_lo, g_hi, g_lo);
+  __syncthreads();
+}
 template <int N, bool COMP, typename T>
 __device__ __forceinline__ void stage_twiddles(cx<T>* sh_hi, cx<T>* sh_lo, const cx<T>* __restrict__ g_hi,
                                                const cx<T>* __restrict__ g_lo) {
-  stage_tw_pass<N, 1, RegPlan<N>::rad[0], COMP>(sh_hi, sh_lo, g_hi, g_lo);
-  __syncthreads();
+  stage_twiddles<RegPlan<N>, COMP>(sh_hi, sh_lo, g_hi, g_lo);
 }
 
 // cos(2π·m/36) for any integer m, from cos(10°·i), i = 0 … 9 (correctly rounded literals).
@@ -212,7 +251,7 @@ __device__ __forceinline__ void dft_r(cx<T>* x) {
 template <int N, int PASS, int NS, bool COMP, bool INV, bool STAGED, typename T>
 __device__ __forceinline__ void reg_pass(px2<T> (&v)[8], cx<T>* lds, int t, const cx<T>* twh, const cx<T>* twl) {
   using PL = RegPlan<N>;
-  static_assert(reg_all_regular<N>(), "the two-component form runs plans of radices 2, 4 and 8");
+  static_assert(plan_all_regular<RegPlan<N>>(), "the two-component form runs plans of radices 2, 4 and 8");
   if constexpr (PASS < PL::P) {
     constexpr int R = PL::rad[PASS];
     constexpr int B = 8 / R;
@@ -272,9 +311,9 @@ __device__ __forceinline__ void reg_pass(px2<T> (&v)[8], cx<T>* lds, int t, cons
 // (the slab-step kernels give the two components of a pixel to two adjacent lanes). Regular passes
 // run from registers; an irregular pass runs from the LDS the previous pass left (see the file
 // comment), so a regular pass followed by an irregular one does not read its output back.
-template <int N, int PASS, int NS, bool COMP, bool INV, bool STAGED, typename T>
+template <typename PL, int PASS, int NS, bool COMP, bool INV, bool STAGED, typename T>
 __device__ __forceinline__ void reg_pass1(cx<T> (&v)[8], cx<T>* lds, int t, const cx<T>* twh, const cx<T>* twl) {
-  using PL = RegPlan<N>;
+  constexpr int N = PL::N;
   if constexpr (PASS < PL::P) {
     constexpr int R = PL::rad[PASS];
     constexpr int TT = N / 8;
@@ -291,7 +330,7 @@ __device__ __forceinline__ void reg_pass1(cx<T> (&v)[8], cx<T>* lds, int t, cons
       if constexpr (NS > 1) {
 #pragma unroll
         for (int r = 1; r < R; ++r) {
-          const int e = STAGED ? reg_tw_off<N>(PASS) + (r - 1) * NS + k : r * k * (N / (NS * R));
+          const int e = STAGED ? plan_tw_off<PL>(PASS) + (r - 1) * NS + k : r * k * (N / (NS * R));
           const cx<T> wh = INV ? conj(twh[e]) : twh[e];
           cx<T> wl{};
           if constexpr (COMP) wl = INV ? conj(twl[e]) : twl[e];
@@ -345,7 +384,7 @@ __device__ __forceinline__ void reg_pass1(cx<T> (&v)[8], cx<T>* lds, int t, cons
         __syncthreads();
       }
     }
-    reg_pass1<N, PASS + 1, NS * R, COMP, INV, STAGED>(v, lds, t, twh, twl);
+    reg_pass1<PL, PASS + 1, NS * R, COMP, INV, STAGED>(v, lds, t, twh, twl);
   }
 }
 
@@ -363,7 +402,13 @@ __device__ __forceinline__ void reg_fft(px2<T> (&v)[8], cx<T>* lds, int t, const
 }
 template <int N, bool COMP, bool INV, bool STAGED = false, typename T>
 __device__ __forceinline__ void reg_fft(cx<T> (&v)[8], cx<T>* lds, int t, const cx<T>* twh, const cx<T>* twl) {
-  reg_pass1<N, 0, 1, COMP, INV, STAGED>(v, lds, t, twh, twl);
+  reg_pass1<RegPlan<N>, 0, 1, COMP, INV, STAGED>(v, lds, t, twh, twl);
+}
+// The same with an explicit plan PL (one component per thread; STAGED twiddles from
+// stage_twiddles<PL>).
+template <typename PL, bool COMP, bool INV, bool STAGED = false, typename T>
+__device__ __forceinline__ void reg_fft(cx<T> (&v)[8], cx<T>* lds, int t, const cx<T>* twh, const cx<T>* twl) {
+  reg_pass1<PL, 0, 1, COMP, INV, STAGED>(v, lds, t, twh, twl);
 }
 
 }  // namespace bwpm

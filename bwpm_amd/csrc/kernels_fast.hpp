@@ -25,11 +25,11 @@ constexpr int col_group_lines(int w) { return 2 * w; }
 constexpr int kRowGroupLines = 2;
 
 // Static LDS of a kernel with `lines` component lines of n points (complex size csize), `group`
-// of them side by side in a bank group, with the staged twiddles (n − 8 entries for plans that
-// start with radix 8: reg_tw_len).
+// of them side by side in a bank group, with room for the staged twiddles (n − rad[0] entries:
+// plan_tw_len, at most n).
 constexpr long long fast_lds_bytes(int n, int lines, int group, int csize, bool comp) {
   return static_cast<long long>(lines) * lds_stride(n + n / 8, group, csize) * csize +
-         static_cast<long long>(n - 8) * csize * (comp ? 2 : 1);
+         static_cast<long long>(n) * csize * (comp ? 2 : 1);
 }
 
 // Lines (columns or rows) per workgroup: a power of two, 256 threads (n/4 per line) where the line
@@ -98,22 +98,23 @@ __device__ __forceinline__ cx<T> apply_prod1(const Prod<T>& pd, unsigned k, int 
 // an LDS stash across the products instead of writing them back to X and reloading them. The
 // arithmetic is the same, so both forms give the same bits where X is stored in the arithmetic
 // precision (ST = T); with narrower storage only the LDS stash keeps the spectrum unrounded.
-template <int N, int W, typename T, bool COMP, bool MULTI, typename ST = T>
-__global__ __launch_bounds__(N / 4 * W) void k_col_fast(const ColDesc<T, ST>* __restrict__ descs, int nx,
+template <typename PL, int W, typename T, bool COMP, bool MULTI, typename ST = T>
+__global__ __launch_bounds__(PL::N / 4 * W) void k_col_fast(const ColDesc<T, ST>* __restrict__ descs, int nx,
                                                         long long plane, const cx<T>* __restrict__ g_twh,
                                                         const cx<T>* __restrict__ g_twl) {
+  constexpr int N = PL::N;
   constexpr int TT = N / 8;
   constexpr int CS = static_cast<int>(sizeof(cx<T>));
   constexpr int LS = lds_stride(reg_lds_len<N>(), col_group_lines(W), CS);
   constexpr int SS = lds_stride(N, col_group_lines(W), CS);  // stash line stride
   constexpr bool TWL = fast_tw_in_lds<N, 2 * W, col_group_lines(W), T, COMP>();
   __shared__ cx<T> sh[2 * W * LS];
-  __shared__ cx<T> tw_hi_s[TWL ? reg_tw_len<N>() : 1];
-  __shared__ cx<T> tw_lo_s[TWL && COMP ? reg_tw_len<N>() : 1];
+  __shared__ cx<T> tw_hi_s[TWL ? plan_tw_len<PL>() : 1];
+  __shared__ cx<T> tw_lo_s[TWL && COMP ? plan_tw_len<PL>() : 1];
   __shared__ cx<T> stash[MULTI ? 2 * W * SS : 1];
   const ColDesc<T, ST> d = descs[blockIdx.y];
   if (!d.in) return;
-  if constexpr (TWL) stage_twiddles<N, COMP>(tw_hi_s, tw_lo_s, g_twh, g_twl);
+  if constexpr (TWL) stage_twiddles<PL, COMP>(tw_hi_s, tw_lo_s, g_twh, g_twl);
   const cx<T>* tw_hi = TWL ? tw_hi_s : g_twh;
   const cx<T>* tw_lo = TWL ? tw_lo_s : g_twl;
   const int comp = threadIdx.x & 1;
@@ -139,7 +140,7 @@ __global__ __launch_bounds__(N / 4 * W) void k_col_fast(const ColDesc<T, ST>* __
   cx<T> s[8];
 #pragma unroll
   for (int m = 0; m < 8; ++m) s[m] = sload<T>(xio, at(m));
-  reg_fft<N, COMP, false, TWL>(s, line, t, tw_hi, tw_lo);
+  reg_fft<PL, COMP, false, TWL>(s, line, t, tw_hi, tw_lo);
   // several products: keep the spectrum (in the LDS stash, or in the thread's own positions of X);
   // stash entry m of this lane is own[m·T] on its component line
   cx<T>* own = stash;
@@ -167,7 +168,7 @@ __global__ __launch_bounds__(N / 4 * W) void k_col_fast(const ColDesc<T, ST>* __
     const Prod<T> pd{uniform(gload1(&pp->type, 0)), uniform(gload1(&pp->hi, 0)), uniform(gload1(&pp->lo, 0))};
 #pragma unroll
     for (int m = 0; m < 8; ++m) s[m] = apply_prod1<T, COMP>(pd, pix(m), comp, s[m]);
-    reg_fft<N, COMP, true, TWL>(s, line, t, tw_hi, tw_lo);
+    reg_fft<PL, COMP, true, TWL>(s, line, t, tw_hi, tw_lo);
     px2<ST>* out = d.out + o * plane;
     if constexpr (MULTI) {
       // a product is stored only in the rows whose row transforms use it: rows with a pixel that
@@ -188,22 +189,23 @@ __global__ __launch_bounds__(N / 4 * W) void k_col_fast(const ColDesc<T, ST>* __
 // Row kernel: RW rows per workgroup, N/8 threads per row and component (thread index
 // (r·N/8 + t)·2 + comp). MULTI (launched for steps with several transmitted products): the
 // selected pixels of each product are merged in an LDS stash instead of product plane 0.
-template <int N, int RW, typename T, bool COMP, bool MULTI, typename ST = T>
-__global__ __launch_bounds__(N / 4 * RW) void k_row_fast(const RowDesc<T, ST>* __restrict__ descs, long long plane,
+template <typename PL, int RW, typename T, bool COMP, bool MULTI, typename ST = T>
+__global__ __launch_bounds__(PL::N / 4 * RW) void k_row_fast(const RowDesc<T, ST>* __restrict__ descs, long long plane,
                                                          const cx<T>* __restrict__ g_twh,
                                                          const cx<T>* __restrict__ g_twl) {
+  constexpr int N = PL::N;
   constexpr int TT = N / 8;
   constexpr int CS = static_cast<int>(sizeof(cx<T>));
   constexpr int LS = lds_stride(reg_lds_len<N>(), kRowGroupLines, CS);
   constexpr int SS = lds_stride(N, kRowGroupLines, CS);  // stash line stride
   constexpr bool TWL = fast_tw_in_lds<N, 2 * RW, kRowGroupLines, T, COMP>();
   __shared__ cx<T> sh[2 * RW * LS];
-  __shared__ cx<T> tw_hi_s[TWL ? reg_tw_len<N>() : 1];
-  __shared__ cx<T> tw_lo_s[TWL && COMP ? reg_tw_len<N>() : 1];
+  __shared__ cx<T> tw_hi_s[TWL ? plan_tw_len<PL>() : 1];
+  __shared__ cx<T> tw_lo_s[TWL && COMP ? plan_tw_len<PL>() : 1];
   __shared__ cx<T> stash[MULTI ? 2 * RW * SS : 1];
   const RowDesc<T, ST> a = descs[blockIdx.y];
   if (!a.active) return;
-  if constexpr (TWL) stage_twiddles<N, COMP>(tw_hi_s, tw_lo_s, g_twh, g_twl);
+  if constexpr (TWL) stage_twiddles<PL, COMP>(tw_hi_s, tw_lo_s, g_twh, g_twl);
   const cx<T>* tw_hi = TWL ? tw_hi_s : g_twh;
   const cx<T>* tw_lo = TWL ? tw_lo_s : g_twl;
   const int comp = threadIdx.x & 1;
@@ -238,7 +240,7 @@ __global__ __launch_bounds__(N / 4 * RW) void k_row_fast(const RowDesc<T, ST>* _
       const px2<ST>* src = a.prod + (a.nt + o) * plane;
 #pragma unroll
       for (int m = 0; m < 8; ++m) acc[m] = sload<T>(src, at(m));
-      reg_fft<N, COMP, true, TWL>(acc, line, t, tw_hi, tw_lo);
+      reg_fft<PL, COMP, true, TWL>(acc, line, t, tw_hi, tw_lo);
 #pragma unroll
       for (int m = 0; m < 8; ++m) line[lpad(t + TT * m)] = acc[m];
       __syncthreads();
@@ -258,7 +260,7 @@ __global__ __launch_bounds__(N / 4 * RW) void k_row_fast(const RowDesc<T, ST>* _
     const int o = a.nt == 1 ? 0 : __builtin_ctz(static_cast<unsigned>(use) & kRowUseT);
 #pragma unroll
     for (int m = 0; m < 8; ++m) acc[m] = sload<T>(a.prod + o * plane, at(m));
-    reg_fft<N, COMP, true, TWL>(acc, line, t, tw_hi, tw_lo);
+    reg_fft<PL, COMP, true, TWL>(acc, line, t, tw_hi, tw_lo);
   } else {
     // the selected pixels are collected in the LDS stash or in product plane 0
     px2<ST>* merged = const_cast<px2<ST>*>(a.prod);
@@ -269,7 +271,7 @@ __global__ __launch_bounds__(N / 4 * RW) void k_row_fast(const RowDesc<T, ST>* _
       const px2<ST>* src = a.prod + o * plane;
 #pragma unroll
       for (int m = 0; m < 8; ++m) acc[m] = sload<T>(src, at(m));
-      reg_fft<N, COMP, true, TWL>(acc, line, t, tw_hi, tw_lo);
+      reg_fft<PL, COMP, true, TWL>(acc, line, t, tw_hi, tw_lo);
 #pragma unroll
       for (int m = 0; m < 8; ++m)
         if (gload1(a.tsel, row + t + TT * m) == o) {
@@ -336,7 +338,7 @@ __global__ __launch_bounds__(N / 4 * RW) void k_row_fast(const RowDesc<T, ST>* _
 
   // 5. row FFT for the next slab step
   if (a.x_out) {
-    reg_fft<N, COMP, false, TWL>(acc, line, t, tw_hi, tw_lo);
+    reg_fft<PL, COMP, false, TWL>(acc, line, t, tw_hi, tw_lo);
 #pragma unroll
     for (int m = 0; m < 8; ++m) sstore<T>(a.x_out, at(m), acc[m]);
   }

@@ -50,17 +50,18 @@ __device__ __host__ __forceinline__ int xcd_tile(int u, int groups) {
   return (k / g8) * groups + (u & 7) * g8 + k % g8;
 }
 
-template <int N, int W, typename T, bool COMP, typename ST>
-__global__ __launch_bounds__(N / 4 * W) void k_col_plain(const ColDesc<T, ST>* __restrict__ descs, int S,
+template <typename PL, int W, typename T, bool COMP, typename ST>
+__global__ __launch_bounds__(PL::N / 4 * W) void k_col_plain(const ColDesc<T, ST>* __restrict__ descs, int S,
                                                          int groups, int nx, const cx<T>* __restrict__ g_twh,
                                                          const cx<T>* __restrict__ g_twl) {
+  constexpr int N = PL::N;
   constexpr int TT = N / 8;
   constexpr int CS = static_cast<int>(sizeof(cx<T>));
   constexpr int LS = lds_stride(reg_lds_len<N>(), col_group_lines(W), CS);
   constexpr bool TWL = fast_tw_in_lds<N, 2 * W, col_group_lines(W), T, COMP>();
   __shared__ cx<T> sh[2 * W * LS];
-  __shared__ cx<T> tw_hi_s[TWL ? reg_tw_len<N>() : 1];
-  __shared__ cx<T> tw_lo_s[TWL && COMP ? reg_tw_len<N>() : 1];
+  __shared__ cx<T> tw_hi_s[TWL ? plan_tw_len<PL>() : 1];
+  __shared__ cx<T> tw_lo_s[TWL && COMP ? plan_tw_len<PL>() : 1];
   const int ntiles = S * groups;
   // the next tile (dispatch index) of this workgroup whose solve has column work; tile indices are
   // uniform, so descriptors are read with scalar loads (which do not wait behind vector loads)
@@ -71,7 +72,7 @@ __global__ __launch_bounds__(N / 4 * W) void k_col_plain(const ColDesc<T, ST>* _
   };
   int u = advance(static_cast<int>(blockIdx.x));
   if (u >= ntiles) return;
-  if constexpr (TWL) stage_twiddles<N, COMP>(tw_hi_s, tw_lo_s, g_twh, g_twl);
+  if constexpr (TWL) stage_twiddles<PL, COMP>(tw_hi_s, tw_lo_s, g_twh, g_twl);
   const cx<T>* tw_hi = TWL ? tw_hi_s : g_twh;
   const cx<T>* tw_lo = TWL ? tw_lo_s : g_twl;
   const int comp = threadIdx.x & 1;
@@ -101,7 +102,7 @@ __global__ __launch_bounds__(N / 4 * W) void k_col_plain(const ColDesc<T, ST>* _
     cx<T> v[8];
 #pragma unroll
     for (int m = 0; m < 8; ++m) v[m] = widen<T>(cur[m]);
-    reg_fft<N, COMP, false, TWL>(v, line, t, tw_hi, tw_lo);
+    reg_fft<PL, COMP, false, TWL>(v, line, t, tw_hi, tw_lo);
     // the phase values are loaded before the next tile's prefetch is issued, so that waiting for
     // them does not wait for the prefetch (loads complete in order); the prefetch then overlaps the
     // operator, the inverse transform and the stores. The host launches this kernel for phase
@@ -123,7 +124,7 @@ __global__ __launch_bounds__(N / 4 * W) void k_col_plain(const ColDesc<T, ST>* _
 #pragma unroll
       for (int m = 0; m < 8; ++m) v[m] = cmul2<COMP>(v[m], ph[m], pl[m]);
     }
-    reg_fft<N, COMP, true, TWL>(v, line, t, tw_hi, tw_lo);
+    reg_fft<PL, COMP, true, TWL>(v, line, t, tw_hi, tw_lo);
 #pragma unroll
     for (int m = 0; m < 8; ++m) sstore<T>(out, at(x, m), v[m]);
     u = un;
@@ -138,17 +139,18 @@ __global__ __launch_bounds__(N / 4 * W) void k_col_plain(const ColDesc<T, ST>* _
   }
 }
 
-template <int N, int RW, typename T, bool COMP, typename ST>
-__global__ __launch_bounds__(N / 4 * RW) void k_row_plain(const RowDesc<T, ST>* __restrict__ descs, int S,
+template <typename PL, int RW, typename T, bool COMP, typename ST>
+__global__ __launch_bounds__(PL::N / 4 * RW) void k_row_plain(const RowDesc<T, ST>* __restrict__ descs, int S,
                                                           int groups, const cx<T>* __restrict__ g_twh,
                                                           const cx<T>* __restrict__ g_twl) {
+  constexpr int N = PL::N;
   constexpr int TT = N / 8;
   constexpr int CS = static_cast<int>(sizeof(cx<T>));
   constexpr int LS = lds_stride(reg_lds_len<N>(), kRowGroupLines, CS);
   constexpr bool TWL = fast_tw_in_lds<N, 2 * RW, kRowGroupLines, T, COMP>();
   __shared__ cx<T> sh[2 * RW * LS];
-  __shared__ cx<T> tw_hi_s[TWL ? reg_tw_len<N>() : 1];
-  __shared__ cx<T> tw_lo_s[TWL && COMP ? reg_tw_len<N>() : 1];
+  __shared__ cx<T> tw_hi_s[TWL ? plan_tw_len<PL>() : 1];
+  __shared__ cx<T> tw_lo_s[TWL && COMP ? plan_tw_len<PL>() : 1];
   const int ntiles = S * groups;
   auto advance = [&](int u) {
     for (u = uniform(u); u < ntiles; u = uniform(u + static_cast<int>(gridDim.x)))
@@ -157,7 +159,7 @@ __global__ __launch_bounds__(N / 4 * RW) void k_row_plain(const RowDesc<T, ST>* 
   };
   int u = advance(static_cast<int>(blockIdx.x));
   if (u >= ntiles) return;
-  if constexpr (TWL) stage_twiddles<N, COMP>(tw_hi_s, tw_lo_s, g_twh, g_twl);
+  if constexpr (TWL) stage_twiddles<PL, COMP>(tw_hi_s, tw_lo_s, g_twh, g_twl);
   const cx<T>* tw_hi = TWL ? tw_hi_s : g_twh;
   const cx<T>* tw_lo = TWL ? tw_lo_s : g_twl;
   const int comp = threadIdx.x & 1;
@@ -198,7 +200,7 @@ __global__ __launch_bounds__(N / 4 * RW) void k_row_plain(const RowDesc<T, ST>* 
     cx<T> acc[8];
 #pragma unroll
     for (int m = 0; m < 8; ++m) acc[m] = widen<T>(cur[m]);
-    reg_fft<N, COMP, true, TWL>(acc, line, t, tw_hi, tw_lo);
+    reg_fft<PL, COMP, true, TWL>(acc, line, t, tw_hi, tw_lo);
     if (mask_y || mask) {
 #pragma unroll
       for (int m = 0; m < 8; ++m) acc[m] = acc[m] * fac[m];
@@ -214,7 +216,7 @@ __global__ __launch_bounds__(N / 4 * RW) void k_row_plain(const RowDesc<T, ST>* 
       for (int m = 0; m < 8; ++m) gstore(o, at(y, m), acc[m]);
     }
     if (x_out) {
-      reg_fft<N, COMP, false, TWL>(acc, line, t, tw_hi, tw_lo);
+      reg_fft<PL, COMP, false, TWL>(acc, line, t, tw_hi, tw_lo);
 #pragma unroll
       for (int m = 0; m < 8; ++m) sstore<T>(x_out, at(y, m), acc[m]);
     }

@@ -239,6 +239,13 @@ int bwpm_queue_stats(bwpm_queue* q, double* out);
  * capture), graph replays, and lanes that ran their chunks as direct launches because a graph
  * could not be built (no counterpart in the reference). */
 int bwpm_lane_graph_stats(int64_t* built, int64_t* replays, int64_t* fallbacks);
+/* The radix plans the plan tuner chose in this process for the register FFT kernels, as text: one
+ * line "<precision code> <col|row> <line length> <plan 0|1> <µs of plan 0> <µs of plan 1>" per
+ * tuned key. Writes at most cap − 1 characters and a terminating NUL into buf (cap may be 0 to
+ * query the length) and the full length into *len. Plans are chosen once per device, precision,
+ * line length and direction before the first solve of that key; BWPM_PLAN=0 or 1 fixes them (no
+ * counterpart in the reference, whose FFTs are pocketfft/cuFFT plans). */
+int bwpm_fft_plans(char* buf, int64_t cap, int64_t* len);
 
 /* ---- analysis reductions over planes (the step after the path) ---------------------------------
  *

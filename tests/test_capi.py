@@ -46,6 +46,16 @@ def test_binding_covers_the_reference_module_surface() -> None:
     assert list(inspect.signature(_core.CudaSolver).parameters)[:12] == head + ["device", "fluctuation"]
 
 
+def test_fft_plan_report_needs_no_device() -> None:
+    """The plan tuner's report is host state: empty before any solve, readable without a GPU."""
+    from bwpm_amd import _core
+
+    plans = _core.fft_plans()
+    assert isinstance(plans, list)
+    for p in plans:
+        assert p["plan"] in (0, 1) and p["direction"] in ("col", "row")
+
+
 def test_invalid_arguments_raise_value_error_before_any_device_work() -> None:
     from bwpm_amd import _core
 

@@ -74,6 +74,51 @@ def test_register_kernels_match_oracle(shape: tuple[int, int], precision: str) -
             assert rel_l2(got, want) < (1e-6 if precision in ("mixed", "bfp") else 1e-5)
 
 
+@pytest.mark.parametrize("shape", [(128, 128), (256, 128), (512, 512), (1024, 2), (2048, 2), (192, 192), (384, 384),
+                                   (768, 64), (1, 576), (1, 1152), (192, 1536), (2, 3072)])
+@pytest.mark.parametrize("precision", ["fp64", "mixed", "fp32"])
+def test_alternative_radix_plans_match_oracle(shape: tuple[int, int], precision: str, monkeypatch) -> None:
+    """BWPM_PLAN=1 runs every register transform on its length's alternative radix plan (the one the
+    plan tuner times against the default): the same results as the oracle."""
+    monkeypatch.setenv("BWPM_PLAN", "1")
+    ny, nx = shape
+    nz = 7
+    a = layered_volume(ny, nx, nz, seed=3 * ny + nx)
+    ref = O.Solver(a["ny"], a["nx"], a["dy"], a["dx"], a["dz"], a["k0"], a["indices"], a["labels"],
+                   a["index_before"], a["mask"]).run(a["field"], 2)
+    out = _solver(a, precision).run(a["field"], 2)
+    scale = float(np.max(np.abs(a["field"])))
+    for key in ("transmitted", "reflected"):
+        got, want = out[key], ref[key]
+        if precision == "fp64":
+            assert np.max(np.abs(got - want)) < 1e-10 * scale, key
+        elif np.max(np.abs(want)) < 1e-6 * scale:
+            assert np.max(np.abs(got)) < 1e-6 * scale, key
+        else:
+            assert rel_l2(got, want) < (1e-6 if precision == "mixed" else 1e-5), key
+
+
+def test_plan_tuner_times_both_plans() -> None:
+    """Without BWPM_PLAN the tuner times both radix plans of a length once, before its first solve,
+    and every later solve of that length takes the faster one: solves after the tuning give the same
+    bits as each other."""
+    import os
+
+    from bwpm_amd import _core
+
+    assert "BWPM_PLAN" not in os.environ
+    a = layered_volume(384, 384, 6, seed=11)
+    first = _solver(a, "mixed").run(a["field"], 1)
+    plans = [p for p in _core.fft_plans() if p["n"] == 384 and p["precision"] == 48]
+    assert {p["direction"] for p in plans} == {"col", "row"}
+    for p in plans:
+        assert p["us"][0] > 0 and p["us"][1] > 0
+        assert p["plan"] == (1 if p["us"][1] < p["us"][0] else 0)
+    again = _solver(a, "mixed").run(a["field"], 1)
+    for key in ("transmitted", "reflected"):
+        assert np.array_equal(first[key], again[key])
+
+
 @pytest.mark.parametrize("precision", ["fp32", "mixed", "bfp", "fp64"])
 def test_batch_equals_alone_bitwise(precision: str) -> None:
     """Solves of two grid sizes, uneven depths and different geometries: the batched lanes give
