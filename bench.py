@@ -271,10 +271,13 @@ def configs(c3_nz: int, c4_count: int, precision: str, which: str = "c1,c2,c3,c4
             for pol in ("x", "y")}
     if "c3" in sel and c3_nz > 0:
         # the δn volume is generated on the device (a (1024, 512, 512) complex array would be
-        # 4 GiB on the host)
+        # 4 GiB on the host); the transverse edges carry the reference's absorbing zone of 20
+        # wavelengths (its validity rule): in the bare periodic window the iteration diverges
+        # from the fourth iteration on, in FP64 as in every narrow mode
         out[f"c3_born_512x512x{c3_nz}_it6"] = {
-            **with_modes(workloads.born_3d_synth(n=512, nz=c3_nz, iterations=6)),
-            "input": "Gaussian-correlated δn (σ = 0.01, correlation 1 µm, seed 3) generated on the device"}
+            **with_modes(workloads.born_3d_synth(n=512, nz=c3_nz, iterations=6, absorber_wavelengths=20.0)),
+            "input": "Gaussian-correlated δn (σ = 0.01, correlation 1 µm, seed 3) generated on the device; "
+                     "20-wavelength absorbing zone"}
     if c4_count > 0 and "c4" in sel:
         out[f"c4_uneven_born_{c4_count}"] = uneven_batch(c4_count, precision)
     return out

@@ -155,7 +155,9 @@ __global__ __launch_bounds__(N / 8 * RW) void k_born_op_rows_inv(const px2<T>* _
                                                                 const T* __restrict__ kxT, const T* __restrict__ kyT,
                                                                 T d, px2<T>* __restrict__ xy, cx<T>* __restrict__ zl,
                                                                 const cx<T>* __restrict__ g_twh,
-                                                                const cx<T>* __restrict__ g_twl) {
+                                                                const cx<T>* __restrict__ g_twl,
+                                                                const int* __restrict__ skip) {
+  if (skip && *skip) return;
   constexpr int TT = N / 8;
   constexpr int LQ = lds_stride(2 * reg_lds_len<N>(), 1, sizeof(cx<T>));  // a line: E_x and E_y planes
   __shared__ cx<T> sh[RW * LQ];
@@ -205,7 +207,9 @@ template <int N, int W, typename T, bool COMP>
 __global__ __launch_bounds__(N / 8 * W) void k_born_op_cols(px2<T>* xy, cx<T>* zl, int nx,
                                                            const cx<T>* __restrict__ pot, T scale,
                                                            const cx<T>* __restrict__ g_twh,
-                                                           const cx<T>* __restrict__ g_twl) {
+                                                           const cx<T>* __restrict__ g_twl,
+                                                           const int* __restrict__ skip) {
+  if (skip && *skip) return;
   constexpr int TT = N / 8;
   constexpr int LQ = lds_stride(2 * reg_lds_len<N>(), W, sizeof(cx<T>));  // a line: E_x and E_y planes
   __shared__ cx<T> sh[W * LQ];
@@ -250,7 +254,9 @@ __global__ __launch_bounds__(N / 8 * RW) void k_born_op_rows_fwd(const px2<T>* _
                                                                 T d, cx<T> inv_k2, const px2<T>* __restrict__ v,
                                                                 px2<T>* __restrict__ out, int mode,
                                                                 const cx<T>* __restrict__ g_twh,
-                                                                const cx<T>* __restrict__ g_twl) {
+                                                                const cx<T>* __restrict__ g_twl,
+                                                                const int* __restrict__ skip) {
+  if (skip && *skip) return;
   constexpr int TT = N / 8;
   constexpr int LQ = lds_stride(2 * reg_lds_len<N>(), 1, sizeof(cx<T>));  // a line: E_x and E_y planes
   __shared__ cx<T> sh[RW * LQ];
@@ -347,14 +353,18 @@ __global__ void k_born_emit(const px2<T>* __restrict__ xy0, const px2<T>* __rest
 inline constexpr int kRedBlocks = 512;
 inline constexpr int kRedThreads = 256;
 
-__global__ __launch_bounds__(kRedThreads) void k_dot_final(const double2* __restrict__ partial, double2* __restrict__ out);
+// Kernels with a `skip` flag (nullptr: never) return at once while *skip is set: the Arnoldi steps
+// that the host enqueues past a GMRES cycle's convergence (which the device detects) do no work.
+__global__ __launch_bounds__(kRedThreads) void k_dot_final(const double2* __restrict__ partial, double2* __restrict__ out,
+                                                           const int* __restrict__ skip);
 
 // Partial dots <B_i, v> of the basis vectors B_i = basis + i·stride (blockIdx.y = i); the same
 // summation order as k_dot_partial.
 template <typename T>
 __global__ __launch_bounds__(kRedThreads) void k_dot_basis(const px2<T>* __restrict__ basis, long long stride,
                                                            const px2<T>* __restrict__ v, long long npix,
-                                                           double2* __restrict__ partial) {
+                                                           double2* __restrict__ partial, const int* __restrict__ skip) {
+  if (skip && *skip) return;
   __shared__ double sr[kRedThreads], si[kRedThreads];
   const px2<T>* u = basis + blockIdx.y * stride;
   double re = 0, im = 0;
@@ -397,6 +407,30 @@ __global__ void k_sub_basis(px2<T>* w, const px2<T>* __restrict__ basis, long lo
 
 // Largest basis the fused projections below take (the restart length plus one fits).
 inline constexpr int kMaxBasis = 32;
+
+// GMRES restart length (solver.cpp:341) and the layout of the reduction results of one Arnoldi
+// step: h1 at [0, k), h2 at [kGmH2, kGmH2 + k), ‖w‖² at kGmN2.
+inline constexpr int kGmRestart = 30;
+inline constexpr int kGmH2 = kGmRestart + 2;
+inline constexpr int kGmN2 = 2 * (kGmRestart + 2);
+
+// Device state of one GMRES restart cycle. The Givens rotations, the rotated Hessenberg matrix and
+// the residual vector g are updated on the device after every Arnoldi step (k_gm_givens), so the
+// host enqueues Arnoldi steps without waiting for them and reads this state once per chunk of
+// steps. `done` gates the later steps of the cycle (their kernels take &done as `skip`).
+struct GmState {
+  int done;   // converged (|g[used]| ≤ tol·‖b‖), broken down (‖w‖ = 0), or ‖r₀‖ ≤ tol·‖b‖ at the start
+  int used;   // Arnoldi steps of the cycle
+  double beta;  // ‖r₀‖
+  double2 g[kGmRestart + 1];
+  double2 cs[kGmRestart], sn[kGmRestart];
+  double2 h[(kGmRestart + 1) * kGmRestart];  // row i, column j at i·kGmRestart + j (rotated)
+};
+
+// Arnoldi step j's Hessenberg column from the reductions `res` (h1 + h2, ‖w‖), the previous
+// rotations applied to it, the new rotation, and the convergence test against tol_abs = tol·‖b‖
+// (solver.cpp:160-190). One thread; returns at once when the cycle is done.
+__global__ void k_gm_givens(GmState* st, const double2* __restrict__ res, int j, double tol_abs);
 
 // Sum over the 64 lanes of a wave in a fixed butterfly order; every lane receives the same value.
 __device__ __forceinline__ double wave_sum(double v) {
@@ -473,7 +507,9 @@ template <typename T>
 __global__ __launch_bounds__(kRedThreads) void k_sub_dot_basis(px2<T>* w, const px2<T>* __restrict__ basis,
                                                                long long stride, const double2* __restrict__ c,
                                                                int k, long long npix,
-                                                               double2* __restrict__ partial) {
+                                                               double2* __restrict__ partial,
+                                                               const int* __restrict__ skip) {
+  if (skip && *skip) return;
   double ar[kMaxBasis], ai[kMaxBasis];
 #pragma unroll
   for (int i = 0; i < kMaxBasis; ++i) ar[i] = ai[i] = 0;
@@ -510,7 +546,9 @@ template <typename T>
 __global__ __launch_bounds__(kRedThreads) void k_sub_norm_basis(px2<T>* w, const px2<T>* __restrict__ basis,
                                                                 long long stride, const double2* __restrict__ c,
                                                                 int k, long long npix,
-                                                                double2* __restrict__ partial) {
+                                                                double2* __restrict__ partial,
+                                                                const int* __restrict__ skip) {
+  if (skip && *skip) return;
   double ar[kMaxBasis], ai[kMaxBasis];
 #pragma unroll
   for (int i = 0; i < kMaxBasis; ++i) ar[i] = ai[i] = 0;
@@ -539,7 +577,8 @@ template <typename T>
 __global__ __launch_bounds__(kRedThreads) void k_normalize_sum(const px2<T>* __restrict__ w,
                                                                const double2* __restrict__ npart,
                                                                double2* __restrict__ res, px2<T>* out,
-                                                               long long npix) {
+                                                               long long npix, const int* __restrict__ skip) {
+  if (skip && *skip) return;
   __shared__ double2 n2[1];
   final_sums(npart, 1, n2, res);
   const long long q = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;
@@ -549,6 +588,29 @@ __global__ __launch_bounds__(kRedThreads) void k_normalize_sum(const px2<T>* __r
   const T s = static_cast<T>(1.0 / nrm);
   const px2<T> u = w[q];
   out[q] = px2<T>{u.a * s, u.b * s};
+}
+
+// Start of a GMRES cycle: β = ‖r₀‖ from the partial sums `npart` of <r₀, r₀> (the second stage in
+// every workgroup, as k_normalize_sum), the cycle state (g = β·e₀, done if β ≤ tol_abs), and
+// r₀ ← r₀/β in place (the first basis vector).
+template <typename T>
+__global__ __launch_bounds__(kRedThreads) void k_gm_begin(px2<T>* r0, const double2* __restrict__ npart, GmState* st,
+                                                          double tol_abs, long long npix) {
+  __shared__ double2 n2[1];
+  final_sums(npart, 1, n2, nullptr);
+  const double beta = sqrt(n2[0].x);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->beta = beta;
+    st->used = 0;
+    st->done = beta <= tol_abs ? 1 : 0;
+    st->g[0] = double2{beta, 0.0};
+    for (int i = 1; i <= kGmRestart; ++i) st->g[i] = double2{0.0, 0.0};
+  }
+  const long long q = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;
+  if (q >= npix || !(beta > 0.0)) return;
+  const T sc = static_cast<T>(1.0 / beta);
+  const px2<T> u = r0[q];
+  r0[q] = px2<T>{u.a * sc, u.b * sc};
 }
 
 // out = a·x + b·y (real a, b).
@@ -709,6 +771,8 @@ class Born {
     dres_.alloc((2 * (kRestart + 2) + 1) * sizeof(double2));
     coef_.alloc((kRestart + 2) * sizeof(double2));
     HIP_CHECK(hipHostMalloc(&hres_, (2 * (kRestart + 2) + 1) * sizeof(double2), hipHostMallocDefault));
+    gm_.alloc(sizeof(GmState));
+    HIP_CHECK(hipHostMalloc(&hgm_, sizeof(GmState), hipHostMallocDefault));
     if (s.synth.on) synthesise(s, host);
     fft_.init(ny_, nx_, px_, py_, stream_);
     HIP_CHECK(hipStreamSynchronize(stream_));
@@ -716,6 +780,7 @@ class Born {
   }
   ~Born() {
     if (hres_) (void)hipHostFree(hres_);
+    if (hgm_) (void)hipHostFree(hgm_);
   }
   Born(const Born&) = delete;
   Born& operator=(const Born&) = delete;
@@ -758,7 +823,7 @@ class Born {
   }
 
  private:
-  static constexpr int kRestart = 30;      // solver.cpp:341
+  static constexpr int kRestart = kGmRestart;  // solver.cpp:341
   // Workgroup size of the fused operator passes: 128 threads ran a 512² FP64 Born solve 2 % faster
   // than 256 and 7 % faster than 64.
   static constexpr int kOpThreads = 128;
@@ -799,10 +864,12 @@ class Born {
   // ---- the operators --------------------------------------------------------------------------------
 
   // out = G_d·v (mode 0) or (1 − G_d/2)·v (mode 1) (solver.cpp:262-311).
-  void apply_g(const px2<T>* v, int d, long long slab, px2<T>* out, int mode) {
+  // `skip` (nullptr: none) gates the fused passes (kernels with a skip flag); the generic path runs
+  // regardless, and its output is then unused.
+  void apply_g(const px2<T>* v, int d, long long slab, px2<T>* out, int mode, const int* skip = nullptr) {
     if constexpr (COMP == (sizeof(T) == 4)) {
       if (fft_.fast()) {
-        apply_g_fused(v, d, slab, out, mode);
+        apply_g_fused(v, d, slab, out, mode, skip);
         return;
       }
     }
@@ -819,35 +886,35 @@ class Born {
   }
 
   // The same operator in three fused passes (power-of-two grids in the register kernels' precision).
-  void apply_g_fused(const px2<T>* v, int d, long long slab, px2<T>* out, int mode) {
+  void apply_g_fused(const px2<T>* v, int d, long long slab, px2<T>* out, int mode, const int* skip) {
     px2<T>* xy = xy0_.template as<px2<T>>();
     cx<T>* zl = zz_.template as<cx<T>>();
     const cx<T>* ot = optab_.template as<cx<T>>() + 3 * slab_tab_[slab] * npix_;
     const BornOp<T> op{ot, ot + npix_, ot + 2 * npix_};
     const T dd = static_cast<T>(d);
     switch (nx_) {
-      case 64: op_rows<64>(v, op, dd, xy, zl, inv_k2(slab), out, mode, false); break;
-      case 128: op_rows<128>(v, op, dd, xy, zl, inv_k2(slab), out, mode, false); break;
-      case 256: op_rows<256>(v, op, dd, xy, zl, inv_k2(slab), out, mode, false); break;
-      case 512: op_rows<512>(v, op, dd, xy, zl, inv_k2(slab), out, mode, false); break;
-      case 1024: op_rows<1024>(v, op, dd, xy, zl, inv_k2(slab), out, mode, false); break;
-      default: op_rows<2048>(v, op, dd, xy, zl, inv_k2(slab), out, mode, false); break;
+      case 64: op_rows<64>(v, op, dd, xy, zl, inv_k2(slab), out, mode, false, skip); break;
+      case 128: op_rows<128>(v, op, dd, xy, zl, inv_k2(slab), out, mode, false, skip); break;
+      case 256: op_rows<256>(v, op, dd, xy, zl, inv_k2(slab), out, mode, false, skip); break;
+      case 512: op_rows<512>(v, op, dd, xy, zl, inv_k2(slab), out, mode, false, skip); break;
+      case 1024: op_rows<1024>(v, op, dd, xy, zl, inv_k2(slab), out, mode, false, skip); break;
+      default: op_rows<2048>(v, op, dd, xy, zl, inv_k2(slab), out, mode, false, skip); break;
     }
     switch (ny_) {
-      case 64: op_cols<64>(xy, zl, potential(slab)); break;
-      case 128: op_cols<128>(xy, zl, potential(slab)); break;
-      case 256: op_cols<256>(xy, zl, potential(slab)); break;
-      case 512: op_cols<512>(xy, zl, potential(slab)); break;
-      case 1024: op_cols<1024>(xy, zl, potential(slab)); break;
-      default: op_cols<2048>(xy, zl, potential(slab)); break;
+      case 64: op_cols<64>(xy, zl, potential(slab), skip); break;
+      case 128: op_cols<128>(xy, zl, potential(slab), skip); break;
+      case 256: op_cols<256>(xy, zl, potential(slab), skip); break;
+      case 512: op_cols<512>(xy, zl, potential(slab), skip); break;
+      case 1024: op_cols<1024>(xy, zl, potential(slab), skip); break;
+      default: op_cols<2048>(xy, zl, potential(slab), skip); break;
     }
     switch (nx_) {
-      case 64: op_rows<64>(v, op, dd, xy, zl, inv_k2(slab), out, mode, true); break;
-      case 128: op_rows<128>(v, op, dd, xy, zl, inv_k2(slab), out, mode, true); break;
-      case 256: op_rows<256>(v, op, dd, xy, zl, inv_k2(slab), out, mode, true); break;
-      case 512: op_rows<512>(v, op, dd, xy, zl, inv_k2(slab), out, mode, true); break;
-      case 1024: op_rows<1024>(v, op, dd, xy, zl, inv_k2(slab), out, mode, true); break;
-      default: op_rows<2048>(v, op, dd, xy, zl, inv_k2(slab), out, mode, true); break;
+      case 64: op_rows<64>(v, op, dd, xy, zl, inv_k2(slab), out, mode, true, skip); break;
+      case 128: op_rows<128>(v, op, dd, xy, zl, inv_k2(slab), out, mode, true, skip); break;
+      case 256: op_rows<256>(v, op, dd, xy, zl, inv_k2(slab), out, mode, true, skip); break;
+      case 512: op_rows<512>(v, op, dd, xy, zl, inv_k2(slab), out, mode, true, skip); break;
+      case 1024: op_rows<1024>(v, op, dd, xy, zl, inv_k2(slab), out, mode, true, skip); break;
+      default: op_rows<2048>(v, op, dd, xy, zl, inv_k2(slab), out, mode, true, skip); break;
     }
   }
   // Lines per workgroup of the fused passes: `threads` threads per workgroup, at least one line.
@@ -858,7 +925,7 @@ class Born {
   // Pass 1 (forward = false) or pass 3 (forward = true) of the fused operator; N = nx.
   template <int N>
   void op_rows(const px2<T>* v, const BornOp<T>& op, T d, px2<T>* xy, cx<T>* zl, cx<T> ik2, px2<T>* out, int mode,
-               bool forward) {
+               bool forward, const int* skip) {
     constexpr int RW = op_lines<N>(kOpThreads);
     static_assert(RW <= 64, "a power-of-two ny of at least 64 holds whole workgroups");
     const FftPlan& p = fft_.row_plan();
@@ -868,21 +935,21 @@ class Born {
     if (forward)
       hipLaunchKernelGGL((k_born_op_rows_fwd<N, RW, T, COMP>), dim3(rows), dim3(N / 8 * RW), 0, stream_,
                          static_cast<const px2<T>*>(xy), static_cast<const cx<T>*>(zl), op, kxT_.template as<T>(),
-                         kyT_.template as<T>(), d, ik2, v, out, mode, th, tl);
+                         kyT_.template as<T>(), d, ik2, v, out, mode, th, tl, skip);
     else
       hipLaunchKernelGGL((k_born_op_rows_inv<N, RW, T, COMP>), dim3(rows, 2), dim3(N / 8 * RW), 0, stream_, v, op,
-                         kxT_.template as<T>(), kyT_.template as<T>(), d, xy, zl, th, tl);
+                         kxT_.template as<T>(), kyT_.template as<T>(), d, xy, zl, th, tl, skip);
     HIP_CHECK(hipGetLastError());
   }
   // Pass 2 of the fused operator; N = ny.
   template <int N>
-  void op_cols(px2<T>* xy, cx<T>* zl, const cx<T>* pot) {
+  void op_cols(px2<T>* xy, cx<T>* zl, const cx<T>* pot, const int* skip) {
     constexpr int W = op_lines<N>(kOpThreads);
     static_assert(W <= 64, "a power-of-two nx of at least 64 holds whole workgroups");
     const FftPlan& p = fft_.col_plan();
     hipLaunchKernelGGL((k_born_op_cols<N, W, T, COMP>), dim3(static_cast<unsigned>(nx_ / W), 2), dim3(N / 8 * W), 0,
                        stream_, xy, zl, static_cast<int>(nx_), pot, static_cast<T>(1.0 / static_cast<double>(npix_)),
-                       static_cast<const cx<T>*>(p.tw_hi), static_cast<const cx<T>*>(p.tw_lo));
+                       static_cast<const cx<T>*>(p.tw_hi), static_cast<const cx<T>*>(p.tw_lo), skip);
     HIP_CHECK(hipGetLastError());
   }
 
@@ -925,9 +992,11 @@ class Born {
   double norm(const px2<T>* v) {
     double2* res = dres_.template as<double2>();
     hipLaunchKernelGGL(k_dot_basis<T>, dim3(kRedBlocks, 1), dim3(kRedThreads), 0, stream_, v,
-                       static_cast<long long>(npix_), v, npix_, partial_.template as<double2>());
+                       static_cast<long long>(npix_), v, npix_, partial_.template as<double2>(),
+                       static_cast<const int*>(nullptr));
     HIP_CHECK(hipGetLastError());
-    hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(kRedThreads), 0, stream_, partial_.template as<double2>(), res);
+    hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(kRedThreads), 0, stream_, partial_.template as<double2>(), res,
+                       static_cast<const int*>(nullptr));
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipMemcpyAsync(hres_, res, sizeof(double2), hipMemcpyDeviceToHost, stream_));
     HIP_CHECK(hipStreamSynchronize(stream_));
@@ -980,7 +1049,7 @@ class Born {
                        static_cast<const cx<T>*>(v), n, partial_.template as<double2>());
     HIP_CHECK(hipGetLastError());
     hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(kRedThreads), 0, stream_, partial_.template as<double2>(),
-                       dres_.template as<double2>());
+                       dres_.template as<double2>(), static_cast<const int*>(nullptr));
     HIP_CHECK(hipGetLastError());
     double2 mom{};
     HIP_CHECK(hipMemcpyAsync(&mom, dres_.template as<void>(), sizeof(double2), hipMemcpyDeviceToHost, stream_));
@@ -1001,45 +1070,44 @@ class Born {
     HIP_CHECK(hipStreamSynchronize(stream_));
   }
 
-  // One Arnoldi step after w = G·B_j: classical Gram–Schmidt twice against B_0 … B_j and the new
-  // basis vector B_{j+1} = w/‖w‖, all on the device; the host receives the two projections and
-  // ‖w‖² in one copy (hv = h1 + h2, the Hessenberg column) and returns ‖w‖.
-  double arnoldi(int j, px2<T>* w, std::complex<double>* hv) {
+  // One Arnoldi step after w = G·B_j, enqueued only: classical Gram–Schmidt twice against
+  // B_0 … B_j and the next basis vector B_{j+1} = w/‖w‖, with the two projections and ‖w‖² left in
+  // dres_ (h1, h2, ‖w‖²), then the Hessenberg column's rotations and the convergence test
+  // (k_gm_givens). Every kernel is gated by the cycle's `done` flag.
+  void arnoldi(int j, px2<T>* w, const int* skip) {
     const int k = j + 1;
     const px2<T>* basis = basis_.template as<px2<T>>();
     double2* part = partial_.template as<double2>();
     double2* res = dres_.template as<double2>();
-    constexpr int kH2 = kRestart + 2, kN2 = 2 * (kRestart + 2);  // offsets of h2 and ‖w‖² in res
-    auto dot = [&](const px2<T>* base, int n, const px2<T>* v, double2* out) {
-      hipLaunchKernelGGL(k_dot_basis<T>, dim3(kRedBlocks, n), dim3(kRedThreads), 0, stream_, base,
-                         static_cast<long long>(npix_), v, npix_, part);
-      HIP_CHECK(hipGetLastError());
-      hipLaunchKernelGGL(k_dot_final, dim3(n), dim3(kRedThreads), 0, stream_, part, out);
-      HIP_CHECK(hipGetLastError());
-    };
     static_assert(kRestart + 1 <= kMaxBasis, "the fused projections hold every basis vector");
-    dot(basis, k, w, res);
+    hipLaunchKernelGGL(k_dot_basis<T>, dim3(kRedBlocks, k), dim3(kRedThreads), 0, stream_, basis,
+                       static_cast<long long>(npix_), static_cast<const px2<T>*>(w), npix_, part, skip);
+    HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_dot_final, dim3(k), dim3(kRedThreads), 0, stream_, static_cast<const double2*>(part), res, skip);
+    HIP_CHECK(hipGetLastError());
     // w −= B·h1 with the dots h2 = Bᴴw of the result, then w −= B·h2 with ‖w‖², then B_{j+1}
     // (the normalisation finishes the norm's reduction itself)
     hipLaunchKernelGGL(k_sub_dot_basis<T>, dim3(kRedBlocks), dim3(kRedThreads), 0, stream_, w, basis,
-                       static_cast<long long>(npix_), static_cast<const double2*>(res), k, npix_, part);
+                       static_cast<long long>(npix_), static_cast<const double2*>(res), k, npix_, part, skip);
     HIP_CHECK(hipGetLastError());
-    hipLaunchKernelGGL(k_dot_final, dim3(k), dim3(kRedThreads), 0, stream_, part, res + kH2);
+    hipLaunchKernelGGL(k_dot_final, dim3(k), dim3(kRedThreads), 0, stream_, static_cast<const double2*>(part),
+                       res + kGmH2, skip);
     HIP_CHECK(hipGetLastError());
     hipLaunchKernelGGL(k_sub_norm_basis<T>, dim3(kRedBlocks), dim3(kRedThreads), 0, stream_, w, basis,
-                       static_cast<long long>(npix_), static_cast<const double2*>(res + kH2), k, npix_, part);
+                       static_cast<long long>(npix_), static_cast<const double2*>(res + kGmH2), k, npix_, part, skip);
     HIP_CHECK(hipGetLastError());
-    launch(k_normalize_sum<T>, static_cast<const px2<T>*>(w), static_cast<const double2*>(part), res + kN2, B(j + 1),
-           npix_);
-    HIP_CHECK(hipMemcpyAsync(hres_, res, (kN2 + 1) * sizeof(double2), hipMemcpyDeviceToHost, stream_));
-    HIP_CHECK(hipStreamSynchronize(stream_));
-    for (int i = 0; i < k; ++i)
-      hv[i] = std::complex<double>(hres_[i].x, hres_[i].y) + std::complex<double>(hres_[kH2 + i].x, hres_[kH2 + i].y);
-    return std::sqrt(hres_[kN2].x);
+    launch(k_normalize_sum<T>, static_cast<const px2<T>*>(w), static_cast<const double2*>(part), res + kGmN2, B(j + 1),
+           npix_, skip);
+    hipLaunchKernelGGL(k_gm_givens, dim3(1), dim3(64), 0, stream_, gm_.template as<GmState>(),
+                       static_cast<const double2*>(res), j, tol_abs_);
+    HIP_CHECK(hipGetLastError());
   }
 
   // Restarted GMRES on (1 − G/2)x = b with b = rhs_, x = x_ (start value), following the structure
-  // of the reference's solver (solver.cpp:130-207) with block Gram–Schmidt.
+  // of the reference's solver (solver.cpp:130-207) with block Gram–Schmidt. The Arnoldi steps of a
+  // cycle and their Givens rotations run on the device; the host enqueues them in chunks, the
+  // first as long as the last cycle that converged, and reads the cycle state once per chunk
+  // instead of once per step. Steps enqueued past convergence skip their work.
   bool gmres(int d, long long slab) {
     const px2<T>* b = rhs_.template as<px2<T>>();
     px2<T>* x = x_.template as<px2<T>>();
@@ -1050,59 +1118,51 @@ class Born {
              static_cast<T>(0), x, npix_);
       return true;
     }
+    tol_abs_ = tol_ * bnorm;
     const int m = kRestart;
-    std::vector<std::complex<double>> h((m + 1) * m), g(m + 1), cs(m), sn(m), hv(m + 1);
+    GmState* st = gm_.template as<GmState>();
+    const int* skip = &st->done;
+    double2* part = partial_.template as<double2>();
+    std::vector<std::complex<double>> y(m);
     for (int outer = 0; outer < kMaxRestarts; ++outer) {
       apply_g(x, d, slab, w, 1);
       launch(k_axpby<T>, b, static_cast<T>(1), static_cast<const px2<T>*>(w), static_cast<T>(-1), B(0), npix_);
-      const double beta = norm(B(0));
-      if (beta <= tol_ * bnorm) return true;
-      launch(k_axpby<T>, static_cast<const px2<T>*>(B(0)), static_cast<T>(1.0 / beta),
-             static_cast<const px2<T>*>(nullptr), static_cast<T>(0), B(0), npix_);
-      std::fill(g.begin(), g.end(), std::complex<double>(0.0, 0.0));
-      g[0] = beta;
-      int used = 0;
-      bool done = false;
-      for (int j = 0; j < m; ++j) {
-        apply_g(B(j), d, slab, w, 1);
-        ++iterations_;
-        // classical Gram–Schmidt, twice, and the next basis vector (device)
-        const double hnext = arnoldi(j, w, hv.data());
-        for (int i = 0; i <= j; ++i) h[i * m + j] = hv[i];
-        for (int i = 0; i < j; ++i) {  // previous rotations on column j
-          const std::complex<double> a = h[i * m + j], c = h[(i + 1) * m + j];
-          h[i * m + j] = cs[i] * a + sn[i] * c;
-          h[(i + 1) * m + j] = -std::conj(sn[i]) * a + cs[i] * c;
+      hipLaunchKernelGGL(k_dot_basis<T>, dim3(kRedBlocks, 1), dim3(kRedThreads), 0, stream_,
+                         static_cast<const px2<T>*>(B(0)), static_cast<long long>(npix_),
+                         static_cast<const px2<T>*>(B(0)), npix_, part, static_cast<const int*>(nullptr));
+      HIP_CHECK(hipGetLastError());
+      launch(k_gm_begin<T>, B(0), static_cast<const double2*>(part), st, tol_abs_, npix_);
+      int j = 0;
+      for (;;) {
+        const int chunk = std::min(m - j, std::max(1, (j == 0 ? expect_ : 1)));
+        for (int c = 0; c < chunk; ++c, ++j) {
+          apply_g(B(j), d, slab, w, 1, skip);
+          arnoldi(j, w, skip);
         }
-        const std::complex<double> a = h[j * m + j];
-        const double r = std::sqrt(std::norm(a) + hnext * hnext);
-        if (std::abs(a) == 0.0) {
-          cs[j] = 0.0;
-          sn[j] = 1.0;
-        } else {
-          cs[j] = std::abs(a) / r;
-          sn[j] = (a / std::abs(a)) * hnext / r;
-        }
-        h[j * m + j] = cs[j] * a + sn[j] * hnext;
-        g[j + 1] = -std::conj(sn[j]) * g[j];
-        g[j] = cs[j] * g[j];
-        used = j + 1;
-        if (std::abs(g[j + 1]) <= tol_ * bnorm || hnext == 0.0) {
-          done = true;
-          break;
-        }
+        HIP_CHECK(hipMemcpyAsync(hgm_, st, sizeof(GmState), hipMemcpyDeviceToHost, stream_));
+        HIP_CHECK(hipStreamSynchronize(stream_));
+        if (hgm_->done || j >= m) break;
       }
-      std::vector<std::complex<double>> y(used);
+      const int used = hgm_->used;
+      const bool done = hgm_->done != 0;
+      iterations_ += used;
+      if (used == 0) {
+        if (done) return true;  // ‖r₀‖ ≤ tol·‖b‖
+        continue;
+      }
+      if (done) expect_ = used;
+      // y = R⁻¹g on the rotated Hessenberg matrix (host; used ≤ 30)
       for (int i = used - 1; i >= 0; --i) {
-        std::complex<double> s = g[i];
-        for (int k = i + 1; k < used; ++k) s -= h[i * m + k] * y[k];
-        y[i] = s / h[i * m + i];
+        std::complex<double> sacc(hgm_->g[i].x, hgm_->g[i].y);
+        for (int k = i + 1; k < used; ++k)
+          sacc -= std::complex<double>(hgm_->h[i * m + k].x, hgm_->h[i * m + k].y) * y[k];
+        y[i] = sacc / std::complex<double>(hgm_->h[i * m + i].x, hgm_->h[i * m + i].y);
       }
       add_basis(x, y.data(), used);
       if (done) {
         apply_g(x, d, slab, w, 1);
         launch(k_axpby<T>, b, static_cast<T>(1), static_cast<const px2<T>*>(w), static_cast<T>(-1), w, npix_);
-        if (norm(w) <= tol_ * bnorm * 10.0) return true;
+        if (norm(w) <= tol_abs_ * 10.0) return true;
       }
     }
     return false;
@@ -1139,6 +1199,10 @@ class Born {
   Dev ehat_, xh_, xy0_, xy1_, zz_, rhs_, x_, w_, tmp_, inc_, basis_, planes_;
   Dev partial_, dres_, coef_;
   double2* hres_ = nullptr;
+  Dev gm_;                      // GmState of the current GMRES cycle
+  GmState* hgm_ = nullptr;      // its host copy (pinned)
+  double tol_abs_ = 0.0;        // tol·‖b‖ of the current solve
+  int expect_ = 4;              // Arnoldi steps of the last converged cycle (the first chunk's length)
   double2 host_coef_[kRestart + 2] = {};
   Fft2D<T, COMP> fft_;
   long long iterations_ = 0;

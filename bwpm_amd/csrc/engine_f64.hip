@@ -71,7 +71,7 @@ double flux_of_plane(px2<double>* plane, const double* dkx, const double* dky, c
   hipLaunchKernelGGL(k_flux_partial, dim3(kRedBlocks), dim3(kRedThreads), 0, s, plane, dkx, dky, static_cast<int>(nx),
                      npix, zd{index.real(), index.imag()}, k0, static_cast<double>(direction), partial);
   HIP_CHECK(hipGetLastError());
-  hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(kRedThreads), 0, s, partial, out);
+  hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(kRedThreads), 0, s, partial, out, nullptr);
   HIP_CHECK(hipGetLastError());
   double2 h{};
   HIP_CHECK(hipMemcpyAsync(&h, out, sizeof(double2), hipMemcpyDeviceToHost, s));

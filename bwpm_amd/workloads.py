@@ -161,20 +161,29 @@ def born_3d(n: int = 512, nz: int = 1024, iterations: int = 6, sigma: float = 0.
 
 def born_3d_synth(n: int = 512, nz: int = 1024, iterations: int = 6, sigma: float = 0.01, corr_um: float = 1.0,
                   seed: int = 3, wavelength_um: float = 0.8, step_um: float = 0.1, angle_deg: float = 5.0,
-                  host: float = 1.5) -> Problem:
+                  host: float = 1.5, dz_um: float | None = None, absorber_wavelengths: float = 0.0) -> Problem:
     """Config 3 geometry with the δn generated on the device (no (nz, ny, nx) arrays on the host):
-    a uniform host volume and a FluctuationSynthesis of the same σ and correlation length."""
+    a uniform host volume and a FluctuationSynthesis of the same σ and correlation length. The slab
+    thickness is `dz_um` (default: the transverse step); `absorber_wavelengths` > 0 adds the
+    reference's absorbing zone at the transverse edges."""
     lam = wavelength_um
     d = step_um * 1e-3
+    dz = (dz_um if dz_um is not None else step_um) * 1e-3
     k0 = 2 * np.pi / (lam * 1e-3)
     dk = 2 * np.pi / (n * d)
     m = int(round(host * k0 * np.sin(np.deg2rad(angle_deg)) / dk))
     x = centred(n, d)
     f = np.zeros((n, n, 2), dtype=np.complex128)
     f[..., 0] = np.exp(1j * m * dk * x)[None, :]
-    return Problem(f"born_3d_synth_{n}x{n}x{nz}", n, n, d, d, d, lam, [complex(host)], Uniform(nz, 0),
-                   complex(host), f, iterations, fluctuation=FluctuationSynthesis(sigma, corr_um * 1e-3, seed),
-                   meta={"m": m, "sigma": sigma})
+    mask = None
+    if absorber_wavelengths > 0:
+        from bwpm_amd.volume3d import Grid3D, absorbing_mask_2d
+
+        mask = absorbing_mask_2d(Grid3D(n, n, d, d, dz, lam), absorber_wavelengths * lam * 1e-3)
+    return Problem(f"born_3d_synth_{n}x{n}x{nz}", n, n, d, d, dz, lam, [complex(host)], Uniform(nz, 0),
+                   complex(host), f, iterations, mask=mask,
+                   fluctuation=FluctuationSynthesis(sigma, corr_um * 1e-3, seed),
+                   meta={"m": m, "sigma": sigma, "absorber_wavelengths": absorber_wavelengths})
 
 
 def uneven_batch(count: int = 64, seed0: int = 1000, sizes: tuple[int, ...] = (128, 256, 384, 512),
