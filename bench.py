@@ -405,7 +405,7 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--batch", type=int, default=16)
+    ap.add_argument("--batch", type=int, default=24)
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--nz", type=int, default=1024)
     ap.add_argument("--iterations", type=int, default=4)

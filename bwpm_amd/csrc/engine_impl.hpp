@@ -1450,7 +1450,8 @@ class Engine final : public EngineBase {
     }
     if (std::any_of(has_host_.begin(), has_host_.end(), [](char c) { return c != 0; }))
       born_ = std::make_unique<Born<T, COMP>>(s, has_host_, host_, kx_.as<double>(), ky_.as<double>(), twx_->plan,
-                                              twy_->plan, stream_);
+                                              twy_->plan, stream_,
+                                              born_tolerance(sizeof(T) == 8, !std::is_same<ST, T>::value));
     HIP_CHECK(hipStreamSynchronize(stream_));
     bytes_ = stage_.bytes() + field_in_.bytes() + X_.bytes() + prod_.bytes() + e_.bytes() + fwd_vals_.bytes() +
              bwd_vals_.bytes() + mask_.bytes() + mask_y_.bytes() + mask_x_.bytes() + ops_bytes_ + tsel_.bytes() + rowuse_.bytes() + sp_pix_.bytes() +
