@@ -16,6 +16,8 @@
 // rotations on the host. On power-of-two grids the operator G runs as three fused passes.
 #pragma once
 
+#include <cstdlib>
+
 #include <cmath>
 #include <complex>
 #include <map>
@@ -1134,7 +1136,7 @@ class Born {
       return true;
     }
     tol_abs_ = tol_ * bnorm;
-    const int m = kRestart;
+    const int m = restart_;
     GmState* st = gm_.template as<GmState>();
     const int* skip = &st->done;
     double2* part = partial_.template as<double2>();
@@ -1170,8 +1172,8 @@ class Born {
       for (int i = used - 1; i >= 0; --i) {
         std::complex<double> sacc(hgm_->g[i].x, hgm_->g[i].y);
         for (int k = i + 1; k < used; ++k)
-          sacc -= std::complex<double>(hgm_->h[i * m + k].x, hgm_->h[i * m + k].y) * y[k];
-        y[i] = sacc / std::complex<double>(hgm_->h[i * m + i].x, hgm_->h[i * m + i].y);
+          sacc -= std::complex<double>(hgm_->h[i * kGmRestart + k].x, hgm_->h[i * kGmRestart + k].y) * y[k];
+        y[i] = sacc / std::complex<double>(hgm_->h[i * kGmRestart + i].x, hgm_->h[i * kGmRestart + i].y);
       }
       add_basis(x, y.data(), used);
       if (done) {
@@ -1218,6 +1220,13 @@ class Born {
   GmState* hgm_ = nullptr;      // its host copy (pinned)
   double tol_abs_ = 0.0;        // tol·‖b‖ of the current solve
   int expect_ = 4;              // Arnoldi steps of the last converged cycle (the first chunk's length)
+  // Restart length: the reference's 30, or BWPM_GMRES_RESTART (2 … 30), which tests use to run the
+  // restart path (cycles that end without converging) on small problems.
+  int restart_ = [] {
+    const char* env = std::getenv("BWPM_GMRES_RESTART");
+    const int v = env ? std::atoi(env) : 0;
+    return v >= 2 && v <= kGmRestart ? v : kGmRestart;
+  }();
   double2 host_coef_[kRestart + 2] = {};
   Fft2D<T, COMP> fft_;
   long long iterations_ = 0;

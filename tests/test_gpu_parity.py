@@ -361,6 +361,38 @@ def test_device_equals_reference_core_tilted_256(pol: str) -> None:
     assert got["stored_values"] == ref["stored_values"]
 
 
+@pytest.mark.parametrize("restart", [2, 3])
+def test_born_gmres_restarts_match_the_oracle(restart: int, monkeypatch) -> None:
+    """A GMRES restart length of 2 or 3 (BWPM_GMRES_RESTART) makes every Born step end cycles
+    without converging, so the restart path of the device cycle (a full cycle, the update of x,
+    a new r₀) runs; the field still matches the oracle (restart 30) to 1e-10."""
+    from bwpm_amd import _core
+
+    monkeypatch.setenv("BWPM_GMRES_RESTART", str(restart))
+    rng = np.random.default_rng(11)
+    ny = nx = 32
+    nz = 8
+    dx, dz, lam = 0.1e-3, 0.15e-3, 0.8
+    k0 = 2 * np.pi / (lam * 1e-3)
+    labels = np.zeros((nz, ny, nx), dtype=np.int32)
+    fl = np.zeros((nz, ny, nx), complex)
+    fl[1:nz - 1] = 0.05 * rng.normal(size=(nz - 2, ny, nx))
+    x = (np.arange(nx) - nx / 2) * dx
+    field = np.zeros((ny, nx, 2), complex)
+    field[..., 0] = np.exp(-(x[None, :] ** 2 + x[:, None] ** 2) / (1.0e-3) ** 2)
+    indices = np.array([1.5])
+    s = _core.GpuSolver(ny, nx, dx, dx, dz, k0, indices, labels, 1.5 + 0j, None, fluctuation=fl, precision="fp64")
+    try:
+        got = s.run(field, 2)
+    finally:
+        s.close()
+    ref = O.Solver(ny, nx, dx, dx, dz, k0, indices, labels, 1.5 + 0j, None, fluctuation=fl).run(field, 2)
+    scale = float(np.max(np.abs(field)))
+    assert got["gmres_iterations"] > (nz - 2) * 2 * 2 * restart  # more than one cycle per Born step
+    for key in ("transmitted", "reflected"):
+        assert np.max(np.abs(got[key] - ref[key])) < 1e-10 * scale, key
+
+
 def test_device_equals_reference_core_born() -> None:
     """Born slabs (δn, device GMRES) with the back-scattering paths over three iterations against
     the reference's C++ core, 64² × 12 slabs: agreement to 1e-10 of the field's maximum (GMRES stops
