@@ -66,6 +66,14 @@ def profiled_traffic(precision: str, n: int, batch: int, absorber: float) -> tup
     return None, None
 
 
+_T0 = time.perf_counter()
+
+
+def progress(msg: str) -> None:
+    """A progress line on stderr (stdout carries only the JSON line)."""
+    print(f"bench [{time.perf_counter() - _T0:7.1f} s] {msg}", file=sys.stderr, flush=True)
+
+
 def rel_l2(a: np.ndarray, b: np.ndarray) -> float:
     nb = float(np.linalg.norm(b))
     return float(np.linalg.norm(a - b)) / nb if nb > 0 else float(np.linalg.norm(a))
@@ -265,10 +273,12 @@ def configs(c3_nz: int, c4_count: int, precision: str, which: str = "c1,c2,c3,c4
     out: dict = {}
     if "c1" in sel:
         out["c1_2d_1024x200_it3"] = with_modes(workloads.plain_2d(ny=1024, nz=200, iterations=3))
+        progress("config 1 done")
     if "c2" in sel:
         out["c2_tilted_256x256x512_it4"] = {
             f"E{pol}": with_modes(workloads.tilted_3d(n=256, nz=512, iterations=4, polarisation=pol))
             for pol in ("x", "y")}
+        progress("config 2 done")
     if "c3" in sel and c3_nz > 0:
         # the δn volume is generated on the device (a (1024, 512, 512) complex array would be
         # 4 GiB on the host); the transverse edges carry the reference's absorbing zone of 20
@@ -278,8 +288,10 @@ def configs(c3_nz: int, c4_count: int, precision: str, which: str = "c1,c2,c3,c4
             **with_modes(workloads.born_3d_synth(n=512, nz=c3_nz, iterations=6, absorber_wavelengths=20.0)),
             "input": "Gaussian-correlated δn (σ = 0.01, correlation 1 µm, seed 3) generated on the device; "
                      "20-wavelength absorbing zone"}
+        progress("config 3 done")
     if c4_count > 0 and "c4" in sel:
         out[f"c4_uneven_born_{c4_count}"] = uneven_batch(c4_count, precision)
+        progress("config 4 done")
     return out
 
 
@@ -435,6 +447,7 @@ def main() -> int:
                                   absorber_wavelengths=args.absorber) for c in "xy"}
     probs = [pol["x" if i % 2 == 0 else "y"] for i in range(args.batch)]
     solvers, elapsed, kt, batch_res = lane(probs, args.precision, args.iterations, args.warmup, args.steps)
+    progress(f"headline lane timed ({elapsed:.2f} s for {args.steps} steps)")
     updates = sum(p.updates for p in probs) * args.steps
     value = updates / elapsed
     ms_per_step = 1e3 * elapsed / max(args.steps, 1)
@@ -503,6 +516,7 @@ def main() -> int:
         checks["batched_over_plain_fp32"] = value / plain32
         checks["plain_fp64_one_solve_per_launch"] = refs["x"][1]
         checks["small_grids"] = small_lane()
+        progress("checks done")
         if not args.no_modes:
             modes = {}
             for m in ("fp64", "mixed", "bfp", "fp32"):
@@ -520,10 +534,15 @@ def main() -> int:
                     for i, c in enumerate("xy"):
                         entry[f"E{c}"] = accuracy(pol[c], res[i], refs[c][0], ref_rta[c])
                 modes[m] = entry
+                progress(f"precision mode {m} done")
     del pol, probs
     other = configs(args.c3_nz, args.c4_count, args.precision, args.configs) if args.configs else None
     serving = served(args.serve_requests, args.precision) if args.serve_requests > 0 else None
+    if serving is not None:
+        progress("serving done")
     cpu = None if args.no_cpu_baseline else cpu_baseline(args.cpu_reps)
+    if cpu is not None:
+        progress("CPU baseline done")
     line = {
         "metric": "grid-point·slab updates/s (3D vector 512²)",
         "value": value,

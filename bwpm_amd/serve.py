@@ -89,8 +89,10 @@ def planar_3d(n: int, nz: int, iterations: int, wavelength_um: float, n1: comple
     return p
 
 
-def served_request(seed: int, max_n: int = 512, max_nz: int = 1024, born_n: int = 128) -> workloads.Problem:
-    """One request of the served mix, drawn from `seed` (reproducible)."""
+def served_request(seed: int, max_n: int = 512, max_nz: int = 1024, born_n: int | None = None) -> workloads.Problem:
+    """One request of the served mix, drawn from `seed` (reproducible). δn requests have the sizes of
+    the other 3D requests (n² up to max_n², 200 … max_nz slabs, 1 … 3 iterations); `born_n` fixes
+    them to born_n² × 50 … 128 slabs × 1 iteration instead (small test mixes)."""
     rng = np.random.default_rng(seed)
     kind = rng.choice(["2d", "planar", "tilted", "silver", "born"], p=[0.2, 0.25, 0.3, 0.15, 0.1])
     its = int(rng.integers(1, 4))
@@ -113,6 +115,10 @@ def served_request(seed: int, max_n: int = 512, max_nz: int = 1024, born_n: int 
     elif kind == "silver":
         z0 = int(rng.integers(nz // 4, nz // 2))
         p = planar_3d(n, nz, its, lam, 1.0, SILVER, layer=(z0, z0 + int(rng.integers(2, 8))), polarisation=pol)
+    elif born_n is None:
+        p = workloads.born_3d_synth(n=n, nz=nz, iterations=its, sigma=float(rng.uniform(0.005, 0.02)),
+                                    seed=int(rng.integers(1 << 30)), wavelength_um=lam, step_um=lam / 6,
+                                    dz_um=lam / 4)
     else:
         nzb = int(rng.integers(50, 129))
         p = workloads.born_3d_synth(n=born_n, nz=nzb, iterations=1, sigma=float(rng.uniform(0.005, 0.02)),
