@@ -71,4 +71,27 @@ __global__ void k_gm_givens(GmState* st, const double2* __restrict__ res, int j,
   if (zabs(st->g[j + 1]) <= tol_abs || hnext == 0.0) st->done = 1;
 }
 
+__global__ void k_rich_begin(GmState* st) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  st->done = 0;
+  st->used = 0;
+  st->beta = __builtin_huge_val();
+}
+
+__global__ __launch_bounds__(kRedThreads) void k_rich_check(const double2* __restrict__ partial, GmState* st,
+                                                            double tol_abs, int max_steps) {
+  if (st->done) return;  // uniform: every thread reads the same flag before any synchronisation
+  __shared__ double2 n2[1];
+  final_sums(partial, 1, n2, nullptr);
+  if (threadIdx.x != 0) return;
+  const double step = sqrt(n2[0].x);
+  const int used = st->used + 1;
+  st->used = used;
+  if (step <= tol_abs)
+    st->done = 1;
+  else if ((used >= 2 && step > 0.5 * st->beta) || used >= max_steps || !(step == step))
+    st->done = 2;
+  st->beta = step;
+}
+
 }  // namespace bwpm

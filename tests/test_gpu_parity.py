@@ -369,6 +369,7 @@ def test_born_gmres_restarts_match_the_oracle(restart: int, monkeypatch) -> None
     from bwpm_amd import _core
 
     monkeypatch.setenv("BWPM_GMRES_RESTART", str(restart))
+    monkeypatch.setenv("BWPM_BORN_SOLVER", "gmres")
     rng = np.random.default_rng(11)
     ny = nx = 32
     nz = 8
@@ -391,6 +392,40 @@ def test_born_gmres_restarts_match_the_oracle(restart: int, monkeypatch) -> None
     assert got["gmres_iterations"] > (nz - 2) * 2 * 2 * restart  # more than one cycle per Born step
     for key in ("transmitted", "reflected"):
         assert np.max(np.abs(got[key] - ref[key])) < 1e-10 * scale, key
+
+
+@pytest.mark.parametrize("solver", ["gmres", "default"])
+def test_born_solvers_agree_with_the_oracle(solver: str, monkeypatch) -> None:
+    """The Cayley system solved by GMRES alone (BWPM_BORN_SOLVER=gmres) and by the default
+    (Richardson steps, GMRES only where they stall) on weak, strong and very strong fluctuations
+    (σ = 0.02, 0.5 and 2, the last beyond Richardson's contraction, so the default falls back to
+    GMRES): both match the oracle to 1e-10 of the field's maximum."""
+    from bwpm_amd import _core
+
+    if solver == "gmres":
+        monkeypatch.setenv("BWPM_BORN_SOLVER", "gmres")
+    for sigma in (0.02, 0.5, 2.0):
+        rng = np.random.default_rng(5)
+        ny = nx = 32
+        nz = 6
+        dx, dz, lam = 0.1e-3, 0.15e-3, 0.8
+        k0 = 2 * np.pi / (lam * 1e-3)
+        labels = np.zeros((nz, ny, nx), dtype=np.int32)
+        fl = np.zeros((nz, ny, nx), complex)
+        fl[1:nz - 1] = sigma * rng.normal(size=(nz - 2, ny, nx))
+        x = (np.arange(nx) - nx / 2) * dx
+        field = np.zeros((ny, nx, 2), complex)
+        field[..., 0] = np.exp(-(x[None, :] ** 2 + x[:, None] ** 2) / (1.0e-3) ** 2)
+        s = _core.GpuSolver(ny, nx, dx, dx, dz, k0, np.array([1.5]), labels, 1.5 + 0j, None, fluctuation=fl,
+                            precision="fp64")
+        try:
+            got = s.run(field, 2)
+        finally:
+            s.close()
+        ref = O.Solver(ny, nx, dx, dx, dz, k0, [1.5], labels, 1.5 + 0j, None, fluctuation=fl).run(field, 2)
+        for key in ("transmitted", "reflected"):
+            scale = max(float(np.max(np.abs(field))), float(np.max(np.abs(ref[key]))))
+            assert np.max(np.abs(got[key] - ref[key])) < 1e-10 * scale, (sigma, key)
 
 
 def test_device_equals_reference_core_born() -> None:
