@@ -78,13 +78,19 @@ __global__ void k_rich_begin(GmState* st) {
   st->beta = __builtin_huge_val();
 }
 
-__global__ __launch_bounds__(kRedThreads) void k_rich_check(const double2* __restrict__ partial, GmState* st,
-                                                            double tol_abs, int max_steps) {
+__global__ __launch_bounds__(kRedThreads) void k_rich_check(const double2* __restrict__ partial, int count,
+                                                            GmState* st, double tol_abs, int max_steps) {
   if (st->done) return;  // uniform: every thread reads the same flag before any synchronisation
-  __shared__ double2 n2[1];
-  final_sums(partial, 1, n2, nullptr);
+  __shared__ double red[kRedThreads / 64];
+  double acc = 0;
+  for (int i = static_cast<int>(threadIdx.x); i < count; i += kRedThreads) acc += partial[i].x;
+  acc = wave_sum(acc);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
   if (threadIdx.x != 0) return;
-  const double step = sqrt(n2[0].x);
+  double tot = 0;
+  for (int w = 0; w < kRedThreads / 64; ++w) tot += red[w];
+  const double step = sqrt(tot);
   const int used = st->used + 1;
   st->used = used;
   if (step <= tol_abs)

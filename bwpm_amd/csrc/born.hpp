@@ -115,6 +115,13 @@ __global__ void k_born_finish_op(const px2<T>* __restrict__ xy, const px2<T>* __
   out[q] = g;
 }
 
+// Sum over the 64 lanes of a wave in a fixed butterfly order; every lane receives the same value.
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
 // ---- G_d in three fused passes (power-of-two grids) ------------------------------------------------
 //
 // The same operator as k_born_prepare_op → F⁻¹ → ×V → F → k_born_finish_op, with the transform
@@ -253,8 +260,9 @@ template <int N, int RW, typename T, bool COMP>
 __global__ __launch_bounds__(N / 8 * RW) void k_born_op_rows_fwd(const px2<T>* __restrict__ xy,
                                                                 const cx<T>* __restrict__ zl, BornOp<T> op,
                                                                 const T* __restrict__ kxT, const T* __restrict__ kyT,
-                                                                T d, cx<T> inv_k2, const px2<T>* __restrict__ v,
-                                                                px2<T>* __restrict__ out, int mode,
+                                                                T d, cx<T> inv_k2, const px2<T>* v, px2<T>* out,
+                                                                int mode, const px2<T>* __restrict__ rb,
+                                                                double2* __restrict__ rpart,
                                                                 const cx<T>* __restrict__ g_twh,
                                                                 const cx<T>* __restrict__ g_twl,
                                                                 const int* __restrict__ skip) {
@@ -281,6 +289,7 @@ __global__ __launch_bounds__(N / 8 * RW) void k_born_op_rows_fwd(const px2<T>* _
   reg_fft<N, COMP, false, TWL>(a, sh + r * LQ, t, twh, twl);
   reg_fft<N, COMP, false, TWL>(z, sh + r * LQ, t, twh, twl);
   const T ky = kyT[y];
+  double step2 = 0;
 #pragma unroll
   for (int m = 0; m < 8; ++m) {
     const long long q = row + t + TT * m;
@@ -292,8 +301,29 @@ __global__ __launch_bounds__(N / 8 * RW) void k_born_op_rows_fwd(const px2<T>* _
       const px2<T> w = v[q];
       const T h = static_cast<T>(0.5);
       g = px2<T>{w.a - g.a * h, w.b - g.b * h};
+    } else if (mode == 2) {
+      // the Richardson step of k_rich_update (same arithmetic), with out == v == x
+      const px2<T> xo = v[q], bb = rb[q];
+      const T h = static_cast<T>(0.5);
+      g = px2<T>{bb.a + g.a * h, bb.b + g.b * h};
+      const double d0 = static_cast<double>(g.a.x) - static_cast<double>(xo.a.x);
+      const double d1 = static_cast<double>(g.a.y) - static_cast<double>(xo.a.y);
+      const double d2 = static_cast<double>(g.b.x) - static_cast<double>(xo.b.x);
+      const double d3 = static_cast<double>(g.b.y) - static_cast<double>(xo.b.y);
+      step2 += d0 * d0 + d1 * d1 + d2 * d2 + d3 * d3;
     }
     out[q] = g;
+  }
+  if (mode == 2) {  // the workgroup's sum of ‖x_new − x_old‖² into rpart[block] (waves in order)
+    __shared__ double red[16];
+    const double w = wave_sum(step2);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = w;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double acc = 0;
+      for (int i = 0; i < static_cast<int>(blockDim.x + 63) / 64; ++i) acc += red[i];
+      rpart[blockIdx.x] = double2{acc, 0.0};
+    }
   }
 }
 
@@ -430,26 +460,20 @@ struct GmState {
 };
 
 // Richardson iteration state in a GmState: k_rich_begin starts it (done = 0, used = 0, the last
-// step size in beta = ∞); k_rich_check takes ‖x_new − x_old‖ from the partial sums of
-// k_rich_update, counts the step, and sets done = 1 when it is at most tol_abs, or done = 2 when
+// step size in beta = ∞); k_rich_check takes ‖x_new − x_old‖ from the `count` partial sums of the
+// step (k_rich_update, or the operator's last pass in mode 2), counts the step, and sets done = 1 when it is at most tol_abs, or done = 2 when
 // the iteration does not contract fast enough (a step larger than half the previous one, or
 // max_steps steps), after which the caller switches to GMRES. One workgroup of
 // kRedThreads threads; both return at once when the state is done.
 __global__ void k_rich_begin(GmState* st);
-__global__ __launch_bounds__(kRedThreads) void k_rich_check(const double2* __restrict__ partial, GmState* st,
-                                                            double tol_abs, int max_steps);
+__global__ __launch_bounds__(kRedThreads) void k_rich_check(const double2* __restrict__ partial, int count,
+                                                            GmState* st, double tol_abs, int max_steps);
 
 // Arnoldi step j's Hessenberg column from the reductions `res` (h1 + h2, ‖w‖), the previous
 // rotations applied to it, the new rotation, and the convergence test against tol_abs = tol·‖b‖
 // (solver.cpp:160-190). One thread; returns at once when the cycle is done.
 __global__ void k_gm_givens(GmState* st, const double2* __restrict__ res, int j, double tol_abs);
 
-// Sum over the 64 lanes of a wave in a fixed butterfly order; every lane receives the same value.
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
-}
 
 // The workgroup's share of the sums acc[i] (i < k), per wave then over the waves in order, into
 // partial[i·kRedBlocks + block].
@@ -918,7 +942,8 @@ class Born {
 
   // out = G_d·v (mode 0) or (1 − G_d/2)·v (mode 1) (solver.cpp:262-311).
   // `skip` (nullptr: none) gates the fused passes (kernels with a skip flag); the generic path runs
-  // regardless, and its output is then unused.
+  // regardless, and its output is then unused. Mode 2 (fused passes only, out == v == x): the
+  // Richardson step x ← b + ½·G·x with b = rich_b_ and its partial sums into rich_part_.
   void apply_g(const px2<T>* v, int d, long long slab, px2<T>* out, int mode, const int* skip = nullptr) {
     if constexpr (COMP == (sizeof(T) == 4)) {
       if (fft_.fast()) {
@@ -985,11 +1010,13 @@ class Born {
     const auto* th = static_cast<const cx<T>*>(p.tw_hi);
     const auto* tl = static_cast<const cx<T>*>(p.tw_lo);
     const unsigned rows = static_cast<unsigned>(ny_ / RW);
-    if (forward)
+    if (forward) {
       hipLaunchKernelGGL((k_born_op_rows_fwd<N, RW, T, COMP>), dim3(rows), dim3(N / 8 * RW), 0, stream_,
                          static_cast<const px2<T>*>(xy), static_cast<const cx<T>*>(zl), op, kxT_.template as<T>(),
-                         kyT_.template as<T>(), d, ik2, v, out, mode, th, tl, skip);
-    else
+                         kyT_.template as<T>(), d, ik2, v, out, mode, mode == 2 ? rich_b_ : nullptr,
+                         mode == 2 ? rich_part_ : nullptr, th, tl, skip);
+      if (mode == 2) rich_count_ = static_cast<int>(rows);
+    } else
       hipLaunchKernelGGL((k_born_op_rows_inv<N, RW, T, COMP>), dim3(rows, 2), dim3(N / 8 * RW), 0, stream_, v, op,
                          kxT_.template as<T>(), kyT_.template as<T>(), d, xy, zl, th, tl, skip);
     HIP_CHECK(hipGetLastError());
@@ -1194,15 +1221,29 @@ class Born {
     double2* part = partial_.template as<double2>();
     hipLaunchKernelGGL(k_rich_begin, dim3(1), dim3(1), 0, stream_, st);
     HIP_CHECK(hipGetLastError());
+    // on the fused operator the step itself runs in its last pass (mode 2: x ← b + ½·G·x in place,
+    // with one partial sum of the step per workgroup)
+    bool fused = false;
+    if constexpr (COMP == (sizeof(T) == 4)) fused = fft_.fast();
     for (int issued = 0;;) {
       const int chunk = issued == 0 ? rich_expect_ : 1;
       for (int c = 0; c < chunk; ++c, ++issued) {
-        apply_g(x, d, slab, w, 0, skip);
-        hipLaunchKernelGGL(k_rich_update<T>, dim3(kRedBlocks), dim3(kRedThreads), 0, stream_, x, b,
-                           static_cast<const px2<T>*>(w), static_cast<long long>(npix_), part, skip);
-        HIP_CHECK(hipGetLastError());
+        int count = kRedBlocks;
+        if (fused) {
+          rich_b_ = b;
+          rich_part_ = part;
+          apply_g(x, d, slab, x, 2, skip);
+          rich_b_ = nullptr;
+          rich_part_ = nullptr;
+          count = rich_count_;
+        } else {
+          apply_g(x, d, slab, w, 0, skip);
+          hipLaunchKernelGGL(k_rich_update<T>, dim3(kRedBlocks), dim3(kRedThreads), 0, stream_, x, b,
+                             static_cast<const px2<T>*>(w), static_cast<long long>(npix_), part, skip);
+          HIP_CHECK(hipGetLastError());
+        }
         hipLaunchKernelGGL(k_rich_check, dim3(1), dim3(kRedThreads), 0, stream_, static_cast<const double2*>(part),
-                           st, tol_abs_, kRichMaxSteps);
+                           count, st, tol_abs_, kRichMaxSteps);
         HIP_CHECK(hipGetLastError());
       }
       HIP_CHECK(hipMemcpyAsync(hgm_, st, sizeof(GmState), hipMemcpyDeviceToHost, stream_));
@@ -1304,6 +1345,9 @@ class Born {
   double tol_abs_ = 0.0;        // tol·‖b‖ of the current solve
   int expect_ = 4;              // Arnoldi steps of the last converged cycle (the first chunk's length)
   static constexpr int kRichMaxSteps = 40;
+  const px2<T>* rich_b_ = nullptr;  // b of the fused Richardson step (operator mode 2)
+  double2* rich_part_ = nullptr;    // its partial sums, one per workgroup of the last pass
+  int rich_count_ = 0;              // how many
   int rich_expect_ = 4;         // Richardson steps of the last converged solve (the first chunk's length)
   // Richardson steps before GMRES, unless BWPM_BORN_SOLVER=gmres
   bool richardson_ = [] {
