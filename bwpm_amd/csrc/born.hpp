@@ -1058,7 +1058,6 @@ class Born {
     px2<T>* b = rhs_.template as<px2<T>>();
     apply_g(xh, d, slab, b, 0);
     px2<T>* x = x_.template as<px2<T>>();
-    HIP_CHECK(hipMemcpyAsync(x, b, npix_ * sizeof(px2<T>), hipMemcpyDeviceToDevice, stream_));
     if (!solve(d, slab))
       throw Error(kEnoconv, "GMRES did not converge in the Born step");
     px2<T>* y = tmp_.template as<px2<T>>();
@@ -1188,9 +1187,10 @@ class Born {
   // cycle and their Givens rotations run on the device; the host enqueues them in chunks, the
   // first as long as the last cycle that converged, and reads the cycle state once per chunk
   // instead of once per step. Steps enqueued past convergence skip their work.
-  // The Cayley system (1 − G/2)x = b, x_ holding the start value b: Richardson steps first, then
+  // The Cayley system (1 − G/2)x = b from the start value x = b: Richardson steps first, then
   // GMRES from where they stopped if they do not contract (or always GMRES with
-  // BWPM_BORN_SOLVER=gmres). Both stop at ‖b − (1 − G/2)x‖ ≤ tol·‖b‖.
+  // BWPM_BORN_SOLVER=gmres). Both stop at ‖b − (1 − G/2)x‖ ≤ tol·‖b‖. The fused Richardson step
+  // reads the start value from b itself; every other path copies it into x_ first.
   bool solve(int d, long long slab) {
     const px2<T>* b = rhs_.template as<px2<T>>();
     px2<T>* x = x_.template as<px2<T>>();
@@ -1201,7 +1201,11 @@ class Born {
       return true;
     }
     tol_abs_ = tol_ * bnorm;
-    if (richardson_ && richardson(d, slab)) return true;
+    bool fused = false;
+    if constexpr (COMP == (sizeof(T) == 4)) fused = fft_.fast();
+    if (!(richardson_ && fused))
+      HIP_CHECK(hipMemcpyAsync(x, b, npix_ * sizeof(px2<T>), hipMemcpyDeviceToDevice, stream_));
+    if (richardson_ && richardson(d, slab, fused)) return true;
     return gmres(d, slab);
   }
 
@@ -1212,7 +1216,7 @@ class Born {
   // iterate, and the returned iterate's residual is smaller still). The steps are enqueued in
   // chunks, gated by the state's done flag like the Arnoldi steps; returns false (x holding the
   // last iterate) when the contraction is too slow.
-  bool richardson(int d, long long slab) {
+  bool richardson(int d, long long slab, bool fused) {
     const px2<T>* b = rhs_.template as<px2<T>>();
     px2<T>* x = x_.template as<px2<T>>();
     px2<T>* w = w_.template as<px2<T>>();
@@ -1222,9 +1226,7 @@ class Born {
     hipLaunchKernelGGL(k_rich_begin, dim3(1), dim3(1), 0, stream_, st);
     HIP_CHECK(hipGetLastError());
     // on the fused operator the step itself runs in its last pass (mode 2: x ← b + ½·G·x in place,
-    // with one partial sum of the step per workgroup)
-    bool fused = false;
-    if constexpr (COMP == (sizeof(T) == 4)) fused = fft_.fast();
+    // the first step from x = b read from b, with one partial sum of the step per workgroup)
     for (int issued = 0;;) {
       const int chunk = issued == 0 ? rich_expect_ : 1;
       for (int c = 0; c < chunk; ++c, ++issued) {
@@ -1232,7 +1234,7 @@ class Born {
         if (fused) {
           rich_b_ = b;
           rich_part_ = part;
-          apply_g(x, d, slab, x, 2, skip);
+          apply_g(issued == 0 ? b : x, d, slab, x, 2, skip);
           rich_b_ = nullptr;
           rich_part_ = nullptr;
           count = rich_count_;
