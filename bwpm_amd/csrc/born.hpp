@@ -157,6 +157,39 @@ __global__ void k_born_optab(const BornK<T>* __restrict__ tb, long long npix, cx
   out[2 * npix + q] = tb[q].green;
 }
 
+// The per-k fields the pointwise kernels read, one plane each (the table above is an array of
+// 208-byte structs, of which those kernels use 16 … 48 bytes per pixel): P_½, P_½⁻¹, 1/k_z or 0,
+// k_z, the source phases ps[d][m] and the emission factors ge[d′][m].
+inline constexpr int kBornFields = 12;
+template <typename T>
+struct BornF {
+  const cx<T>* f[kBornFields];
+  BWPM_HD const cx<T>* half() const { return f[0]; }
+  BWPM_HD const cx<T>* halfinv() const { return f[1]; }
+  BWPM_HD const cx<T>* cz() const { return f[2]; }
+  BWPM_HD const cx<T>* kz() const { return f[3]; }
+  BWPM_HD const cx<T>* ps(int dir, int m) const { return f[4 + 2 * dir + m]; }
+  BWPM_HD const cx<T>* ge(int dir, int m) const { return f[8 + 2 * dir + m]; }
+};
+
+template <typename T>
+__global__ void k_born_fields(const BornK<T>* __restrict__ tb, long long npix, cx<T>* __restrict__ out) {
+  const long long q = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;
+  if (q >= npix) return;
+  const BornK<T>& b = tb[q];
+  out[q] = b.half;
+  out[npix + q] = b.halfinv;
+  out[2 * npix + q] = b.cz;
+  out[3 * npix + q] = b.kz;
+#pragma unroll
+  for (int d = 0; d < 2; ++d)
+#pragma unroll
+    for (int m = 0; m < 2; ++m) {
+      out[(4 + 2 * d + m) * npix + q] = b.ps[d][m];
+      out[(8 + 2 * d + m) * npix + q] = b.ge[d][m];
+    }
+}
+
 // Pass 1, one row per RW·N/8 threads: blockIdx.y = 0 transforms the propagating part of
 // (S_x, S_y), blockIdx.y = 1 the line S_z = −d(k_x S_x + k_y S_y)/k_z (inverse, unscaled).
 template <int N, int RW, typename T, bool COMP>
@@ -329,27 +362,26 @@ __global__ __launch_bounds__(N / 8 * RW) void k_born_op_rows_fwd(const px2<T>* _
 
 // x = P_½·Ê (which = 0), y·P_½ (which = 1, direction +1) or y·P_½⁻¹ (which = 2, direction −1).
 template <typename T>
-__global__ void k_born_half(const px2<T>* __restrict__ in, const BornK<T>* __restrict__ tb, long long npix,
-                            int which, px2<T>* out) {
+__global__ void k_born_half(const px2<T>* __restrict__ in, const cx<T>* __restrict__ fac, long long npix,
+                            px2<T>* out) {
   const long long q = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;
   if (q >= npix) return;
-  const cx<T> f = which == 2 ? tb[q].halfinv : tb[q].half;
+  const cx<T> f = fac[q];
   const px2<T> v = in[q];
   out[q] = px2<T>{v.a * f, v.b * f};
 }
 
 // Sources: (Ê_x, Ê_y, Ê_z)·e^{i k_z s_m} for both Gauss points → planes (x, y)₀, (x, y)₁, (z₀, z₁).
 template <typename T>
-__global__ void k_born_src_prepare(const px2<T>* __restrict__ e, const BornK<T>* __restrict__ tb,
-                                   const T* __restrict__ kxT, const T* __restrict__ kyT, int nx, long long npix,
-                                   T d, int dir, px2<T>* xy0, px2<T>* xy1, px2<T>* zz) {
+__global__ void k_born_src_prepare(const px2<T>* __restrict__ e, BornF<T> tb, const T* __restrict__ kxT,
+                                   const T* __restrict__ kyT, int nx, long long npix, T d, int dir, px2<T>* xy0,
+                                   px2<T>* xy1, px2<T>* zz) {
   const long long q = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;
   if (q >= npix) return;
   const T kx = kxT[q % nx], ky = kyT[q / nx];
-  const BornK<T>& b = tb[q];
   const px2<T> s = e[q];
-  const cx<T> ez = (s.a * kx + s.b * ky) * b.cz * (-d);
-  const cx<T> p0 = b.ps[dir][0], p1 = b.ps[dir][1];
+  const cx<T> ez = (s.a * kx + s.b * ky) * tb.cz()[q] * (-d);
+  const cx<T> p0 = tb.ps(dir, 0)[q], p1 = tb.ps(dir, 1)[q];
   xy0[q] = px2<T>{s.a * p0, s.b * p0};
   xy1[q] = px2<T>{s.a * p1, s.b * p1};
   zz[q] = px2<T>{ez * p0, ez * p1};
@@ -358,14 +390,13 @@ __global__ void k_born_src_prepare(const px2<T>* __restrict__ e, const BornK<T>*
 // Emission in direction d′: Σ_m ge_m·[P(k_d′)·S_m]⊥.
 template <typename T>
 __global__ void k_born_emit(const px2<T>* __restrict__ xy0, const px2<T>* __restrict__ xy1,
-                            const px2<T>* __restrict__ zz, const BornK<T>* __restrict__ tb,
-                            const T* __restrict__ kxT, const T* __restrict__ kyT, int nx, long long npix, T dp,
-                            int dir, cx<T> inv_k2, px2<T>* out) {
+                            const px2<T>* __restrict__ zz, BornF<T> tb, const T* __restrict__ kxT,
+                            const T* __restrict__ kyT, int nx, long long npix, T dp, int dir, cx<T> inv_k2,
+                            px2<T>* out) {
   const long long q = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;
   if (q >= npix) return;
   const T kx = kxT[q % nx], ky = kyT[q / nx];
-  const BornK<T>& b = tb[q];
-  const cx<T> skz = b.kz * dp;
+  const cx<T> skz = tb.kz()[q] * dp;
   const px2<T> z = zz[q];
   px2<T> tot{};
 #pragma unroll
@@ -373,7 +404,7 @@ __global__ void k_born_emit(const px2<T>* __restrict__ xy0, const px2<T>* __rest
     const px2<T> s = m == 0 ? xy0[q] : xy1[q];
     const cx<T> s2 = m == 0 ? z.a : z.b;
     const cx<T> lon = (s.a * kx + s.b * ky + skz * s2) * inv_k2;
-    const cx<T> g = b.ge[dir][m];
+    const cx<T> g = tb.ge(dir, m)[q];
     tot.a = tot.a + g * (s.a - lon * kx);
     tot.b = tot.b + g * (s.b - lon * ky);
   }
@@ -826,6 +857,10 @@ class Born {
     for (size_t i = 0; i < hosts.size(); ++i)
       launch(k_born_table<T>, kx, ky, static_cast<int>(ny_), static_cast<int>(nx_), zd{hosts[i].real(), hosts[i].imag()},
              k0_, dz_, tabs_.template as<BornK<T>>() + i * npix_);
+    fields_.alloc(hosts.size() * kBornFields * npix_ * sizeof(cx<T>));
+    for (size_t i = 0; i < hosts.size(); ++i)
+      launch(k_born_fields<T>, tabs_.template as<const BornK<T>>() + i * npix_, npix_,
+             fields_.template as<cx<T>>() + kBornFields * i * npix_);
     optab_.alloc(hosts.size() * 3 * npix_ * sizeof(cx<T>));
     for (size_t i = 0; i < hosts.size(); ++i)
       launch(k_born_optab<T>, tabs_.template as<const BornK<T>>() + i * npix_, npix_,
@@ -853,7 +888,8 @@ class Born {
     if (s.synth.on) synthesise(s, host);
     fft_.init(ny_, nx_, px_, py_, stream_);
     HIP_CHECK(hipStreamSynchronize(stream_));
-    bytes_ = V_.bytes() + tabs_.bytes() + optab_.bytes() + 10 * pb + basis_.bytes() + planes_.bytes();
+    bytes_ = V_.bytes() + tabs_.bytes() + fields_.bytes() + optab_.bytes() + 10 * pb + basis_.bytes() +
+             planes_.bytes();
   }
   ~Born() {
     if (hres_) (void)hipHostFree(hres_);
@@ -911,6 +947,12 @@ class Born {
     return planes_.template as<px2<T>>() + (2 * slab_v_[slab] + which) * npix_;
   }
   const BornK<T>* table(long long slab) const { return tabs_.template as<BornK<T>>() + slab_tab_[slab] * npix_; }
+  BornF<T> fields(long long slab) const {
+    BornF<T> f;
+    const cx<T>* base = fields_.template as<const cx<T>>() + kBornFields * slab_tab_[slab] * npix_;
+    for (int i = 0; i < kBornFields; ++i) f.f[i] = base + i * npix_;
+    return f;
+  }
   const cx<T>* potential(long long slab) const { return V_.template as<cx<T>>() + slab_v_[slab] * npix_; }
   cx<T> inv_k2(long long slab) const {
     const c128 k = hosts_[slab_tab_[slab]] * k0_;
@@ -1035,33 +1077,34 @@ class Born {
 
   // R1 / R3: sources of direction d from Ê (in ehat_), emitted into direction dp, real space.
   void emission(int d, int dp, long long slab, px2<T>* out) {
-    const BornK<T>* tb = table(slab);
     px2<T>* x0 = xy0_.template as<px2<T>>();
     px2<T>* x1 = xy1_.template as<px2<T>>();
     px2<T>* zz = zz_.template as<px2<T>>();
-    launch(k_born_src_prepare<T>, ehat_.template as<px2<T>>(), tb, kxT_.template as<T>(), kyT_.template as<T>(),
-           static_cast<int>(nx_), npix_, static_cast<T>(d), d > 0 ? 0 : 1, x0, x1, zz);
+    const BornF<T> tf = fields(slab);
+    launch(k_born_src_prepare<T>, static_cast<const px2<T>*>(ehat_.template as<px2<T>>()), tf, kxT_.template as<T>(),
+           kyT_.template as<T>(), static_cast<int>(nx_), npix_, static_cast<T>(d), d > 0 ? 0 : 1, x0, x1, zz);
     px2<T>* planes[3] = {x0, x1, zz};
     fft2_planes(planes, 3, true, potential(slab));
     fft2_planes(planes, 3, false, nullptr);
     px2<T>* tot = tmp_.template as<px2<T>>();
-    launch(k_born_emit<T>, x0, x1, zz, tb, kxT_.template as<T>(), kyT_.template as<T>(), static_cast<int>(nx_), npix_,
-           static_cast<T>(dp), dp > 0 ? 0 : 1, inv_k2(slab), tot);
+    launch(k_born_emit<T>, static_cast<const px2<T>*>(x0), static_cast<const px2<T>*>(x1),
+           static_cast<const px2<T>*>(zz), tf, kxT_.template as<T>(), kyT_.template as<T>(), static_cast<int>(nx_),
+           npix_, static_cast<T>(dp), dp > 0 ? 0 : 1, inv_k2(slab), tot);
     fft2(tot, out, true);
   }
 
   // Cayley increment P_½(1 − G/2)⁻¹G·P_½Ê, real space, into inc_ (solver.cpp:313-350).
   void step(int d, long long slab) {
-    const BornK<T>* tb = table(slab);
     px2<T>* xh = xh_.template as<px2<T>>();
-    launch(k_born_half<T>, ehat_.template as<px2<T>>(), tb, npix_, 0, xh);
+    launch(k_born_half<T>, static_cast<const px2<T>*>(ehat_.template as<px2<T>>()), fields(slab).half(), npix_, xh);
     px2<T>* b = rhs_.template as<px2<T>>();
     apply_g(xh, d, slab, b, 0);
     px2<T>* x = x_.template as<px2<T>>();
     if (!solve(d, slab))
       throw Error(kEnoconv, "GMRES did not converge in the Born step");
     px2<T>* y = tmp_.template as<px2<T>>();
-    launch(k_born_half<T>, x, tb, npix_, d > 0 ? 1 : 2, y);
+    launch(k_born_half<T>, static_cast<const px2<T>*>(x), d > 0 ? fields(slab).half() : fields(slab).halfinv(), npix_,
+           y);
     fft2(y, inc_.template as<px2<T>>(), true);
   }
 
@@ -1338,7 +1381,7 @@ class Born {
   std::vector<long long> slab_v_;
   std::vector<int> slab_tab_;
   std::vector<c128> hosts_;
-  Dev V_, tabs_, optab_, kxT_, kyT_;
+  Dev V_, tabs_, fields_, optab_, kxT_, kyT_;
   Dev ehat_, xh_, xy0_, xy1_, zz_, rhs_, x_, w_, tmp_, inc_, basis_, planes_;
   Dev partial_, dres_, coef_;
   double2* hres_ = nullptr;
