@@ -538,6 +538,9 @@ inline std::string plan_report() {
 }
 
 template <typename T, bool COMP, typename ST>
+PlanChoice plan_time(long long n, bool col, const Geometry& g0, hipStream_t stream);
+
+template <typename T, bool COMP, typename ST>
 int plan_pick(int device, int precision, long long n, bool col, const Geometry& g0, hipStream_t stream) {
   if (!with_fast_size(n, [&]<int N>() { return reg_plan_count<N>() > 1; })) return 0;
   if (const char* env = std::getenv("BWPM_PLAN")) {
@@ -548,6 +551,20 @@ int plan_pick(int device, int precision, long long n, bool col, const Geometry& 
   auto it = plan_cache().find(key);
   if (it != plan_cache().end()) return it->second.variant;
 
+  // a device too full for the scratch planes keeps the default plan (and does not retry)
+  try {
+    plan_cache()[key] = plan_time<T, COMP, ST>(n, col, g0, stream);
+  } catch (const Error& e) {
+    if (e.code != kEnomem) throw;
+    (void)hipGetLastError();
+    plan_cache()[key] = PlanChoice{};
+  }
+  return plan_cache()[key].variant;
+}
+
+// Times both radix plans of line length n (plan_pick) on zeroed scratch planes.
+template <typename T, bool COMP, typename ST>
+PlanChoice plan_time(long long n, bool col, const Geometry& g0, hipStream_t stream) {
   const long long lines = std::min<long long>(n, 512);
   const int S = static_cast<int>(std::max<long long>(1, std::min<long long>(16, (4LL << 20) / (lines * n))));
   Launcher<T, COMP, ST> L;
@@ -629,8 +646,7 @@ int plan_pick(int device, int precision, long long n, bool col, const Geometry& 
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   c.variant = c.us[1] < c.us[0] ? 1 : 0;
-  plan_cache()[key] = c;
-  return c.variant;
+  return c;
 }
 
 // A lane step: one column launch and one row launch over every solve of the lane, or a Born
