@@ -16,11 +16,13 @@
 // rotations on the host. On power-of-two grids the operator G runs as three fused passes.
 #pragma once
 
+#include <atomic>
 #include <cstdlib>
 
 #include <cmath>
 #include <complex>
 #include <map>
+#include <mutex>
 #include <vector>
 
 #include "engine.hpp"
@@ -812,6 +814,39 @@ inline double born_tolerance(bool fp64_arith, bool narrow_storage) {
   return narrow_storage ? 1e-9 : 1e-12;
 }
 
+// Born steps running in the process (Born::forward/backward calls, every precision), for the
+// choice of running the emission on a second stream.
+inline std::atomic<int>& born_active_steps() {
+  static std::atomic<int> n{0};
+  return n;
+}
+struct ActiveSteps {
+  int n;
+  ActiveSteps() : n(++born_active_steps()) {}
+  ~ActiveSteps() { --born_active_steps(); }
+  ActiveSteps(const ActiveSteps&) = delete;
+  ActiveSteps& operator=(const ActiveSteps&) = delete;
+};
+
+// The emission streams: two per device for the whole process (default priority, non-blocking),
+// made on first use and shared by every Born solve there. A few shared streams rather than one per
+// solve: streams share the process's few hardware queues, and 64 extra streams for the 64 solves
+// of a queue put the solves' own streams on fewer queues (that batch ran 16 % slower).
+inline hipStream_t born_side_stream(int device) {
+  static std::mutex mu;
+  static std::map<int, std::vector<hipStream_t>> pools;
+  static int next = 0;
+  std::lock_guard<std::mutex> lock(mu);
+  std::vector<hipStream_t>& pool = pools[device];
+  if (pool.empty())
+    for (int i = 0; i < 2; ++i) {
+      hipStream_t st = nullptr;
+      HIP_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+      pool.push_back(st);
+    }
+  return pool[static_cast<size_t>(next++ % 2)];
+}
+
 template <typename T, bool COMP>
 class Born {
  public:
@@ -875,7 +910,7 @@ class Born {
       HIP_CHECK(hipMemcpy(kyT_.template as<void>(), b.data(), ny_ * sizeof(T), hipMemcpyHostToDevice));
     }
     const size_t pb = npix_ * sizeof(px2<T>);
-    for (auto* b : {&ehat_, &xh_, &xy0_, &xy1_, &zz_, &rhs_, &x_, &w_, &tmp_, &inc_}) b->alloc(pb);
+    for (auto* b : {&ehat_, &xh_, &xy0_, &xy1_, &zz_, &rhs_, &x_, &w_, &tmp_, &inc_, &ex0_, &ezz_, &etot_}) b->alloc(pb);
     basis_.alloc((kRestart + 1) * pb);
     // Born planes: born_bwd[p + 1] (R1) and born_fwd[p + 1] (R3) of each Born slab p
     planes_.alloc(2 * static_cast<size_t>(std::max<long long>(nborn, 1)) * pb);
@@ -887,11 +922,16 @@ class Born {
     HIP_CHECK(hipHostMalloc(&hgm_, sizeof(GmState), hipHostMallocDefault));
     if (s.synth.on) synthesise(s, host);
     fft_.init(ny_, nx_, px_, py_, stream_);
+    HIP_CHECK(hipEventCreateWithFlags(&ev_hat_, hipEventDisableTiming));
+    HIP_CHECK(hipEventCreateWithFlags(&ev_emit_, hipEventDisableTiming));
     HIP_CHECK(hipStreamSynchronize(stream_));
-    bytes_ = V_.bytes() + tabs_.bytes() + fields_.bytes() + optab_.bytes() + 10 * pb + basis_.bytes() +
+    bytes_ = V_.bytes() + tabs_.bytes() + fields_.bytes() + optab_.bytes() + 13 * pb + basis_.bytes() +
              planes_.bytes();
   }
   ~Born() {
+    if (estream_) (void)hipStreamSynchronize(estream_);
+    if (ev_hat_) (void)hipEventDestroy(ev_hat_);
+    if (ev_emit_) (void)hipEventDestroy(ev_emit_);
     if (hres_) (void)hipHostFree(hres_);
     if (hgm_) (void)hipHostFree(hgm_);
   }
@@ -923,16 +963,32 @@ class Born {
   }
 
   // Forward step p: increment of slab p into increment(); with `bidir` also R1 → bwd_plane(p + 1).
+  // The emission and the Cayley step both only read Ê. While at most kSideMax Born steps run in the
+  // process, the emission runs on a second stream beside the step and the caller's stream waits
+  // for it before the call returns; with more, the solves already overlap one another and the
+  // emission runs in line (a stream waiting on an event holds up the hardware queue it shares with
+  // other streams: C4's four Born workers ran 16 % slower with a second stream each). Either way
+  // the same kernels compute the same values.
   void forward(const px2<T>* e, long long p, bool bidir) {
+    const ActiveSteps active;
+    const bool side = active.n <= kSideMax;
     fft2(e, ehat_.template as<px2<T>>(), false);
+    if (bidir && side) emission_async(+1, -1, p, const_cast<px2<T>*>(plane(p, 0)));
     step(+1, p);
-    if (bidir) emission(+1, -1, p, const_cast<px2<T>*>(plane(p, 0)));
+    if (bidir && side) HIP_CHECK(hipStreamWaitEvent(stream_, ev_emit_, 0));
+    if (bidir && !side) emission(stream_, fft_, +1, -1, p, const_cast<px2<T>*>(plane(p, 0)));
   }
-  // Backward step at slab: R3 → fwd_plane(slab + 1), then the R2 increment into increment().
+  // Backward step at slab: R3 → fwd_plane(slab + 1), and the R2 increment into increment().
   void backward(const px2<T>* e, long long slab) {
+    const ActiveSteps active;
+    const bool side = active.n <= kSideMax;
     fft2(e, ehat_.template as<px2<T>>(), false);
-    emission(-1, +1, slab, const_cast<px2<T>*>(plane(slab, 1)));
+    if (side)
+      emission_async(-1, +1, slab, const_cast<px2<T>*>(plane(slab, 1)));
+    else
+      emission(stream_, fft_, -1, +1, slab, const_cast<px2<T>*>(plane(slab, 1)));
     step(-1, slab);
+    if (side) HIP_CHECK(hipStreamWaitEvent(stream_, ev_emit_, 0));
   }
 
  private:
@@ -965,6 +1021,12 @@ class Born {
   void launch(K kern, A... args) {
     const unsigned grid = static_cast<unsigned>((npix_ + 255) / 256);
     hipLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, stream_, args...);
+    HIP_CHECK(hipGetLastError());
+  }
+  template <typename K, typename... A>
+  void launch_on(hipStream_t st, K kern, A... args) {
+    const unsigned grid = static_cast<unsigned>((npix_ + 255) / 256);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, st, args...);
     HIP_CHECK(hipGetLastError());
   }
   template <typename K, typename... A>
@@ -1075,22 +1137,41 @@ class Born {
     HIP_CHECK(hipGetLastError());
   }
 
-  // R1 / R3: sources of direction d from Ê (in ehat_), emitted into direction dp, real space.
-  void emission(int d, int dp, long long slab, px2<T>* out) {
-    px2<T>* x0 = xy0_.template as<px2<T>>();
+  static constexpr int kSideMax = 2;
+
+  // The emission on the emission stream after Ê is ready; ev_emit_ marks its end. Its work planes
+  // (ex0_, xy1_, ezz_, etot_) are its own, so it may run beside the Cayley step, which reads Ê too
+  // but writes neither Ê nor them.
+  void emission_async(int d, int dp, long long slab, px2<T>* out) {
+    if (!estream_) {
+      int dev = 0;
+      HIP_CHECK(hipGetDevice(&dev));
+      estream_ = born_side_stream(dev);
+      efft_.init(ny_, nx_, px_, py_, estream_);
+    }
+    HIP_CHECK(hipEventRecord(ev_hat_, stream_));
+    HIP_CHECK(hipStreamWaitEvent(estream_, ev_hat_, 0));
+    emission(estream_, efft_, d, dp, slab, out);
+    HIP_CHECK(hipEventRecord(ev_emit_, estream_));
+  }
+  // R1 / R3: sources of direction d from Ê (in ehat_), emitted into direction dp, real space, on
+  // stream `st` with the transforms `f` (bound to st).
+  void emission(hipStream_t st, Fft2D<T, COMP>& f, int d, int dp, long long slab, px2<T>* out) {
+    px2<T>* x0 = ex0_.template as<px2<T>>();
     px2<T>* x1 = xy1_.template as<px2<T>>();
-    px2<T>* zz = zz_.template as<px2<T>>();
+    px2<T>* zz = ezz_.template as<px2<T>>();
     const BornF<T> tf = fields(slab);
-    launch(k_born_src_prepare<T>, static_cast<const px2<T>*>(ehat_.template as<px2<T>>()), tf, kxT_.template as<T>(),
-           kyT_.template as<T>(), static_cast<int>(nx_), npix_, static_cast<T>(d), d > 0 ? 0 : 1, x0, x1, zz);
+    launch_on(st, k_born_src_prepare<T>, static_cast<const px2<T>*>(ehat_.template as<px2<T>>()), tf,
+              kxT_.template as<T>(), kyT_.template as<T>(), static_cast<int>(nx_), npix_, static_cast<T>(d),
+              d > 0 ? 0 : 1, x0, x1, zz);
     px2<T>* planes[3] = {x0, x1, zz};
-    fft2_planes(planes, 3, true, potential(slab));
-    fft2_planes(planes, 3, false, nullptr);
-    px2<T>* tot = tmp_.template as<px2<T>>();
-    launch(k_born_emit<T>, static_cast<const px2<T>*>(x0), static_cast<const px2<T>*>(x1),
-           static_cast<const px2<T>*>(zz), tf, kxT_.template as<T>(), kyT_.template as<T>(), static_cast<int>(nx_),
-           npix_, static_cast<T>(dp), dp > 0 ? 0 : 1, inv_k2(slab), tot);
-    fft2(tot, out, true);
+    f.planes(planes, 3, true, potential(slab));
+    f.planes(planes, 3, false, nullptr);
+    px2<T>* tot = etot_.template as<px2<T>>();
+    launch_on(st, k_born_emit<T>, static_cast<const px2<T>*>(x0), static_cast<const px2<T>*>(x1),
+              static_cast<const px2<T>*>(zz), tf, kxT_.template as<T>(), kyT_.template as<T>(), static_cast<int>(nx_),
+              npix_, static_cast<T>(dp), dp > 0 ? 0 : 1, inv_k2(slab), tot);
+    f.one(tot, out, true);
   }
 
   // Cayley increment P_½(1 − G/2)⁻¹G·P_½Ê, real space, into inc_ (solver.cpp:313-350).
@@ -1383,6 +1464,10 @@ class Born {
   std::vector<c128> hosts_;
   Dev V_, tabs_, fields_, optab_, kxT_, kyT_;
   Dev ehat_, xh_, xy0_, xy1_, zz_, rhs_, x_, w_, tmp_, inc_, basis_, planes_;
+  Dev ex0_, ezz_, etot_;        // the emission's own work planes (with xy1_)
+  hipStream_t estream_ = nullptr;  // the emission's stream (born_side_stream), from the first use
+  hipEvent_t ev_hat_ = nullptr;    // Ê written (caller's stream)
+  hipEvent_t ev_emit_ = nullptr;   // emission done (emission stream)
   Dev partial_, dres_, coef_;
   double2* hres_ = nullptr;
   Dev gm_;                      // GmState of the current GMRES cycle
@@ -1408,6 +1493,7 @@ class Born {
   }();
   double2 host_coef_[kRestart + 2] = {};
   Fft2D<T, COMP> fft_;
+  Fft2D<T, COMP> efft_;  // the same transforms on the emission stream
   long long iterations_ = 0;
   size_t bytes_ = 0;
 };

@@ -282,9 +282,10 @@ size_t estimate_device_bytes(const Spec& s, int iterations) {
   bytes += m * 2 * npix * elem + (m > 1 ? m * (m - 1) * 4 * 6 * npix * elem : 0);
   // sparse interface values (two sets) and lists, bounded by two planes
   bytes += 2 * plane + 2 * npix * 8;
-  // Born: potential and emitted planes per slab, per-host tables, Krylov basis and work planes
+  // Born: potential and emitted planes per slab, per-host tables (the per-k struct of 13 complex
+  // values, 12 field planes and 3 operator planes), Krylov basis and 13 work planes
   if (!s.fluct.empty() || (s.synth.on && s.synth.sigma > 0))
-    bytes += static_cast<size_t>(s.nz) * (npix * elem + 2 * plane) + m * npix * 16 * elem + (31 + 12) * plane;
+    bytes += static_cast<size_t>(s.nz) * (npix * elem + 2 * plane) + m * npix * 28 * elem + (31 + 13) * plane;
   return bytes;
 }
 
