@@ -71,33 +71,20 @@ __global__ void k_gm_givens(GmState* st, const double2* __restrict__ res, int j,
   if (zabs(st->g[j + 1]) <= tol_abs || hnext == 0.0) st->done = 1;
 }
 
-__global__ void k_rich_begin(GmState* st) {
+__global__ void k_rich_begin(GmState* st, const double2* __restrict__ bb, double tol) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   st->done = 0;
   st->used = 0;
   st->beta = __builtin_huge_val();
+  st->tol_abs = tol * sqrt(bb->x);
 }
 
 __global__ __launch_bounds__(kRedThreads) void k_rich_check(const double2* __restrict__ partial, int count,
-                                                            GmState* st, double tol_abs, int max_steps) {
+                                                            GmState* st, int max_steps) {
   if (st->done) return;  // uniform: every thread reads the same flag before any synchronisation
   __shared__ double red[kRedThreads / 64];
-  double acc = 0;
-  for (int i = static_cast<int>(threadIdx.x); i < count; i += kRedThreads) acc += partial[i].x;
-  acc = wave_sum(acc);
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
-  __syncthreads();
-  if (threadIdx.x != 0) return;
-  double tot = 0;
-  for (int w = 0; w < kRedThreads / 64; ++w) tot += red[w];
-  const double step = sqrt(tot);
-  const int used = st->used + 1;
-  st->used = used;
-  if (step <= tol_abs)
-    st->done = 1;
-  else if ((used >= 2 && step > 0.5 * st->beta) || used >= max_steps || !(step == step))
-    st->done = 2;
-  st->beta = step;
+  const double tot = rich_sum(partial, count, red);
+  if (threadIdx.x == 0) rich_update_state(st, tot, max_steps);
 }
 
 }  // namespace bwpm

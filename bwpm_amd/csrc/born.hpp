@@ -17,6 +17,7 @@
 #pragma once
 
 #include <atomic>
+#include <cstddef>
 #include <cstdlib>
 
 #include <cmath>
@@ -122,6 +123,72 @@ __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
   return v;
+}
+
+// Reduction geometry of the GMRES vector kernels and the Richardson checks.
+inline constexpr int kRedBlocks = 512;
+inline constexpr int kRedThreads = 256;
+
+// GMRES restart length (solver.cpp:341) and the layout of the reduction results of one Arnoldi
+// step: h1 at [0, k), h2 at [kGmH2, kGmH2 + k), ‖w‖² at kGmN2.
+inline constexpr int kGmRestart = 30;
+inline constexpr int kGmH2 = kGmRestart + 2;
+inline constexpr int kGmN2 = 2 * (kGmRestart + 2);
+
+// Device state of one GMRES restart cycle. The Givens rotations, the rotated Hessenberg matrix and
+// the residual vector g are updated on the device after every Arnoldi step (k_gm_givens), so the
+// host enqueues Arnoldi steps without waiting for them and reads this state once per chunk of
+// steps. `done` gates the later steps of the cycle (their kernels take &done as `skip`).
+struct GmState {
+  int done;   // converged (|g[used]| ≤ tol·‖b‖), broken down (‖w‖ = 0), or ‖r₀‖ ≤ tol·‖b‖ at the start
+  int used;   // Arnoldi steps of the cycle
+  double beta;  // ‖r₀‖
+  double tol_abs;  // Richardson: tol·‖b‖, set on the device by k_rich_begin
+  double2 g[kGmRestart + 1];
+  double2 cs[kGmRestart], sn[kGmRestart];
+  double2 h[(kGmRestart + 1) * kGmRestart];  // row i, column j at i·kGmRestart + j (rotated)
+};
+
+// Richardson iteration state in a GmState: k_rich_begin starts it (done = 0, used = 0, the last
+// step size in beta = ∞, tol_abs = tol·‖b‖ from bb = (‖b‖², ·) of a reduction);
+// k_rich_check takes ‖x_new − x_old‖ from the `count` partial sums of the step (k_rich_update, or
+// the operator's last pass in mode 2), counts the step, and sets done = 1 when it is at most tol_abs, or done = 2 when the iteration
+// does not contract fast enough (a step larger than half the previous one, or max_steps steps),
+// after which the caller switches to GMRES. One workgroup of kRedThreads threads; both return at
+// once when the state is done.
+__global__ void k_rich_begin(GmState* st, const double2* __restrict__ bb, double tol);
+__global__ __launch_bounds__(kRedThreads) void k_rich_check(const double2* __restrict__ partial, int count,
+                                                            GmState* st, int max_steps);
+
+// Σ partial[i].x, i < count, by one workgroup of 64·k threads, in the order of a kRedThreads-thread
+// reduction whatever k is: virtual thread v < kRedThreads sums partial[v], partial[v + kRedThreads],
+// …; wave_sum sums each virtual wave, and thread 0 the waves in order. red: kRedThreads / 64
+// doubles of LDS. Thread 0 returns the sum.
+__device__ inline double rich_sum(const double2* partial, int count, double* red) {
+  for (int v0 = 0; v0 < kRedThreads; v0 += static_cast<int>(blockDim.x)) {
+    const int v = v0 + static_cast<int>(threadIdx.x);
+    double acc = 0;
+    if (v < kRedThreads)
+      for (int i = v; i < count; i += kRedThreads) acc += partial[i].x;
+    acc = wave_sum(acc);
+    if ((threadIdx.x & 63) == 0 && v < kRedThreads) red[v >> 6] = acc;
+  }
+  __syncthreads();
+  double tot = 0;
+  if (threadIdx.x == 0)
+    for (int w = 0; w < kRedThreads / 64; ++w) tot += red[w];
+  return tot;
+}
+// The state update of a Richardson step whose ‖x_new − x_old‖² is `tot` (thread 0 only).
+__device__ inline void rich_update_state(GmState* st, double tot, int max_steps) {
+  const double step = sqrt(tot);
+  const int used = st->used + 1;
+  st->used = used;
+  if (step <= st->tol_abs)
+    st->done = 1;
+  else if ((used >= 2 && step > 0.5 * st->beta) || used >= max_steps || !(step == step))
+    st->done = 2;
+  st->beta = step;
 }
 
 // ---- G_d in three fused passes (power-of-two grids) ------------------------------------------------
@@ -290,7 +357,10 @@ __global__ __launch_bounds__(N / 8 * W) void k_born_op_cols(px2<T>* xy, cx<T>* z
 }
 
 // Pass 3, one row per RW·N/8 threads: forward row transforms of (x, y) and z, then
-// out = i dz/(2k_z)·[P(k_d)·S]⊥ (mode 0) or v − ½ that (mode 1), as k_born_finish_op.
+// out = i dz/(2k_z)·[P(k_d)·S]⊥ (mode 0) or v − ½ that (mode 1), as k_born_finish_op. Mode 2 is
+// the Richardson step x ← b + ½·that (b = rb, out == v == x): each workgroup stores its sum of
+// ‖x_new − x_old‖² in rpart[block] for k_rich_check. (The check in the last workgroup to arrive,
+// behind a device-wide fence, made this pass 7 µs longer at 512² than the separate check.)
 template <int N, int RW, typename T, bool COMP>
 __global__ __launch_bounds__(N / 8 * RW) void k_born_op_rows_fwd(const px2<T>* __restrict__ xy,
                                                                 const cx<T>* __restrict__ zl, BornOp<T> op,
@@ -415,9 +485,6 @@ __global__ void k_born_emit(const px2<T>* __restrict__ xy0, const px2<T>* __rest
 
 // ---- GMRES vector kernels (deterministic: fixed grid, fixed reduction order) ----------------------
 
-inline constexpr int kRedBlocks = 512;
-inline constexpr int kRedThreads = 256;
-
 // Kernels with a `skip` flag (nullptr: never) return at once while *skip is set: the Arnoldi steps
 // that the host enqueues past a GMRES cycle's convergence (which the device detects) do no work.
 __global__ __launch_bounds__(kRedThreads) void k_dot_final(const double2* __restrict__ partial, double2* __restrict__ out,
@@ -473,34 +540,6 @@ __global__ void k_sub_basis(px2<T>* w, const px2<T>* __restrict__ basis, long lo
 // Largest basis the fused projections below take (the restart length plus one fits).
 inline constexpr int kMaxBasis = 32;
 
-// GMRES restart length (solver.cpp:341) and the layout of the reduction results of one Arnoldi
-// step: h1 at [0, k), h2 at [kGmH2, kGmH2 + k), ‖w‖² at kGmN2.
-inline constexpr int kGmRestart = 30;
-inline constexpr int kGmH2 = kGmRestart + 2;
-inline constexpr int kGmN2 = 2 * (kGmRestart + 2);
-
-// Device state of one GMRES restart cycle. The Givens rotations, the rotated Hessenberg matrix and
-// the residual vector g are updated on the device after every Arnoldi step (k_gm_givens), so the
-// host enqueues Arnoldi steps without waiting for them and reads this state once per chunk of
-// steps. `done` gates the later steps of the cycle (their kernels take &done as `skip`).
-struct GmState {
-  int done;   // converged (|g[used]| ≤ tol·‖b‖), broken down (‖w‖ = 0), or ‖r₀‖ ≤ tol·‖b‖ at the start
-  int used;   // Arnoldi steps of the cycle
-  double beta;  // ‖r₀‖
-  double2 g[kGmRestart + 1];
-  double2 cs[kGmRestart], sn[kGmRestart];
-  double2 h[(kGmRestart + 1) * kGmRestart];  // row i, column j at i·kGmRestart + j (rotated)
-};
-
-// Richardson iteration state in a GmState: k_rich_begin starts it (done = 0, used = 0, the last
-// step size in beta = ∞); k_rich_check takes ‖x_new − x_old‖ from the `count` partial sums of the
-// step (k_rich_update, or the operator's last pass in mode 2), counts the step, and sets done = 1 when it is at most tol_abs, or done = 2 when
-// the iteration does not contract fast enough (a step larger than half the previous one, or
-// max_steps steps), after which the caller switches to GMRES. One workgroup of
-// kRedThreads threads; both return at once when the state is done.
-__global__ void k_rich_begin(GmState* st);
-__global__ __launch_bounds__(kRedThreads) void k_rich_check(const double2* __restrict__ partial, int count,
-                                                            GmState* st, double tol_abs, int max_steps);
 
 // Arnoldi step j's Hessenberg column from the reductions `res` (h1 + h2, ‖w‖), the previous
 // rotations applied to it, the new rotation, and the convergence test against tol_abs = tol·‖b‖
@@ -1318,18 +1357,31 @@ class Born {
   bool solve(int d, long long slab) {
     const px2<T>* b = rhs_.template as<px2<T>>();
     px2<T>* x = x_.template as<px2<T>>();
-    const double bnorm = norm(b);
-    if (bnorm == 0.0) {
-      launch(k_axpby<T>, static_cast<const px2<T>*>(x), static_cast<T>(0), static_cast<const px2<T>*>(nullptr),
-             static_cast<T>(0), x, npix_);
-      return true;
+    if (!richardson_) {
+      const double bnorm = norm(b);
+      if (bnorm == 0.0) {
+        launch(k_axpby<T>, static_cast<const px2<T>*>(x), static_cast<T>(0), static_cast<const px2<T>*>(nullptr),
+               static_cast<T>(0), x, npix_);
+        return true;
+      }
+      tol_abs_ = tol_ * bnorm;
+      HIP_CHECK(hipMemcpyAsync(x, b, npix_ * sizeof(px2<T>), hipMemcpyDeviceToDevice, stream_));
+      return gmres(d, slab);
     }
-    tol_abs_ = tol_ * bnorm;
     bool fused = false;
     if constexpr (COMP == (sizeof(T) == 4)) fused = fft_.fast();
-    if (!(richardson_ && fused))
-      HIP_CHECK(hipMemcpyAsync(x, b, npix_ * sizeof(px2<T>), hipMemcpyDeviceToDevice, stream_));
-    if (richardson_ && richardson(d, slab, fused)) return true;
+    if (!fused) HIP_CHECK(hipMemcpyAsync(x, b, npix_ * sizeof(px2<T>), hipMemcpyDeviceToDevice, stream_));
+    // ‖b‖² into dres_[0], from which k_rich_begin sets tol·‖b‖ on the device (no host round trip;
+    // b = 0 gives x = 0 in the first step, whose size 0 meets the tolerance 0)
+    hipLaunchKernelGGL(k_dot_basis<T>, dim3(kRedBlocks, 1), dim3(kRedThreads), 0, stream_, b,
+                       static_cast<long long>(npix_), b, npix_, partial_.template as<double2>(),
+                       static_cast<const int*>(nullptr));
+    HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(kRedThreads), 0, stream_, partial_.template as<double2>(),
+                       dres_.template as<double2>(), static_cast<const int*>(nullptr));
+    HIP_CHECK(hipGetLastError());
+    if (richardson(d, slab, fused)) return true;
+    tol_abs_ = hgm_->tol_abs;  // from the state read after the last chunk of steps
     return gmres(d, slab);
   }
 
@@ -1347,7 +1399,7 @@ class Born {
     GmState* st = gm_.template as<GmState>();
     const int* skip = &st->done;
     double2* part = partial_.template as<double2>();
-    hipLaunchKernelGGL(k_rich_begin, dim3(1), dim3(1), 0, stream_, st);
+    hipLaunchKernelGGL(k_rich_begin, dim3(1), dim3(1), 0, stream_, st, static_cast<const double2*>(dres_.template as<double2>()), tol_);
     HIP_CHECK(hipGetLastError());
     // on the fused operator the step itself runs in its last pass (mode 2: x ← b + ½·G·x in place,
     // the first step from x = b read from b, with one partial sum of the step per workgroup)
@@ -1369,10 +1421,11 @@ class Born {
           HIP_CHECK(hipGetLastError());
         }
         hipLaunchKernelGGL(k_rich_check, dim3(1), dim3(kRedThreads), 0, stream_, static_cast<const double2*>(part),
-                           count, st, tol_abs_, kRichMaxSteps);
+                           count, st, kRichMaxSteps);
         HIP_CHECK(hipGetLastError());
       }
-      HIP_CHECK(hipMemcpyAsync(hgm_, st, sizeof(GmState), hipMemcpyDeviceToHost, stream_));
+      // the state's head (done, used, the step sizes and tol·‖b‖), not the GMRES arrays
+      HIP_CHECK(hipMemcpyAsync(hgm_, st, offsetof(GmState, g), hipMemcpyDeviceToHost, stream_));
       HIP_CHECK(hipStreamSynchronize(stream_));
       if (hgm_->done != 0) {
         iterations_ += hgm_->used;
