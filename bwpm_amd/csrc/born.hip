@@ -71,12 +71,15 @@ __global__ void k_gm_givens(GmState* st, const double2* __restrict__ res, int j,
   if (zabs(st->g[j + 1]) <= tol_abs || hnext == 0.0) st->done = 1;
 }
 
-__global__ void k_rich_begin(GmState* st, const double2* __restrict__ bb, double tol) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+__global__ __launch_bounds__(kRedThreads) void k_rich_begin(GmState* st, const double2* __restrict__ bsq, int count,
+                                                            double tol) {
+  __shared__ double red[kRedThreads / 64];
+  const double tot = rich_sum(bsq, count, red);
+  if (threadIdx.x != 0) return;
   st->done = 0;
   st->used = 0;
   st->beta = __builtin_huge_val();
-  st->tol_abs = tol * sqrt(bb->x);
+  st->tol_abs = tol * sqrt(tot);
 }
 
 __global__ __launch_bounds__(kRedThreads) void k_rich_check(const double2* __restrict__ partial, int count,

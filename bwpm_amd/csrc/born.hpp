@@ -150,13 +150,14 @@ struct GmState {
 };
 
 // Richardson iteration state in a GmState: k_rich_begin starts it (done = 0, used = 0, the last
-// step size in beta = ∞, tol_abs = tol·‖b‖ from bb = (‖b‖², ·) of a reduction);
+// step size in beta = ∞, tol_abs = tol·‖b‖ from the `count` partial sums of ‖b‖² in bsq[i].x);
 // k_rich_check takes ‖x_new − x_old‖ from the `count` partial sums of the step (k_rich_update, or
 // the operator's last pass in mode 2), counts the step, and sets done = 1 when it is at most tol_abs, or done = 2 when the iteration
 // does not contract fast enough (a step larger than half the previous one, or max_steps steps),
 // after which the caller switches to GMRES. One workgroup of kRedThreads threads; both return at
 // once when the state is done.
-__global__ void k_rich_begin(GmState* st, const double2* __restrict__ bb, double tol);
+__global__ __launch_bounds__(kRedThreads) void k_rich_begin(GmState* st, const double2* __restrict__ bsq, int count,
+                                                            double tol);
 __global__ __launch_bounds__(kRedThreads) void k_rich_check(const double2* __restrict__ partial, int count,
                                                             GmState* st, int max_steps);
 
@@ -260,9 +261,11 @@ __global__ void k_born_fields(const BornK<T>* __restrict__ tb, long long npix, c
 }
 
 // Pass 1, one row per RW·N/8 threads: blockIdx.y = 0 transforms the propagating part of
-// (S_x, S_y), blockIdx.y = 1 the line S_z = −d(k_x S_x + k_y S_y)/k_z (inverse, unscaled).
+// (S_x, S_y), blockIdx.y = 1 the line S_z = −d(k_x S_x + k_y S_y)/k_z (inverse, unscaled), with
+// S = v, or S = v·pre per pixel (pre: P_½, which saves the half step's own pass; nullptr: none).
 template <int N, int RW, typename T, bool COMP>
-__global__ __launch_bounds__(N / 8 * RW) void k_born_op_rows_inv(const px2<T>* __restrict__ v, BornOp<T> op,
+__global__ __launch_bounds__(N / 8 * RW) void k_born_op_rows_inv(const px2<T>* __restrict__ v,
+                                                                const cx<T>* __restrict__ pre, BornOp<T> op,
                                                                 const T* __restrict__ kxT, const T* __restrict__ kyT,
                                                                 T d, px2<T>* __restrict__ xy, cx<T>* __restrict__ zl,
                                                                 const cx<T>* __restrict__ g_twh,
@@ -288,7 +291,11 @@ __global__ __launch_bounds__(N / 8 * RW) void k_born_op_rows_inv(const px2<T>* _
       const long long q = row + t + TT * m;
       const cx<T> cz = op.cz[q];
       const bool prop = cz.x != 0 || cz.y != 0;
-      const px2<T> s = v[q];
+      px2<T> s = v[q];
+      if (pre) {
+        const cx<T> f = pre[q];
+        s = px2<T>{s.a * f, s.b * f};
+      }
       a[m] = prop ? s : px2<T>{};
     }
     reg_fft<N, COMP, true, TWL>(a, sh + r * LQ, t, twh, twl);
@@ -302,7 +309,11 @@ __global__ __launch_bounds__(N / 8 * RW) void k_born_op_rows_inv(const px2<T>* _
       const long long q = row + t + TT * m;
       const cx<T> cz = op.cz[q];
       const bool prop = cz.x != 0 || cz.y != 0;
-      const px2<T> s = v[q];
+      px2<T> s = v[q];
+      if (pre) {
+        const cx<T> f = pre[q];
+        s = px2<T>{s.a * f, s.b * f};
+      }
       const cx<T> ex = prop ? s.a : cx<T>{0, 0}, ey = prop ? s.b : cx<T>{0, 0};
       z[m] = (ex * kxT[t + TT * m] + ey * ky) * cz * (-d);
     }
@@ -360,7 +371,8 @@ __global__ __launch_bounds__(N / 8 * W) void k_born_op_cols(px2<T>* xy, cx<T>* z
 // out = i dz/(2k_z)·[P(k_d)·S]⊥ (mode 0) or v − ½ that (mode 1), as k_born_finish_op. Mode 2 is
 // the Richardson step x ← b + ½·that (b = rb, out == v == x): each workgroup stores its sum of
 // ‖x_new − x_old‖² in rpart[block] for k_rich_check. (The check in the last workgroup to arrive,
-// behind a device-wide fence, made this pass 7 µs longer at 512² than the separate check.)
+// behind a device-wide fence, made this pass 7 µs longer at 512² than the separate check.) Mode 3
+// is mode 0 with each workgroup's sum of ‖out‖² in rpart[block] (‖b‖ of the Cayley system).
 template <int N, int RW, typename T, bool COMP>
 __global__ __launch_bounds__(N / 8 * RW) void k_born_op_rows_fwd(const px2<T>* __restrict__ xy,
                                                                 const cx<T>* __restrict__ zl, BornOp<T> op,
@@ -416,10 +428,14 @@ __global__ __launch_bounds__(N / 8 * RW) void k_born_op_rows_fwd(const px2<T>* _
       const double d2 = static_cast<double>(g.b.x) - static_cast<double>(xo.b.x);
       const double d3 = static_cast<double>(g.b.y) - static_cast<double>(xo.b.y);
       step2 += d0 * d0 + d1 * d1 + d2 * d2 + d3 * d3;
+    } else if (mode == 3) {
+      const double d0 = static_cast<double>(g.a.x), d1 = static_cast<double>(g.a.y);
+      const double d2 = static_cast<double>(g.b.x), d3 = static_cast<double>(g.b.y);
+      step2 += d0 * d0 + d1 * d1 + d2 * d2 + d3 * d3;
     }
     out[q] = g;
   }
-  if (mode == 2) {  // the workgroup's sum of ‖x_new − x_old‖² into rpart[block] (waves in order)
+  if (mode >= 2) {  // the workgroup's sum of squares into rpart[block] (waves in order)
     __shared__ double red[16];
     const double w = wave_sum(step2);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = w;
@@ -1087,13 +1103,15 @@ class Born {
   // `skip` (nullptr: none) gates the fused passes (kernels with a skip flag); the generic path runs
   // regardless, and its output is then unused. Mode 2 (fused passes only, out == v == x): the
   // Richardson step x ← b + ½·G·x with b = rich_b_ and its partial sums into rich_part_.
-  void apply_g(const px2<T>* v, int d, long long slab, px2<T>* out, int mode, const int* skip = nullptr) {
+  void apply_g(const px2<T>* v, int d, long long slab, px2<T>* out, int mode, const int* skip = nullptr,
+               const cx<T>* pre = nullptr) {
     if constexpr (COMP == (sizeof(T) == 4)) {
       if (fft_.fast()) {
-        apply_g_fused(v, d, slab, out, mode, skip);
+        apply_g_fused(v, d, slab, out, mode, skip, pre);
         return;
       }
     }
+    require(pre == nullptr && mode <= 2, "the generic operator takes no pre-multiplied operand and no mode 3");
     const BornK<T>* tb = table(slab);
     px2<T>* xy = xy0_.template as<px2<T>>();
     px2<T>* zz = zz_.template as<px2<T>>();
@@ -1107,19 +1125,21 @@ class Born {
   }
 
   // The same operator in three fused passes (power-of-two grids in the register kernels' precision).
-  void apply_g_fused(const px2<T>* v, int d, long long slab, px2<T>* out, int mode, const int* skip) {
+  void apply_g_fused(const px2<T>* v, int d, long long slab, px2<T>* out, int mode, const int* skip,
+                     const cx<T>* pre) {
+    require(pre == nullptr || mode == 0 || mode == 3, "a pre-multiplied operand serves modes 0 and 3 only");
     px2<T>* xy = xy0_.template as<px2<T>>();
     cx<T>* zl = zz_.template as<cx<T>>();
     const cx<T>* ot = optab_.template as<cx<T>>() + 3 * slab_tab_[slab] * npix_;
     const BornOp<T> op{ot, ot + npix_, ot + 2 * npix_};
     const T dd = static_cast<T>(d);
     switch (nx_) {
-      case 64: op_rows<64>(v, op, dd, xy, zl, inv_k2(slab), out, mode, false, skip); break;
-      case 128: op_rows<128>(v, op, dd, xy, zl, inv_k2(slab), out, mode, false, skip); break;
-      case 256: op_rows<256>(v, op, dd, xy, zl, inv_k2(slab), out, mode, false, skip); break;
-      case 512: op_rows<512>(v, op, dd, xy, zl, inv_k2(slab), out, mode, false, skip); break;
-      case 1024: op_rows<1024>(v, op, dd, xy, zl, inv_k2(slab), out, mode, false, skip); break;
-      default: op_rows<2048>(v, op, dd, xy, zl, inv_k2(slab), out, mode, false, skip); break;
+      case 64: op_rows<64>(v, pre, op, dd, xy, zl, inv_k2(slab), out, mode, false, skip); break;
+      case 128: op_rows<128>(v, pre, op, dd, xy, zl, inv_k2(slab), out, mode, false, skip); break;
+      case 256: op_rows<256>(v, pre, op, dd, xy, zl, inv_k2(slab), out, mode, false, skip); break;
+      case 512: op_rows<512>(v, pre, op, dd, xy, zl, inv_k2(slab), out, mode, false, skip); break;
+      case 1024: op_rows<1024>(v, pre, op, dd, xy, zl, inv_k2(slab), out, mode, false, skip); break;
+      default: op_rows<2048>(v, pre, op, dd, xy, zl, inv_k2(slab), out, mode, false, skip); break;
     }
     switch (ny_) {
       case 64: op_cols<64>(xy, zl, potential(slab), skip); break;
@@ -1130,12 +1150,12 @@ class Born {
       default: op_cols<2048>(xy, zl, potential(slab), skip); break;
     }
     switch (nx_) {
-      case 64: op_rows<64>(v, op, dd, xy, zl, inv_k2(slab), out, mode, true, skip); break;
-      case 128: op_rows<128>(v, op, dd, xy, zl, inv_k2(slab), out, mode, true, skip); break;
-      case 256: op_rows<256>(v, op, dd, xy, zl, inv_k2(slab), out, mode, true, skip); break;
-      case 512: op_rows<512>(v, op, dd, xy, zl, inv_k2(slab), out, mode, true, skip); break;
-      case 1024: op_rows<1024>(v, op, dd, xy, zl, inv_k2(slab), out, mode, true, skip); break;
-      default: op_rows<2048>(v, op, dd, xy, zl, inv_k2(slab), out, mode, true, skip); break;
+      case 64: op_rows<64>(v, pre, op, dd, xy, zl, inv_k2(slab), out, mode, true, skip); break;
+      case 128: op_rows<128>(v, pre, op, dd, xy, zl, inv_k2(slab), out, mode, true, skip); break;
+      case 256: op_rows<256>(v, pre, op, dd, xy, zl, inv_k2(slab), out, mode, true, skip); break;
+      case 512: op_rows<512>(v, pre, op, dd, xy, zl, inv_k2(slab), out, mode, true, skip); break;
+      case 1024: op_rows<1024>(v, pre, op, dd, xy, zl, inv_k2(slab), out, mode, true, skip); break;
+      default: op_rows<2048>(v, pre, op, dd, xy, zl, inv_k2(slab), out, mode, true, skip); break;
     }
   }
   // Lines per workgroup of the fused passes: `threads` threads per workgroup, at least one line.
@@ -1145,8 +1165,8 @@ class Born {
   }
   // Pass 1 (forward = false) or pass 3 (forward = true) of the fused operator; N = nx.
   template <int N>
-  void op_rows(const px2<T>* v, const BornOp<T>& op, T d, px2<T>* xy, cx<T>* zl, cx<T> ik2, px2<T>* out, int mode,
-               bool forward, const int* skip) {
+  void op_rows(const px2<T>* v, const cx<T>* pre, const BornOp<T>& op, T d, px2<T>* xy, cx<T>* zl, cx<T> ik2,
+               px2<T>* out, int mode, bool forward, const int* skip) {
     constexpr int RW = op_lines<N>(kOpThreads);
     static_assert(RW <= 64, "a power-of-two ny of at least 64 holds whole workgroups");
     const FftPlan& p = fft_.row_plan();
@@ -1157,10 +1177,10 @@ class Born {
       hipLaunchKernelGGL((k_born_op_rows_fwd<N, RW, T, COMP>), dim3(rows), dim3(N / 8 * RW), 0, stream_,
                          static_cast<const px2<T>*>(xy), static_cast<const cx<T>*>(zl), op, kxT_.template as<T>(),
                          kyT_.template as<T>(), d, ik2, v, out, mode, mode == 2 ? rich_b_ : nullptr,
-                         mode == 2 ? rich_part_ : nullptr, th, tl, skip);
-      if (mode == 2) rich_count_ = static_cast<int>(rows);
+                         mode >= 2 ? rich_part_ : nullptr, th, tl, skip);
+      if (mode >= 2) rich_count_ = static_cast<int>(rows);
     } else
-      hipLaunchKernelGGL((k_born_op_rows_inv<N, RW, T, COMP>), dim3(rows, 2), dim3(N / 8 * RW), 0, stream_, v, op,
+      hipLaunchKernelGGL((k_born_op_rows_inv<N, RW, T, COMP>), dim3(rows, 2), dim3(N / 8 * RW), 0, stream_, v, pre, op,
                          kxT_.template as<T>(), kyT_.template as<T>(), d, xy, zl, th, tl, skip);
     HIP_CHECK(hipGetLastError());
   }
@@ -1214,17 +1234,30 @@ class Born {
   }
 
   // Cayley increment P_½(1 − G/2)⁻¹G·P_½Ê, real space, into inc_ (solver.cpp:313-350).
+  // On the fused operator the half steps ride in the passes beside them: P_½Ê in the operator's
+  // first pass, ‖b‖² in its last (mode 3), and x·P_½^{±1} in the inverse transform's row pass.
   void step(int d, long long slab) {
+    bool fused = false;
+    if constexpr (COMP == (sizeof(T) == 4)) fused = fft_.fast();
+    const cx<T>* hout = d > 0 ? fields(slab).half() : fields(slab).halfinv();
+    px2<T>* b = rhs_.template as<px2<T>>();
+    px2<T>* x = x_.template as<px2<T>>();
+    if (fused) {
+      rich_part_ = partial_.template as<double2>();
+      apply_g(ehat_.template as<px2<T>>(), d, slab, b, 3, nullptr, fields(slab).half());
+      rich_part_ = nullptr;
+      if (!solve(d, slab, rich_count_))
+        throw Error(kEnoconv, "GMRES did not converge in the Born step");
+      fft_.one(x, inc_.template as<px2<T>>(), true, hout);
+      return;
+    }
     px2<T>* xh = xh_.template as<px2<T>>();
     launch(k_born_half<T>, static_cast<const px2<T>*>(ehat_.template as<px2<T>>()), fields(slab).half(), npix_, xh);
-    px2<T>* b = rhs_.template as<px2<T>>();
     apply_g(xh, d, slab, b, 0);
-    px2<T>* x = x_.template as<px2<T>>();
-    if (!solve(d, slab))
+    if (!solve(d, slab, 0))
       throw Error(kEnoconv, "GMRES did not converge in the Born step");
     px2<T>* y = tmp_.template as<px2<T>>();
-    launch(k_born_half<T>, static_cast<const px2<T>*>(x), d > 0 ? fields(slab).half() : fields(slab).halfinv(), npix_,
-           y);
+    launch(k_born_half<T>, static_cast<const px2<T>*>(x), hout, npix_, y);
     fft2(y, inc_.template as<px2<T>>(), true);
   }
 
@@ -1354,7 +1387,8 @@ class Born {
   // GMRES from where they stopped if they do not contract (or always GMRES with
   // BWPM_BORN_SOLVER=gmres). Both stop at ‖b − (1 − G/2)x‖ ≤ tol·‖b‖. The fused Richardson step
   // reads the start value from b itself; every other path copies it into x_ first.
-  bool solve(int d, long long slab) {
+  // `bparts` > 0: partial_ already holds that many partial sums of ‖b‖² (the operator's mode 3).
+  bool solve(int d, long long slab, int bparts) {
     const px2<T>* b = rhs_.template as<px2<T>>();
     px2<T>* x = x_.template as<px2<T>>();
     if (!richardson_) {
@@ -1371,16 +1405,18 @@ class Born {
     bool fused = false;
     if constexpr (COMP == (sizeof(T) == 4)) fused = fft_.fast();
     if (!fused) HIP_CHECK(hipMemcpyAsync(x, b, npix_ * sizeof(px2<T>), hipMemcpyDeviceToDevice, stream_));
-    // ‖b‖² into dres_[0], from which k_rich_begin sets tol·‖b‖ on the device (no host round trip;
-    // b = 0 gives x = 0 in the first step, whose size 0 meets the tolerance 0)
-    hipLaunchKernelGGL(k_dot_basis<T>, dim3(kRedBlocks, 1), dim3(kRedThreads), 0, stream_, b,
-                       static_cast<long long>(npix_), b, npix_, partial_.template as<double2>(),
-                       static_cast<const int*>(nullptr));
-    HIP_CHECK(hipGetLastError());
-    hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(kRedThreads), 0, stream_, partial_.template as<double2>(),
-                       dres_.template as<double2>(), static_cast<const int*>(nullptr));
-    HIP_CHECK(hipGetLastError());
-    if (richardson(d, slab, fused)) return true;
+    // the partial sums of ‖b‖², from which k_rich_begin sets tol·‖b‖ on the device (no host round
+    // trip; b = 0 gives x = 0 in the first step, whose size 0 meets the tolerance 0)
+    const double2* bsq = partial_.template as<double2>();
+    int nbsq = bparts;
+    if (nbsq <= 0) {
+      hipLaunchKernelGGL(k_dot_basis<T>, dim3(kRedBlocks, 1), dim3(kRedThreads), 0, stream_, b,
+                         static_cast<long long>(npix_), b, npix_, partial_.template as<double2>(),
+                         static_cast<const int*>(nullptr));
+      HIP_CHECK(hipGetLastError());
+      nbsq = kRedBlocks;
+    }
+    if (richardson(d, slab, fused, bsq, nbsq)) return true;
     tol_abs_ = hgm_->tol_abs;  // from the state read after the last chunk of steps
     return gmres(d, slab);
   }
@@ -1392,14 +1428,14 @@ class Born {
   // iterate, and the returned iterate's residual is smaller still). The steps are enqueued in
   // chunks, gated by the state's done flag like the Arnoldi steps; returns false (x holding the
   // last iterate) when the contraction is too slow.
-  bool richardson(int d, long long slab, bool fused) {
+  bool richardson(int d, long long slab, bool fused, const double2* bsq, int nbsq) {
     const px2<T>* b = rhs_.template as<px2<T>>();
     px2<T>* x = x_.template as<px2<T>>();
     px2<T>* w = w_.template as<px2<T>>();
     GmState* st = gm_.template as<GmState>();
     const int* skip = &st->done;
     double2* part = partial_.template as<double2>();
-    hipLaunchKernelGGL(k_rich_begin, dim3(1), dim3(1), 0, stream_, st, static_cast<const double2*>(dres_.template as<double2>()), tol_);
+    hipLaunchKernelGGL(k_rich_begin, dim3(1), dim3(kRedThreads), 0, stream_, st, bsq, nbsq, tol_);
     HIP_CHECK(hipGetLastError());
     // on the fused operator the step itself runs in its last pass (mode 2: x ← b + ½·G·x in place,
     // the first step from x = b read from b, with one partial sum of the step per workgroup)

@@ -20,6 +20,7 @@ struct PlaneSet {
   const px2<T>* in[kMaxPlanes];
   px2<T>* out[kMaxPlanes];
   const cx<T>* mul[kMaxPlanes];  // per-pixel multiplier of both components (nullptr: none)
+  const cx<T>* pre[kMaxPlanes];  // per-pixel multiplier of the input (register row kernels only)
   T scale;                       // applied to every output value
 };
 
@@ -95,11 +96,19 @@ __global__ __launch_bounds__(N / 8 * RW) void k_fft2_rows_fast(PlaneSet<T> ps, c
   const px2<T>* in = ps.in[blockIdx.y];
   px2<T>* out = ps.out[blockIdx.y];
   const cx<T>* mul = ps.mul[blockIdx.y];
+  const cx<T>* pre = ps.pre[blockIdx.y];
   const int r = threadIdx.x / TT, t = threadIdx.x % TT;
   const long long row = static_cast<long long>(blockIdx.x * RW + r) * N;
   px2<T> v[8];
 #pragma unroll
   for (int m = 0; m < 8; ++m) v[m] = in[row + t + TT * m];
+  if (pre) {
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+      const cx<T> f = pre[row + t + TT * m];
+      v[m] = px2<T>{v[m].a * f, v[m].b * f};
+    }
+  }
   reg_fft<N, COMP, INV>(v, sh + r * LQ, t, twh, twl);
 #pragma unroll
   for (int m = 0; m < 8; ++m) out[row + t + TT * m] = post(v[m], mul, row + t + TT * m, ps.scale);
@@ -191,9 +200,13 @@ class Fft2D {
   const FftPlan& row_plan() const { return px_; }
   const FftPlan& col_plan() const { return py_; }
 
-  // out = F(in) or F⁻¹(in) (1/N scaled); `in` is left unchanged unless it is `out`.
-  void one(const px2<T>* in, px2<T>* out, bool inverse) {
+  // out = F(in·pre) or F⁻¹(in·pre) (1/N scaled; pre: per-pixel multiplier of the input, nullptr:
+  // none, only where the row pass runs the register kernels); `in` is left unchanged unless it is
+  // `out`.
+  void one(const px2<T>* in, px2<T>* out, bool inverse, const cx<T>* pre = nullptr) {
+    require(pre == nullptr || rows_fast_, "an input multiplier needs the register row kernels");
     PlaneSet<T> a{}, b{};
+    a.pre[0] = pre;
     a.in[0] = in;
     a.out[0] = out;
     b.in[0] = out;
