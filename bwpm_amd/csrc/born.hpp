@@ -1139,7 +1139,12 @@ class Born {
       case 256: op_rows<256>(v, pre, op, dd, xy, zl, inv_k2(slab), out, mode, false, skip); break;
       case 512: op_rows<512>(v, pre, op, dd, xy, zl, inv_k2(slab), out, mode, false, skip); break;
       case 1024: op_rows<1024>(v, pre, op, dd, xy, zl, inv_k2(slab), out, mode, false, skip); break;
-      default: op_rows<2048>(v, pre, op, dd, xy, zl, inv_k2(slab), out, mode, false, skip); break;
+      case 2048: op_rows<2048>(v, pre, op, dd, xy, zl, inv_k2(slab), out, mode, false, skip); break;
+      case 192: op_rows<192>(v, pre, op, dd, xy, zl, inv_k2(slab), out, mode, false, skip); break;
+      case 384: op_rows<384>(v, pre, op, dd, xy, zl, inv_k2(slab), out, mode, false, skip); break;
+      case 768: op_rows<768>(v, pre, op, dd, xy, zl, inv_k2(slab), out, mode, false, skip); break;
+      case 1536: op_rows<1536>(v, pre, op, dd, xy, zl, inv_k2(slab), out, mode, false, skip); break;
+      default: throw Error(kEinternal, "fused Born operator: unsupported row length");
     }
     switch (ny_) {
       case 64: op_cols<64>(xy, zl, potential(slab), skip); break;
@@ -1147,7 +1152,12 @@ class Born {
       case 256: op_cols<256>(xy, zl, potential(slab), skip); break;
       case 512: op_cols<512>(xy, zl, potential(slab), skip); break;
       case 1024: op_cols<1024>(xy, zl, potential(slab), skip); break;
-      default: op_cols<2048>(xy, zl, potential(slab), skip); break;
+      case 2048: op_cols<2048>(xy, zl, potential(slab), skip); break;
+      case 192: op_cols<192>(xy, zl, potential(slab), skip); break;
+      case 384: op_cols<384>(xy, zl, potential(slab), skip); break;
+      case 768: op_cols<768>(xy, zl, potential(slab), skip); break;
+      case 1536: op_cols<1536>(xy, zl, potential(slab), skip); break;
+      default: throw Error(kEinternal, "fused Born operator: unsupported column length");
     }
     switch (nx_) {
       case 64: op_rows<64>(v, pre, op, dd, xy, zl, inv_k2(slab), out, mode, true, skip); break;
@@ -1155,13 +1165,23 @@ class Born {
       case 256: op_rows<256>(v, pre, op, dd, xy, zl, inv_k2(slab), out, mode, true, skip); break;
       case 512: op_rows<512>(v, pre, op, dd, xy, zl, inv_k2(slab), out, mode, true, skip); break;
       case 1024: op_rows<1024>(v, pre, op, dd, xy, zl, inv_k2(slab), out, mode, true, skip); break;
-      default: op_rows<2048>(v, pre, op, dd, xy, zl, inv_k2(slab), out, mode, true, skip); break;
+      case 2048: op_rows<2048>(v, pre, op, dd, xy, zl, inv_k2(slab), out, mode, true, skip); break;
+      case 192: op_rows<192>(v, pre, op, dd, xy, zl, inv_k2(slab), out, mode, true, skip); break;
+      case 384: op_rows<384>(v, pre, op, dd, xy, zl, inv_k2(slab), out, mode, true, skip); break;
+      case 768: op_rows<768>(v, pre, op, dd, xy, zl, inv_k2(slab), out, mode, true, skip); break;
+      case 1536: op_rows<1536>(v, pre, op, dd, xy, zl, inv_k2(slab), out, mode, true, skip); break;
+      default: throw Error(kEinternal, "fused Born operator: unsupported row length");
     }
   }
-  // Lines per workgroup of the fused passes: `threads` threads per workgroup, at least one line.
+  // Lines per workgroup of the fused passes: about `threads` threads per workgroup (a power of two
+  // of lines, at least one), and whole waves, which the partial sums of the last pass need (a line
+  // of 384 points has 48 threads: four lines, 192 threads).
   template <int N>
   static constexpr int op_lines(int threads) {
-    return threads / (N / 8) > 0 ? threads / (N / 8) : 1;
+    int l = 1;
+    while (2 * l * (N / 8) <= threads) l *= 2;
+    while ((l * (N / 8)) % 64 != 0) l *= 2;
+    return l;
   }
   // Pass 1 (forward = false) or pass 3 (forward = true) of the fused operator; N = nx.
   template <int N>

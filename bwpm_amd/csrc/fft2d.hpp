@@ -216,7 +216,16 @@ class Fft2D {
 
  private:
   static bool fast_n(long long n) {
-    return n == 64 || n == 128 || n == 256 || n == 512 || n == 1024 || n == 2048;
+    return n == 64 || n == 128 || n == 256 || n == 512 || n == 1024 || n == 2048 || n == 192 || n == 384 ||
+           n == 768 || n == 1536;
+  }
+  // Lines per workgroup of the register kernels: a power of two with at most `threads` threads
+  // (at least one line).
+  template <int N>
+  static constexpr int fit_lines(int threads) {
+    int l = 1;
+    while (2 * l * (N / 8) <= threads) l *= 2;
+    return l;
   }
   void run(PlaneSet<T>& a, PlaneSet<T>& b, int np, bool inverse) {
     a.scale = 1;
@@ -238,7 +247,11 @@ class Fft2D {
         case 256: rows_fast_n<256, INV>(ps, np); return;
         case 512: rows_fast_n<512, INV>(ps, np); return;
         case 1024: rows_fast_n<1024, INV>(ps, np); return;
-        default: rows_fast_n<2048, INV>(ps, np); return;
+        case 2048: rows_fast_n<2048, INV>(ps, np); return;
+        case 192: rows_fast_n<192, INV>(ps, np); return;
+        case 384: rows_fast_n<384, INV>(ps, np); return;
+        case 768: rows_fast_n<768, INV>(ps, np); return;
+        default: rows_fast_n<1536, INV>(ps, np); return;
       }
     }
     const bool narrow = (ny_ + rows_ - 1) / rows_ * np < kNarrowBelow;
@@ -255,7 +268,7 @@ class Fft2D {
   }
   template <int N, bool INV>
   void rows_fast_n(const PlaneSet<T>& ps, int np) {
-    constexpr int RW = (sizeof(T) == 4 ? 512 : 256) / (N / 8) > 0 ? (sizeof(T) == 4 ? 512 : 256) / (N / 8) : 1;
+    constexpr int RW = fit_lines<N>(sizeof(T) == 4 ? 512 : 256);
     const auto* th = static_cast<const cx<T>*>(px_.tw_hi);
     const auto* tl = static_cast<const cx<T>*>(px_.tw_lo);
     if (ny_ % RW == 0)
@@ -275,7 +288,11 @@ class Fft2D {
         case 256: cols_fast_n<256, INV>(ps, np); return;
         case 512: cols_fast_n<512, INV>(ps, np); return;
         case 1024: cols_fast_n<1024, INV>(ps, np); return;
-        default: cols_fast_n<2048, INV>(ps, np); return;
+        case 2048: cols_fast_n<2048, INV>(ps, np); return;
+        case 192: cols_fast_n<192, INV>(ps, np); return;
+        case 384: cols_fast_n<384, INV>(ps, np); return;
+        case 768: cols_fast_n<768, INV>(ps, np); return;
+        default: cols_fast_n<1536, INV>(ps, np); return;
       }
     }
     const bool narrow = (nx_ + cols_ - 1) / cols_ * np < kNarrowBelow;
@@ -292,8 +309,8 @@ class Fft2D {
   }
   template <int N, bool INV>
   void cols_fast_n(const PlaneSet<T>& ps, int np) {
-    constexpr int W0 = (sizeof(T) == 4 ? 512 : 256) / (N / 8);
-    constexpr int W = W0 > 16 ? 16 : (W0 < 1 ? 1 : W0);
+    constexpr int W0 = fit_lines<N>(sizeof(T) == 4 ? 512 : 256);
+    constexpr int W = W0 > 16 ? 16 : W0;
     const auto* th = static_cast<const cx<T>*>(py_.tw_hi);
     const auto* tl = static_cast<const cx<T>*>(py_.tw_lo);
     if (nx_ % W == 0)

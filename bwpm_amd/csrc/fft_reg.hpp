@@ -388,6 +388,109 @@ __device__ __forceinline__ void reg_pass1(cx<T> (&v)[8], cx<T>* lds, int t, cons
   }
 }
 
+// Both components per thread, for plans with an irregular pass (lines of 3·2^k and 9·2^k points in
+// the 2D transforms and the Born operator): the passes of reg_pass1 on the two LDS planes, the E_x
+// plane and then the E_y plane (reg_pass keeps serving the all-regular plans).
+template <typename PL, int PASS, int NS, bool COMP, bool INV, bool STAGED, typename T>
+__device__ __forceinline__ void reg_pass2(px2<T> (&v)[8], cx<T>* lds, int t, const cx<T>* twh, const cx<T>* twl) {
+  constexpr int N = PL::N;
+  constexpr int L = reg_lds_len<N>();
+  if constexpr (PASS < PL::P) {
+    constexpr int R = PL::rad[PASS];
+    constexpr int TT = N / 8;
+    constexpr int M = N / R;
+    constexpr bool last = PASS == PL::P - 1;
+    constexpr bool regular = reg_regular(R);
+    constexpr bool in_regs = last && R == 8;
+    constexpr bool next_lds = !last && !reg_regular(PL::rad[PASS + 1 < PL::P ? PASS + 1 : 0]);
+    static_assert(PASS > 0 || (regular && NS == 1), "the first pass takes natural order from registers");
+    // the twiddles of butterfly j (shared by both components)
+    auto twiddles = [&](int j, cx<T>* wh, cx<T>* wl) {
+      const int k = j % NS;
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        const int e = STAGED ? plan_tw_off<PL>(PASS) + (r - 1) * NS + k : r * k * (N / (NS * R));
+        wh[r] = INV ? conj(twh[e]) : twh[e];
+        wl[r] = cx<T>{};
+        if constexpr (COMP) wl[r] = INV ? conj(twl[e]) : twl[e];
+      }
+    };
+    auto butterfly = [&](int j, int comp, cx<T>* x, const cx<T>* wh, const cx<T>* wl) {
+      const int k = j % NS;
+      if constexpr (NS > 1) {
+#pragma unroll
+        for (int r = 1; r < R; ++r) x[r] = cmul2<COMP>(x[r], wh[r], wl[r]);
+      }
+      dft_r<R, COMP, INV>(x);
+      if constexpr (!in_regs) {
+        const int base = (j - k) * R + k;
+#pragma unroll
+        for (int r = 0; r < R; ++r) lds[comp * L + lpad(base + r * NS)] = x[r];
+      }
+    };
+    if constexpr (regular) {
+      constexpr int B = 8 / R;
+#pragma unroll
+      for (int b = 0; b < B; ++b) {
+        const int j = t + TT * b;
+        cx<T> wh[R], wl[R];
+        if constexpr (NS > 1) twiddles(j, wh, wl);
+#pragma unroll
+        for (int comp = 0; comp < 2; ++comp) {
+          cx<T> x[R];
+#pragma unroll
+          for (int r = 0; r < R; ++r) x[r] = comp == 0 ? v[b + r * B].a : v[b + r * B].b;
+          butterfly(j, comp, x, wh, wl);
+          if constexpr (in_regs) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+              if (comp == 0) v[r].a = x[r];
+              else v[r].b = x[r];
+            }
+          }
+        }
+      }
+    } else {
+      // butterflies j = t + T·b of both planes; the last round covers j < M only
+      constexpr int B = (M + TT - 1) / TT;
+      cx<T> x[2][B][R];
+#pragma unroll
+      for (int b = 0; b < B; ++b) {
+        const int j = t + TT * b;
+        if (b * TT + TT <= M || j < M) {
+#pragma unroll
+          for (int comp = 0; comp < 2; ++comp)
+#pragma unroll
+            for (int r = 0; r < R; ++r) x[comp][b][r] = lds[comp * L + lpad(j + r * M)];
+        }
+      }
+      __syncthreads();  // every input is read before the outputs overwrite the lines
+#pragma unroll
+      for (int b = 0; b < B; ++b) {
+        const int j = t + TT * b;
+        if (b * TT + TT <= M || j < M) {
+          cx<T> wh[R], wl[R];
+          if constexpr (NS > 1) twiddles(j, wh, wl);
+          butterfly(j, 0, x[0][b], wh, wl);
+          butterfly(j, 1, x[1][b], wh, wl);
+        }
+      }
+    }
+    if constexpr (!in_regs) {
+      __syncthreads();
+      if constexpr (!next_lds) {
+#pragma unroll
+        for (int m = 0; m < 8; ++m) {
+          v[m].a = lds[lpad(t + TT * m)];
+          v[m].b = lds[L + lpad(t + TT * m)];
+        }
+        __syncthreads();
+      }
+    }
+    reg_pass2<PL, PASS + 1, NS * R, COMP, INV, STAGED>(v, lds, t, twh, twl);
+  }
+}
+
 // Transform one line held as v[m] = a[t + T·m]; on return v[m] = A[t + T·m]. All threads of the
 // workgroup must call it together (it synchronises the workgroup between passes). The px2 form
 // exchanges through 2·reg_lds_len<N>() LDS entries, the E_x plane and then the E_y plane, so that
@@ -398,7 +501,10 @@ __device__ __forceinline__ void reg_pass1(cx<T> (&v)[8], cx<T>* lds, int t, cons
 // the cx form one component.
 template <int N, bool COMP, bool INV, bool STAGED = false, typename T>
 __device__ __forceinline__ void reg_fft(px2<T> (&v)[8], cx<T>* lds, int t, const cx<T>* twh, const cx<T>* twl) {
-  reg_pass<N, 0, 1, COMP, INV, STAGED>(v, lds, t, twh, twl);
+  if constexpr (plan_all_regular<RegPlan<N>>())
+    reg_pass<N, 0, 1, COMP, INV, STAGED>(v, lds, t, twh, twl);
+  else
+    reg_pass2<RegPlan<N>, 0, 1, COMP, INV, STAGED>(v, lds, t, twh, twl);
 }
 template <int N, bool COMP, bool INV, bool STAGED = false, typename T>
 __device__ __forceinline__ void reg_fft(cx<T> (&v)[8], cx<T>* lds, int t, const cx<T>* twh, const cx<T>* twl) {

@@ -428,6 +428,40 @@ def test_born_solvers_agree_with_the_oracle(solver: str, monkeypatch) -> None:
             assert np.max(np.abs(got[key] - ref[key])) < 1e-10 * scale, (sigma, key)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize(("ny", "nx"), [(192, 192), (192, 384)])
+def test_born_on_three_times_power_of_two_grids(precision: str, ny: int, nx: int) -> None:
+    """Born slabs on grids of 3·2^k points, which run the register 2D transforms and the fused
+    operator with a radix-3 or radix-6 pass (both components per thread), against the oracle over
+    two iterations: FP64 to 1e-10 of the field's maximum, FP32 to 2e-5."""
+    from bwpm_amd import _core
+
+    rng = np.random.default_rng(11)
+    nz = 4
+    dx, dz, lam = 0.1e-3, 0.15e-3, 0.8
+    k0 = 2 * np.pi / (lam * 1e-3)
+    labels = np.zeros((nz, ny, nx), dtype=np.int32)
+    fl = np.zeros((nz, ny, nx), complex)
+    fl[1:nz - 1] = 0.02 * rng.normal(size=(nz - 2, ny, nx))
+    x = (np.arange(nx) - nx / 2) * dx
+    y = (np.arange(ny) - ny / 2) * dx
+    field = np.zeros((ny, nx, 2), complex)
+    field[..., 0] = np.exp(-(x[None, :] ** 2 + y[:, None] ** 2) / (2.0e-3) ** 2)
+    field[..., 1] = 0.3j * field[..., 0]
+    print("born 3·2^k", precision, ny, nx, flush=True)
+    s = _core.GpuSolver(ny, nx, dx, dx, dz, k0, np.array([1.5]), labels, 1.5 + 0j, None, fluctuation=fl,
+                        precision=precision)
+    try:
+        got = s.run(field, 2)
+    finally:
+        s.close()
+    ref = O.Solver(ny, nx, dx, dx, dz, k0, [1.5], labels, 1.5 + 0j, None, fluctuation=fl).run(field, 2)
+    tol = 1e-10 if precision == "fp64" else 2e-5
+    for key in ("transmitted", "reflected"):
+        scale = max(float(np.max(np.abs(field))), float(np.max(np.abs(ref[key]))))
+        assert np.max(np.abs(got[key] - ref[key])) < tol * scale, key
+
+
 def test_device_equals_reference_core_born() -> None:
     """Born slabs (δn, device GMRES) with the back-scattering paths over three iterations against
     the reference's C++ core, 64² × 12 slabs: agreement to 1e-10 of the field's maximum (GMRES stops
