@@ -200,6 +200,19 @@ class Fft2D {
   const FftPlan& row_plan() const { return px_; }
   const FftPlan& col_plan() const { return py_; }
 
+  // The column pass alone, for callers that made the row pass themselves: out = F_y(in) or
+  // F_y⁻¹(in)/N_pixels (the scale of the whole inverse 2D transform).
+  void cols(const px2<T>* in, px2<T>* out, bool inverse) {
+    PlaneSet<T> b{};
+    b.in[0] = in;
+    b.out[0] = out;
+    b.scale = inverse ? static_cast<T>(1.0 / static_cast<double>(npix_)) : static_cast<T>(1);
+    if (inverse)
+      cols_pass<true>(b, 1);
+    else
+      cols_pass<false>(b, 1);
+  }
+
   // out = F(in·pre) or F⁻¹(in·pre) (1/N scaled; pre: per-pixel multiplier of the input, nullptr:
   // none, only where the row pass runs the register kernels); `in` is left unchanged unless it is
   // `out`.
