@@ -1,6 +1,8 @@
 // Request queue with continuous batching on one device (see serve.hpp).
 #include "serve.hpp"
 
+#include <cstdlib>
+
 #include <algorithm>
 #include <set>
 
@@ -50,7 +52,13 @@ Queue::Queue(int device, double budget_bytes, int slots, int chunk)
   budget_ = budget_bytes > 0 ? std::min(budget_bytes, static_cast<double>(total_b)) : 0.9 * static_cast<double>(free_b);
   for (int i = 0; i < kAdmitWorkers; ++i) admit_threads_.emplace_back([this] { admit_loop(); });
   complete_thread_ = std::thread([this] { complete_loop(); });
-  for (int i = 0; i < kBornWorkers; ++i) born_threads_.emplace_back([this] { born_loop(); });
+  // Born solves run one per worker thread and stream (BWPM_BORN_WORKERS, 1 … 16, overrides the count)
+  int born_workers = kBornWorkers;
+  if (const char* env = std::getenv("BWPM_BORN_WORKERS")) {
+    const int v = std::atoi(env);
+    if (v >= 1 && v <= 16) born_workers = v;
+  }
+  for (int i = 0; i < born_workers; ++i) born_threads_.emplace_back([this] { born_loop(); });
 }
 
 Queue::~Queue() {

@@ -102,7 +102,7 @@ class Queue {
   bool any_submit_ = false;
 
   static constexpr int kAdmitWorkers = 4;
-  static constexpr int kBornWorkers = 4;
+  static constexpr int kBornWorkers = 8;
   std::thread complete_thread_;
   std::vector<std::thread> admit_threads_, born_threads_;
 };
