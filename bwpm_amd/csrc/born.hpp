@@ -453,8 +453,9 @@ __global__ __launch_bounds__(N / 8 * RW) void k_born_op_rows_fwd(const px2<T>* _
 // The same values as k_born_src_prepare → F⁻¹ → ×V → F (three planes) → k_born_emit → F⁻¹, with
 // the transform axes ordered like the fused operator: inverse rows of the three source planes
 // (with the preparation in front), inverse columns × V·(1/N_pixels) × forward columns, forward rows
-// with the emission and the inverse row transform of its result behind, and the inverse column
-// transform (Fft2D). 16 plane passes over HBM instead of 36, and four launches instead of eight.
+// with the emission and the inverse row transform of its result behind (one Gauss point per
+// workgroup), and the inverse column transform of the sum of the two terms. 19 plane passes over
+// HBM instead of 36, and four launches instead of eight.
 
 // Pass 1, one row per RW·N/8 threads: blockIdx.y = m < 2 the sources (Ê_x, Ê_y)·e^{i k_z s_m} of
 // Gauss point m, blockIdx.y = 2 the z pair (Ê_z·e^{i k_z s_0}, Ê_z·e^{i k_z s_1}), Ê_z =
@@ -529,15 +530,16 @@ __global__ __launch_bounds__(N / 8 * W) void k_born_em_cols(px2<T>* p0, px2<T>* 
   for (int m = 0; m < 8; ++m) pl[static_cast<long long>(t + TT * m) * nx + x] = a[m];
 }
 
-// Pass 3, one row per RW·N/8 threads: forward row transforms of pz, p0 and p1, the emission in
-// direction d′ = dp, Σ_m ge_m·[P(k_d′)·S_m]⊥ (as k_born_emit), and its inverse row transform
-// (unscaled) into tot.
+// Pass 3, one row per RW·N/8 threads, Gauss point m = blockIdx.y: forward row transforms of plane
+// p_m and of component m of pz, the emission term ge_m·[P(k_d′)·S_m]⊥ in direction d′ = dp (as
+// k_born_emit), and its inverse row transform (unscaled), in place into p_m. The two terms are
+// summed by the column pass (the transform is linear); one term per workgroup keeps the kernel
+// within its registers (both in one spilled: 34 µs at 512² FP64).
 template <int N, int RW, typename T, bool COMP>
-__global__ __launch_bounds__(N / 8 * RW) void k_born_em_rows_fwd(const px2<T>* __restrict__ p0,
-                                                                const px2<T>* __restrict__ p1,
+__global__ __launch_bounds__(N / 8 * RW) void k_born_em_rows_fwd(px2<T>* p0, px2<T>* p1,
                                                                 const px2<T>* __restrict__ pz, BornF<T> tb,
                                                                 const T* __restrict__ kxT, const T* __restrict__ kyT,
-                                                                T dp, int dir, cx<T> inv_k2, px2<T>* __restrict__ tot,
+                                                                T dp, int dir, cx<T> inv_k2,
                                                                 const cx<T>* __restrict__ g_twh,
                                                                 const cx<T>* __restrict__ g_twl) {
   constexpr int TT = N / 8;
@@ -552,38 +554,63 @@ __global__ __launch_bounds__(N / 8 * RW) void k_born_em_rows_fwd(const px2<T>* _
   const int r = threadIdx.x / TT, t = threadIdx.x % TT;
   const int y = blockIdx.x * RW + r;
   const long long row = static_cast<long long>(y) * N;
+  const int g = blockIdx.y;
   cx<T>* lds = sh + r * LQ;
+  px2<T>* pg = g == 0 ? p0 : p1;
   const T ky = kyT[y];
-  px2<T> z[8], acc[8];
+  cx<T> z[8];
 #pragma unroll
-  for (int m = 0; m < 8; ++m) z[m] = pz[row + t + TT * m];
-  reg_fft<N, COMP, false, TWL>(z, lds, t, twh, twl);
-#pragma unroll
-  for (int g = 0; g < 2; ++g) {
-    px2<T> s[8];
-#pragma unroll
-    for (int m = 0; m < 8; ++m) s[m] = (g == 0 ? p0 : p1)[row + t + TT * m];
-    reg_fft<N, COMP, false, TWL>(s, lds, t, twh, twl);
-#pragma unroll
-    for (int m = 0; m < 8; ++m) {
-      const long long q = row + t + TT * m;
-      const T kx = kxT[t + TT * m];
-      const cx<T> skz = tb.kz()[q] * dp;
-      const cx<T> s2 = g == 0 ? z[m].a : z[m].b;
-      const cx<T> lon = (s[m].a * kx + s[m].b * ky + skz * s2) * inv_k2;
-      const cx<T> ge = tb.ge(dir, g)[q];
-      const px2<T> term{ge * (s[m].a - lon * kx), ge * (s[m].b - lon * ky)};
-      if (g == 0) {
-        acc[m] = px2<T>{cx<T>{0, 0} + term.a, cx<T>{0, 0} + term.b};
-      } else {
-        acc[m].a = acc[m].a + term.a;
-        acc[m].b = acc[m].b + term.b;
-      }
-    }
+  for (int m = 0; m < 8; ++m) {
+    const px2<T> v = pz[row + t + TT * m];
+    z[m] = g == 0 ? v.a : v.b;
   }
-  reg_fft<N, COMP, true, TWL>(acc, lds, t, twh, twl);
+  reg_fft<N, COMP, false, TWL>(z, lds, t, twh, twl);
+  px2<T> s[8];
 #pragma unroll
-  for (int m = 0; m < 8; ++m) tot[row + t + TT * m] = acc[m];
+  for (int m = 0; m < 8; ++m) s[m] = pg[row + t + TT * m];
+  reg_fft<N, COMP, false, TWL>(s, lds, t, twh, twl);
+#pragma unroll
+  for (int m = 0; m < 8; ++m) {
+    const long long q = row + t + TT * m;
+    const T kx = kxT[t + TT * m];
+    const cx<T> skz = tb.kz()[q] * dp;
+    const cx<T> lon = (s[m].a * kx + s[m].b * ky + skz * z[m]) * inv_k2;
+    const cx<T> ge = tb.ge(dir, g)[q];
+    s[m] = px2<T>{ge * (s[m].a - lon * kx), ge * (s[m].b - lon * ky)};
+  }
+  reg_fft<N, COMP, true, TWL>(s, lds, t, twh, twl);
+#pragma unroll
+  for (int m = 0; m < 8; ++m) pg[row + t + TT * m] = s[m];
+}
+
+// Pass 4, W columns per workgroup: the sum of the two row-transformed terms, inverse column
+// transform, ×scale (1/N_pixels of the inverse 2D transform), into out.
+template <int N, int W, typename T, bool COMP>
+__global__ __launch_bounds__(N / 8 * W) void k_born_em_cols_inv(const px2<T>* __restrict__ t0,
+                                                               const px2<T>* __restrict__ t1, px2<T>* __restrict__ out,
+                                                               int nx, T scale, const cx<T>* __restrict__ g_twh,
+                                                               const cx<T>* __restrict__ g_twl) {
+  constexpr int TT = N / 8;
+  constexpr int LQ = lds_stride(2 * reg_lds_len<N>(), W, sizeof(cx<T>));
+  __shared__ cx<T> sh[W * LQ];
+  constexpr bool TWL = op_tw_in_lds<N, T, COMP>(sizeof(sh));
+  __shared__ cx<T> tw_hi_s[TWL ? reg_tw_len<N>() : 1];
+  __shared__ cx<T> tw_lo_s[TWL && COMP ? reg_tw_len<N>() : 1];
+  if constexpr (TWL) stage_twiddles<N, COMP>(tw_hi_s, tw_lo_s, g_twh, g_twl);
+  const cx<T>* twh = TWL ? tw_hi_s : g_twh;
+  const cx<T>* twl = TWL ? tw_lo_s : g_twl;
+  const int c = threadIdx.x % W, t = threadIdx.x / W;
+  const int x = xcd_group(blockIdx.x, gridDim.x) * W + c;
+  px2<T> a[8];
+#pragma unroll
+  for (int m = 0; m < 8; ++m) {
+    const long long q = static_cast<long long>(t + TT * m) * nx + x;
+    const px2<T> u = t0[q], v = t1[q];
+    a[m] = px2<T>{u.a + v.a, u.b + v.b};
+  }
+  reg_fft<N, COMP, true, TWL>(a, sh + c * LQ, t, twh, twl);
+#pragma unroll
+  for (int m = 0; m < 8; ++m) out[static_cast<long long>(t + TT * m) * nx + x] = px2<T>{a[m].a * scale, a[m].b * scale};
 }
 
 // x = P_½·Ê (which = 0), y·P_½ (which = 1, direction +1) or y·P_½⁻¹ (which = 2, direction −1).
@@ -1400,7 +1427,6 @@ class Born {
     bool fused = false;
     if constexpr (COMP == (sizeof(T) == 4)) fused = f.fast();
     if (fused) {
-      px2<T>* tot = etot_.template as<px2<T>>();
       const auto* rth = static_cast<const cx<T>*>(f.row_plan().tw_hi);
       const auto* rtl = static_cast<const cx<T>*>(f.row_plan().tw_lo);
       const auto* cth = static_cast<const cx<T>*>(f.col_plan().tw_hi);
@@ -1423,13 +1449,18 @@ class Born {
       });
       with_line(nx_, [&]<int N>() {
         constexpr int RW = op_lines<N>(kOpThreads);
-        hipLaunchKernelGGL((k_born_em_rows_fwd<N, RW, T, COMP>), dim3(static_cast<unsigned>(ny_ / RW)),
-                           dim3(N / 8 * RW), 0, st, static_cast<const px2<T>*>(x0), static_cast<const px2<T>*>(x1),
-                           static_cast<const px2<T>*>(zz), tf, kx, ky, static_cast<T>(dp), dp > 0 ? 0 : 1,
-                           inv_k2(slab), tot, rth, rtl);
+        hipLaunchKernelGGL((k_born_em_rows_fwd<N, RW, T, COMP>), dim3(static_cast<unsigned>(ny_ / RW), 2),
+                           dim3(N / 8 * RW), 0, st, x0, x1, static_cast<const px2<T>*>(zz), tf, kx, ky,
+                           static_cast<T>(dp), dp > 0 ? 0 : 1, inv_k2(slab), rth, rtl);
         HIP_CHECK(hipGetLastError());
       });
-      f.cols(tot, out, true);
+      with_line(ny_, [&]<int N>() {
+        constexpr int W = op_lines<N>(kOpThreads);
+        hipLaunchKernelGGL((k_born_em_cols_inv<N, W, T, COMP>), dim3(static_cast<unsigned>(nx_ / W)),
+                           dim3(N / 8 * W), 0, st, static_cast<const px2<T>*>(x0), static_cast<const px2<T>*>(x1),
+                           out, static_cast<int>(nx_), static_cast<T>(1.0 / static_cast<double>(npix_)), cth, ctl);
+        HIP_CHECK(hipGetLastError());
+      });
       return;
     }
     launch_on(st, k_born_src_prepare<T>, static_cast<const px2<T>*>(ehat_.template as<px2<T>>()), tf,
