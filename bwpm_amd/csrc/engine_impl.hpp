@@ -891,8 +891,26 @@ class Engine final : public EngineBase {
         ++steps;
       }
     }
+    // Rows of one point (2D solves, nx = 1): the row transforms are the identity, so a step whose
+    // row work only copies its one product into X (no mask, sources, Born terms or outputs) has the
+    // column kernel write straight into X, in place, and no row launch.
+    if (sp_.nx == 1)
+      for (Op<T, ST>& o : ops)
+        if (o.kind == kOpPair && row_is_copy(o, X)) {
+          o.col.out = X;
+          o.row = RowDesc<T, ST>{};
+        }
     last_steps_ = steps;
     return ops;
+  }
+
+  static bool row_is_copy(const Op<T, ST>& o, const px2<ST>* X) {
+    const RowDesc<T, ST>& r = o.row;
+    return o.col.in != nullptr && o.col.nprod == 1 && o.col.nt == 1 && o.col.rowuse == nullptr && r.active != 0 &&
+           r.nt == 1 && r.nr == 0 && r.prod == o.col.out && r.tsel == nullptr && r.rowuse == nullptr &&
+           r.mask == nullptr && r.mask_y == nullptr && r.e_in == nullptr && r.inc == nullptr && r.rowptr == nullptr &&
+           r.sp_add == nullptr && r.sp_store == nullptr && r.dense == nullptr && r.e_out == nullptr &&
+           r.e_out2 == nullptr && r.x_out == X;
   }
 
   // Run the programs of every member (same grid and precision) in lockstep on `stream`.
@@ -1132,10 +1150,13 @@ class Engine final : public EngineBase {
           }
         HIP_CHECK(hipMemcpyAsync(dcol.as<void>(), hc, G * S * sizeof(ColDesc<T, ST>), hipMemcpyHostToDevice, stream));
         HIP_CHECK(hipMemcpyAsync(drow.as<void>(), hr, G * S * sizeof(RowDesc<T, ST>), hipMemcpyHostToDevice, stream));
-        // uniform chunk: every step is one plain column and one plain row launch
+        // uniform chunk: every step is one plain column launch and either one plain row launch
+        // or, in every step, none (rows of one point whose row work is a copy)
         bool uniform = true;
+        const bool rows_all = G > 0 && has_row[0];
         for (size_t g = 0; g < G; ++g)
-          uniform = uniform && has_col[g] && has_row[g] && !multi_col[g] && !multi_row[g] && plain_row[g] && plain_col[g];
+          uniform = uniform && has_col[g] && has_row[g] == rows_all && !multi_col[g] && plain_col[g] &&
+                    (!rows_all || (!multi_row[g] && plain_row[g]));
         int active = 0;
         for (int i = 0; i < S; ++i) active += slot[i] != nullptr;
         const TuneKey tk{slot_device(slot), precision_code(), S, L.ny, L.nx};
@@ -1164,7 +1185,7 @@ class Engine final : public EngineBase {
         };
         Emit eager;
         eager.stream = stream;
-        if (uniform && trial >= 0) {
+        if (uniform && rows_all && trial >= 0) {
           // a timed trial of a tuning candidate
           hipEvent_t a, b;
           HIP_CHECK(hipEventCreate(&a));
@@ -1181,7 +1202,7 @@ class Engine final : public EngineBase {
         } else if (uniform && graphs_ok) {
           // uniform chunks replay one captured graph per shape (descriptors live in dcol/drow)
           const auto gk = std::make_tuple(G, Lc.geo.col_w, Lc.geo.rw, Lc.narrow_mask(S),
-                                          Lc.geo.plain_col * 2 + Lc.geo.plain_row);
+                                          Lc.geo.plain_col * 2 + Lc.geo.plain_row + (rows_all ? 0 : 4));
           auto it = graphs.find(gk);
           if (it == graphs.end()) {
             // the graph is built node by node (no stream capture, so other threads of the
