@@ -22,6 +22,7 @@
 
 #include <cmath>
 #include <complex>
+#include <functional>
 #include <map>
 #include <mutex>
 #include <vector>
@@ -1144,6 +1145,7 @@ class Born {
     fft_.init(ny_, nx_, px_, py_, stream_);
     HIP_CHECK(hipEventCreateWithFlags(&ev_hat_, hipEventDisableTiming));
     HIP_CHECK(hipEventCreateWithFlags(&ev_emit_, hipEventDisableTiming));
+    HIP_CHECK(hipEventCreateWithFlags(&ev_state_, hipEventDisableTiming));
     HIP_CHECK(hipStreamSynchronize(stream_));
     bytes_ = V_.bytes() + tabs_.bytes() + fields_.bytes() + optab_.bytes() + 13 * pb + basis_.bytes() +
              planes_.bytes();
@@ -1152,6 +1154,7 @@ class Born {
     if (estream_) (void)hipStreamSynchronize(estream_);
     if (ev_hat_) (void)hipEventDestroy(ev_hat_);
     if (ev_emit_) (void)hipEventDestroy(ev_emit_);
+    if (ev_state_) (void)hipEventDestroy(ev_state_);
     if (hres_) (void)hipHostFree(hres_);
     if (hgm_) (void)hipHostFree(hgm_);
   }
@@ -1489,15 +1492,22 @@ class Born {
       rich_part_ = partial_.template as<double2>();
       apply_g(ehat_.template as<px2<T>>(), d, slab, b, 3, nullptr, fields(slab).half());
       rich_part_ = nullptr;
-      if (!solve(d, slab, rich_count_))
+      // While few Born steps run in the process, the increment's transform is enqueued behind
+      // every chunk of Richardson steps, before the host waits for the chunk's state, so the
+      // device has it to run while the host wakes up; a chunk that did not converge has it redone
+      // behind the next one (it only writes inc_). With more, the device has their work instead.
+      const auto tail = [&] { fft_.one(x, inc_.template as<px2<T>>(), true, hout); };
+      const bool ahead = born_active_steps().load() <= kSideMax;
+      bool tail_done = false;
+      if (!solve(d, slab, rich_count_, ahead ? std::function<void()>(tail) : std::function<void()>(), &tail_done))
         throw Error(kEnoconv, "GMRES did not converge in the Born step");
-      fft_.one(x, inc_.template as<px2<T>>(), true, hout);
+      if (!tail_done) tail();
       return;
     }
     px2<T>* xh = xh_.template as<px2<T>>();
     launch(k_born_half<T>, static_cast<const px2<T>*>(ehat_.template as<px2<T>>()), fields(slab).half(), npix_, xh);
     apply_g(xh, d, slab, b, 0);
-    if (!solve(d, slab, 0))
+    if (!solve(d, slab, 0, nullptr, nullptr))
       throw Error(kEnoconv, "GMRES did not converge in the Born step");
     px2<T>* y = tmp_.template as<px2<T>>();
     launch(k_born_half<T>, static_cast<const px2<T>*>(x), hout, npix_, y);
@@ -1631,7 +1641,10 @@ class Born {
   // BWPM_BORN_SOLVER=gmres). Both stop at ‖b − (1 − G/2)x‖ ≤ tol·‖b‖. The fused Richardson step
   // reads the start value from b itself; every other path copies it into x_ first.
   // `bparts` > 0: partial_ already holds that many partial sums of ‖b‖² (the operator's mode 3).
-  bool solve(int d, long long slab, int bparts) {
+  // `tail` (may be empty): work that only reads x, enqueued behind each chunk of Richardson steps;
+  // *tail_done tells whether it ran behind the chunk that converged.
+  bool solve(int d, long long slab, int bparts, const std::function<void()>& tail, bool* tail_done) {
+    if (tail_done) *tail_done = false;
     const px2<T>* b = rhs_.template as<px2<T>>();
     px2<T>* x = x_.template as<px2<T>>();
     if (!richardson_) {
@@ -1659,7 +1672,10 @@ class Born {
       HIP_CHECK(hipGetLastError());
       nbsq = kRedBlocks;
     }
-    if (richardson(d, slab, fused, bsq, nbsq)) return true;
+    if (richardson(d, slab, fused, bsq, nbsq, tail)) {
+      if (tail_done) *tail_done = static_cast<bool>(tail);
+      return true;
+    }
     tol_abs_ = hgm_->tol_abs;  // from the state read after the last chunk of steps
     return gmres(d, slab);
   }
@@ -1671,7 +1687,8 @@ class Born {
   // iterate, and the returned iterate's residual is smaller still). The steps are enqueued in
   // chunks, gated by the state's done flag like the Arnoldi steps; returns false (x holding the
   // last iterate) when the contraction is too slow.
-  bool richardson(int d, long long slab, bool fused, const double2* bsq, int nbsq) {
+  bool richardson(int d, long long slab, bool fused, const double2* bsq, int nbsq,
+                  const std::function<void()>& tail) {
     const px2<T>* b = rhs_.template as<px2<T>>();
     px2<T>* x = x_.template as<px2<T>>();
     px2<T>* w = w_.template as<px2<T>>();
@@ -1705,7 +1722,13 @@ class Born {
       }
       // the state's head (done, used, the step sizes and tol·‖b‖), not the GMRES arrays
       HIP_CHECK(hipMemcpyAsync(hgm_, st, offsetof(GmState, g), hipMemcpyDeviceToHost, stream_));
-      HIP_CHECK(hipStreamSynchronize(stream_));
+      if (tail) {
+        HIP_CHECK(hipEventRecord(ev_state_, stream_));
+        tail();
+        HIP_CHECK(hipEventSynchronize(ev_state_));
+      } else {
+        HIP_CHECK(hipStreamSynchronize(stream_));
+      }
       if (hgm_->done != 0) {
         iterations_ += hgm_->used;
         if (hgm_->done == 1) rich_expect_ = std::max(1, hgm_->used);
@@ -1800,6 +1823,7 @@ class Born {
   hipStream_t estream_ = nullptr;  // the emission's stream (born_side_stream), from the first use
   hipEvent_t ev_hat_ = nullptr;    // Ê written (caller's stream)
   hipEvent_t ev_emit_ = nullptr;   // emission done (emission stream)
+  hipEvent_t ev_state_ = nullptr;  // the Richardson state copied to the host (caller's stream)
   Dev partial_, dres_, coef_;
   double2* hres_ = nullptr;
   Dev gm_;                      // GmState of the current GMRES cycle
