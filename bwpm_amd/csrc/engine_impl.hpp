@@ -189,9 +189,14 @@ struct TuneChoice {
   bool pc = true, pr = true;
   bool operator==(const TuneChoice& o) const { return cw == o.cw && rw == o.rw && pc == o.pc && pr == o.pr; }
 };
+// Timed stretches per candidate: each candidate is timed this many times, in rounds over all
+// candidates, and keeps its fastest time (one sample each let a noisy stretch pick a column
+// kernel 12 % slower than the best for the headline lane).
+inline constexpr int kTuneSamples = 3;
 struct TuneState {
   std::vector<TuneChoice> cand;
-  std::vector<double> us;  // measured time per solve-step, < 0: not yet
+  std::vector<double> us;  // fastest measured time per solve-step, < 0: not yet
+  std::vector<int> n;      // samples taken
   int best = -1;
 };
 inline std::mutex& tune_mutex() {
@@ -219,6 +224,7 @@ inline TuneChoice tune_pick(const TuneKey& key, const Geometry& g, int& trial) {
             if (std::find(st.cand.begin(), st.cand.end(), c) == st.cand.end()) st.cand.push_back(c);
           }
     st.us.assign(st.cand.size(), -1.0);
+    st.n.assign(st.cand.size(), 0);
     if (st.cand.size() == 1) st.best = 0;
     // BWPM_LINES="<columns>,<rows>[,<plain columns>,<plain rows>]" fixes the choice when it is a
     // candidate (reproducible profiles: hardware counters slow kernels unevenly, which can sway the
@@ -234,20 +240,19 @@ inline TuneChoice tune_pick(const TuneKey& key, const Geometry& g, int& trial) {
   }
   trial = -1;
   if (st.best >= 0) return st.cand[st.best];
-  for (size_t i = 0; i < st.cand.size(); ++i)
-    if (st.us[i] < 0) {
-      trial = static_cast<int>(i);
-      return st.cand[i];
-    }
-  return st.cand[0];
+  // the candidate with the fewest samples (the first of them): rounds over all candidates
+  const auto it = std::min_element(st.n.begin(), st.n.end());
+  trial = static_cast<int>(it - st.n.begin());
+  return st.cand[trial];
 }
 
 inline void tune_report(const TuneKey& key, int trial, double us_per_solve_step) {
   std::lock_guard<std::mutex> lock(tune_mutex());
   TuneState& st = tune_cache()[key];
   if (trial < 0 || trial >= static_cast<int>(st.us.size()) || st.best >= 0) return;
-  st.us[trial] = us_per_solve_step;
-  if (std::all_of(st.us.begin(), st.us.end(), [](double v) { return v >= 0; }))
+  if (st.us[trial] < 0 || us_per_solve_step < st.us[trial]) st.us[trial] = us_per_solve_step;
+  ++st.n[trial];
+  if (std::all_of(st.n.begin(), st.n.end(), [](int k) { return k >= kTuneSamples; }))
     st.best = static_cast<int>(std::min_element(st.us.begin(), st.us.end()) - st.us.begin());
 }
 

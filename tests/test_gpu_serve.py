@@ -121,9 +121,10 @@ def test_poisson_stream_of_the_served_mix() -> None:
 def test_replayed_lane_graphs_match_eager_solves() -> None:
     # slab steps in a homogeneous medium make uniform chunks: the lane captures one graph and
     # replays it for every chunk of every request, with eager work of other requests in between
+    # (deep enough that chunks remain after the tile tuner has timed its candidates)
     reqs = []
     for i in range(6):
-        p = workloads.tilted_3d(n=64, nz=48 + 8 * i, iterations=1 + i % 2, polarisation="xy"[i % 2])
+        p = workloads.tilted_3d(n=64, nz=200 + 8 * i, iterations=1 + i % 2, polarisation="xy"[i % 2])
         p.labels = np.zeros_like(p.labels)
         p.indices = [1.33 + 0j]
         p.index_before = 1.33 + 0j
