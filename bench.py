@@ -345,6 +345,14 @@ def served(count: int, precision: str, check: int = 3) -> dict:
 
 # ---- CPU baseline ------------------------------------------------------------------------------
 
+def fft_plans_chosen() -> list:
+    """The radix plans the plan tuner chose in this process (per precision, direction and line
+    length: 0 default, 1 alternative, with the timed launch of each in µs)."""
+    from bwpm_amd import _core
+
+    return _core.fft_plans()
+
+
 def host_cores() -> dict:
     """Physical cores from lscpu, its CPU model, and the CPUs this process may run on."""
     info: dict = {"allowed_cpus": len(os.sched_getaffinity(0))}
@@ -576,6 +584,7 @@ def main() -> int:
         "configs": other,
         "serving": serving,
         "cpu_baseline": cpu,
+        "fft_plans": fft_plans_chosen(),
     }
     print(json.dumps(line), flush=True)
     return 0
