@@ -153,10 +153,10 @@ struct GmState {
 // Richardson iteration state in a GmState: k_rich_begin starts it (done = 0, used = 0, the last
 // step size in beta = ∞, tol_abs = tol·‖b‖ from the `count` partial sums of ‖b‖² in bsq[i].x);
 // k_rich_check takes ‖x_new − x_old‖ from the `count` partial sums of the step (k_rich_update, or
-// the operator's last pass in mode 2), counts the step, and sets done = 1 when it is at most tol_abs, or done = 2 when the iteration
-// does not contract fast enough (a step larger than half the previous one, or max_steps steps),
-// after which the caller switches to GMRES. One workgroup of kRedThreads threads; both return at
-// once when the state is done.
+// the operator's last pass in mode 2), counts the step, and sets done = 1 when it is at most
+// tol_abs, or done = 2 when the iteration does not contract fast enough (a step larger than half
+// the previous one, or max_steps steps), after which the caller switches to GMRES. One workgroup
+// of kRedThreads threads; the check returns at once when the state is done.
 __global__ __launch_bounds__(kRedThreads) void k_rich_begin(GmState* st, const double2* __restrict__ bsq, int count,
                                                             double tol);
 __global__ __launch_bounds__(kRedThreads) void k_rich_check(const double2* __restrict__ partial, int count,
